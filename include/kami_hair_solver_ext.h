@@ -1,0 +1,98 @@
+/*
+ * kami_hair_solver_ext.h - additive entry points of the MI355X hair solver.
+ *
+ * None of these has a counterpart in the reference ABI v5; KHS_ABI_VERSION
+ * stays 5 so existing callers load the library unchanged. They serve three
+ * kinds of caller:
+ *  - parity tests, which read and write the solver state and evaluate the
+ *    incremental potential at a given state (khsx* "state" and "evaluate");
+ *  - benchmarks, which select the convergence rule and arithmetic mode;
+ *  - the bake server and batched runs (khsxJob*, in kami_hair_server.h).
+ * The CPU oracle library exports the same khsx functions, so one test
+ * drives both.
+ */
+#ifndef KAMI_HAIR_SOLVER_EXT_H
+#define KAMI_HAIR_SOLVER_EXT_H
+
+#include "kami_hair_solver.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Number of doubles in q (and in v): 6 per internal node. */
+KHS_API uint64_t khsxGetDofCount(const KhsSolver *solver);
+
+/* Copies the generalized coordinates q and velocities v (6 doubles per node:
+   x, y, z, rotation vector). Either pointer may be NULL. */
+KHS_API KhsResult khsxGetState(const KhsSolver *solver, double *q, double *v,
+                               uint64_t count);
+
+/* Sets q, the previous-substep coordinates q_old and v. q_old may be NULL,
+   in which case q_old = q. */
+KHS_API KhsResult khsxSetState(KhsSolver *solver, const double *q,
+                               const double *q_old, const double *v,
+                               uint64_t count);
+
+/* Rest model built by khsBuild. Per node (count = internal node count):
+   reference frame, row-major 9 doubles; mass; rotational mass; flags
+   (bit 0 fixed, bit 1 virtual extension). Per element (count = element
+   count): node pair (2 uint32), strand, contact flag; rest length, rest
+   shear (3), rest curvature (3). Any pointer may be NULL. */
+KHS_API KhsResult khsxCopyNodeModel(const KhsSolver *solver, double *frames9,
+                                    double *mass, double *rotational_mass,
+                                    uint32_t *flags, uint32_t capacity);
+KHS_API KhsResult khsxCopyElementModel(const KhsSolver *solver, uint32_t *nodes2,
+                                       uint32_t *strand, uint32_t *contact,
+                                       double *rest7, uint32_t capacity);
+
+typedef struct KhsxEvaluation {
+    uint32_t struct_size;
+    uint32_t feasible;
+    uint32_t solve_failed;
+    uint32_t reserved;
+    uint64_t candidate_pairs;
+    uint64_t active_contacts;
+    double objective;
+    double elastic;
+    double contact;
+    double kinetic;
+    double minimum_gap;
+} KhsxEvaluation;
+
+/* Evaluates the incremental potential of one substep of length dt at the
+   current state (q, q_old, v, current collider): energies, the gradient
+   (6 per node, zero on fixed nodes) and, if direction is not NULL, the
+   block-tridiagonal Gauss-Newton direction -H^-1 g before trust-radius
+   scaling. gradient may be NULL. */
+KHS_API KhsResult khsxEvaluate(KhsSolver *solver, double dt, double *gradient,
+                               double *direction, KhsxEvaluation *out);
+
+/* Broad phase at the current state: every (element, triangle) pair whose
+   axis-aligned boxes overlap with padding radius + offset + barrier
+   distance, for contact-enabled elements with a free left node. Pairs are
+   written as (element, triangle) uint32 pairs, sorted; *count receives the
+   total even when capacity is too small. */
+KHS_API KhsResult khsxBroadphasePairs(KhsSolver *solver, uint32_t *pairs,
+                                      uint64_t capacity, uint64_t *count);
+
+/* Convergence rule. 0 (default): the reference rule, with the relative and
+   absolute tolerances floored at 1e-4 and 1e-5 sqrt(dof). 1: floors lifted,
+   so the desc tolerances apply as given (used to run to a 1e-6 residual). */
+KHS_API KhsResult khsxSetConvergenceMode(KhsSolver *solver, uint32_t mode);
+
+/* Arithmetic mode of the GPU step. 0 (default): FP64 throughout.
+   1: mixed precision (FP32 assembly and block solve with FP64 residual and
+   refinement; FP64 barrier, CCD and convergence test). The oracle accepts
+   only 0. */
+KHS_API KhsResult khsxSetPrecisionMode(KhsSolver *solver, uint32_t mode);
+
+/* Worker threads the CPU oracle uses over strands (oracle only; the GPU
+   library ignores it and returns KHS_OK). */
+KHS_API KhsResult khsxSetThreadCount(KhsSolver *solver, uint32_t threads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
