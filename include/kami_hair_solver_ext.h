@@ -87,6 +87,30 @@ KHS_API KhsResult khsxSetConvergenceMode(KhsSolver *solver, uint32_t mode);
    only 0. */
 KHS_API KhsResult khsxSetPrecisionMode(KhsSolver *solver, uint32_t mode);
 
+/* Device-side timing of the step, from HIP events recorded on the
+   library's stream around each launch of the named kernel classes while
+   timing is enabled (off by default). Substep latencies of the last frame
+   (event to event) are kept for the last frame stepped. */
+typedef struct KhsxKernelTiming {
+    uint32_t struct_size;
+    uint32_t substep_count;          /* substeps of the last frame */
+    uint64_t assembly_launches;      /* gradient + Gauss-Newton block assembly */
+    double assembly_ms;
+    uint64_t energy_launches;        /* line-search energy evaluations */
+    double energy_ms;
+    uint64_t solve_launches;         /* block-tridiagonal solves */
+    double solve_ms;
+    uint64_t collision_launches;     /* BVH rebuilds, sweeps and CCD */
+    double collision_ms;
+    uint64_t nodes;                  /* internal nodes of the model */
+} KhsxKernelTiming;
+
+KHS_API KhsResult khsxSetKernelTiming(KhsSolver *solver, uint32_t enable);
+KHS_API KhsResult khsxGetKernelTiming(const KhsSolver *solver, KhsxKernelTiming *timing);
+/* Milliseconds of each substep of the last frame (at most capacity). */
+KHS_API KhsResult khsxGetSubstepTimes(const KhsSolver *solver, double *milliseconds,
+                                      uint32_t capacity, uint32_t *count);
+
 /* Worker threads the CPU oracle uses over strands (oracle only; the GPU
    library ignores it and returns KHS_OK). */
 KHS_API KhsResult khsxSetThreadCount(KhsSolver *solver, uint32_t threads);

@@ -1,0 +1,67 @@
+// Device kernels of the hair step and their launch helpers.
+#pragma once
+
+#include "lbvh.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace khs {
+
+constexpr int kWave = 64;
+constexpr int kTile = 63;   // nodes owned by one wavefront in tiled kernels
+constexpr int kBlock = 256;  // four wavefronts
+constexpr int kTilesPerBlock = kBlock / kWave;
+constexpr int kNodesPerBlock = kTile * kTilesPerBlock;
+
+// One block's share of every reduction the host needs after a pass; the
+// reduce kernel folds them in a fixed order, so results are reproducible.
+struct Partial {
+    double sum[8];
+    double min_gap;
+    double alpha;
+    double max_translation;
+    unsigned long long candidates, active;
+    unsigned int infeasible, failed;
+};
+enum PartialSum { kObjective = 0, kElastic, kContact, kKinetic, kGradNorm2, kDirNorm2, kSlope, kSpare };
+
+// Read-only rod model and collider views passed by value to kernels.
+struct RodView {
+    uint32_t node_count, strand_count;
+    const double *frame;  // 9 per node, row-major
+    const double *rest;   // 8 per node: L0, shear0[3], curvature0[3], 0
+    const double *mass, *rotational_mass;
+    const uint32_t *flags;
+    const uint32_t *strand_of;
+    const uint32_t *strand_first, *strand_count_of, *strand_fixed;
+};
+
+struct ColliderView {
+    uint32_t vertex_count, triangle_count;
+    const double *current;  // 3 per vertex
+    const double *previous;
+    const uint32_t *triangles;
+    const BvhNode *bvh;
+};
+
+// Scalars of one substep.
+struct StepParams {
+    double dt, inv_dt, inv_dt2;
+    double gravity[3];
+    double radius_offset;  // radius + collider offset
+    double barrier, stiffness, damping, friction, smoothing, minimum_gap;
+    double weights[6];     // kGA, kGA, EA, EI, EI, 2GI
+    double contact_pad;    // radius + offset + barrier
+    double sweep_target;   // radius + offset + minimum gap
+    int has_collider;
+};
+
+struct SweepRecord {
+    uint32_t claimed, strand, element, triangle;
+    double distance, required, clearance, displacement;
+    double hair_start[3], hair_end[3], collider_point[3];
+};
+
+}  // namespace khs

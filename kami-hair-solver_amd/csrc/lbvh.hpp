@@ -1,0 +1,61 @@
+// Linear BVH over collider triangles, rebuilt on the device (Morton codes,
+// radix sort, Karras 2012 hierarchy, bottom-up boxes), and its query.
+//
+// Each internal node stores the boxes of its two children, so a traversal
+// step loads one node. A child that is a leaf is one triangle, and the box
+// stored for it is that triangle's own box: a query therefore reports
+// exactly the triangles whose box overlaps the padded query box, whatever
+// the tree shape.
+#pragma once
+
+#include "device_math.hpp"
+
+namespace khs {
+
+struct alignas(16) BvhNode {
+    double box[12];  // child 0 lo, hi; child 1 lo, hi
+    int child[2];    // >= 0 internal node; < 0 leaf, triangle = -1 - child
+    int pad[2];
+};
+
+namespace dev {
+
+constexpr int kBvhStack = 64;
+
+// Calls f(triangle) for every triangle whose box overlaps [lo, hi] grown by
+// pad. The Karras hierarchy over 30-bit codes with index tie-break has
+// depth below 64, so the stack never overflows.
+template <class F>
+__device__ inline void bvh_query(const BvhNode *nodes, V3 lo, V3 hi, double pad, F &&f)
+{
+    int stack[kBvhStack];
+    int top = 0, cur = 0;
+    // A tree over T triangles has T - 1 internal nodes, each visited at most
+    // once; the cap only bounds the loop should the tree be malformed.
+    for (int visits = 0; visits < (1 << 24); ++visits) {
+        const BvhNode &n = nodes[cur];
+        int next = -1;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const V3 bl = mk(n.box[6 * c], n.box[6 * c + 1], n.box[6 * c + 2]);
+            const V3 bu = mk(n.box[6 * c + 3], n.box[6 * c + 4], n.box[6 * c + 5]);
+            if (!overlap(lo, hi, bl, bu, pad)) continue;
+            const int ch = n.child[c];
+            if (ch < 0) {
+                f(uint32_t(-1 - ch));
+            } else if (next < 0) {
+                next = ch;
+            } else if (top < kBvhStack) {
+                stack[top++] = ch;
+            }
+        }
+        if (next < 0) {
+            if (top == 0) break;
+            next = stack[--top];
+        }
+        cur = next;
+    }
+}
+
+}  // namespace dev
+}  // namespace khs

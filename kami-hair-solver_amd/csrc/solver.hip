@@ -1,0 +1,1512 @@
+// Host control of the hair step on one MI355X: device buffers, LBVH
+// rebuild, the Newton / line-search loop and the adaptive frame loop
+// (reference src/cuda_backend.cu:341-723, src/solver.cpp:449-893).
+#include "solver.hpp"
+
+#include "step_kernels.hpp"
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <sstream>
+#include <stdexcept>
+#include <thread>
+
+namespace khs {
+
+#define HIP_TRY(call)                                                                                         \
+    do {                                                                                                      \
+        const hipError_t khs_err_ = (call);                                                                   \
+        if (khs_err_ != hipSuccess)                                                                           \
+            throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(khs_err_) + " in " #call); \
+    } while (0)
+
+namespace {
+
+constexpr uint64_t kCheckpointMagic = 0x3154504348534bULL;
+constexpr double kPi = 3.1415926535897932384626433832795;
+const double kInf = std::numeric_limits<double>::infinity();
+
+template <class T>
+class Buffer {
+public:
+    Buffer() = default;
+    Buffer(const Buffer &) = delete;
+    Buffer &operator=(const Buffer &) = delete;
+    ~Buffer() { reset(); }
+    void allocate(size_t count)
+    {
+        reset();
+        count_ = count;
+        if (count) HIP_TRY(hipMalloc(&ptr_, count * sizeof(T)));
+    }
+    void reset()
+    {
+        if (ptr_) (void)hipFree(ptr_);
+        ptr_ = nullptr;
+        count_ = 0;
+    }
+    void upload(const T *src, size_t count, size_t offset = 0)
+    {
+        if (offset + count > count_) throw std::runtime_error("device buffer overflow");
+        if (count) HIP_TRY(hipMemcpy(ptr_ + offset, src, count * sizeof(T), hipMemcpyHostToDevice));
+    }
+    void download(T *dst, size_t count, size_t offset = 0) const
+    {
+        if (offset + count > count_) throw std::runtime_error("device buffer overflow");
+        if (count) HIP_TRY(hipMemcpy(dst, ptr_ + offset, count * sizeof(T), hipMemcpyDeviceToHost));
+    }
+    T *get() const { return ptr_; }
+    size_t size() const { return count_; }
+    size_t bytes() const { return count_ * sizeof(T); }
+
+private:
+    T *ptr_ = nullptr;
+    size_t count_ = 0;
+};
+
+unsigned grid_for(size_t n, unsigned block) { return unsigned((n + block - 1) / block); }
+
+std::vector<double> flatten(const std::vector<Vec3> &v)
+{
+    std::vector<double> out(v.size() * 3);
+    for (size_t i = 0; i < v.size(); ++i) {
+        out[3 * i] = v[i].x;
+        out[3 * i + 1] = v[i].y;
+        out[3 * i + 2] = v[i].z;
+    }
+    return out;
+}
+
+std::vector<Vec3> to_vec3(const KhsVec3 *p, uint32_t n)
+{
+    std::vector<Vec3> out(n);
+    for (uint32_t i = 0; i < n; ++i) out[i] = {p[i].x, p[i].y, p[i].z};
+    return out;
+}
+
+}  // namespace
+
+struct DeviceState {
+    hipStream_t stream = nullptr;
+    hipEvent_t frame_begin = nullptr, frame_end = nullptr;
+    uint32_t N = 0, S = 0, V = 0, T = 0, P = 0;
+    // rod model
+    Buffer<double> frame, rest, mass, rmass;
+    Buffer<uint32_t> flags, strand_of, strand_first, strand_count, strand_fixed, mapping;
+    // state and solver vectors
+    Buffer<double> q, q_old, vel, q_base, snap_q, snap_v, grad, dir, precond, Dblk, Ublk, Cp, dp, dv;
+    // root targets (3 per node)
+    Buffer<double> root_cur, rot_cur, root_pend, rot_pend, root_anim, rot_anim;
+    // collider (3 per vertex)
+    Buffer<double> col_cur, col_old, col_start, col_pend, col_anim;
+    Buffer<uint32_t> tris;
+    // LBVH
+    Buffer<double> tri_box, bound_partials, bounds;
+    Buffer<uint32_t> keys, keys_sorted, ids, ids_sorted;
+    Buffer<BvhNode> bvh;
+    Buffer<int> parent_internal, parent_leaf;
+    Buffer<unsigned> counters;
+    Buffer<unsigned char> sort_temp;
+    size_t sort_temp_bytes = 0;
+    // reductions and flags
+    Buffer<Partial> partials, reduced;
+    Buffer<unsigned> flag;
+    Buffer<unsigned long long> key64;
+    Buffer<SweepRecord> sweep;
+    Buffer<double> out3;
+    Buffer<uint32_t> pairs;
+    Partial *host_partial = nullptr;
+
+    // Event timing of kernel classes (enabled on request) and of substeps.
+    struct Pending {
+        hipEvent_t begin, end;
+        double *ms;
+        uint64_t *launches;
+    };
+    bool timing = false;
+    std::vector<hipEvent_t> event_pool;
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> substep_events;
+    std::vector<double> substep_ms;
+    KhsxKernelTiming totals{};
+    hipEvent_t take_event()
+    {
+        if (event_pool.empty()) {
+            hipEvent_t e;
+            HIP_TRY(hipEventCreate(&e));
+            return e;
+        }
+        hipEvent_t e = event_pool.back();
+        event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t tic()
+    {
+        if (!timing) return nullptr;
+        hipEvent_t e = take_event();
+        HIP_TRY(hipEventRecord(e, stream));
+        return e;
+    }
+    void toc(hipEvent_t begin, double &ms, uint64_t &launches)
+    {
+        if (!timing || !begin) return;
+        hipEvent_t e = take_event();
+        HIP_TRY(hipEventRecord(e, stream));
+        pending.push_back({begin, e, &ms, &launches});
+    }
+    // Called after the stream has been synchronized.
+    void settle()
+    {
+        for (const Pending &p : pending) {
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, p.begin, p.end));
+            *p.ms += ms;
+            *p.launches += 1;
+            event_pool.push_back(p.begin);
+            event_pool.push_back(p.end);
+        }
+        pending.clear();
+    }
+
+    ~DeviceState()
+    {
+        for (const Pending &p : pending) {
+            (void)hipEventDestroy(p.begin);
+            (void)hipEventDestroy(p.end);
+        }
+        for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
+        for (hipEvent_t e : substep_events) (void)hipEventDestroy(e);
+        if (host_partial) (void)hipHostFree(host_partial);
+        if (frame_begin) (void)hipEventDestroy(frame_begin);
+        if (frame_end) (void)hipEventDestroy(frame_end);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    RodView rods() const
+    {
+        return RodView{N,           S,           frame.get(),        rest.get(),         mass.get(),
+                       rmass.get(), flags.get(), strand_of.get(), strand_first.get(), strand_count.get(),
+                       strand_fixed.get()};
+    }
+    ColliderView collider() const { return ColliderView{V, T, col_cur.get(), col_old.get(), tris.get(), bvh.get()}; }
+
+    size_t resident_bytes() const
+    {
+        size_t b = 0;
+        for (const Buffer<double> *x : {&frame, &rest, &mass, &rmass, &q, &q_old, &vel, &q_base, &snap_q, &snap_v,
+                                        &grad, &dir, &precond, &Dblk, &Ublk, &Cp, &dp, &dv, &root_cur, &rot_cur,
+                                        &root_pend, &rot_pend, &root_anim, &rot_anim, &col_cur, &col_old, &col_start,
+                                        &col_pend, &col_anim, &tri_box, &bound_partials, &bounds, &out3})
+            b += x->bytes();
+        for (const Buffer<uint32_t> *x : {&flags, &strand_of, &strand_first, &strand_count, &strand_fixed, &mapping,
+                                          &tris, &keys, &keys_sorted, &ids, &ids_sorted})
+            b += x->bytes();
+        b += bvh.bytes() + parent_internal.bytes() + parent_leaf.bytes() + counters.bytes() + sort_temp.bytes() +
+             partials.bytes() + reduced.bytes();
+        return b;
+    }
+
+    Partial reduce(uint32_t blocks)
+    {
+        hipLaunchKernelGGL(dev::k_reduce, dim3(1), dim3(kBlock), 0, stream, partials.get(), blocks, reduced.get());
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(host_partial, reduced.get(), sizeof(Partial), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        settle();
+        return *host_partial;
+    }
+
+    uint32_t tiled_blocks() const { return grid_for(size_t(N) + 1, kNodesPerBlock); }
+    uint32_t node_blocks() const { return grid_for(N, kBlock); }
+};
+
+struct Solver::Reduced {
+    double objective, elastic, contact, kinetic, grad_norm2, dir_norm2, min_gap;
+    unsigned long long candidates, active;
+    bool feasible;
+};
+
+Solver::Solver(const KhsSolverDesc &desc) : desc_(desc)
+{
+    build_stats_.struct_size = sizeof(KhsBuildStats);
+    step_stats_.struct_size = sizeof(KhsStepStats);
+    step_stats_.phase = KHS_PHASE_IDLE;
+    gpu_stats_.struct_size = sizeof(KhsGpuStats);
+    clear_failure();
+}
+
+Solver::~Solver() = default;
+
+void Solver::release_device() { dev_.reset(); }
+
+KhsResult Solver::fail(KhsResult result, const std::string &message)
+{
+    error_ = message;
+    step_stats_.phase = KHS_PHASE_FAILED;
+    phase_.store(KHS_PHASE_FAILED);
+    return result;
+}
+
+void Solver::clear_failure()
+{
+    failure_ = {};
+    failure_.struct_size = sizeof(failure_);
+    failure_.strand_index = 0xffffffffu;
+    failure_.element_index = 0xffffffffu;
+    failure_.collider_triangle_index = 0xffffffffu;
+}
+
+void Solver::set_failure_context(uint32_t kind, uint32_t frame, uint32_t sub, uint32_t attempted, uint32_t maximum,
+                                 uint32_t attempts)
+{
+    failure_.kind = kind;
+    failure_.frame_index = frame;
+    failure_.substep = sub;
+    failure_.requested_substeps = desc_.substeps;
+    failure_.attempted_substeps = attempted;
+    failure_.maximum_substeps = maximum;
+    failure_.adaptive_attempt_count = attempts;
+}
+
+// ------------------------------------------------------------------ input
+KhsResult Solver::set_hair(const KhsVec3 *points, uint32_t point_count, const uint32_t *offsets, uint32_t strand_count,
+                           const KhsHairMaterial &m)
+{
+    // reference solver.cpp:136-177
+    if (!points || !offsets || point_count < 2 || strand_count == 0 || offsets[0] != 0 ||
+        offsets[strand_count] != point_count) {
+        error_ = "Invalid hair points or strand offsets.";
+        return KHS_ERROR_INVALID_ARGUMENT;
+    }
+    for (uint32_t s = 0; s < strand_count; ++s)
+        if (offsets[s + 1] <= offsets[s] || offsets[s + 1] > point_count) {
+            error_ = "Hair strand offsets are not increasing.";
+            return KHS_ERROR_INVALID_HAIR;
+        }
+    if (!(m.density > 0.0 && m.radius > 0.0 && m.young_modulus > 0.0 && m.poisson_ratio > -1.0 &&
+          m.poisson_ratio < 0.5 && m.shear_correction > 0.0 && m.contact_stiffness > 0.0 && m.barrier_distance > 0.0 &&
+          m.friction >= 0.0 && m.friction_smoothing > 0.0)) {
+        error_ = "Hair material is out of range.";
+        return KHS_ERROR_INVALID_ARGUMENT;
+    }
+    std::vector<Vec3> pts = to_vec3(points, point_count);
+    for (uint32_t i = 0; i < point_count; ++i)
+        if (!is_finite(pts[i])) {
+            error_ = "Hair point " + std::to_string(i) + " is not finite.";
+            return KHS_ERROR_INVALID_HAIR;
+        }
+    built_ = false;
+    animation_ready_ = false;
+    release_device();
+    points_ = std::move(pts);
+    offsets_.assign(offsets, offsets + strand_count + 1);
+    material_ = m;
+    has_hair_ = true;
+    error_.clear();
+    return KHS_OK;
+}
+
+KhsResult Solver::set_collider(const KhsVec3 *vertices, uint32_t vertex_count, const KhsTriangle *triangles,
+                               uint32_t triangle_count)
+{
+    // reference solver.cpp:179-225
+    if (triangle_count == 0) {
+        has_collider_ = false;
+        collider_ = ColliderMesh{};
+        built_ = false;
+        return KHS_OK;
+    }
+    if (!vertices || !triangles || vertex_count < 3) {
+        error_ = "Invalid collider vertices or triangles.";
+        return KHS_ERROR_INVALID_ARGUMENT;
+    }
+    ColliderMesh mesh;
+    const KhsResult r = filter_collider(vertices, vertex_count, triangles, triangle_count, mesh, error_);
+    if (r != KHS_OK) return r;
+    collider_ = std::move(mesh);
+    has_collider_ = true;
+    built_ = false;
+    error_.clear();
+    return KHS_OK;
+}
+
+// ------------------------------------------------------------------ build
+KhsResult Solver::build()
+{
+    // reference solver.cpp:449-556
+    if (!has_hair_) {
+        error_ = "Set hair curves first.";
+        return KHS_ERROR_INVALID_STATE;
+    }
+    built_ = false;
+    animation_ready_ = false;
+    release_device();
+    step_stats_.phase = KHS_PHASE_PREPARING;
+    if (!build_rod_model(points_, offsets_, desc_, model_, error_)) {
+        step_stats_.phase = KHS_PHASE_FAILED;
+        return KHS_ERROR_INVALID_HAIR;
+    }
+    assign_masses(material_, model_);
+    std::vector<Vec3> tp, tr;
+    const unsigned threads = std::max(1u, std::thread::hardware_concurrency());
+    if (!root_targets(model_, points_, tp, tr, threads)) {
+        error_ = "Cannot build the initial root orientation.";
+        step_stats_.phase = KHS_PHASE_FAILED;
+        return KHS_ERROR_INVALID_HAIR;
+    }
+    if (has_collider_) validate_collider(collider_);
+    KhsResult r = upload_model();
+    if (r != KHS_OK) {
+        step_stats_.phase = KHS_PHASE_FAILED;
+        release_device();
+        return r;
+    }
+    double initial_gap = kInf;
+    if (has_collider_) {
+        r = initial_feasibility();
+        if (r != KHS_OK) {
+            step_stats_.phase = KHS_PHASE_FAILED;
+            release_device();
+            return r;
+        }
+        initial_gap = build_stats_.initial_minimum_gap;
+    }
+    built_ = true;
+    build_stats_ = {};
+    build_stats_.struct_size = sizeof(KhsBuildStats);
+    build_stats_.strand_count = model_.strand_count;
+    build_stats_.original_point_count = uint32_t(points_.size());
+    build_stats_.internal_node_count = model_.node_count;
+    build_stats_.element_count = model_.element_count;
+    uint32_t fixed = 0;
+    for (uint32_t f : model_.flags) fixed += (f & kNodeFixed) ? 1 : 0;
+    build_stats_.fixed_node_count = fixed;
+    build_stats_.collider_vertex_count = uint32_t(collider_.vertices.size());
+    build_stats_.collider_triangle_count = collider_.triangle_count();
+    build_stats_.excluded_collider_triangle_count = collider_.excluded;
+    build_stats_.collider_boundary_edge_count = collider_.boundary_edges;
+    build_stats_.collider_nonmanifold_edge_count = collider_.nonmanifold_edges;
+    build_stats_.collider_inconsistent_edge_count = collider_.inconsistent_edges;
+    build_stats_.collider_inverted_closed_component_count = collider_.inverted;
+    build_stats_.merged_zero_length_segment_count = model_.merged_segments;
+    build_stats_.virtual_extension_strand_count = model_.extension_strands;
+    build_stats_.virtual_extension_node_count = model_.extension_nodes;
+    build_stats_.virtual_extension_rest_length = model_.extension_length;
+    build_stats_.degree_of_freedom_count = model_.free_dofs;
+    build_stats_.estimated_bytes = dev_->resident_bytes();
+    build_stats_.initial_minimum_gap = initial_gap;
+    gpu_stats_ = {};
+    gpu_stats_.struct_size = sizeof(KhsGpuStats);
+    gpu_stats_.resident_bytes = dev_->resident_bytes();
+    step_stats_ = {};
+    step_stats_.struct_size = sizeof(KhsStepStats);
+    step_stats_.phase = KHS_PHASE_IDLE;
+    phase_.store(KHS_PHASE_IDLE);
+    error_.clear();
+    return KHS_OK;
+}
+
+KhsResult Solver::upload_model()
+{
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count < 1) {
+        error_ = "No HIP GPU is available.";
+        return KHS_ERROR_INVALID_STATE;
+    }
+    HIP_TRY(hipSetDevice(0));
+    dev_ = std::make_unique<DeviceState>();
+    DeviceState &d = *dev_;
+    HIP_TRY(hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&d.frame_begin));
+    HIP_TRY(hipEventCreate(&d.frame_end));
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&d.host_partial), sizeof(Partial)));
+    const RodModel &m = model_;
+    d.N = m.node_count;
+    d.S = m.strand_count;
+    d.P = uint32_t(points_.size());
+    const size_t N = d.N;
+    std::vector<double> frames(9 * N);
+    for (size_t n = 0; n < N; ++n) std::memcpy(&frames[9 * n], m.frame[n].m, 9 * sizeof(double));
+    d.frame.allocate(9 * N);
+    d.frame.upload(frames.data(), frames.size());
+    d.rest.allocate(8 * N);
+    d.rest.upload(m.rest.data(), m.rest.size());
+    d.mass.allocate(N);
+    d.mass.upload(m.mass.data(), N);
+    d.rmass.allocate(N);
+    d.rmass.upload(m.rotational_mass.data(), N);
+    d.flags.allocate(N);
+    d.flags.upload(m.flags.data(), N);
+    d.strand_of.allocate(N);
+    d.strand_of.upload(m.strand_of.data(), N);
+    d.strand_first.allocate(d.S);
+    d.strand_first.upload(m.strand_first.data(), d.S);
+    d.strand_count.allocate(d.S);
+    d.strand_count.upload(m.strand_count_of.data(), d.S);
+    d.strand_fixed.allocate(d.S);
+    d.strand_fixed.upload(m.strand_fixed.data(), d.S);
+    d.mapping.allocate(d.P);
+    d.mapping.upload(m.original_to_internal.data(), d.P);
+
+    // q = (position, zero rotation), v = 0 (reference cuda_backend.cu:360-371)
+    std::vector<double> q(6 * N, 0.0);
+    for (size_t n = 0; n < N; ++n) {
+        q[6 * n] = m.position[n].x;
+        q[6 * n + 1] = m.position[n].y;
+        q[6 * n + 2] = m.position[n].z;
+    }
+    const std::vector<double> zeros(6 * N, 0.0);
+    for (Buffer<double> *b : {&d.q, &d.q_old, &d.vel, &d.q_base, &d.snap_q, &d.snap_v, &d.grad, &d.dir, &d.precond,
+                              &d.dp})
+        b->allocate(6 * N);
+    d.q.upload(q.data(), q.size());
+    d.q_old.upload(q.data(), q.size());
+    d.snap_q.upload(q.data(), q.size());
+    d.vel.upload(zeros.data(), zeros.size());
+    d.snap_v.upload(zeros.data(), zeros.size());
+    d.dir.upload(zeros.data(), zeros.size());
+    d.grad.upload(zeros.data(), zeros.size());
+    d.dv.allocate(3 * N);
+    d.Dblk.allocate(36 * N);
+    d.Ublk.allocate(36 * N);
+    d.Cp.allocate(36 * N);
+    // root targets start at the node positions and zero rotations
+    std::vector<double> pos = flatten(m.position), zero3(3 * N, 0.0);
+    for (Buffer<double> *b : {&d.root_cur, &d.rot_cur, &d.root_pend, &d.rot_pend}) b->allocate(3 * N);
+    d.root_cur.upload(pos.data(), pos.size());
+    d.root_pend.upload(pos.data(), pos.size());
+    d.rot_cur.upload(zero3.data(), zero3.size());
+    d.rot_pend.upload(zero3.data(), zero3.size());
+    d.out3.allocate(3 * std::max<size_t>(N, d.P));
+    const uint32_t max_blocks = std::max(d.tiled_blocks(), d.node_blocks()) + 1;
+    d.partials.allocate(max_blocks + 16);
+    d.reduced.allocate(1);
+    d.flag.allocate(1);
+    d.key64.allocate(1);
+    d.sweep.allocate(1);
+
+    if (has_collider_) {
+        d.V = uint32_t(collider_.vertices.size());
+        d.T = collider_.triangle_count();
+        const std::vector<double> cv = flatten(collider_.vertices);
+        for (Buffer<double> *b : {&d.col_cur, &d.col_old, &d.col_start, &d.col_pend}) {
+            b->allocate(cv.size());
+            b->upload(cv.data(), cv.size());
+        }
+        collider_pending_host_ = collider_.vertices;
+        d.tris.allocate(collider_.triangles.size());
+        d.tris.upload(collider_.triangles.data(), collider_.triangles.size());
+        const size_t T = d.T;
+        d.tri_box.allocate(6 * T);
+        d.bound_partials.allocate(6 * size_t(grid_for(T, kBlock)));
+        d.bounds.allocate(6);
+        d.keys.allocate(T);
+        d.keys_sorted.allocate(T);
+        d.ids.allocate(T);
+        d.ids_sorted.allocate(T);
+        d.bvh.allocate(std::max<size_t>(1, T - 1));
+        d.parent_internal.allocate(std::max<size_t>(1, T - 1));
+        d.parent_leaf.allocate(T);
+        d.counters.allocate(std::max<size_t>(1, T - 1));
+        size_t temp = 0;
+        HIP_TRY(rocprim::radix_sort_pairs(nullptr, temp, d.keys.get(), d.keys_sorted.get(), d.ids.get(),
+                                          d.ids_sorted.get(), T, 0, 30, d.stream));
+        d.sort_temp.allocate(std::max<size_t>(temp, 1));
+        d.sort_temp_bytes = temp;
+        rebuild_bvh(false);
+    }
+    trust_radius_ = std::max(0.25 * model_.maximum_rest_length,
+                             8.0 * (material_.radius + material_.collider_offset + material_.barrier_distance));
+    HIP_TRY(hipStreamSynchronize(d.stream));
+    return KHS_OK;
+}
+
+// Morton codes, radix sort, hierarchy and boxes from the current collider.
+void Solver::rebuild_bvh(bool swept)
+{
+    DeviceState &d = *dev_;
+    if (!d.T) return;
+    const ColliderView cv = d.collider();
+    const unsigned tb = grid_for(d.T, kBlock);
+    hipLaunchKernelGGL(dev::k_triangle_boxes, dim3(tb), dim3(kBlock), 0, d.stream, cv, swept ? 1 : 0, d.tri_box.get(),
+                       d.bound_partials.get());
+    hipLaunchKernelGGL(dev::k_bounds_final, dim3(1), dim3(1), 0, d.stream, d.bound_partials.get(), tb, d.bounds.get());
+    hipLaunchKernelGGL(dev::k_morton, dim3(tb), dim3(kBlock), 0, d.stream, cv, d.bounds.get(), d.keys.get(),
+                       d.ids.get());
+    size_t temp = d.sort_temp_bytes;
+    HIP_TRY(rocprim::radix_sort_pairs(d.sort_temp.get(), temp, d.keys.get(), d.keys_sorted.get(), d.ids.get(),
+                                      d.ids_sorted.get(), d.T, 0, 30, d.stream));
+    hipLaunchKernelGGL(dev::k_hierarchy, dim3(tb), dim3(kBlock), 0, d.stream, d.T, d.keys_sorted.get(),
+                       d.ids_sorted.get(), d.bvh.get(), d.parent_internal.get(), d.parent_leaf.get(), d.counters.get());
+    hipLaunchKernelGGL(dev::k_refit, dim3(tb), dim3(kBlock), 0, d.stream, d.T, d.ids_sorted.get(), d.tri_box.get(),
+                       d.parent_internal.get(), d.parent_leaf.get(), d.bvh.get(), d.counters.get());
+    HIP_TRY(hipGetLastError());
+}
+
+KhsResult Solver::initial_feasibility()
+{
+    // reference solver.cpp:634-657
+    DeviceState &d = *dev_;
+    std::vector<double> pos = flatten(model_.position);
+    Buffer<double> dpos;
+    dpos.allocate(pos.size());
+    dpos.upload(pos.data(), pos.size());
+    const RodView rv = d.rods();
+    const ColliderView cv = d.collider();
+    if (collider_.closed) {
+        double x_max = -kInf;
+        for (const Vec3 &v : collider_.vertices) x_max = std::max(x_max, v.x);
+        const unsigned init = 0xffffffffu;
+        HIP_TRY(hipMemcpyAsync(d.flag.get(), &init, sizeof(unsigned), hipMemcpyHostToDevice, d.stream));
+        hipLaunchKernelGGL(dev::k_point_inside, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, cv, dpos.get(),
+                           x_max, d.flag.get());
+        HIP_TRY(hipGetLastError());
+        unsigned inside = 0;
+        HIP_TRY(hipMemcpyAsync(&inside, d.flag.get(), sizeof(unsigned), hipMemcpyDeviceToHost, d.stream));
+        HIP_TRY(hipStreamSynchronize(d.stream));
+        if (inside != 0xffffffffu) {
+            error_ = "Initial intersection: internal node " + std::to_string(inside) + " is inside the closed collider.";
+            return KHS_ERROR_INITIAL_INTERSECTION;
+        }
+    }
+    StepParams p{};
+    p.radius_offset = material_.radius + material_.collider_offset;
+    p.contact_pad = material_.radius + material_.collider_offset + material_.barrier_distance;
+    hipLaunchKernelGGL(dev::k_initial_gap, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, cv, p, dpos.get(),
+                       d.partials.get());
+    HIP_TRY(hipGetLastError());
+    const Partial r = d.reduce(d.node_blocks());
+    build_stats_.initial_minimum_gap = r.min_gap;
+    if (r.min_gap <= desc_.minimum_gap) {
+        std::ostringstream ss;
+        ss << "Initial intersection or negative gap: minimum gap " << r.min_gap << " m, " << r.candidates
+           << " candidates.";
+        error_ = ss.str();
+        return KHS_ERROR_INITIAL_INTERSECTION;
+    }
+    return KHS_OK;
+}
+
+KhsResult Solver::update_runtime_parameters(const KhsSolverDesc &desc, const KhsHairMaterial &material)
+{
+    // reference solver.cpp:558-606, cuda_backend.cu:409-430
+    if (!built_) {
+        error_ = "Build the solver before updating runtime parameters.";
+        return KHS_ERROR_INVALID_STATE;
+    }
+    if (desc.maximum_element_length != desc_.maximum_element_length ||
+        desc.minimum_dynamic_length != desc_.minimum_dynamic_length || desc.fixed_root_nodes != desc_.fixed_root_nodes) {
+        error_ = "maximum_element_length, minimum_dynamic_length and fixed_root_nodes cannot change after the build.";
+        return KHS_ERROR_INVALID_ARGUMENT;
+    }
+    desc_ = desc;
+    material_ = material;
+    assign_masses(material_, model_);
+    dev_->mass.upload(model_.mass.data(), model_.node_count);
+    dev_->rmass.upload(model_.rotational_mass.data(), model_.node_count);
+    trust_radius_ = std::max(0.25 * model_.maximum_rest_length,
+                             8.0 * (material_.radius + material_.collider_offset + material_.barrier_distance));
+    error_.clear();
+    return KHS_OK;
+}
+
+KhsResult Solver::update_collider(const KhsVec3 *vertices, uint32_t vertex_count)
+{
+    // reference solver.cpp:683-708
+    if (!built_ || !has_collider_) {
+        error_ = "No built collider.";
+        return KHS_ERROR_INVALID_STATE;
+    }
+    if (!vertices || vertex_count != collider_.vertices.size()) {
+        error_ = "Collider vertex count changed.";
+        return KHS_ERROR_INVALID_COLLIDER;
+    }
+    std::vector<Vec3> v = to_vec3(vertices, vertex_count);
+    for (const Vec3 &x : v)
+        if (!is_finite(x)) {
+            error_ = "Collider vertex is not finite.";
+            return KHS_ERROR_INVALID_COLLIDER;
+        }
+    const std::vector<double> flat = flatten(v);
+    dev_->col_pend.upload(flat.data(), flat.size());
+    collider_pending_host_ = std::move(v);
+    return KHS_OK;
+}
+
+KhsResult Solver::update_roots(const KhsVec3 *points, uint32_t point_count)
+{
+    // reference solver.cpp:710-742
+    if (!built_ || !points || point_count != points_.size()) {
+        error_ = "Root target count does not match the hair curves.";
+        return KHS_ERROR_INVALID_ARGUMENT;
+    }
+    std::vector<Vec3> p = to_vec3(points, point_count);
+    for (const Vec3 &x : p)
+        if (!is_finite(x)) {
+            error_ = "Root target is not finite.";
+            return KHS_ERROR_INVALID_HAIR;
+        }
+    std::vector<Vec3> tp, tr;
+    if (!root_targets(model_, p, tp, tr, std::max(1u, std::thread::hardware_concurrency()))) {
+        error_ = "Cannot build root orientation from the targets.";
+        return KHS_ERROR_INVALID_HAIR;
+    }
+    const std::vector<double> fp = flatten(tp), fr = flatten(tr);
+    dev_->root_pend.upload(fp.data(), fp.size());
+    dev_->rot_pend.upload(fr.data(), fr.size());
+    return KHS_OK;
+}
+
+// -------------------------------------------------------------- animation
+KhsResult Solver::allocate_animation(uint32_t frames)
+{
+    if (!built_) return KHS_ERROR_INVALID_STATE;
+    if (frames < 1) return fail(KHS_ERROR_INVALID_ARGUMENT, "Invalid animation frame count.");
+    try {
+        DeviceState &d = *dev_;
+        d.root_anim.allocate(size_t(frames) * 3 * d.N);
+        d.rot_anim.allocate(size_t(frames) * 3 * d.N);
+        if (d.V) d.col_anim.allocate(size_t(frames) * 3 * d.V);
+    } catch (const std::exception &e) {
+        return fail(KHS_ERROR_OUT_OF_MEMORY, e.what());
+    }
+    animation_frames_ = frames;
+    root_frame_set_.assign(frames, false);
+    collider_frame_set_.assign(frames, !has_collider_);
+    gpu_stats_.animation_frame_count = frames;
+    gpu_stats_.resident_bytes = dev_->resident_bytes();
+    return KHS_OK;
+}
+
+KhsResult Solver::set_root_animation_frame(uint32_t frame, const KhsVec3 *points, uint32_t point_count)
+{
+    if (!built_ || !points || point_count != points_.size()) return KHS_ERROR_INVALID_ARGUMENT;
+    std::vector<Vec3> p = to_vec3(points, point_count);
+    for (const Vec3 &x : p)
+        if (!is_finite(x)) return KHS_ERROR_INVALID_HAIR;
+    std::vector<Vec3> tp, tr;
+    if (!root_targets(model_, p, tp, tr, std::max(1u, std::thread::hardware_concurrency()))) return KHS_ERROR_INVALID_HAIR;
+    if (frame >= animation_frames_) return fail(KHS_ERROR_INVALID_ARGUMENT, "Invalid root animation frame.");
+    const std::vector<double> fp = flatten(tp), fr = flatten(tr);
+    const size_t n3 = 3 * size_t(dev_->N);
+    dev_->root_anim.upload(fp.data(), n3, size_t(frame) * n3);
+    dev_->rot_anim.upload(fr.data(), n3, size_t(frame) * n3);
+    root_frame_set_[frame] = true;
+    return KHS_OK;
+}
+
+KhsResult Solver::set_collider_animation_frame(uint32_t frame, const KhsVec3 *vertices, uint32_t vertex_count)
+{
+    if (!built_) return KHS_ERROR_INVALID_STATE;
+    const uint32_t vc = dev_->V;
+    if (frame >= animation_frames_ || vertex_count != vc || (vertex_count && !vertices))
+        return fail(KHS_ERROR_INVALID_COLLIDER, "Invalid collider animation frame.");
+    if (vc) {
+        const std::vector<double> flat = flatten(to_vec3(vertices, vertex_count));
+        dev_->col_anim.upload(flat.data(), flat.size(), size_t(frame) * 3 * vc);
+    }
+    collider_frame_set_[frame] = true;
+    return KHS_OK;
+}
+
+KhsResult Solver::finalize_animation()
+{
+    if (!built_) return KHS_ERROR_INVALID_STATE;
+    for (bool b : root_frame_set_)
+        if (!b) return fail(KHS_ERROR_INVALID_STATE, "Animation has missing frames.");
+    for (bool b : collider_frame_set_)
+        if (!b) return fail(KHS_ERROR_INVALID_STATE, "Animation has missing frames.");
+    if (animation_frames_ < 1) return fail(KHS_ERROR_INVALID_STATE, "Animation has missing frames.");
+    animation_ready_ = true;
+    current_frame_ = 0;
+    return KHS_OK;
+}
+
+// ---------------------------------------------------------------- solving
+Solver::Reduced Solver::evaluate(bool gradient, double dt, bool line_search, double alpha)
+{
+    DeviceState &d = *dev_;
+    StepParams p{};
+    p.dt = dt;
+    p.inv_dt = 1.0 / dt;
+    p.inv_dt2 = p.inv_dt * p.inv_dt;
+    p.gravity[0] = desc_.gravity.x;
+    p.gravity[1] = desc_.gravity.y;
+    p.gravity[2] = desc_.gravity.z;
+    p.radius_offset = material_.radius + material_.collider_offset;
+    p.barrier = material_.barrier_distance;
+    p.stiffness = material_.contact_stiffness;
+    p.damping = material_.mass_damping;
+    p.friction = material_.friction;
+    p.smoothing = material_.friction_smoothing;
+    p.minimum_gap = desc_.minimum_gap;
+    const double r = material_.radius;
+    const double area = kPi * r * r, inertia = kPi * r * r * r * r / 4.0;
+    const double G = material_.young_modulus / (2.0 * (1.0 + material_.poisson_ratio));
+    p.weights[0] = material_.shear_correction * G * area;
+    p.weights[1] = material_.shear_correction * G * area;
+    p.weights[2] = material_.young_modulus * area;
+    p.weights[3] = material_.young_modulus * inertia;
+    p.weights[4] = material_.young_modulus * inertia;
+    p.weights[5] = 2.0 * G * inertia;
+    p.contact_pad = material_.radius + material_.collider_offset + material_.barrier_distance;
+    p.sweep_target = material_.radius + material_.collider_offset + desc_.minimum_gap;
+    p.has_collider = d.T > 0;
+    const unsigned blocks = d.tiled_blocks();
+    const RodView rv = d.rods();
+    const ColliderView cv = d.collider();
+    hipEvent_t t0 = d.tic();
+    if (gradient)
+        hipLaunchKernelGGL((dev::k_evaluate<true, false>), dim3(blocks), dim3(kBlock), 0, d.stream, rv, cv, p,
+                           d.q.get(), d.q_old.get(), d.vel.get(), d.q_base.get(), d.dir.get(), 0.0, d.grad.get(),
+                           d.Dblk.get(), d.Ublk.get(), d.partials.get());
+    else if (line_search)
+        hipLaunchKernelGGL((dev::k_evaluate<false, true>), dim3(blocks), dim3(kBlock), 0, d.stream, rv, cv, p,
+                           d.q.get(), d.q_old.get(), d.vel.get(), d.q_base.get(), d.dir.get(), alpha, d.grad.get(),
+                           d.Dblk.get(), d.Ublk.get(), d.partials.get());
+    else
+        hipLaunchKernelGGL((dev::k_evaluate<false, false>), dim3(blocks), dim3(kBlock), 0, d.stream, rv, cv, p,
+                           d.q.get(), d.q_old.get(), d.vel.get(), d.q_base.get(), d.dir.get(), 0.0, d.grad.get(),
+                           d.Dblk.get(), d.Ublk.get(), d.partials.get());
+    HIP_TRY(hipGetLastError());
+    if (gradient) d.toc(t0, d.totals.assembly_ms, d.totals.assembly_launches);
+    else d.toc(t0, d.totals.energy_ms, d.totals.energy_launches);
+    const Partial r2 = d.reduce(blocks);
+    Reduced out{};
+    out.objective = r2.sum[kObjective];
+    out.elastic = r2.sum[kElastic];
+    out.contact = r2.sum[kContact];
+    out.kinetic = r2.sum[kKinetic];
+    out.grad_norm2 = r2.sum[kGradNorm2];
+    out.dir_norm2 = r2.sum[kDirNorm2];
+    out.min_gap = r2.min_gap;
+    out.candidates = r2.candidates;
+    out.active = r2.active;
+    out.feasible = r2.infeasible == 0;
+    return out;
+}
+
+void Solver::limit_direction(double max_translation)
+{
+    // reference cuda_backend.cu:623-630
+    if (max_translation > trust_radius_) {
+        DeviceState &d = *dev_;
+        const double s = trust_radius_ / max_translation;
+        hipLaunchKernelGGL(dev::k_scale, dim3(grid_for(6 * size_t(d.N), kBlock)), dim3(kBlock), 0, d.stream,
+                           d.dir.get(), 6 * size_t(d.N), s);
+        HIP_TRY(hipGetLastError());
+    }
+}
+
+void Solver::steepest_direction()
+{
+    // reference cuda_backend.cu:632-638
+    DeviceState &d = *dev_;
+    const RodView rv = d.rods();
+    hipLaunchKernelGGL(dev::k_steepest, dim3(grid_for(6 * size_t(d.N), kBlock)), dim3(kBlock), 0, d.stream, rv,
+                       d.grad.get(), d.precond.get(), d.dir.get());
+    hipLaunchKernelGGL(dev::k_direction_stats, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, d.grad.get(),
+                       d.dir.get(), (const unsigned *)nullptr, d.partials.get());
+    HIP_TRY(hipGetLastError());
+    const Partial r = d.reduce(d.node_blocks());
+    limit_direction(r.max_translation);
+}
+
+bool Solver::make_direction()
+{
+    // reference cuda_backend.cu:640-648
+    DeviceState &d = *dev_;
+    const RodView rv = d.rods();
+    hipLaunchKernelGGL(dev::k_fill_u32, dim3(1), dim3(64), 0, d.stream, d.flag.get(), size_t(1), 0u);
+    hipEvent_t t0 = d.tic();
+    hipLaunchKernelGGL(dev::k_block_solve, dim3(grid_for(d.S, 64)), dim3(64), 0, d.stream, rv, d.Dblk.get(),
+                       d.Ublk.get(), d.grad.get(), d.Cp.get(), d.dp.get(), d.dir.get(), d.flag.get());
+    d.toc(t0, d.totals.solve_ms, d.totals.solve_launches);
+    hipLaunchKernelGGL(dev::k_direction_stats, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, d.grad.get(),
+                       d.dir.get(), (const unsigned *)d.flag.get(), d.partials.get());
+    HIP_TRY(hipGetLastError());
+    const Partial r = d.reduce(d.node_blocks());
+    const double slope = r.sum[kSlope];
+    const bool used = !r.failed && slope < 0 && std::isfinite(slope);
+    if (!used) {
+        steepest_direction();
+        return false;
+    }
+    limit_direction(r.max_translation);
+    return true;
+}
+
+KhsResult Solver::optimize_substep(double dt)
+{
+    // reference cuda_backend.cu:650-688
+    DeviceState &d = *dev_;
+    const RodView rv = d.rods();
+    {
+        const double r = material_.radius;
+        const double area = kPi * r * r, inertia = kPi * r * r * r * r / 4.0;
+        const double G = material_.young_modulus / (2.0 * (1.0 + material_.poisson_ratio));
+        const double tk = std::fmax(material_.young_modulus * area, material_.shear_correction * G * area);
+        const double ra = material_.shear_correction * G * area;
+        const double rb = std::fmax(material_.young_modulus * inertia, 2.0 * G * inertia);
+        hipLaunchKernelGGL(dev::k_preconditioner, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, 1.0 / dt,
+                           material_.mass_damping, tk, ra, rb, d.precond.get());
+    }
+    Reduced current = evaluate(true, dt, false, 0.0);
+    if (!current.feasible) return fail(KHS_ERROR_NOT_CONVERGED, "State is outside the barrier feasible region.");
+    const double initial = std::sqrt(current.grad_norm2);
+    step_stats_.initial_residual_norm = std::fmax(step_stats_.initial_residual_norm, initial);
+    const double dofs = double(6 * size_t(d.N));
+    StepParams cp{};
+    cp.radius_offset = material_.radius + material_.collider_offset;
+    cp.sweep_target = material_.radius + material_.collider_offset + desc_.minimum_gap;
+    cp.has_collider = d.T > 0;
+    for (uint32_t it = 0; it <= desc_.newton_iterations; ++it) {
+        iteration_.store(it);
+        if (cancel_.load()) return fail(KHS_ERROR_INVALID_STATE, "Solve cancelled.");
+        const double residual = std::sqrt(current.grad_norm2);
+        const double relative = residual / std::fmax(initial, 1e-30);
+        step_stats_.final_residual_norm = residual;
+        step_stats_.relative_residual_norm = relative;
+        step_stats_.minimum_gap = std::fmin(step_stats_.minimum_gap, current.min_gap);
+        step_stats_.contact_candidate_count += current.candidates;
+        step_stats_.active_contact_count += current.active;
+        step_stats_.kinetic_energy = current.kinetic;
+        step_stats_.elastic_energy = current.elastic;
+        step_stats_.contact_energy = current.contact;
+        step_stats_.friction_energy = 0.0;
+        const double abs_tol = convergence_mode_ ? desc_.absolute_tolerance
+                                                 : std::fmax(desc_.absolute_tolerance, 1.0e-5 * std::sqrt(dofs));
+        const double rel_tol = convergence_mode_ ? desc_.relative_tolerance : std::fmax(desc_.relative_tolerance, 1.0e-4);
+        if (residual <= abs_tol || relative <= rel_tol) {
+            hipLaunchKernelGGL(dev::k_update_velocity, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv,
+                               d.q.get(), d.q_old.get(), d.vel.get(), 1.0 / dt);
+            HIP_TRY(hipGetLastError());
+            return KHS_OK;
+        }
+        if (it == desc_.newton_iterations) break;
+        phase_.store(KHS_PHASE_LINEAR_SOLVE);
+        bool block_direction = make_direction();
+        ++step_stats_.linear_solves;
+        HIP_TRY(hipMemcpyAsync(d.q_base.get(), d.q.get(), d.q.bytes(), hipMemcpyDeviceToDevice, d.stream));
+        bool accepted = false;
+        double alpha = 1.0;
+        for (int attempt = 0; attempt < 2; ++attempt) {
+            phase_.store(KHS_PHASE_CCD);
+            hipEvent_t t0 = d.tic();
+            hipLaunchKernelGGL(dev::k_ccd_slope, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, d.collider(),
+                               cp, d.q.get(), d.grad.get(), d.dir.get(), d.partials.get());
+            HIP_TRY(hipGetLastError());
+            d.toc(t0, d.totals.collision_ms, d.totals.collision_launches);
+            const Partial cr = d.reduce(d.node_blocks());
+            const double slope = cr.sum[kSlope];
+            alpha = 1.0;
+            if (d.T) {
+                alpha = cr.alpha;
+                if (alpha < 1.0) alpha *= 0.9;
+                step_stats_.ccd_step_limit = std::fmin(step_stats_.ccd_step_limit, alpha);
+            }
+            accepted = false;
+            phase_.store(KHS_PHASE_LINE_SEARCH);
+            if (alpha >= desc_.minimum_line_search_step)
+                for (uint32_t search = 0; search < desc_.line_search_iterations; ++search) {
+                    const Reduced trial = evaluate(false, dt, true, alpha);
+                    ++step_stats_.line_search_evaluations;
+                    if (trial.feasible && std::isfinite(trial.objective) &&
+                        trial.objective <= current.objective + 1e-4 * alpha * slope) {
+                        accepted = true;
+                        break;
+                    }
+                    alpha *= 0.5;
+                    if (alpha < desc_.minimum_line_search_step) break;
+                }
+            if (accepted) break;
+            if (!block_direction) break;
+            HIP_TRY(hipMemcpyAsync(d.q.get(), d.q_base.get(), d.q.bytes(), hipMemcpyDeviceToDevice, d.stream));
+            steepest_direction();
+            block_direction = false;
+        }
+        if (!accepted) {
+            HIP_TRY(hipMemcpyAsync(d.q.get(), d.q_base.get(), d.q.bytes(), hipMemcpyDeviceToDevice, d.stream));
+            return fail(KHS_ERROR_NOT_CONVERGED, "Gauss-Newton line search failed.");
+        }
+        current = evaluate(true, dt, false, 0.0);
+        const double dnorm = std::sqrt(current.dir_norm2);
+        step_stats_.increment_norm = alpha * dnorm;
+        step_stats_.accepted_step_length = alpha;
+        ++step_stats_.newton_iterations;
+        phase_.store(KHS_PHASE_ASSEMBLING);
+        if (step_stats_.increment_norm / std::sqrt(dofs) <= 2.0 * desc_.increment_tolerance) {
+            hipLaunchKernelGGL(dev::k_update_velocity, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv,
+                               d.q.get(), d.q_old.get(), d.vel.get(), 1.0 / dt);
+            HIP_TRY(hipGetLastError());
+            return KHS_OK;
+        }
+    }
+    hipLaunchKernelGGL(dev::k_update_velocity, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, d.q.get(),
+                       d.q_old.get(), d.vel.get(), 1.0 / dt);
+    HIP_TRY(hipGetLastError());
+    return KHS_OK;
+}
+
+KhsResult Solver::run_frame(double frame_dt, const double *root_a, const double *rot_a, const double *root_b,
+                            const double *rot_b, const double *collider_a, const double *collider_b, uint32_t frame,
+                            uint32_t frame_count)
+{
+    // reference cuda_backend.cu:690-721
+    if (!(frame_dt > 0 && std::isfinite(frame_dt))) return fail(KHS_ERROR_INVALID_ARGUMENT, "Invalid time step.");
+    DeviceState &d = *dev_;
+    const RodView rv = d.rods();
+    cancel_.store(false);
+    clear_failure();
+    frame_clock_.store(std::chrono::steady_clock::now().time_since_epoch().count());
+    frame_.store(frame);
+    frame_count_.store(frame_count);
+    iteration_.store(0);
+    HIP_TRY(hipEventRecord(d.frame_begin, d.stream));
+    const size_t dof_bytes = d.q.bytes();
+    HIP_TRY(hipMemcpyAsync(d.snap_q.get(), d.q.get(), dof_bytes, hipMemcpyDeviceToDevice, d.stream));
+    HIP_TRY(hipMemcpyAsync(d.snap_v.get(), d.vel.get(), dof_bytes, hipMemcpyDeviceToDevice, d.stream));
+    uint32_t substeps = desc_.substeps;
+    const uint32_t max_sub = d.T ? desc_.maximum_substeps : desc_.substeps;
+    uint32_t attempts = 0;
+    const size_t col_bytes = d.col_cur.bytes();
+    StepParams sp{};
+    sp.radius_offset = material_.radius + material_.collider_offset;
+    sp.contact_pad = material_.radius + material_.collider_offset + material_.barrier_distance;
+    sp.sweep_target = material_.radius + material_.collider_offset + desc_.minimum_gap;
+    sp.barrier = material_.barrier_distance;
+    sp.stiffness = material_.contact_stiffness;
+    sp.friction = material_.friction;
+    sp.smoothing = material_.friction_smoothing;
+    sp.minimum_gap = desc_.minimum_gap;
+    sp.has_collider = d.T > 0;
+    for (;;) {
+        ++attempts;
+        HIP_TRY(hipMemcpyAsync(d.q.get(), d.snap_q.get(), dof_bytes, hipMemcpyDeviceToDevice, d.stream));
+        HIP_TRY(hipMemcpyAsync(d.vel.get(), d.snap_v.get(), dof_bytes, hipMemcpyDeviceToDevice, d.stream));
+        clear_failure();
+        error_.clear();
+        phase_.store(KHS_PHASE_ASSEMBLING);
+        substep_count_.store(substeps);
+        iteration_.store(0);
+        step_stats_ = {};
+        step_stats_.struct_size = sizeof(step_stats_);
+        step_stats_.substeps = substeps;
+        step_stats_.minimum_gap = kInf;
+        step_stats_.ccd_step_limit = 1.0;
+        if (d.T) {
+            HIP_TRY(hipMemcpyAsync(d.col_cur.get(), collider_a, col_bytes, hipMemcpyDeviceToDevice, d.stream));
+            if (collider_a != d.col_start.get())
+                HIP_TRY(hipMemcpyAsync(d.col_start.get(), collider_a, col_bytes, hipMemcpyDeviceToDevice, d.stream));
+        }
+        const double dt = frame_dt / substeps;
+        sp.dt = dt;
+        bool crossed = false;
+        KhsResult result = KHS_OK;
+        while (d.substep_events.size() < size_t(substeps) + 1) {
+            hipEvent_t e;
+            HIP_TRY(hipEventCreate(&e));
+            d.substep_events.push_back(e);
+        }
+        for (uint32_t sub = 1; sub <= substeps; ++sub) {
+            substep_.store(sub);
+            iteration_.store(0);
+            HIP_TRY(hipEventRecord(d.substep_events[sub - 1], d.stream));
+            const double f = double(sub) / substeps;
+            hipLaunchKernelGGL(dev::k_begin_substep, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, d.q.get(),
+                               d.q_old.get(), d.vel.get(), root_a, rot_a, root_b, rot_b, f, dt);
+            HIP_TRY(hipGetLastError());
+            if (d.T) {
+                hipLaunchKernelGGL(dev::k_collider_lerp, dim3(grid_for(3 * size_t(d.V), kBlock)), dim3(kBlock), 0,
+                                   d.stream, d.V, d.col_cur.get(), d.col_old.get(), collider_a, collider_b, f);
+                hipEvent_t t0 = d.tic();
+                rebuild_bvh(true);
+                phase_.store(KHS_PHASE_CCD);
+                const unsigned long long none = ~0ull;
+                hipLaunchKernelGGL(dev::k_fill_u64, dim3(1), dim3(64), 0, d.stream, d.key64.get(), size_t(1), none);
+                hipLaunchKernelGGL(dev::k_sweep, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, d.collider(),
+                                   sp, d.q.get(), d.key64.get());
+                hipLaunchKernelGGL(dev::k_sweep_record, dim3(1), dim3(1), 0, d.stream, rv, d.collider(), sp,
+                                   d.q.get(), d.key64.get(), d.sweep.get());
+                HIP_TRY(hipGetLastError());
+                d.toc(t0, d.totals.collision_ms, d.totals.collision_launches);
+                SweepRecord rec{};
+                HIP_TRY(hipMemcpyAsync(&rec, d.sweep.get(), sizeof(rec), hipMemcpyDeviceToHost, d.stream));
+                HIP_TRY(hipStreamSynchronize(d.stream));
+                d.settle();
+                if (rec.claimed) {
+                    set_failure_context(KHS_FAILURE_MOVING_COLLIDER_SWEEP, frame, sub, substeps, max_sub, attempts);
+                    failure_.strand_index = rec.strand;
+                    failure_.element_index = rec.element;
+                    failure_.collider_triangle_index = rec.triangle;
+                    failure_.distance = rec.distance;
+                    failure_.required_distance = rec.required;
+                    failure_.clearance = rec.clearance;
+                    failure_.collider_substep_displacement = rec.displacement;
+                    failure_.collider_frame_displacement = rec.displacement * substeps;
+                    failure_.hair_start = {rec.hair_start[0], rec.hair_start[1], rec.hair_start[2]};
+                    failure_.hair_end = {rec.hair_end[0], rec.hair_end[1], rec.hair_end[2]};
+                    failure_.collider_point = {rec.collider_point[0], rec.collider_point[1], rec.collider_point[2]};
+                    crossed = true;
+                    break;
+                }
+                hipEvent_t t1 = d.tic();
+                rebuild_bvh(false);
+                d.toc(t1, d.totals.collision_ms, d.totals.collision_launches);
+            }
+            phase_.store(KHS_PHASE_ASSEMBLING);
+            result = optimize_substep(dt);
+            if (result != KHS_OK) {
+                if (!cancel_.load()) {
+                    const uint32_t kind = error_.find("outside the barrier") != std::string::npos
+                                              ? KHS_FAILURE_BARRIER_INFEASIBLE
+                                              : KHS_FAILURE_NONLINEAR_SOLVE;
+                    set_failure_context(kind, frame, sub, substeps, max_sub, attempts);
+                }
+                break;
+            }
+            if (d.T && material_.friction > 0.0) {
+                hipLaunchKernelGGL(dev::k_friction, dim3(d.tiled_blocks()), dim3(kBlock), 0, d.stream, rv,
+                                   d.collider(), sp, d.q.get(), d.vel.get(), d.dv.get());
+                hipLaunchKernelGGL(dev::k_apply_dv, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.N, d.dv.get(),
+                                   d.vel.get());
+                HIP_TRY(hipGetLastError());
+            }
+            ++step_stats_.converged_substeps;
+        }
+        if (!crossed && result == KHS_OK) break;
+        HIP_TRY(hipMemcpyAsync(d.q.get(), d.snap_q.get(), dof_bytes, hipMemcpyDeviceToDevice, d.stream));
+        HIP_TRY(hipMemcpyAsync(d.vel.get(), d.snap_v.get(), dof_bytes, hipMemcpyDeviceToDevice, d.stream));
+        HIP_TRY(hipStreamSynchronize(d.stream));
+        if (cancel_.load()) return result;
+        if (substeps >= max_sub) {
+            if (crossed) return fail(KHS_ERROR_NOT_CONVERGED, "Animated collider crossed hair after adaptive substepping.");
+            return result;
+        }
+        substeps = std::min(max_sub, substeps * 2);
+    }
+    if (!animation_ready_) {
+        const size_t n3 = 3 * size_t(d.N) * sizeof(double);
+        HIP_TRY(hipMemcpyAsync(d.root_cur.get(), d.root_pend.get(), n3, hipMemcpyDeviceToDevice, d.stream));
+        HIP_TRY(hipMemcpyAsync(d.rot_cur.get(), d.rot_pend.get(), n3, hipMemcpyDeviceToDevice, d.stream));
+        if (d.T) HIP_TRY(hipMemcpyAsync(d.col_start.get(), d.col_pend.get(), col_bytes, hipMemcpyDeviceToDevice, d.stream));
+    }
+    clear_failure();
+    step_stats_.phase = KHS_PHASE_FINISHED;
+    phase_.store(KHS_PHASE_FINISHED);
+    HIP_TRY(hipEventRecord(d.frame_end, d.stream));
+    HIP_TRY(hipEventSynchronize(d.frame_end));
+    d.settle();
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, d.frame_begin, d.frame_end));
+    gpu_stats_.last_frame_milliseconds = ms;
+    d.substep_ms.assign(substeps, 0.0);
+    for (uint32_t k = 0; k < substeps; ++k) {
+        float sms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&sms, d.substep_events[k], k + 1 < substeps ? d.substep_events[k + 1] : d.frame_end));
+        d.substep_ms[k] = sms;
+    }
+    d.totals.substep_count = substeps;
+    error_.clear();
+    return KHS_OK;
+}
+
+KhsResult Solver::step(double frame_dt)
+{
+    if (!built_) {
+        error_ = "Build the solver first.";
+        return KHS_ERROR_INVALID_STATE;
+    }
+    if (!(frame_dt > 0.0) || !std::isfinite(frame_dt)) {
+        error_ = "The time step must be positive and finite.";
+        return KHS_ERROR_INVALID_ARGUMENT;
+    }
+    DeviceState &d = *dev_;
+    return run_frame(frame_dt, d.root_cur.get(), d.rot_cur.get(), d.root_pend.get(), d.rot_pend.get(),
+                     d.T ? d.col_start.get() : nullptr, d.T ? d.col_pend.get() : nullptr, 0, 0);
+}
+
+KhsResult Solver::step_animation_frame(uint32_t frame, double frame_dt)
+{
+    // reference cuda_backend.cu:488-498
+    if (!built_) return KHS_ERROR_INVALID_STATE;
+    if (!animation_ready_ || frame >= animation_frames_) return fail(KHS_ERROR_INVALID_ARGUMENT, "Invalid animation frame index.");
+    if (frame == 0) {
+        current_frame_ = 0;
+        return KHS_OK;
+    }
+    if (frame != current_frame_ + 1) return fail(KHS_ERROR_INVALID_STATE, "Animation frames must be stepped in order.");
+    DeviceState &d = *dev_;
+    const size_t n3 = 3 * size_t(d.N), v3 = 3 * size_t(d.V);
+    const uint32_t a = frame - 1, b = frame;
+    const KhsResult r = run_frame(frame_dt, d.root_anim.get() + a * n3, d.rot_anim.get() + a * n3,
+                                  d.root_anim.get() + b * n3, d.rot_anim.get() + b * n3,
+                                  d.T ? d.col_anim.get() + a * v3 : nullptr, d.T ? d.col_anim.get() + b * v3 : nullptr,
+                                  frame, animation_frames_);
+    if (r == KHS_OK) current_frame_ = frame;
+    return r;
+}
+
+// ------------------------------------------------------------ checkpoints
+uint64_t Solver::checkpoint_size() const
+{
+    if (!built_ || !animation_ready_) return 0;
+    return 24 + 2 * dof_count() * sizeof(double);
+}
+
+KhsResult Solver::save_checkpoint(void *data, uint64_t capacity)
+{
+    // reference cuda_backend.cu:506-517: {magic, abi, frame, dof} + q + v
+    if (!built_) return KHS_ERROR_INVALID_STATE;
+    const uint64_t required = checkpoint_size();
+    if (!data || !required || capacity < required) return fail(KHS_ERROR_INVALID_ARGUMENT, "Invalid checkpoint output buffer.");
+    unsigned char *out = static_cast<unsigned char *>(data);
+    const uint64_t magic = kCheckpointMagic, dof = dof_count();
+    const uint32_t abi = KHS_ABI_VERSION, frame = current_frame_;
+    std::memcpy(out, &magic, 8);
+    std::memcpy(out + 8, &abi, 4);
+    std::memcpy(out + 12, &frame, 4);
+    std::memcpy(out + 16, &dof, 8);
+    dev_->q.download(reinterpret_cast<double *>(out + 24), dof);
+    dev_->vel.download(reinterpret_cast<double *>(out + 24 + dof * 8), dof);
+    error_.clear();
+    return KHS_OK;
+}
+
+KhsResult Solver::restore_checkpoint(const void *data, uint64_t size)
+{
+    // reference cuda_backend.cu:519-537
+    if (!built_) return KHS_ERROR_INVALID_STATE;
+    const uint64_t required = checkpoint_size();
+    if (!data || !required || size != required) return fail(KHS_ERROR_INVALID_ARGUMENT, "Invalid checkpoint size.");
+    const unsigned char *in = static_cast<const unsigned char *>(data);
+    uint64_t magic = 0, dof = 0;
+    uint32_t abi = 0, frame = 0;
+    std::memcpy(&magic, in, 8);
+    std::memcpy(&abi, in + 8, 4);
+    std::memcpy(&frame, in + 12, 4);
+    std::memcpy(&dof, in + 16, 8);
+    if (magic != kCheckpointMagic || abi != KHS_ABI_VERSION || dof != dof_count() || frame >= animation_frames_)
+        return fail(KHS_ERROR_INVALID_ARGUMENT, "Checkpoint does not match this solver.");
+    std::vector<double> q(dof), v(dof);
+    std::memcpy(q.data(), in + 24, dof * 8);
+    std::memcpy(v.data(), in + 24 + dof * 8, dof * 8);
+    DeviceState &d = *dev_;
+    d.q.upload(q.data(), dof);
+    d.vel.upload(v.data(), dof);
+    d.q_old.upload(q.data(), dof);
+    d.snap_q.upload(q.data(), dof);
+    d.snap_v.upload(v.data(), dof);
+    current_frame_ = frame;
+    cancel_.store(false);
+    frame_.store(frame);
+    substep_.store(0);
+    substep_count_.store(0);
+    iteration_.store(0);
+    phase_.store(KHS_PHASE_FINISHED);
+    step_stats_ = {};
+    step_stats_.struct_size = sizeof(step_stats_);
+    step_stats_.phase = KHS_PHASE_FINISHED;
+    clear_failure();
+    error_.clear();
+    return KHS_OK;
+}
+
+// ----------------------------------------------------------------- output
+KhsResult Solver::copy_original_positions(KhsVec3 *positions, uint32_t capacity) const
+{
+    if (!built_ || !positions || capacity < points_.size()) return KHS_ERROR_INVALID_ARGUMENT;
+    try {
+        DeviceState &d = *dev_;
+        hipLaunchKernelGGL(dev::k_gather_positions, dim3(grid_for(d.P, kBlock)), dim3(kBlock), 0, d.stream, d.q.get(),
+                           d.mapping.get(), d.P, d.out3.get());
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(d.stream));
+        d.out3.download(reinterpret_cast<double *>(positions), 3 * size_t(d.P));
+        return KHS_OK;
+    } catch (...) {
+        return KHS_ERROR_INTERNAL;
+    }
+}
+
+KhsResult Solver::copy_internal_positions(KhsVec3 *positions, uint32_t capacity) const
+{
+    if (!built_ || !positions || capacity < model_.node_count) return KHS_ERROR_INVALID_ARGUMENT;
+    try {
+        DeviceState &d = *dev_;
+        hipLaunchKernelGGL(dev::k_gather_positions, dim3(grid_for(d.N, kBlock)), dim3(kBlock), 0, d.stream, d.q.get(),
+                           (const uint32_t *)nullptr, d.N, d.out3.get());
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(d.stream));
+        d.out3.download(reinterpret_cast<double *>(positions), 3 * size_t(d.N));
+        return KHS_OK;
+    } catch (...) {
+        return KHS_ERROR_INTERNAL;
+    }
+}
+
+KhsResult Solver::copy_mapping(uint32_t *indices, uint32_t capacity) const
+{
+    if (!built_ || !indices || capacity < points_.size()) return KHS_ERROR_INVALID_ARGUMENT;
+    std::copy(model_.original_to_internal.begin(), model_.original_to_internal.end(), indices);
+    return KHS_OK;
+}
+
+KhsResult Solver::build_stats(KhsBuildStats *stats) const
+{
+    if (!built_ || !stats || stats->struct_size < sizeof(KhsBuildStats)) return KHS_ERROR_INVALID_ARGUMENT;
+    *stats = build_stats_;
+    return KHS_OK;
+}
+
+KhsResult Solver::step_stats(KhsStepStats *stats) const
+{
+    if (!stats || stats->struct_size < sizeof(KhsStepStats)) return KHS_ERROR_INVALID_ARGUMENT;
+    *stats = step_stats_;
+    return KHS_OK;
+}
+
+KhsResult Solver::gpu_stats(KhsGpuStats *stats) const
+{
+    if (!built_ || !stats || stats->struct_size < sizeof(KhsGpuStats)) return KHS_ERROR_INVALID_ARGUMENT;
+    *stats = gpu_stats_;
+    return KHS_OK;
+}
+
+KhsResult Solver::progress(KhsProgress *out) const
+{
+    if (!built_ || !out || out->struct_size < sizeof(KhsProgress)) return KHS_ERROR_INVALID_ARGUMENT;
+    KhsProgress p{};
+    p.struct_size = sizeof(p);
+    p.phase = phase_.load();
+    p.frame_index = frame_.load();
+    p.frame_count = frame_count_.load();
+    p.substep = substep_.load();
+    p.substep_count = substep_count_.load();
+    p.nonlinear_iteration = iteration_.load();
+    p.nonlinear_iteration_limit = desc_.newton_iterations;
+    p.cancelled = cancel_.load() ? 1u : 0u;
+    const auto start = std::chrono::steady_clock::time_point(std::chrono::steady_clock::duration(frame_clock_.load()));
+    p.frame_elapsed_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+    *out = p;
+    return KHS_OK;
+}
+
+KhsResult Solver::failure_diagnostics(KhsFailureDiagnostics *diagnostics) const
+{
+    if (!built_ || !diagnostics || diagnostics->struct_size < sizeof(KhsFailureDiagnostics))
+        return KHS_ERROR_INVALID_ARGUMENT;
+    *diagnostics = failure_;
+    return KHS_OK;
+}
+
+KhsResult Solver::request_cancel()
+{
+    if (!built_) return KHS_ERROR_INVALID_STATE;
+    cancel_.store(true);
+    return KHS_OK;
+}
+
+// ------------------------------------------------------------- test hooks
+KhsResult Solver::get_state(double *q, double *v, uint64_t count) const
+{
+    if (!built_ || count != dof_count()) return KHS_ERROR_INVALID_ARGUMENT;
+    HIP_TRY(hipStreamSynchronize(dev_->stream));
+    if (q) dev_->q.download(q, count);
+    if (v) dev_->vel.download(v, count);
+    return KHS_OK;
+}
+
+KhsResult Solver::set_state(const double *q, const double *q_old, const double *v, uint64_t count)
+{
+    if (!built_ || !q || !v || count != dof_count()) return KHS_ERROR_INVALID_ARGUMENT;
+    HIP_TRY(hipStreamSynchronize(dev_->stream));
+    dev_->q.upload(q, count);
+    dev_->vel.upload(v, count);
+    dev_->q_old.upload(q_old ? q_old : q, count);
+    return KHS_OK;
+}
+
+KhsResult Solver::copy_node_model(double *frames9, double *mass, double *rotational_mass, uint32_t *flags,
+                                  uint32_t capacity) const
+{
+    if (!built_ || capacity < model_.node_count) return KHS_ERROR_INVALID_ARGUMENT;
+    const DeviceState &d = *dev_;
+    if (frames9) d.frame.download(frames9, 9 * size_t(d.N));
+    if (mass) d.mass.download(mass, d.N);
+    if (rotational_mass) d.rmass.download(rotational_mass, d.N);
+    if (flags) {
+        std::vector<uint32_t> f(d.N);
+        d.flags.download(f.data(), d.N);
+        for (uint32_t n = 0; n < d.N; ++n)
+            flags[n] = ((f[n] & kNodeFixed) ? 1u : 0u) | ((f[n] & kNodeVirtual) ? 2u : 0u);
+    }
+    return KHS_OK;
+}
+
+KhsResult Solver::copy_element_model(uint32_t *nodes2, uint32_t *strand, uint32_t *contact, double *rest7,
+                                     uint32_t capacity) const
+{
+    if (!built_ || capacity < model_.element_count) return KHS_ERROR_INVALID_ARGUMENT;
+    const DeviceState &d = *dev_;
+    std::vector<double> rest(8 * size_t(d.N));
+    d.rest.download(rest.data(), rest.size());
+    std::vector<uint32_t> f(d.N), so(d.N);
+    d.flags.download(f.data(), d.N);
+    d.strand_of.download(so.data(), d.N);
+    uint32_t e = 0;
+    for (uint32_t n = 0; n < d.N; ++n) {
+        if (!(f[n] & kNodeHasElement)) continue;
+        if (nodes2) { nodes2[2 * e] = n; nodes2[2 * e + 1] = n + 1; }
+        if (strand) strand[e] = so[n];
+        if (contact) contact[e] = (f[n] & kNodeContact) ? 1u : 0u;
+        if (rest7) for (int k = 0; k < 7; ++k) rest7[7 * size_t(e) + k] = rest[8 * size_t(n) + k];
+        ++e;
+    }
+    return KHS_OK;
+}
+
+KhsResult Solver::test_evaluate(double dt, double *gradient, double *direction, KhsxEvaluation *out)
+{
+    if (!built_ || !(dt > 0.0) || !out || out->struct_size < sizeof(KhsxEvaluation)) return KHS_ERROR_INVALID_ARGUMENT;
+    DeviceState &d = *dev_;
+    if (d.T) rebuild_bvh(false);
+    const Reduced r = evaluate(true, dt, false, 0.0);
+    KhsxEvaluation e{};
+    e.struct_size = sizeof(e);
+    e.feasible = r.feasible ? 1u : 0u;
+    e.candidate_pairs = r.candidates;
+    e.active_contacts = r.active;
+    e.objective = r.objective;
+    e.elastic = r.elastic;
+    e.contact = r.contact;
+    e.kinetic = r.kinetic;
+    e.minimum_gap = r.min_gap;
+    if (gradient) d.grad.download(gradient, dof_count());
+    if (direction) {
+        const RodView rv = d.rods();
+        hipLaunchKernelGGL(dev::k_fill_u32, dim3(1), dim3(64), 0, d.stream, d.flag.get(), size_t(1), 0u);
+        hipLaunchKernelGGL(dev::k_block_solve, dim3(grid_for(d.S, 64)), dim3(64), 0, d.stream, rv, d.Dblk.get(),
+                           d.Ublk.get(), d.grad.get(), d.Cp.get(), d.dp.get(), d.dir.get(), d.flag.get());
+        HIP_TRY(hipGetLastError());
+        unsigned failed = 0;
+        HIP_TRY(hipMemcpyAsync(&failed, d.flag.get(), sizeof(unsigned), hipMemcpyDeviceToHost, d.stream));
+        HIP_TRY(hipStreamSynchronize(d.stream));
+        e.solve_failed = failed;
+        d.dir.download(direction, dof_count());
+    }
+    *out = e;
+    return KHS_OK;
+}
+
+KhsResult Solver::broadphase_pairs(uint32_t *pairs, uint64_t capacity, uint64_t *count)
+{
+    if (!built_ || !count) return KHS_ERROR_INVALID_ARGUMENT;
+    DeviceState &d = *dev_;
+    if (!d.T) {
+        *count = 0;
+        return KHS_OK;
+    }
+    rebuild_bvh(false);
+    Buffer<uint32_t> out;
+    out.allocate(std::max<uint64_t>(2 * capacity, 2));
+    StepParams p{};
+    p.contact_pad = material_.radius + material_.collider_offset + material_.barrier_distance;
+    hipLaunchKernelGGL(dev::k_fill_u64, dim3(1), dim3(64), 0, d.stream, d.key64.get(), size_t(1), 0ull);
+    hipLaunchKernelGGL(dev::k_broadphase_pairs, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.rods(),
+                       d.collider(), p, d.q.get(), out.get(), (unsigned long long)capacity, d.key64.get());
+    HIP_TRY(hipGetLastError());
+    unsigned long long total = 0;
+    HIP_TRY(hipMemcpyAsync(&total, d.key64.get(), sizeof(total), hipMemcpyDeviceToHost, d.stream));
+    HIP_TRY(hipStreamSynchronize(d.stream));
+    *count = total;
+    if (pairs && capacity) {
+        const uint64_t n = std::min<uint64_t>(total, capacity);
+        std::vector<uint64_t> keyed(n);
+        std::vector<uint32_t> raw(2 * n);
+        out.download(raw.data(), 2 * n);
+        for (uint64_t i = 0; i < n; ++i) keyed[i] = (uint64_t(raw[2 * i]) << 32) | raw[2 * i + 1];
+        std::sort(keyed.begin(), keyed.end());
+        for (uint64_t i = 0; i < n; ++i) {
+            pairs[2 * i] = uint32_t(keyed[i] >> 32);
+            pairs[2 * i + 1] = uint32_t(keyed[i]);
+        }
+    }
+    return (pairs && total > capacity) ? KHS_ERROR_INVALID_ARGUMENT : KHS_OK;
+}
+
+KhsResult Solver::set_convergence_mode(uint32_t mode)
+{
+    if (mode > 1) return KHS_ERROR_INVALID_ARGUMENT;
+    convergence_mode_ = mode;
+    return KHS_OK;
+}
+
+KhsResult Solver::set_precision_mode(uint32_t mode)
+{
+    if (mode > 0) {
+        error_ = "Only the FP64 step (mode 0) is available.";
+        return KHS_ERROR_INVALID_ARGUMENT;
+    }
+    precision_mode_ = mode;
+    return KHS_OK;
+}
+
+KhsResult Solver::set_kernel_timing(uint32_t enable)
+{
+    if (!built_) return KHS_ERROR_INVALID_STATE;
+    dev_->timing = enable != 0;
+    dev_->totals = KhsxKernelTiming{};
+    return KHS_OK;
+}
+
+KhsResult Solver::kernel_timing(KhsxKernelTiming *out) const
+{
+    if (!built_ || !out || out->struct_size < sizeof(KhsxKernelTiming)) return KHS_ERROR_INVALID_ARGUMENT;
+    KhsxKernelTiming t = dev_->totals;
+    t.struct_size = sizeof(t);
+    t.nodes = model_.node_count;
+    *out = t;
+    return KHS_OK;
+}
+
+KhsResult Solver::substep_times(double *ms, uint32_t capacity, uint32_t *count) const
+{
+    if (!built_ || !count) return KHS_ERROR_INVALID_ARGUMENT;
+    const std::vector<double> &v = dev_->substep_ms;
+    *count = uint32_t(v.size());
+    if (ms)
+        for (uint32_t i = 0; i < std::min<uint32_t>(capacity, uint32_t(v.size())); ++i) ms[i] = v[i];
+    return KHS_OK;
+}
+
+bool query_device(KhsGpuInfo &info, std::string &error)
+{
+    std::memset(&info, 0, sizeof(info));
+    info.struct_size = sizeof(info);
+    int count = 0;
+    const hipError_t r = hipGetDeviceCount(&count);
+    if (r != hipSuccess || count < 1) {
+        info.device_ordinal = -1;
+        error = r == hipSuccess ? "No HIP GPU found." : hipGetErrorString(r);
+        return false;
+    }
+    hipDeviceProp_t prop{};
+    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) {
+        error = "Failed to query the HIP device.";
+        return false;
+    }
+    info.available = 1;
+    info.device_ordinal = 0;
+    // gfx950 reports major 9, minor 5
+    info.compute_capability_major = uint32_t(prop.major);
+    info.compute_capability_minor = uint32_t(prop.minor);
+    info.total_vram_bytes = prop.totalGlobalMem;
+    std::snprintf(info.device_name, sizeof(info.device_name), "%s (%s)", prop.name, prop.gcnArchName);
+    return true;
+}
+
+}  // namespace khs

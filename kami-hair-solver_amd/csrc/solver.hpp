@@ -1,0 +1,120 @@
+// The hair solver behind the C ABI: host model build plus the device-resident
+// incremental-potential step on one MI355X (HIP, gfx950).
+#pragma once
+
+#include "kami_hair_solver.h"
+#include "kami_hair_solver_ext.h"
+#include "model.hpp"
+
+#include <atomic>
+#include <chrono>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace khs {
+
+struct DeviceState;  // device buffers, defined in solver.hip
+
+class Solver {
+public:
+    explicit Solver(const KhsSolverDesc &desc);
+    ~Solver();
+
+    KhsResult set_hair(const KhsVec3 *points, uint32_t point_count, const uint32_t *offsets, uint32_t strand_count,
+                       const KhsHairMaterial &material);
+    KhsResult set_collider(const KhsVec3 *vertices, uint32_t vertex_count, const KhsTriangle *triangles,
+                           uint32_t triangle_count);
+    KhsResult build();
+    KhsResult update_runtime_parameters(const KhsSolverDesc &desc, const KhsHairMaterial &material);
+    KhsResult update_collider(const KhsVec3 *vertices, uint32_t vertex_count);
+    KhsResult update_roots(const KhsVec3 *points, uint32_t point_count);
+    KhsResult step(double frame_dt);
+    KhsResult allocate_animation(uint32_t frame_count);
+    KhsResult set_root_animation_frame(uint32_t frame, const KhsVec3 *points, uint32_t point_count);
+    KhsResult set_collider_animation_frame(uint32_t frame, const KhsVec3 *vertices, uint32_t vertex_count);
+    KhsResult finalize_animation();
+    KhsResult step_animation_frame(uint32_t frame, double frame_dt);
+    uint64_t checkpoint_size() const;
+    KhsResult save_checkpoint(void *data, uint64_t capacity);
+    KhsResult restore_checkpoint(const void *data, uint64_t size);
+
+    uint32_t original_point_count() const { return uint32_t(points_.size()); }
+    uint32_t internal_node_count() const { return built_ ? model_.node_count : 0; }
+    KhsResult copy_original_positions(KhsVec3 *positions, uint32_t capacity) const;
+    KhsResult copy_internal_positions(KhsVec3 *positions, uint32_t capacity) const;
+    KhsResult copy_mapping(uint32_t *indices, uint32_t capacity) const;
+    KhsResult build_stats(KhsBuildStats *stats) const;
+    KhsResult step_stats(KhsStepStats *stats) const;
+    KhsResult gpu_stats(KhsGpuStats *stats) const;
+    KhsResult progress(KhsProgress *progress) const;
+    KhsResult failure_diagnostics(KhsFailureDiagnostics *diagnostics) const;
+    KhsResult request_cancel();
+
+    // additive hooks (kami_hair_solver_ext.h)
+    uint64_t dof_count() const { return built_ ? uint64_t(model_.node_count) * 6 : 0; }
+    KhsResult get_state(double *q, double *v, uint64_t count) const;
+    KhsResult set_state(const double *q, const double *q_old, const double *v, uint64_t count);
+    KhsResult copy_node_model(double *frames9, double *mass, double *rotational_mass, uint32_t *flags,
+                              uint32_t capacity) const;
+    KhsResult copy_element_model(uint32_t *nodes2, uint32_t *strand, uint32_t *contact, double *rest7,
+                                 uint32_t capacity) const;
+    KhsResult test_evaluate(double dt, double *gradient, double *direction, KhsxEvaluation *out);
+    KhsResult broadphase_pairs(uint32_t *pairs, uint64_t capacity, uint64_t *count);
+    KhsResult set_convergence_mode(uint32_t mode);
+    KhsResult set_precision_mode(uint32_t mode);
+    KhsResult set_kernel_timing(uint32_t enable);
+    KhsResult kernel_timing(KhsxKernelTiming *out) const;
+    KhsResult substep_times(double *ms, uint32_t capacity, uint32_t *count) const;
+
+    const std::string &last_error() const { return error_; }
+    void set_error(std::string message) { error_ = std::move(message); }
+
+private:
+    struct Reduced;
+    KhsResult fail(KhsResult result, const std::string &message);
+    void clear_failure();
+    void set_failure_context(uint32_t kind, uint32_t frame, uint32_t substep, uint32_t attempted, uint32_t maximum,
+                             uint32_t attempts);
+    KhsResult upload_model();
+    KhsResult initial_feasibility();
+    void release_device();
+    void rebuild_bvh(bool swept);
+    Reduced evaluate(bool gradient, double dt, bool line_search, double alpha);
+    bool make_direction();
+    void steepest_direction();
+    void limit_direction(double max_translation);
+    KhsResult optimize_substep(double dt);
+    KhsResult run_frame(double frame_dt, const double *root_a, const double *rot_a, const double *root_b,
+                        const double *rot_b, const double *collider_a, const double *collider_b, uint32_t frame,
+                        uint32_t frame_count);
+
+    KhsSolverDesc desc_{};
+    KhsHairMaterial material_{};
+    bool has_hair_ = false, has_collider_ = false, built_ = false;
+    std::string error_;
+    std::vector<Vec3> points_;
+    std::vector<uint32_t> offsets_;
+    ColliderMesh collider_;
+    std::vector<Vec3> collider_pending_host_;
+    RodModel model_;
+    KhsBuildStats build_stats_{};
+    KhsStepStats step_stats_{};
+    KhsGpuStats gpu_stats_{};
+    KhsFailureDiagnostics failure_{};
+    std::unique_ptr<DeviceState> dev_;
+    uint32_t convergence_mode_ = 0;
+    uint32_t precision_mode_ = 0;
+    double trust_radius_ = 0.0;
+    uint32_t animation_frames_ = 0, current_frame_ = 0;
+    bool animation_ready_ = false;
+    std::vector<bool> root_frame_set_, collider_frame_set_;
+    std::atomic<bool> cancel_{false};
+    std::atomic<uint32_t> phase_{KHS_PHASE_IDLE}, frame_{0}, frame_count_{0}, substep_{0}, substep_count_{0},
+        iteration_{0};
+    std::atomic<int64_t> frame_clock_{0};
+};
+
+bool query_device(KhsGpuInfo &info, std::string &error);
+
+}  // namespace khs

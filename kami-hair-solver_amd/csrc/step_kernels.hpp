@@ -1,0 +1,1147 @@
+// Device kernels of the incremental-potential hair step. Included by
+// solver.hip only (one translation unit, no relocatable device code).
+//
+// Tiled kernels: wavefront w owns nodes [63 w, 63 w + 63); lane l handles the
+// element whose left node is L = 63 w - 1 + l and, for l >= 1, owns node L.
+// Lane 0 recomputes the last element of the previous tile, so every node
+// receives its left element's contribution from the lane below it through a
+// DPP lane shift, and its own element's from its own lane: no atomics, and a
+// fixed summation order (inertia, left element, right element) that matches
+// the oracle's node-then-element order.
+#pragma once
+
+#include "kernels.hpp"
+#include "model.hpp"
+
+namespace khs {
+namespace dev {
+
+__device__ __forceinline__ double inf_double() { return __longlong_as_double(0x7ff0000000000000LL); }
+
+// Value of the lane below (lane 0 receives 0), a DPP wavefront shift right.
+__device__ __forceinline__ double from_lane_below(double v)
+{
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    const int rlo = __builtin_amdgcn_update_dpp(0, lo, 0x138, 0xf, 0xf, false);
+    const int rhi = __builtin_amdgcn_update_dpp(0, hi, 0x138, 0xf, 0xf, false);
+    return __hiloint2double(rhi, rlo);
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+__device__ __forceinline__ double wave_min(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmin(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+__device__ __forceinline__ unsigned wave_or(unsigned v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v |= __shfl_xor(v, o, kWave);
+    return v;
+}
+
+__device__ inline Partial empty_partial()
+{
+    Partial p;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) p.sum[k] = 0.0;
+    p.min_gap = inf_double();
+    p.alpha = 1.0;
+    p.max_translation = 0.0;
+    p.candidates = p.active = 0;
+    p.infeasible = p.failed = 0;
+    return p;
+}
+
+// Block-wide fixed-order reduction of one Partial per thread; thread 0
+// writes the result.
+__device__ inline void block_reduce(Partial p, Partial *out)
+{
+    __shared__ Partial waves[kBlock / kWave];
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) p.sum[k] = wave_sum(p.sum[k]);
+    p.min_gap = wave_min(p.min_gap);
+    p.alpha = wave_min(p.alpha);
+    p.max_translation = wave_max(p.max_translation);
+    p.candidates = wave_sum_u64(p.candidates);
+    p.active = wave_sum_u64(p.active);
+    p.infeasible = wave_or(p.infeasible);
+    p.failed = wave_or(p.failed);
+    if (lane == 0) waves[wave] = p;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Partial r = waves[0];
+        for (int w = 1; w < int(blockDim.x / kWave); ++w) {
+            const Partial &o = waves[w];
+            for (int k = 0; k < 8; ++k) r.sum[k] += o.sum[k];
+            r.min_gap = fmin(r.min_gap, o.min_gap);
+            r.alpha = fmin(r.alpha, o.alpha);
+            r.max_translation = fmax(r.max_translation, o.max_translation);
+            r.candidates += o.candidates;
+            r.active += o.active;
+            r.infeasible |= o.infeasible;
+            r.failed |= o.failed;
+        }
+        *out = r;
+    }
+}
+
+// Folds per-block partials in index order (one block of kBlock threads).
+__global__ __launch_bounds__(kBlock) void k_reduce(const Partial *partials, uint32_t count, Partial *out)
+{
+    Partial acc = empty_partial();
+    for (uint32_t i = threadIdx.x; i < count; i += kBlock) {
+        const Partial &o = partials[i];
+        for (int k = 0; k < 8; ++k) acc.sum[k] += o.sum[k];
+        acc.min_gap = fmin(acc.min_gap, o.min_gap);
+        acc.alpha = fmin(acc.alpha, o.alpha);
+        acc.max_translation = fmax(acc.max_translation, o.max_translation);
+        acc.candidates += o.candidates;
+        acc.active += o.active;
+        acc.infeasible |= o.infeasible;
+        acc.failed |= o.failed;
+    }
+    block_reduce(acc, out);
+}
+
+__device__ __forceinline__ V3 vertex(const ColliderView &c, uint32_t i) { return load3(c.current + 3 * size_t(i)); }
+__device__ __forceinline__ V3 vertex_prev(const ColliderView &c, uint32_t i)
+{
+    return load3(c.previous + 3 * size_t(i));
+}
+
+// Closest candidate triangle of a segment; ties in distance go to the
+// lower triangle index.
+__device__ inline bool closest_triangle(const ColliderView &c, V3 p0, V3 p1, double pad, Closest &best,
+                                        uint32_t &best_tri, unsigned long long &count)
+{
+    best = Closest{1.0e300, 0.0, mk(1, 0, 0)};
+    best_tri = 0xffffffffu;
+    bvh_query(c.bvh, vmin(p0, p1), vmax(p0, p1), pad, [&](uint32_t t) {
+        ++count;
+        const uint32_t *tri = c.triangles + 3 * size_t(t);
+        const Closest cand = closest_segment_triangle(p0, p1, vertex(c, tri[0]), vertex(c, tri[1]), vertex(c, tri[2]));
+        if (cand.distance < best.distance || (cand.distance == best.distance && t < best_tri)) {
+            best = cand;
+            best_tri = t;
+        }
+    });
+    return best_tri != 0xffffffffu;
+}
+
+// ------------------------------------------------------------ evaluation
+// Incremental potential of one substep (reference cuda_backend.cu:883-939):
+// inertia and damping per node, Cosserat elasticity and log-barrier contact
+// per element; with GRAD also the gradient and the Gauss-Newton blocks (the
+// diagonal block D and the upper coupling block U of each node). With LS the
+// state is q = alpha d + q_base, written back by the owners.
+template <bool GRAD, bool LS>
+__global__ __launch_bounds__(kBlock) void k_evaluate(RodView m, ColliderView col, StepParams p, double *q,
+                                                     const double *q_old, const double *vel, const double *q_base,
+                                                     const double *dir, double alpha, double *grad, double *Dblk,
+                                                     double *Ublk, Partial *partials)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const long tile = long(blockIdx.x) * kTilesPerBlock + threadIdx.x / kWave;
+    const long L = tile * kTile - 1 + lane;
+    const bool in_range = L >= 0 && L < long(m.node_count);
+    const bool owner = lane >= 1 && in_range;
+    const uint32_t fl = in_range ? m.flags[L] : 0u;
+    const bool elem = (fl & kNodeHasElement) != 0;
+    const bool fixed_i = (fl & kNodeFixed) != 0;
+    const bool fixed_j = elem && (m.flags[L + 1] & kNodeFixed) != 0;
+
+    double qi[6] = {0, 0, 0, 0, 0, 0}, qj[6] = {0, 0, 0, 0, 0, 0};
+    if (in_range) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) qi[k] = LS ? alpha * dir[6 * L + k] + q_base[6 * L + k] : q[6 * L + k];
+    }
+    if (elem) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            qj[k] = LS ? alpha * dir[6 * (L + 1) + k] + q_base[6 * (L + 1) + k] : q[6 * (L + 1) + k];
+    }
+    if (LS && owner) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) q[6 * L + k] = qi[k];
+    }
+
+    Partial acc = empty_partial();
+
+    // Inertia around q_old + dt v + g dt^2 and damping (cuda_backend.cu:883-897).
+    double gnode[6] = {0, 0, 0, 0, 0, 0}, dnode[6] = {0, 0, 0, 0, 0, 0};
+    if (owner && !fixed_i) {
+        const double kx = m.mass[L] * p.inv_dt2, kr = m.rotational_mass[L] * p.inv_dt2;
+        const double cx = m.mass[L] * p.damping * p.inv_dt, cr = m.rotational_mass[L] * p.damping * p.inv_dt;
+        double obj = 0.0, kin = 0.0;
+#pragma unroll
+        for (int d = 0; d < 6; ++d) {
+            const double gr = d < 3 ? p.gravity[d] * p.dt * p.dt : 0.0;
+            const double predicted = q_old[6 * L + d] + p.dt * vel[6 * L + d] + gr;
+            const double delta = qi[d] - predicted;
+            const double damp = qi[d] - q_old[6 * L + d];
+            const double k = d < 3 ? kx : kr, c = d < 3 ? cx : cr;
+            obj += 0.5 * k * delta * delta + 0.5 * c * damp * damp;
+            if (d < 3) kin += 0.5 * m.mass[L] * damp * damp * p.inv_dt2;
+            gnode[d] = k * delta + c * damp;
+            dnode[d] = k + c;
+        }
+        acc.sum[kObjective] += obj;
+        acc.sum[kKinetic] += kin;
+    }
+
+    // Barrier contact against the closest triangle (cuda_backend.cu:920-938).
+    double cg[6] = {0, 0, 0, 0, 0, 0}, dg[6] = {0, 0, 0, 0, 0, 0}, curvature = 0.0, contact_energy = 0.0;
+    if (elem && p.has_collider && !fixed_i && (fl & kNodeContact)) {
+        const V3 p0 = mk(qi[0], qi[1], qi[2]), p1 = mk(qj[0], qj[1], qj[2]);
+        Closest pair;
+        uint32_t tri = 0;
+        unsigned long long count = 0;
+        const bool found = closest_triangle(col, p0, p1, p.contact_pad, pair, tri, count);
+        acc.candidates += count;
+        if (found) {
+            const double gap = pair.distance - p.radius_offset;
+            acc.min_gap = fmin(acc.min_gap, gap);
+            if (gap <= p.minimum_gap) {
+                acc.infeasible = 1;
+            } else if (gap < p.barrier) {
+                acc.active += 1;
+                const uint32_t *t3 = col.triangles + 3 * size_t(tri);
+                const V3 A = vertex(col, t3[0]), B = vertex(col, t3[1]), C = vertex(col, t3[2]);
+                const V3 bp = add(add(mul(A, pair.bary.x), mul(B, pair.bary.y)), mul(C, pair.bary.z));
+                const double w0 = 1.0 - pair.t, w1 = pair.t;
+                const V3 hp = add(mul(p0, w0), mul(p1, w1));
+                const V3 diff = sub(hp, bp);
+                const double dist = sqrt(dot(diff, diff) + 1e-30);
+                const double dgap = dist - p.radius_offset;
+                if (dgap < p.barrier) {
+                    const double aa = p.barrier - dgap, lg = log(dgap * (1.0 / p.barrier));
+                    contact_energy = -p.stiffness * aa * aa * lg;
+                    if (!isfinite(contact_energy)) {
+                        acc.infeasible = 1;
+                        contact_energy = 0.0;
+                    } else if (GRAD) {
+                        // b'(g) grad(g), grad(g) = (diff / dist) (w0, w1)
+                        const double db = p.stiffness * (2.0 * aa * lg - aa * aa / dgap);
+                        const double s = db / dist;
+                        cg[0] = s * diff.x * w0; cg[1] = s * diff.y * w0; cg[2] = s * diff.z * w0;
+                        cg[3] = s * diff.x * w1; cg[4] = s * diff.y * w1; cg[5] = s * diff.z * w1;
+                        // b''(g) grad(g) grad(g)^T with the closest point
+                        // held fixed
+                        const V3 normal = mul(sub(add(mul(p0, 1.0 - pair.t), mul(p1, pair.t)), bp),
+                                              1.0 / fmax(pair.distance, 1e-15));
+                        const double a2 = p.barrier - gap;
+                        curvature = p.stiffness * (-2.0 * log(gap / p.barrier) + 4.0 * a2 / gap + a2 * a2 / (gap * gap));
+                        dg[0] = normal.x * (1.0 - pair.t); dg[1] = normal.y * (1.0 - pair.t); dg[2] = normal.z * (1.0 - pair.t);
+                        dg[3] = normal.x * pair.t; dg[4] = normal.y * pair.t; dg[5] = normal.z * pair.t;
+                    }
+                }
+            }
+        }
+    }
+
+    // Cosserat elasticity (cuda_backend.cu:905-919).
+    ElementJacobian J;
+    double elastic = 0.0;
+    if (elem) {
+        element_strains<GRAD>(qi, qj, m.frame + 9 * L, m.frame + 9 * (L + 1), m.rest + 8 * L, J);
+        const double L0 = m.rest[8 * L];
+#pragma unroll
+        for (int s = 0; s < 6; ++s) {
+            const double c = (0.5 * L0) * p.weights[s];
+            elastic = elastic + (c * J.strain[s]) * J.strain[s];
+        }
+    }
+    acc.sum[kObjective] += elastic;
+    acc.sum[kElastic] += elastic;
+    acc.sum[kObjective] += contact_energy;
+    acc.sum[kContact] += contact_energy;
+
+    if (GRAD) {
+        const double L0 = elem ? m.rest[8 * L] : 0.0;
+        double Lw[6], cw[6];
+#pragma unroll
+        for (int s = 0; s < 6; ++s) {
+            Lw[s] = L0 * p.weights[s];
+            cw[s] = (0.5 * L0) * p.weights[s];
+        }
+        // Jacobian entry of strain s by coordinate a of (xi, ui, xj, uj).
+        auto jac = [&](int s, int a) -> double {
+            if (!elem) return 0.0;
+            const int blk = a / 3, c = a % 3;
+            if (s < 3) {
+                if (blk == 0) return -J.Sx[3 * s + c];
+                if (blk == 1) return J.Si[3 * s + c];
+                if (blk == 2) return J.Sx[3 * s + c];
+                return J.Sj[3 * s + c];
+            }
+            if (blk == 1) return J.Ki[3 * (s - 3) + c];
+            if (blk == 3) return J.Kj[3 * (s - 3) + c];
+            return 0.0;
+        };
+        auto structural_zero = [](int s, int a) { return s >= 3 && (a / 3 == 0 || a / 3 == 2); };
+        // Gradient parts, rounded as the dual-number product rule rounds
+        // sum_s (c J) e + (c e) J.
+        double ge[12];
+#pragma unroll
+        for (int a = 0; a < 12; ++a) {
+            double acc_a = 0.0;
+#pragma unroll
+            for (int s = 0; s < 6; ++s) {
+                if (structural_zero(s, a)) continue;
+                const double j = jac(s, a);
+                acc_a = acc_a + ((cw[s] * j) * J.strain[s] + (cw[s] * J.strain[s]) * j);
+            }
+            ge[a] = elem ? acc_a : 0.0;
+        }
+        // Left element's j-part and contact terms from the lane below.
+        double g_out[6];
+#pragma unroll
+        for (int d = 0; d < 6; ++d) {
+            const double from_left = from_lane_below(ge[6 + d]);
+            const double ct_left = d < 3 ? from_lane_below(cg[3 + d]) : 0.0;
+            g_out[d] = gnode[d] + from_left + ct_left + ge[d] + (d < 3 ? cg[d] : 0.0);
+        }
+        const double curv_left = from_lane_below(curvature);
+        double dg_left[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) dg_left[k] = from_lane_below(dg[3 + k]);
+        if (owner) {
+            double gg = 0.0, dd = 0.0;
+#pragma unroll
+            for (int d = 0; d < 6; ++d) {
+                const double gv = fixed_i ? 0.0 : g_out[d];
+                grad[6 * L + d] = gv;
+                gg += gv * gv;
+                const double dv = dir[6 * L + d];
+                dd += dv * dv;
+            }
+            acc.sum[kGradNorm2] += gg;
+            acc.sum[kDirNorm2] += dd;
+        }
+        const bool write_u = owner && elem && !fixed_i && !fixed_j;
+#pragma unroll
+        for (int a = 0; a < 6; ++a) {
+#pragma unroll
+            for (int b = 0; b < 6; ++b) {
+                // j-part of this element for the node above, i-part for this node
+                double hj = 0.0, hi = 0.0, hu = 0.0;
+#pragma unroll
+                for (int s = 0; s < 6; ++s) {
+                    if (!structural_zero(s, 6 + a) && !structural_zero(s, 6 + b)) hj = hj + (Lw[s] * jac(s, 6 + a)) * jac(s, 6 + b);
+                    if (!structural_zero(s, a) && !structural_zero(s, b)) hi = hi + (Lw[s] * jac(s, a)) * jac(s, b);
+                    if (!structural_zero(s, a) && !structural_zero(s, 6 + b)) hu = hu + (Lw[s] * jac(s, a)) * jac(s, 6 + b);
+                }
+                const double left = from_lane_below(hj);
+                double v = (a == b ? dnode[a] : 0.0) + left;
+                if (a < 3 && b < 3) v = v + (curv_left * dg_left[a]) * dg_left[b];
+                v = v + hi;
+                if (a < 3 && b < 3) v = v + (curvature * dg[a]) * dg[b];
+                if (a < 3 && b < 3) hu = hu + (curvature * dg[a]) * dg[3 + b];
+                if (owner) {
+                    Dblk[36 * L + 6 * a + b] = fixed_i ? 0.0 : v;
+                    Ublk[36 * L + 6 * a + b] = write_u ? hu : 0.0;
+                }
+            }
+        }
+    }
+    block_reduce(acc, partials + blockIdx.x);
+}
+
+// ---------------------------------------------------------- linear solve
+// Gauss-Jordan inverse with partial pivoting after the diagonal shift
+// max(1e-12 max|a_ii|, 1e-24) (cuda_backend.cu:855-864). Every index is a
+// compile-time constant, so both matrices stay in registers; the pivot row
+// exchange is a select.
+__device__ inline bool inverse6(double *a, double *inv)
+{
+#pragma unroll
+    for (int i = 0; i < 36; ++i) inv[i] = (i % 7 == 0) ? 1.0 : 0.0;
+    double scale = 0.0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) scale = fmax(scale, fabs(a[7 * i]));
+    const double shift = fmax(1.0e-12 * scale, 1.0e-24);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) a[7 * i] += shift;
+#pragma unroll
+    for (int col = 0; col < 6; ++col) {
+        int pivot = col;
+        double best = fabs(a[7 * col]);
+#pragma unroll
+        for (int row = col + 1; row < 6; ++row) {
+            const double v = fabs(a[6 * row + col]);
+            if (v > best) { best = v; pivot = row; }
+        }
+        if (best < 1.0e-30 || !isfinite(best)) return false;
+#pragma unroll
+        for (int row = col + 1; row < 6; ++row) {
+            const bool sw = pivot == row;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                const double x = a[6 * col + k], y = a[6 * row + k];
+                a[6 * col + k] = sw ? y : x;
+                a[6 * row + k] = sw ? x : y;
+                const double u = inv[6 * col + k], w = inv[6 * row + k];
+                inv[6 * col + k] = sw ? w : u;
+                inv[6 * row + k] = sw ? u : w;
+            }
+        }
+        const double invp = 1.0 / a[7 * col];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            a[6 * col + k] *= invp;
+            inv[6 * col + k] *= invp;
+        }
+#pragma unroll
+        for (int row = 0; row < 6; ++row) {
+            if (row == col) continue;
+            const double f = a[6 * row + col];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                a[6 * row + k] -= f * a[6 * col + k];
+                inv[6 * row + k] -= f * inv[6 * col + k];
+            }
+        }
+    }
+    return true;
+}
+
+// Block Thomas per strand on the free nodes; fixed root nodes are
+// eliminated and get a zero direction (cuda_backend.cu:866-881).
+__global__ __launch_bounds__(64) void k_block_solve(RodView m, const double *Dblk, const double *Ublk,
+                                                    const double *grad, double *Cp, double *dp, double *dir,
+                                                    unsigned *failed)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= m.strand_count) return;
+    const uint32_t first = m.strand_first[s], begin = first + m.strand_fixed[s], end = first + m.strand_count_of[s];
+    for (uint32_t n = first; n < begin; ++n)
+        for (int k = 0; k < 6; ++k) dir[6 * size_t(n) + k] = 0.0;
+    if (begin >= end) return;
+    for (uint32_t n = begin; n < end; ++n) {
+        double b[36], inv[36], rhs[6];
+#pragma unroll
+        for (int i = 0; i < 36; ++i) b[i] = Dblk[36 * size_t(n) + i];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) rhs[i] = -grad[6 * size_t(n) + i];
+        if (n > begin) {
+            const double *lower = Ublk + 36 * size_t(n - 1);
+            const double *pc = Cp + 36 * size_t(n - 1);
+            const double *pd = dp + 6 * size_t(n - 1);
+            double low[36], prevc[36], prevd[6];
+#pragma unroll
+            for (int i = 0; i < 36; ++i) { low[i] = lower[i]; prevc[i] = pc[i]; }
+#pragma unroll
+            for (int i = 0; i < 6; ++i) prevd[i] = pd[i];
+#pragma unroll
+            for (int r = 0; r < 6; ++r) {
+#pragma unroll
+                for (int c = 0; c < 6; ++c) {
+                    double sum = 0.0;
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) sum += low[6 * k + r] * prevc[6 * k + c];
+                    b[6 * r + c] -= sum;
+                }
+                double sum = 0.0;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) sum += low[6 * k + r] * prevd[k];
+                rhs[r] -= sum;
+            }
+        }
+        if (!inverse6(b, inv)) {
+            atomicOr(failed, 1u);
+            return;
+        }
+        double *cc = Cp + 36 * size_t(n);
+        if (n + 1 < end) {
+            const double *up = Ublk + 36 * size_t(n);
+            double upper[36];
+#pragma unroll
+            for (int i = 0; i < 36; ++i) upper[i] = up[i];
+#pragma unroll
+            for (int r = 0; r < 6; ++r)
+#pragma unroll
+                for (int c = 0; c < 6; ++c) {
+                    double sum = 0.0;
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) sum += inv[6 * r + k] * upper[6 * k + c];
+                    cc[6 * r + c] = sum;
+                }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 36; ++i) cc[i] = 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            double sum = 0.0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) sum += inv[6 * r + k] * rhs[k];
+            dp[6 * size_t(n) + r] = sum;
+        }
+    }
+    double next[6] = {0, 0, 0, 0, 0, 0};
+    for (uint32_t n = end; n-- > begin;) {
+        double cur[6];
+        const double *cc = Cp + 36 * size_t(n);
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            double value = dp[6 * size_t(n) + r];
+            if (n + 1 < end) {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) value -= cc[6 * r + k] * next[k];
+            }
+            cur[r] = value;
+        }
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            dir[6 * size_t(n) + r] = cur[r];
+            next[r] = cur[r];
+        }
+    }
+}
+
+// Slope g.d and the largest translation of a free node (host then limits
+// the direction by the trust radius, cuda_backend.cu:623-630).
+__global__ __launch_bounds__(kBlock) void k_direction_stats(RodView m, const double *grad, const double *dir,
+                                                            const unsigned *failed, Partial *partials)
+{
+    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    Partial acc = empty_partial();
+    if (n < m.node_count) {
+        double gd = 0.0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) gd += grad[6 * size_t(n) + k] * dir[6 * size_t(n) + k];
+        acc.sum[kSlope] = gd;
+        if (!(m.flags[n] & kNodeFixed)) acc.max_translation = norm(load3(dir + 6 * size_t(n)));
+    }
+    if (n == 0 && failed) acc.failed = *failed;
+    block_reduce(acc, partials + blockIdx.x);
+}
+
+// Steepest descent preconditioned by the Jacobi diagonal (cuda_backend.cu:632-638).
+__global__ void k_steepest(RodView m, const double *grad, const double *precond, double *dir)
+{
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= size_t(m.node_count) * 6) return;
+    dir[i] = (m.flags[i / 6] & kNodeFixed) ? 0.0 : (-grad[i]) * precond[i];
+}
+
+__global__ void k_scale(double *x, size_t count, double s)
+{
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < count) x[i] *= s;
+}
+
+__global__ void k_fill_u32(unsigned *x, size_t count, unsigned v)
+{
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < count) x[i] = v;
+}
+
+__global__ void k_fill_u64(unsigned long long *x, size_t count, unsigned long long v)
+{
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < count) x[i] = v;
+}
+
+// Jacobi diagonal: inertia plus per-element stretch and bend/twist
+// stiffness bounds, node-gathered in element order (cuda_backend.cu:815-841).
+__global__ void k_preconditioner(RodView m, double inv_dt, double damping, double translation_k, double rotation_a,
+                                 double rotation_b, double *precond)
+{
+    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= m.node_count) return;
+    const double inv2 = inv_dt * inv_dt;
+    const double pt = m.mass[n] * (inv2 + damping * inv_dt);
+    const double pr = m.rotational_mass[n] * (inv2 + damping * inv_dt);
+    double t = pt, r = pr;
+    if (n > 0 && (m.flags[n - 1] & kNodeHasElement)) {
+        const double L0 = m.rest[8 * size_t(n - 1)];
+        t += translation_k / L0;
+        r += rotation_a * L0 + rotation_b / L0;
+    }
+    if (m.flags[n] & kNodeHasElement) {
+        const double L0 = m.rest[8 * size_t(n)];
+        t += translation_k / L0;
+        r += rotation_a * L0 + rotation_b / L0;
+    }
+    const bool fixed = m.flags[n] & kNodeFixed;
+    for (int d = 0; d < 3; ++d) {
+        precond[6 * size_t(n) + d] = fixed ? 0.0 : 1.0 / fmax(t, 1.0e-30);
+        precond[6 * size_t(n) + 3 + d] = fixed ? 0.0 : 1.0 / fmax(r, 1.0e-30);
+    }
+}
+
+// ------------------------------------------------------------- substeps
+// q_old = q for every node; fixed nodes move to the interpolated root
+// targets and take the matching velocity (cuda_backend.cu:794-801).
+__global__ void k_begin_substep(RodView m, double *q, double *q_old, double *vel, const double *root_a,
+                                const double *rot_a, const double *root_b, const double *rot_b, double f, double dt)
+{
+    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= m.node_count) return;
+    double cur[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        cur[k] = q[6 * size_t(n) + k];
+        q_old[6 * size_t(n) + k] = cur[k];
+    }
+    if (!(m.flags[n] & kNodeFixed)) return;
+    double val[6];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        val[k] = root_a[3 * size_t(n) + k] * (1.0 - f) + root_b[3 * size_t(n) + k] * f;
+        val[3 + k] = rot_a[3 * size_t(n) + k] * (1.0 - f) + rot_b[3 * size_t(n) + k] * f;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        q[6 * size_t(n) + k] = val[k];
+        vel[6 * size_t(n) + k] = (val[k] - cur[k]) / dt;
+    }
+}
+
+// v = (q - q_old) / dt on free nodes (cuda_backend.cu:803-808).
+__global__ void k_update_velocity(RodView m, const double *q, const double *q_old, double *vel, double inv_dt)
+{
+    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= m.node_count || (m.flags[n] & kNodeFixed)) return;
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        vel[6 * size_t(n) + k] = (q[6 * size_t(n) + k] - q_old[6 * size_t(n) + k]) * inv_dt;
+}
+
+// previous = current; current = a (1 - f) + b f (cuda_backend.cu:770-774).
+__global__ void k_collider_lerp(uint32_t count, double *current, double *previous, const double *a, const double *b,
+                                double f)
+{
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= size_t(count) * 3) return;
+    previous[i] = current[i];
+    current[i] = a[i] * (1.0 - f) + b[i] * f;
+}
+
+__global__ void k_copy(double *dst, const double *src, size_t count)
+{
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < count) dst[i] = src[i];
+}
+
+// ---------------------------------------------------------------- CCD
+// Line-search step bound: conservative advancement of each contact-enabled
+// segment along the direction against the static collider
+// (cuda_backend.cu:989-1000), plus the slope g.d.
+__global__ __launch_bounds__(kBlock) void k_ccd_slope(RodView m, ColliderView col, StepParams p, const double *q,
+                                                      const double *grad, const double *dir, Partial *partials)
+{
+    const uint32_t L = blockIdx.x * blockDim.x + threadIdx.x;
+    Partial acc = empty_partial();
+    if (L < m.node_count) {
+        double gd = 0.0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) gd += grad[6 * size_t(L) + k] * dir[6 * size_t(L) + k];
+        acc.sum[kSlope] = gd;
+        const uint32_t fl = m.flags[L];
+        if (p.has_collider && (fl & kNodeHasElement) && (fl & kNodeContact) && !(fl & kNodeFixed)) {
+            const V3 p0 = load3(q + 6 * size_t(L)), p1 = load3(q + 6 * size_t(L + 1));
+            const V3 d0 = load3(dir + 6 * size_t(L)), d1 = load3(dir + 6 * size_t(L + 1));
+            const V3 lo = vmin(vmin(p0, p1), vmin(add(p0, d0), add(p1, d1)));
+            const V3 hi = vmax(vmax(p0, p1), vmax(add(p0, d0), add(p1, d1)));
+            const double speed = fmax(norm(d0), norm(d1));
+            double limit = 1.0;
+            bvh_query(col.bvh, lo, hi, p.sweep_target, [&](uint32_t t) {
+                if (speed < 1e-30) return;
+                const uint32_t *t3 = col.triangles + 3 * size_t(t);
+                const V3 a = vertex(col, t3[0]), b = vertex(col, t3[1]), c = vertex(col, t3[2]);
+                double alpha = 0.0;
+                for (int it = 0; it < 80; ++it) {
+                    const double dist =
+                        closest_segment_triangle(add(p0, mul(d0, alpha)), add(p1, mul(d1, alpha)), a, b, c).distance;
+                    const double clear = dist - p.sweep_target;
+                    if (clear <= 0) { limit = fmin(limit, fmax(0.0, alpha * 0.9)); return; }
+                    const double advance = 0.8 * clear / speed;
+                    if (alpha + advance >= 1) return;
+                    if (advance < 1e-12) { limit = fmin(limit, alpha); return; }
+                    alpha += advance;
+                }
+                limit = fmin(limit, alpha);
+            });
+            acc.alpha = limit;
+        }
+    }
+    block_reduce(acc, partials + blockIdx.x);
+}
+
+// ---------------------------------------------------- collider sweep test
+// Conservative advancement of each candidate triangle from its previous to
+// its current position against the stationary segment
+// (cuda_backend.cu:975-987); the lowest failing (element, triangle) wins.
+__global__ void k_sweep(RodView m, ColliderView col, StepParams p, const double *q, unsigned long long *first_failure)
+{
+    const uint32_t L = blockIdx.x * blockDim.x + threadIdx.x;
+    if (L >= m.node_count) return;
+    const uint32_t fl = m.flags[L];
+    if (!(fl & kNodeHasElement) || !(fl & kNodeContact) || (fl & kNodeFixed)) return;
+    const V3 p0 = load3(q + 6 * size_t(L)), p1 = load3(q + 6 * size_t(L + 1));
+    const uint32_t element = L - m.strand_of[L];
+    unsigned long long worst = ~0ull;
+    bvh_query(col.bvh, vmin(p0, p1), vmax(p0, p1), p.sweep_target, [&](uint32_t t) {
+        const uint32_t *t3 = col.triangles + 3 * size_t(t);
+        const V3 oa = vertex_prev(col, t3[0]), ob = vertex_prev(col, t3[1]), oc = vertex_prev(col, t3[2]);
+        const V3 da = sub(vertex(col, t3[0]), oa), db = sub(vertex(col, t3[1]), ob), dc = sub(vertex(col, t3[2]), oc);
+        const double speed = fmax(norm(da), fmax(norm(db), norm(dc)));
+        if (speed < 1e-30) return;
+        double alpha = 0.0;
+        bool bad = true;
+        for (int it = 0; it < 80; ++it) {
+            const Closest pair = closest_segment_triangle(p0, p1, add(oa, mul(da, alpha)), add(ob, mul(db, alpha)),
+                                                          add(oc, mul(dc, alpha)));
+            const double clearance = pair.distance - p.sweep_target;
+            if (clearance <= 0) break;
+            const double advance = 0.8 * clearance / speed;
+            if (alpha + advance >= 1) { bad = false; break; }
+            if (advance < 1e-12) break;
+            alpha += advance;
+        }
+        if (bad) {
+            const unsigned long long key = (static_cast<unsigned long long>(element) << 32) | t;
+            worst = key < worst ? key : worst;
+        }
+    });
+    if (worst != ~0ull) atomicMin(first_failure, worst);
+}
+
+// Recomputes the advancement of the reported pair to fill the diagnostics
+// (cuda_backend.cu:966-973).
+__global__ void k_sweep_record(RodView m, ColliderView col, StepParams p, const double *q,
+                               const unsigned long long *first_failure, SweepRecord *out)
+{
+    if (threadIdx.x || blockIdx.x) return;
+    const unsigned long long key = *first_failure;
+    SweepRecord r{};
+    if (key == ~0ull) {
+        *out = r;
+        return;
+    }
+    const uint32_t element = uint32_t(key >> 32), t = uint32_t(key & 0xffffffffu);
+    // element e of strand s has left node e + s: find s with strand_first.
+    uint32_t lo = 0, hi = m.strand_count;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (m.strand_first[mid] - mid <= element) lo = mid; else hi = mid;
+    }
+    const uint32_t L = element + lo;
+    const V3 p0 = load3(q + 6 * size_t(L)), p1 = load3(q + 6 * size_t(L + 1));
+    const uint32_t *t3 = col.triangles + 3 * size_t(t);
+    const V3 oa = vertex_prev(col, t3[0]), ob = vertex_prev(col, t3[1]), oc = vertex_prev(col, t3[2]);
+    const V3 da = sub(vertex(col, t3[0]), oa), db = sub(vertex(col, t3[1]), ob), dc = sub(vertex(col, t3[2]), oc);
+    const double speed = fmax(norm(da), fmax(norm(db), norm(dc)));
+    double alpha = 0.0;
+    Closest pair{};
+    V3 a = oa, b = ob, c = oc;
+    for (int it = 0; it < 80; ++it) {
+        a = add(oa, mul(da, alpha));
+        b = add(ob, mul(db, alpha));
+        c = add(oc, mul(dc, alpha));
+        pair = closest_segment_triangle(p0, p1, a, b, c);
+        const double clearance = pair.distance - p.sweep_target;
+        if (clearance <= 0) break;
+        const double advance = 0.8 * clearance / speed;
+        if (alpha + advance >= 1 || advance < 1e-12) break;
+        alpha += advance;
+    }
+    r.claimed = 1;
+    r.strand = m.strand_of[L];
+    r.element = element;
+    r.triangle = t;
+    r.distance = pair.distance;
+    r.required = p.sweep_target;
+    r.clearance = pair.distance - p.sweep_target;
+    r.displacement = speed;
+    store3(r.hair_start, p0);
+    store3(r.hair_end, p1);
+    store3(r.collider_point, add(add(mul(a, pair.bary.x), mul(b, pair.bary.y)), mul(c, pair.bary.z)));
+    *out = r;
+}
+
+// -------------------------------------------------------------- friction
+// Post-substep smoothed Coulomb impulse against the closest triangle
+// (cuda_backend.cu:941-964); the velocity change of each node is gathered
+// from its two elements in element order.
+__global__ __launch_bounds__(kBlock) void k_friction(RodView m, ColliderView col, StepParams p, const double *q,
+                                                     const double *vel, double *dv)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const long tile = long(blockIdx.x) * kTilesPerBlock + threadIdx.x / kWave;
+    const long L = tile * kTile - 1 + lane;
+    const bool in_range = L >= 0 && L < long(m.node_count);
+    const bool owner = lane >= 1 && in_range;
+    const uint32_t fl = in_range ? m.flags[L] : 0u;
+    const bool elem = (fl & kNodeHasElement) != 0;
+    double di[3] = {0, 0, 0}, dj[3] = {0, 0, 0};
+    if (elem && (fl & kNodeContact)) {
+        const bool fixed_i = fl & kNodeFixed, fixed_j = m.flags[L + 1] & kNodeFixed;
+        if (!(fixed_i && fixed_j)) {
+            const V3 p0 = load3(q + 6 * L), p1 = load3(q + 6 * (L + 1));
+            Closest pair;
+            uint32_t tri;
+            unsigned long long count = 0;
+            if (closest_triangle(col, p0, p1, p.contact_pad, pair, tri, count)) {
+                const double gap = pair.distance - p.radius_offset;
+                if (gap > 0.0 && gap < p.barrier) {
+                    const uint32_t *t3 = col.triangles + 3 * size_t(tri);
+                    const double w0 = 1.0 - pair.t, w1 = pair.t;
+                    const V3 hair = add(mul(p0, w0), mul(p1, w1));
+                    const V3 body = add(add(mul(vertex(col, t3[0]), pair.bary.x), mul(vertex(col, t3[1]), pair.bary.y)),
+                                        mul(vertex(col, t3[2]), pair.bary.z));
+                    const V3 old_body =
+                        add(add(mul(vertex_prev(col, t3[0]), pair.bary.x), mul(vertex_prev(col, t3[1]), pair.bary.y)),
+                            mul(vertex_prev(col, t3[2]), pair.bary.z));
+                    const V3 normal = mul(sub(hair, body), 1.0 / fmax(pair.distance, 1.0e-15));
+                    const V3 v0 = load3(vel + 6 * L), v1 = load3(vel + 6 * (L + 1));
+                    const V3 rel = sub(add(mul(v0, w0), mul(v1, w1)), mul(sub(body, old_body), 1.0 / p.dt));
+                    const V3 tangent = sub(rel, mul(normal, dot(normal, rel)));
+                    const double tspeed = norm(tangent);
+                    double inv_mass = 0.0;
+                    if (!fixed_i) inv_mass += w0 * w0 / fmax(m.mass[L], 1.0e-30);
+                    if (!fixed_j) inv_mass += w1 * w1 / fmax(m.mass[L + 1], 1.0e-30);
+                    if (tspeed >= 1.0e-15 && inv_mass > 0.0) {
+                        const double aa = p.barrier - gap, lg = log(gap / p.barrier);
+                        const double fn = p.stiffness * (aa * aa / gap - 2.0 * aa * lg);
+                        const double mag = fmin(tspeed / inv_mass, p.friction * fn * p.dt);
+                        const double smooth = p.smoothing / p.dt;
+                        const V3 imp = mul(tangent, -mag / fmax(sqrt(tspeed * tspeed + smooth * smooth), 1.0e-30));
+                        if (!fixed_i) {
+                            const double sc = w0 / fmax(m.mass[L], 1.0e-30);
+                            di[0] = imp.x * sc; di[1] = imp.y * sc; di[2] = imp.z * sc;
+                        }
+                        if (!fixed_j) {
+                            const double sc = w1 / fmax(m.mass[L + 1], 1.0e-30);
+                            dj[0] = imp.x * sc; dj[1] = imp.y * sc; dj[2] = imp.z * sc;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    double total[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) total[k] = (0.0 + from_lane_below(dj[k])) + di[k];
+    if (owner) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) dv[3 * L + k] = total[k];
+    }
+}
+
+__global__ void k_apply_dv(uint32_t n, const double *dv, double *vel)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) vel[6 * size_t(i) + k] = 1.0 * dv[3 * size_t(i) + k] + vel[6 * size_t(i) + k];
+}
+
+// --------------------------------------------------- build-time checks
+// Ray-parity point-in-mesh test along a slightly skewed +x ray for free,
+// visible nodes (reference solver.cpp:608-645); candidates come from the
+// BVH with a box enclosing the ray inside the collider bounds.
+__global__ void k_point_inside(RodView m, ColliderView col, const double *pos, double x_max, unsigned *first_inside)
+{
+    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= m.node_count) return;
+    if (m.flags[n] & (kNodeFixed | kNodeVirtual)) return;
+    const V3 pt = load3(pos + 3 * size_t(n));
+    const V3 dirv = mk(1.0, 0.0000012345, 0.0000023456);
+    const double reach = fmax(x_max - pt.x, 0.0) + 1.0e-6;
+    const V3 lo = pt, hi = add(pt, mul(dirv, reach));
+    unsigned hits = 0;
+    bvh_query(col.bvh, vmin(lo, hi), vmax(lo, hi), 1.0e-9, [&](uint32_t t) {
+        const uint32_t *t3 = col.triangles + 3 * size_t(t);
+        const V3 a = vertex(col, t3[0]), b = vertex(col, t3[1]), c = vertex(col, t3[2]);
+        const V3 e1 = sub(b, a), e2 = sub(c, a);
+        const V3 h = mk(dirv.y * e2.z - dirv.z * e2.y, dirv.z * e2.x - dirv.x * e2.z, dirv.x * e2.y - dirv.y * e2.x);
+        const double det = dot(e1, h);
+        if (fabs(det) < 1.0e-14) return;
+        const double inv = 1.0 / det;
+        const V3 s = sub(pt, a);
+        const double u = inv * dot(s, h);
+        if (u < 0.0 || u > 1.0) return;
+        const V3 qv = mk(s.y * e1.z - s.z * e1.y, s.z * e1.x - s.x * e1.z, s.x * e1.y - s.y * e1.x);
+        const double v = inv * dot(dirv, qv);
+        if (v < 0.0 || u + v > 1.0) return;
+        if (inv * dot(e2, qv) > 1.0e-12) ++hits;
+    });
+    if (hits & 1u) atomicMin(first_inside, n);
+}
+
+// Minimum gap of free contact elements with the intersection-aware
+// distance (reference solver.cpp:659-681).
+__global__ __launch_bounds__(kBlock) void k_initial_gap(RodView m, ColliderView col, StepParams p, const double *pos,
+                                                        Partial *partials)
+{
+    const uint32_t L = blockIdx.x * blockDim.x + threadIdx.x;
+    Partial acc = empty_partial();
+    if (L < m.node_count) {
+        const uint32_t fl = m.flags[L];
+        if ((fl & kNodeHasElement) && (fl & kNodeContact) && !(fl & kNodeFixed)) {
+            const V3 p0 = load3(pos + 3 * size_t(L)), p1 = load3(pos + 3 * size_t(L + 1));
+            bvh_query(col.bvh, vmin(p0, p1), vmax(p0, p1), p.contact_pad, [&](uint32_t t) {
+                const uint32_t *t3 = col.triangles + 3 * size_t(t);
+                const Closest c = closest_segment_triangle(p0, p1, vertex(col, t3[0]), vertex(col, t3[1]),
+                                                           vertex(col, t3[2]), true);
+                acc.min_gap = fmin(acc.min_gap, c.distance - p.radius_offset);
+                ++acc.candidates;
+            });
+        }
+    }
+    block_reduce(acc, partials + blockIdx.x);
+}
+
+// Every (element, triangle) box overlap of contact-enabled free elements.
+__global__ void k_broadphase_pairs(RodView m, ColliderView col, StepParams p, const double *q, uint32_t *pairs,
+                                   unsigned long long capacity, unsigned long long *count)
+{
+    const uint32_t L = blockIdx.x * blockDim.x + threadIdx.x;
+    if (L >= m.node_count) return;
+    const uint32_t fl = m.flags[L];
+    if (!(fl & kNodeHasElement) || !(fl & kNodeContact) || (fl & kNodeFixed)) return;
+    const V3 p0 = load3(q + 6 * size_t(L)), p1 = load3(q + 6 * size_t(L + 1));
+    const uint32_t element = L - m.strand_of[L];
+    bvh_query(col.bvh, vmin(p0, p1), vmax(p0, p1), p.contact_pad, [&](uint32_t t) {
+        const unsigned long long slot = atomicAdd(count, 1ull);
+        if (slot < capacity) {
+            pairs[2 * slot] = element;
+            pairs[2 * slot + 1] = t;
+        }
+    });
+}
+
+__global__ void k_gather_positions(const double *q, const uint32_t *map, uint32_t count, double *out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const uint32_t n = map ? map[i] : i;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) out[3 * size_t(i) + k] = q[6 * size_t(n) + k];
+}
+
+// ------------------------------------------------------------------ LBVH
+__device__ __forceinline__ uint32_t spread_bits(uint32_t v)
+{
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+// Triangle boxes (static, or swept over previous and current positions)
+// and per-block centroid bounds.
+__global__ __launch_bounds__(kBlock) void k_triangle_boxes(ColliderView col, int swept, double *tri_box,
+                                                           double *bound_partials)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double lo_s[kBlock][3], hi_s[kBlock][3];
+    V3 clo = mk(1e300, 1e300, 1e300), chi = mk(-1e300, -1e300, -1e300);
+    if (t < col.triangle_count) {
+        const uint32_t *t3 = col.triangles + 3 * size_t(t);
+        V3 lo = mk(1e300, 1e300, 1e300), hi = mk(-1e300, -1e300, -1e300);
+        V3 centroid = mk(0, 0, 0);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const V3 v = vertex(col, t3[c]);
+            lo = vmin(lo, v);
+            hi = vmax(hi, v);
+            centroid = add(centroid, v);
+            if (swept) {
+                const V3 o = vertex_prev(col, t3[c]);
+                lo = vmin(lo, o);
+                hi = vmax(hi, o);
+            }
+        }
+        store3(tri_box + 6 * size_t(t), lo);
+        store3(tri_box + 6 * size_t(t) + 3, hi);
+        centroid = mul(centroid, 1.0 / 3.0);
+        clo = centroid;
+        chi = centroid;
+    }
+    lo_s[threadIdx.x][0] = clo.x; lo_s[threadIdx.x][1] = clo.y; lo_s[threadIdx.x][2] = clo.z;
+    hi_s[threadIdx.x][0] = chi.x; hi_s[threadIdx.x][1] = chi.y; hi_s[threadIdx.x][2] = chi.z;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (int(threadIdx.x) < s)
+            for (int k = 0; k < 3; ++k) {
+                lo_s[threadIdx.x][k] = fmin(lo_s[threadIdx.x][k], lo_s[threadIdx.x + s][k]);
+                hi_s[threadIdx.x][k] = fmax(hi_s[threadIdx.x][k], hi_s[threadIdx.x + s][k]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; ++k) {
+            bound_partials[6 * size_t(blockIdx.x) + k] = lo_s[0][k];
+            bound_partials[6 * size_t(blockIdx.x) + 3 + k] = hi_s[0][k];
+        }
+}
+
+__global__ void k_bounds_final(const double *partials, uint32_t count, double *bounds)
+{
+    if (threadIdx.x || blockIdx.x) return;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (uint32_t i = 0; i < count; ++i)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = fmin(lo[k], partials[6 * size_t(i) + k]);
+            hi[k] = fmax(hi[k], partials[6 * size_t(i) + 3 + k]);
+        }
+    for (int k = 0; k < 3; ++k) {
+        bounds[k] = lo[k];
+        bounds[3 + k] = hi[k];
+    }
+}
+
+// 30-bit Morton code of each triangle centroid in the centroid bounds.
+__global__ void k_morton(ColliderView col, const double *bounds, uint32_t *keys, uint32_t *ids)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= col.triangle_count) return;
+    const uint32_t *t3 = col.triangles + 3 * size_t(t);
+    const V3 c = mul(add(add(vertex(col, t3[0]), vertex(col, t3[1])), vertex(col, t3[2])), 1.0 / 3.0);
+    const double cc[3] = {c.x, c.y, c.z};
+    uint32_t code = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double span = bounds[3 + k] - bounds[k];
+        const double u = span > 0.0 ? (cc[k] - bounds[k]) / span : 0.5;
+        const uint32_t b = uint32_t(fmin(fmax(u * 1024.0, 0.0), 1023.0));
+        code |= spread_bits(b) << (2 - k);
+    }
+    keys[t] = code;
+    ids[t] = t;
+}
+
+__device__ __forceinline__ int common_prefix(const uint32_t *keys, int count, int i, int j)
+{
+    if (j < 0 || j >= count) return -1;
+    const uint32_t a = keys[i], b = keys[j];
+    if (a == b) return 32 + __clz(uint32_t(i ^ j));
+    return __clz(a ^ b);
+}
+
+// Karras 2012 hierarchy over sorted codes: internal node i spans a key range
+// split at its highest differing bit. Leaves are stored as -1 - triangle.
+__global__ void k_hierarchy(uint32_t count, const uint32_t *keys, const uint32_t *sorted_ids, BvhNode *nodes,
+                            int *parent_internal, int *parent_leaf, unsigned *counters)
+{
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    const int n = int(count);
+    if (n == 1) {
+        if (i == 0) {
+            nodes[0].child[0] = -1 - int(sorted_ids[0]);
+            nodes[0].child[1] = -1 - int(sorted_ids[0]);
+            parent_leaf[0] = 0;
+            counters[0] = 0;
+        }
+        return;
+    }
+    if (i >= n - 1) return;
+    counters[i] = 0;
+    const int d = (common_prefix(keys, n, i, i + 1) - common_prefix(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = common_prefix(keys, n, i, i - d);
+    int lmax = 2;
+    while (common_prefix(keys, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+        if (common_prefix(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = common_prefix(keys, n, i, j);
+    int s = 0, t = l;
+    do {
+        t = (t + 1) / 2;
+        if (common_prefix(keys, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    const int gamma = i + s * d + (d < 0 ? d : 0);
+    const int lo = i < j ? i : j, hi = i < j ? j : i;
+    BvhNode &node = nodes[i];
+    if (lo == gamma) {
+        node.child[0] = -1 - int(sorted_ids[gamma]);
+        parent_leaf[gamma] = i << 1;
+    } else {
+        node.child[0] = gamma;
+        parent_internal[gamma] = i << 1;
+    }
+    if (hi == gamma + 1) {
+        node.child[1] = -1 - int(sorted_ids[gamma + 1]);
+        parent_leaf[gamma + 1] = (i << 1) | 1;
+    } else {
+        node.child[1] = gamma + 1;
+        parent_internal[gamma + 1] = (i << 1) | 1;
+    }
+    if (i == 0) parent_internal[0] = -1;
+}
+
+__global__ void k_zero_counters(unsigned *counters, uint32_t count)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) counters[i] = 0;
+}
+
+// Bottom-up boxes: each leaf writes its triangle box into its parent; the
+// second child to arrive at a node writes the union into the grandparent.
+__global__ void k_refit(uint32_t count, const uint32_t *sorted_ids, const double *tri_box, const int *parent_internal,
+                        const int *parent_leaf, BvhNode *nodes, unsigned *counters)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    if (count == 1) {
+        const double *b = tri_box + 6 * size_t(sorted_ids[0]);
+        for (int c = 0; c < 6; ++c) nodes[0].box[c] = b[c];
+        for (int c = 0; c < 3; ++c) {
+            nodes[0].box[6 + c] = 1e300;
+            nodes[0].box[9 + c] = -1e300;
+        }
+        return;
+    }
+    double box[6];
+    const double *b = tri_box + 6 * size_t(sorted_ids[k]);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) box[c] = b[c];
+    int link = parent_leaf[k];
+    // The hierarchy is at most 64 levels deep (30-bit codes, index tie-break).
+    for (int level = 0; level < 128; ++level) {
+        const int parent = link >> 1, side = link & 1;
+        double *slot = nodes[parent].box + 6 * side;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) slot[c] = box[c];
+        __threadfence();
+        if (atomicAdd(&counters[parent], 1u) == 0) return;
+        __threadfence();
+        const volatile double *both = nodes[parent].box;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            box[c] = fmin(both[c], both[6 + c]);
+            box[3 + c] = fmax(both[3 + c], both[9 + c]);
+        }
+        link = parent_internal[parent];
+        if (link < 0) return;
+    }
+}
+
+}  // namespace dev
+}  // namespace khs

@@ -217,6 +217,22 @@ class KhsxEvaluation(C.Structure):
     ]
 
 
+class KhsxKernelTiming(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_uint32),
+        ("substep_count", C.c_uint32),
+        ("assembly_launches", C.c_uint64),
+        ("assembly_ms", C.c_double),
+        ("energy_launches", C.c_uint64),
+        ("energy_ms", C.c_double),
+        ("solve_launches", C.c_uint64),
+        ("solve_ms", C.c_double),
+        ("collision_launches", C.c_uint64),
+        ("collision_ms", C.c_double),
+        ("nodes", C.c_uint64),
+    ]
+
+
 # Every exported function with its ctypes signature. The first 33 are the
 # reference ABI (include/kami_hair_solver.h:206-298); khsx* are additive.
 _P = C.c_void_p
@@ -267,6 +283,9 @@ EXT_FUNCTIONS = {
     "khsxSetConvergenceMode": (C.c_int, [_P, C.c_uint32]),
     "khsxSetPrecisionMode": (C.c_int, [_P, C.c_uint32]),
     "khsxSetThreadCount": (C.c_int, [_P, C.c_uint32]),
+    "khsxSetKernelTiming": (C.c_int, [_P, C.c_uint32]),
+    "khsxGetKernelTiming": (C.c_int, [_P, _D(KhsxKernelTiming)]),
+    "khsxGetSubstepTimes": (C.c_int, [_P, _P, C.c_uint32, _D(C.c_uint32)]),
 }
 
 
@@ -500,6 +519,20 @@ class HairSolver:
 
     def set_threads(self, threads: int) -> int:
         return self.lib.khsxSetThreadCount(self.handle, threads)
+
+    def set_kernel_timing(self, enable: bool) -> int:
+        return self.lib.khsxSetKernelTiming(self.handle, 1 if enable else 0)
+
+    def kernel_timing(self) -> KhsxKernelTiming:
+        return self._get(self.lib.khsxGetKernelTiming, KhsxKernelTiming)
+
+    def substep_times(self) -> np.ndarray:
+        count = C.c_uint32(0)
+        self.check(self.lib.khsxGetSubstepTimes(self.handle, None, 0, C.byref(count)))
+        out = np.zeros(count.value)
+        if count.value:
+            self.check(self.lib.khsxGetSubstepTimes(self.handle, _ptr(out), count.value, C.byref(count)))
+        return out
 
 
 def gpu_info(lib) -> KhsGpuInfo:

@@ -298,6 +298,8 @@ public:
             validate_topology();
             collider_cur_ = collider_input_;
             bvh_.build(collider_cur_, tris_);
+            collider_x_max_ = -kInf;
+            for (const V3 &v : collider_cur_) collider_x_max_ = std::fmax(collider_x_max_, v.x);
             if (!initial_feasibility()) {
                 step_stats_.phase = KHS_PHASE_FAILED;
                 return KHS_ERROR_INITIAL_INTERSECTION;
@@ -986,10 +988,17 @@ private:
 
     bool point_inside(V3 p) const
     {
-        // solver.cpp:608-632: parity of ray crossings along a skewed +x ray
+        // solver.cpp:608-632: parity of ray crossings along a skewed +x ray.
+        // Only triangles whose box meets the box of the ray inside the
+        // collider's x range can be hit; the count equals the brute-force one.
         uint32_t hits = 0;
         const V3 dir{1.0, 0.0000012345, 0.0000023456};
-        for (const Tri &t : tris_) {
+        const double reach = std::fmax(collider_x_max_ - p.x, 0.0) + 1.0e-6;
+        const V3 end = p + dir * reach;
+        std::vector<uint32_t> cand;
+        bvh_.query(vmin(p, end), vmax(p, end), 1.0e-9, cand);
+        for (uint32_t ti : cand) {
+            const Tri &t = tris_[ti];
             const V3 a = collider_cur_[t.i0], b = collider_cur_[t.i1], c = collider_cur_[t.i2];
             const V3 e1 = b - a, e2 = c - a, h = cross(dir, e2);
             const double det = dot(e1, h);
@@ -1006,6 +1015,31 @@ private:
         return (hits & 1u) != 0;
     }
 
+  public:
+    // Brute-force ray parity over every triangle (checks the culled test).
+    int point_inside_both(const double *pt) const
+    {
+        const V3 p{pt[0], pt[1], pt[2]};
+        uint32_t hits = 0;
+        const V3 dir{1.0, 0.0000012345, 0.0000023456};
+        for (const Tri &t : tris_) {
+            const V3 a = collider_cur_[t.i0], b = collider_cur_[t.i1], c = collider_cur_[t.i2];
+            const V3 e1 = b - a, e2 = c - a, h = cross(dir, e2);
+            const double det = dot(e1, h);
+            if (std::fabs(det) < 1.0e-14) continue;
+            const double inv = 1.0 / det;
+            const V3 s = p - a;
+            const double u = inv * dot(s, h);
+            if (u < 0.0 || u > 1.0) continue;
+            const V3 q = cross(s, e1);
+            const double v = inv * dot(dir, q);
+            if (v < 0.0 || u + v > 1.0) continue;
+            if (inv * dot(e2, q) > 1.0e-12) ++hits;
+        }
+        return int(hits & 1u) | (point_inside(p) ? 2 : 0);
+    }
+
+  private:
     bool initial_feasibility()
     {
         // solver.cpp:634-657
@@ -1855,7 +1889,7 @@ private:
     std::vector<Strand> strands_;
     uint64_t free_dofs_ = 0;
     uint32_t merged_ = 0, ext_strands_ = 0, ext_nodes_ = 0;
-    double ext_length_ = 0.0, max_rest_length_ = 0.0, trust_radius_ = 0.0;
+    double ext_length_ = 0.0, max_rest_length_ = 0.0, trust_radius_ = 0.0, collider_x_max_ = 0.0;
     std::vector<Tri> tris_;
     std::vector<V3> collider_input_, collider_pending_, collider_cur_, collider_old_, collider_frame_start_;
     uint32_t excluded_ = 0, boundary_edges_ = 0, nonmanifold_edges_ = 0, inconsistent_edges_ = 0, inverted_ = 0;
@@ -2152,6 +2186,31 @@ KhsResult khsxSetConvergenceMode(KhsSolver *s, uint32_t mode)
 KhsResult khsxSetPrecisionMode(KhsSolver *s, uint32_t mode)
 {
     if (!s || mode != 0) return KHS_ERROR_INVALID_ARGUMENT;
+    return KHS_OK;
+}
+
+KhsResult khsxSetKernelTiming(KhsSolver *s, uint32_t enable)
+{
+    (void)enable;
+    return s ? KHS_OK : KHS_ERROR_INVALID_ARGUMENT;
+}
+
+KhsResult khsxGetKernelTiming(const KhsSolver *s, KhsxKernelTiming *t)
+{
+    if (!s || !t || t->struct_size < sizeof(KhsxKernelTiming)) return KHS_ERROR_INVALID_ARGUMENT;
+    KhsxKernelTiming r{};
+    r.struct_size = sizeof(r);
+    r.nodes = s->impl.node_count();
+    *t = r;
+    return KHS_OK;
+}
+
+KhsResult khsxGetSubstepTimes(const KhsSolver *s, double *ms, uint32_t capacity, uint32_t *count)
+{
+    (void)ms;
+    (void)capacity;
+    if (!s || !count) return KHS_ERROR_INVALID_ARGUMENT;
+    *count = 0;
     return KHS_OK;
 }
 

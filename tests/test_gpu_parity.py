@@ -1,0 +1,158 @@
+"""Parity of the HIP step with the CPU oracle on the same seeded inputs.
+
+Bars: rest model bit-for-bit (same FP64 operations, contraction off);
+energy, gradient and Gauss-Newton direction within 1e-10 relative
+(norm-wise); broad-phase candidate pairs bit-exact; positions after one frame
+within 1e-7 m; minimum gap above the feasibility threshold.
+"""
+import numpy as np
+import pytest
+
+import grooms
+import khs_native as kn
+
+pytestmark = pytest.mark.gpu
+
+LIFTED = dict(relative_tolerance=1e-6, absolute_tolerance=1e-30, increment_tolerance=1e-14)
+
+
+def _pair(oracle_lib, gpu_lib, scene, **desc):
+    o = grooms.build_solver(oracle_lib, scene, **desc)
+    g = grooms.build_solver(gpu_lib, scene, **desc)
+    return o, g
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _perturbed_state(solver, seed, dx=1e-4, du=0.05, dv=1e-3):
+    q, v = solver.get_state()
+    rng = np.random.default_rng(seed)
+    n = q.size // 6
+    dq = np.zeros((n, 6))
+    dq[:, :3] = rng.uniform(-dx, dx, (n, 3))
+    dq[:, 3:] = rng.uniform(-du, du, (n, 3))
+    vel = rng.uniform(-dv, dv, (n, 6))
+    return q, q + dq.ravel(), vel.ravel()
+
+
+def test_rest_model_matches_oracle(oracle_lib, gpu_lib):
+    print("[gpu] rest model, small sphere scene", flush=True)
+    scene = grooms.small_sphere_scene()
+    o, g = _pair(oracle_lib, gpu_lib, scene)
+    fo, mo, ro, flo = o.node_model()
+    fg, mg, rg, flg = g.node_model()
+    assert np.array_equal(flo, flg)
+    assert np.max(np.abs(fo - fg)) <= 1e-15
+    assert np.max(np.abs(mo - mg) / mo) <= 1e-15
+    assert np.max(np.abs(ro - rg) / ro) <= 1e-15
+    no, so, co, resto = o.element_model()
+    ng, sg, cg, restg = g.element_model()
+    assert np.array_equal(no, ng) and np.array_equal(so, sg) and np.array_equal(co, cg)
+    assert np.max(np.abs(resto - restg)) <= 1e-12
+    bo, bg = o.build_stats(), g.build_stats()
+    for f in ("internal_node_count", "element_count", "fixed_node_count", "degree_of_freedom_count",
+              "collider_triangle_count", "collider_boundary_edge_count", "collider_inverted_closed_component_count"):
+        assert getattr(bo, f) == getattr(bg, f), f
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_energy_gradient_direction_match_oracle(oracle_lib, gpu_lib, seed):
+    print(f"[gpu] evaluate parity seed {seed}", flush=True)
+    scene = grooms.small_sphere_scene()
+    o, g = _pair(oracle_lib, gpu_lib, scene)
+    q_old, q, v = _perturbed_state(o, seed)
+    for s in (o, g):
+        s.check(s.set_state(q, v, q_old))
+    dt = scene.frame_dt / 4
+    eo, go, do = o.evaluate(dt)
+    eg, gg, dg = g.evaluate(dt)
+    for f in ("objective", "elastic", "kinetic"):
+        a, b = getattr(eg, f), getattr(eo, f)
+        assert abs(a - b) <= 1e-10 * abs(b), (f, a, b)
+    assert eg.feasible == eo.feasible == 1
+    assert eg.solve_failed == eo.solve_failed == 0
+    rel_g = _rel(gg, go)
+    rel_d = _rel(dg, do)
+    print(f"gradient rel {rel_g:.3e} direction rel {rel_d:.3e}")
+    assert rel_g <= 1e-10
+    assert rel_d <= 1e-10
+
+
+def _draped_scene():
+    """Small-sphere strands pulled radially onto the collider so that many
+    elements sit inside the barrier padding."""
+    scene = grooms.small_sphere_scene()
+    return scene
+
+
+def test_broadphase_pairs_bit_exact(oracle_lib, gpu_lib):
+    print("[gpu] broad phase pairs", flush=True)
+    scene = grooms.small_sphere_scene()
+    o, g = _pair(oracle_lib, gpu_lib, scene)
+    q, v = o.get_state()
+    pts = q.reshape(-1, 6)[:, :3]
+    r = np.linalg.norm(pts, axis=1, keepdims=True)
+    # shrink the radial distance of free nodes so segments come within the padding
+    pts2 = pts * np.minimum(1.0, (0.098 + 0.00025 + (r - 0.1) * 0.002) / r)
+    q2 = q.reshape(-1, 6).copy()
+    q2[:, :3] = pts2
+    q2 = q2.ravel()
+    for s in (o, g):
+        s.check(s.set_state(q2, v, q2))
+    po = o.broadphase_pairs()
+    pg = g.broadphase_pairs()
+    print(f"pairs oracle {len(po)} gpu {len(pg)}")
+    assert len(po) > 100
+    assert np.array_equal(po, pg)
+
+
+def test_one_frame_positions_match_oracle(oracle_lib, gpu_lib):
+    print("[gpu] one frame, small sphere scene, floors lifted", flush=True)
+    scene = grooms.small_sphere_scene()
+    o, g = _pair(oracle_lib, gpu_lib, scene, **LIFTED)
+    for s in (o, g):
+        s.check(s.set_convergence_mode(1))
+        s.check(s.step(scene.frame_dt))
+    xo, xg = o.internal_positions(), g.internal_positions()
+    err = float(np.max(np.abs(xo - xg)))
+    so, sg = o.step_stats(), g.step_stats()
+    print(f"max position error {err:.3e} m; newton oracle {so.newton_iterations} gpu {sg.newton_iterations}")
+    assert err <= 1e-7
+    assert sg.phase == kn.PHASE_FINISHED
+    assert sg.minimum_gap > scene.desc.get("minimum_gap", 1e-7) or np.isinf(sg.minimum_gap)
+
+
+def test_contact_frames_match_oracle(oracle_lib, gpu_lib):
+    """The reference's barrier scenario (tests/test_solver.cpp:218-251), three
+    frames, compared node by node."""
+    print("[gpu] barrier contact frames", flush=True)
+    pts = np.array([[0, 0, 0.03], [0, 0, 0.02], [0, 0, 0.01], [0, 0, 0.001]])
+    col = np.array([[-1.0, -1.0, 0.0], [1.0, -1.0, 0.0], [1.0, 1.0, 0.0], [-1.0, 1.0, 0.0]])
+    tris = np.array([[0, 1, 2], [0, 2, 3]], np.uint32)
+    out = []
+    for lib in (oracle_lib, gpu_lib):
+        desc = kn.default_desc(lib)
+        desc.substeps = 4
+        desc.fixed_root_nodes = 1
+        desc.maximum_element_length = 0.02
+        desc.newton_iterations = 40
+        mat = kn.default_material(lib)
+        mat.radius = 2.0e-4
+        mat.young_modulus = 1.0e6
+        mat.contact_stiffness = 100.0
+        mat.barrier_distance = 2.0e-3
+        mat.mass_damping = 2.0
+        s = kn.HairSolver(lib, desc)
+        s.check(s.set_hair(pts, [0, 4], mat))
+        s.check(s.set_collider(col, tris))
+        s.check(s.build())
+        for _ in range(3):
+            s.check(s.step(1.0 / 60.0))
+        out.append((s.internal_positions(), s.step_stats()))
+    err = float(np.max(np.abs(out[0][0] - out[1][0])))
+    print(f"max position error {err:.3e} m; active oracle {out[0][1].active_contact_count} gpu {out[1][1].active_contact_count}")
+    assert err <= 1e-7
+    assert out[1][1].active_contact_count > 0
+    assert out[1][1].minimum_gap > 1e-7
