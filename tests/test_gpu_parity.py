@@ -68,16 +68,42 @@ def test_energy_gradient_direction_match_oracle(oracle_lib, gpu_lib, seed):
     dt = scene.frame_dt / 4
     eo, go, do = o.evaluate(dt)
     eg, gg, dg = g.evaluate(dt)
-    for f in ("objective", "elastic", "kinetic"):
-        a, b = getattr(eg, f), getattr(eo, f)
-        assert abs(a - b) <= 1e-10 * abs(b), (f, a, b)
-    assert eg.feasible == eo.feasible == 1
-    assert eg.solve_failed == eo.solve_failed == 0
+    energy_rel = {f: abs(getattr(eg, f) - getattr(eo, f)) / abs(getattr(eo, f)) for f in ("objective", "elastic", "kinetic")}
     rel_g = _rel(gg, go)
     rel_d = _rel(dg, do)
-    print(f"gradient rel {rel_g:.3e} direction rel {rel_d:.3e}")
+    print(f"energy rel {energy_rel} gradient rel {rel_g:.3e} direction rel {rel_d:.3e}")
+    for f, e in energy_rel.items():
+        assert e <= 1e-10, (f, e)
+    assert eg.feasible == eo.feasible == 1
+    assert eg.solve_failed == eo.solve_failed == 0
     assert rel_g <= 1e-10
-    assert rel_d <= 1e-10
+    # The per-strand Newton systems are ill-conditioned (stretch stiffness
+    # EA/L ~ 1e3 against twist GJ/L ~ 1e-7), so the direction is compared by
+    # its backward error in the oracle's own system: the GPU direction must
+    # solve H_oracle d = -g as well as the oracle's direction does.
+    Do, Uo = o.blocks()
+    Dg, Ug = g.blocks()
+    rel_h = max(_rel(Dg, Do), _rel(Ug, Uo))
+    res_o = _newton_residual(Do, Uo, go, do, o)
+    res_g = _newton_residual(Do, Uo, go, dg, o)
+    print(f"blocks rel {rel_h:.3e} backward error oracle {res_o:.3e} gpu {res_g:.3e}")
+    assert rel_h <= 1e-10
+    assert res_g <= max(10 * res_o, 1e-10)
+
+
+def _newton_residual(D, U, g, d, solver):
+    """||H d + g|| / ||g|| over free nodes, H block tridiagonal per strand."""
+    _, _, _, flags = solver.node_model()
+    n = D.shape[0]
+    dd = d.reshape(n, 6)
+    r = np.einsum("nab,nb->na", D, dd)
+    nodes2, _, _, _ = solver.element_model()
+    i, j = nodes2[:, 0], nodes2[:, 1]
+    r[i] += np.einsum("nab,nb->na", U[i], dd[j])
+    r[j] += np.einsum("nba,nb->na", U[i], dd[i])
+    r += g.reshape(n, 6)
+    free = (flags & 1) == 0
+    return float(np.linalg.norm(r[free]) / np.linalg.norm(g.reshape(n, 6)[free]))
 
 
 def _draped_scene():
@@ -101,6 +127,9 @@ def test_broadphase_pairs_bit_exact(oracle_lib, gpu_lib):
     q2 = q2.ravel()
     for s in (o, g):
         s.check(s.set_state(q2, v, q2))
+    check = g.check_broadphase()
+    print(f"broad-phase self-check {check}")
+    assert check[:5] == [0, 0, 0, 0, 0]
     po = o.broadphase_pairs()
     pg = g.broadphase_pairs()
     print(f"pairs oracle {len(po)} gpu {len(pg)}")
