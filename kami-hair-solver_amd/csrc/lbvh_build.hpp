@@ -78,19 +78,34 @@ __global__ __launch_bounds__(256) void k_triangle_boxes(ColliderView col, int sw
         }
 }
 
-__global__ void k_bounds_final(const double *partials, uint32_t count, double *bounds)
+// Folds the per-block centroid bounds (one block of 256 threads).
+__global__ __launch_bounds__(256) void k_bounds_final(const double *partials, uint32_t count, double *bounds)
 {
-    if (threadIdx.x || blockIdx.x) return;
+    __shared__ double lo_s[256][3], hi_s[256][3];
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    for (uint32_t i = 0; i < count; ++i)
+    for (uint32_t i = threadIdx.x; i < count; i += 256)
         for (int k = 0; k < 3; ++k) {
             lo[k] = fmin(lo[k], partials[6 * size_t(i) + k]);
             hi[k] = fmax(hi[k], partials[6 * size_t(i) + 3 + k]);
         }
     for (int k = 0; k < 3; ++k) {
-        bounds[k] = lo[k];
-        bounds[3 + k] = hi[k];
+        lo_s[threadIdx.x][k] = lo[k];
+        hi_s[threadIdx.x][k] = hi[k];
     }
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (int(threadIdx.x) < s)
+            for (int k = 0; k < 3; ++k) {
+                lo_s[threadIdx.x][k] = fmin(lo_s[threadIdx.x][k], lo_s[threadIdx.x + s][k]);
+                hi_s[threadIdx.x][k] = fmax(hi_s[threadIdx.x][k], hi_s[threadIdx.x + s][k]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; ++k) {
+            bounds[k] = lo_s[0][k];
+            bounds[3 + k] = hi_s[0][k];
+        }
 }
 
 // 30-bit Morton code of each triangle centroid in the centroid bounds.

@@ -533,7 +533,7 @@ void Solver::rebuild_bvh(bool swept)
     const unsigned tb = grid_for(d.T, kBlock);
     hipLaunchKernelGGL(dev::k_triangle_boxes, dim3(tb), dim3(kBlock), 0, d.stream, cv, swept ? 1 : 0, d.tri_box.get(),
                        d.bound_partials.get());
-    hipLaunchKernelGGL(dev::k_bounds_final, dim3(1), dim3(1), 0, d.stream, d.bound_partials.get(), tb, d.bounds.get());
+    hipLaunchKernelGGL(dev::k_bounds_final, dim3(1), dim3(256), 0, d.stream, d.bound_partials.get(), tb, d.bounds.get());
     hipLaunchKernelGGL(dev::k_morton, dim3(tb), dim3(kBlock), 0, d.stream, cv, d.bounds.get(), d.keys.get(),
                        d.ids.get());
     size_t temp = d.sort_temp_bytes;
@@ -541,6 +541,21 @@ void Solver::rebuild_bvh(bool swept)
                                       d.ids_sorted.get(), d.T, 0, 30, d.stream));
     hipLaunchKernelGGL(dev::k_hierarchy, dim3(tb), dim3(kBlock), 0, d.stream, d.T, d.keys_sorted.get(),
                        d.ids_sorted.get(), d.bvh.get(), d.parent_internal.get(), d.parent_leaf.get(), d.counters.get());
+    hipLaunchKernelGGL(dev::k_refit, dim3(tb), dim3(kBlock), 0, d.stream, d.T, d.ids_sorted.get(), d.tri_box.get(),
+                       d.parent_internal.get(), d.parent_leaf.get(), d.bvh.get(), d.counters.get());
+    HIP_TRY(hipGetLastError());
+}
+
+// New boxes (static or swept) on the current hierarchy.
+void Solver::refit_bvh(bool swept)
+{
+    DeviceState &d = *dev_;
+    if (!d.T) return;
+    const unsigned tb = grid_for(d.T, kBlock);
+    hipLaunchKernelGGL(dev::k_triangle_boxes, dim3(tb), dim3(kBlock), 0, d.stream, d.collider(), swept ? 1 : 0,
+                       d.tri_box.get(), d.bound_partials.get());
+    hipLaunchKernelGGL(dev::k_zero_counters, dim3(tb), dim3(kBlock), 0, d.stream, d.counters.get(),
+                       d.T > 1 ? d.T - 1 : 1u);
     hipLaunchKernelGGL(dev::k_refit, dim3(tb), dim3(kBlock), 0, d.stream, d.T, d.ids_sorted.get(), d.tri_box.get(),
                        d.parent_internal.get(), d.parent_leaf.get(), d.bvh.get(), d.counters.get());
     HIP_TRY(hipGetLastError());
@@ -1054,7 +1069,7 @@ KhsResult Solver::run_frame(double frame_dt, const double *root_a, const double 
                     break;
                 }
                 hipEvent_t t1 = d.tic();
-                rebuild_bvh(false);
+                refit_bvh(false);
                 d.toc(t1, d.totals.collision_ms, d.totals.collision_launches);
             }
             phase_.store(KHS_PHASE_ASSEMBLING);

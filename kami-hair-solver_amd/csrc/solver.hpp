@@ -82,6 +82,7 @@ private:
     KhsResult initial_feasibility();
     void release_device();
     void rebuild_bvh(bool swept);
+    void refit_bvh(bool swept);
     Reduced evaluate(bool gradient, double dt, bool line_search, double alpha);
     bool make_direction();
     void steepest_direction();

@@ -426,7 +426,9 @@ __device__ inline bool inverse6(double *a, double *inv)
 }
 
 // Block Thomas per strand on the free nodes; fixed root nodes are
-// eliminated and get a zero direction (cuda_backend.cu:866-881).
+// eliminated and get a zero direction (cuda_backend.cu:866-881). The
+// previous node's C', d' and coupling block stay in registers between
+// iterations; C' and d' are written once for the back substitution.
 __global__ __launch_bounds__(64) void k_block_solve(RodView m, const double *Dblk, const double *Ublk,
                                                     const double *grad, double *Cp, double *dp, double *dir,
                                                     unsigned *failed)
@@ -437,6 +439,7 @@ __global__ __launch_bounds__(64) void k_block_solve(RodView m, const double *Dbl
     for (uint32_t n = first; n < begin; ++n)
         for (int k = 0; k < 6; ++k) dir[6 * size_t(n) + k] = 0.0;
     if (begin >= end) return;
+    double low[36], prevc[36], prevd[6];
     for (uint32_t n = begin; n < end; ++n) {
         double b[36], inv[36], rhs[6];
 #pragma unroll
@@ -444,14 +447,6 @@ __global__ __launch_bounds__(64) void k_block_solve(RodView m, const double *Dbl
 #pragma unroll
         for (int i = 0; i < 6; ++i) rhs[i] = -grad[6 * size_t(n) + i];
         if (n > begin) {
-            const double *lower = Ublk + 36 * size_t(n - 1);
-            const double *pc = Cp + 36 * size_t(n - 1);
-            const double *pd = dp + 6 * size_t(n - 1);
-            double low[36], prevc[36], prevd[6];
-#pragma unroll
-            for (int i = 0; i < 36; ++i) { low[i] = lower[i]; prevc[i] = pc[i]; }
-#pragma unroll
-            for (int i = 0; i < 6; ++i) prevd[i] = pd[i];
 #pragma unroll
             for (int r = 0; r < 6; ++r) {
 #pragma unroll
@@ -473,41 +468,46 @@ __global__ __launch_bounds__(64) void k_block_solve(RodView m, const double *Dbl
         }
         double *cc = Cp + 36 * size_t(n);
         if (n + 1 < end) {
-            const double *up = Ublk + 36 * size_t(n);
-            double upper[36];
 #pragma unroll
-            for (int i = 0; i < 36; ++i) upper[i] = up[i];
+            for (int i = 0; i < 36; ++i) low[i] = Ublk[36 * size_t(n) + i];
 #pragma unroll
             for (int r = 0; r < 6; ++r)
 #pragma unroll
                 for (int c = 0; c < 6; ++c) {
                     double sum = 0.0;
 #pragma unroll
-                    for (int k = 0; k < 6; ++k) sum += inv[6 * r + k] * upper[6 * k + c];
-                    cc[6 * r + c] = sum;
+                    for (int k = 0; k < 6; ++k) sum += inv[6 * r + k] * low[6 * k + c];
+                    prevc[6 * r + c] = sum;
                 }
         } else {
 #pragma unroll
-            for (int i = 0; i < 36; ++i) cc[i] = 0.0;
+            for (int i = 0; i < 36; ++i) prevc[i] = 0.0;
         }
+#pragma unroll
+        for (int i = 0; i < 36; ++i) cc[i] = prevc[i];
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
             double sum = 0.0;
 #pragma unroll
             for (int k = 0; k < 6; ++k) sum += inv[6 * r + k] * rhs[k];
+            prevd[r] = sum;
             dp[6 * size_t(n) + r] = sum;
         }
     }
     double next[6] = {0, 0, 0, 0, 0, 0};
     for (uint32_t n = end; n-- > begin;) {
-        double cur[6];
+        double cur[6], cn[36], dn[6];
         const double *cc = Cp + 36 * size_t(n);
 #pragma unroll
+        for (int i = 0; i < 36; ++i) cn[i] = cc[i];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) dn[i] = dp[6 * size_t(n) + i];
+#pragma unroll
         for (int r = 0; r < 6; ++r) {
-            double value = dp[6 * size_t(n) + r];
+            double value = dn[r];
             if (n + 1 < end) {
 #pragma unroll
-                for (int k = 0; k < 6; ++k) value -= cc[6 * r + k] * next[k];
+                for (int k = 0; k < 6; ++k) value -= cn[6 * r + k] * next[k];
             }
             cur[r] = value;
         }
