@@ -78,7 +78,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=29, help="timed frames")
     ap.add_argument("--warmup", type=int, default=1, help="untimed frames before the timed ones")
-    ap.add_argument("--cpu-strands", type=int, default=384)
+    ap.add_argument("--cpu-strands", type=int, default=4096)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()

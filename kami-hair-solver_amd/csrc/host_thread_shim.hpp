@@ -50,3 +50,6 @@ template <class T> inline T __hip_atomic_fetch_add(T *p, T v, int, int)
 #ifndef __HIP_MEMORY_SCOPE_AGENT
 #define __HIP_MEMORY_SCOPE_AGENT 0
 #endif
+#ifndef __HIP_MEMORY_SCOPE_WORKGROUP
+#define __HIP_MEMORY_SCOPE_WORKGROUP 0
+#endif

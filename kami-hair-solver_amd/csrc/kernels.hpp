@@ -48,6 +48,28 @@ struct ColliderView {
     const BvhNode *bvh;
 };
 
+// Per-element lists of the collider triangles whose leaf box overlaps a
+// cached box around the element. Before each query pass the cached box is
+// checked to contain the pass's query box (grown by the pass's pad plus a
+// small rounding margin); only elements that left it walk the hierarchy
+// again, gathering for their query box grown by a skin. A covered query thus
+// finds in the list every triangle the hierarchy would report. Lists longer
+// than kCacheCapacity are marked overflowed and their queries walk the tree.
+constexpr uint32_t kCacheCapacity = 256;
+constexpr uint32_t kCacheOverflow = 0xffffffffu;
+constexpr uint32_t kNoTriangle = 0xffffffffu;
+constexpr uint32_t kPairBlocks = 1024;  // grid of the per-element narrow-phase passes
+struct ContactCache {
+    double skin;
+    uint32_t *count;        // per element: list length, or kCacheOverflow
+    uint32_t *tris;         // kCacheCapacity per element, contiguous
+    double *box;            // 6 per element: lo, hi of the covered box
+    uint32_t *listed;       // per element: 1 once appended to the list below
+    uint32_t *list;         // elements whose list became non-empty
+    uint32_t *list_count;
+    const double *tri_box;  // 6 per triangle, the leaf boxes of the hierarchy
+};
+
 // Scalars of one substep.
 struct StepParams {
     double dt, inv_dt, inv_dt2;

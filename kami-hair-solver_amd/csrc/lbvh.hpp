@@ -15,7 +15,7 @@ namespace khs {
 struct alignas(16) BvhNode {
     double box[12];  // child 0 lo, hi; child 1 lo, hi
     int child[2];    // >= 0 internal node; < 0 leaf, triangle = -1 - child
-    int pad[2];
+    int range[2];    // first and last sorted leaf below the node
 };
 
 namespace dev {

@@ -147,6 +147,7 @@ __global__ void k_hierarchy(uint32_t count, const uint32_t *keys, const uint32_t
         if (i == 0) {
             nodes[0].child[0] = -1 - int(sorted_ids[0]);
             nodes[0].child[1] = -1 - int(sorted_ids[0]);
+            nodes[0].range[0] = nodes[0].range[1] = 0;
             parent_leaf[0] = 0;
             counters[0] = 0;
         }
@@ -171,6 +172,8 @@ __global__ void k_hierarchy(uint32_t count, const uint32_t *keys, const uint32_t
     const int gamma = i + s * d + (d < 0 ? d : 0);
     const int lo = i < j ? i : j, hi = i < j ? j : i;
     BvhNode &node = nodes[i];
+    node.range[0] = lo;
+    node.range[1] = hi;
     if (lo == gamma) {
         node.child[0] = -1 - int(sorted_ids[gamma]);
         parent_leaf[gamma] = i << 1;
@@ -196,6 +199,11 @@ __global__ void k_zero_counters(unsigned *counters, uint32_t count)
 
 // Bottom-up boxes: each leaf writes its triangle box into its parent; the
 // second child to arrive at a node writes the union into the grandparent.
+// Workgroup w starts at sorted leaves [256 w, 256 w + 256). A node whose
+// leaf range lies inside that chunk is reached only by threads of w, so its
+// boxes and counter need workgroup-scope ordering only; agent scope, whose
+// release writes back the XCD's L2, is paid only on the few nodes whose
+// range crosses a chunk boundary.
 __global__ void k_refit(uint32_t count, const uint32_t *sorted_ids, const double *tri_box, const int *parent_internal,
                         const int *parent_leaf, BvhNode *nodes, unsigned *counters)
 {
@@ -220,19 +228,33 @@ __global__ void k_refit(uint32_t count, const uint32_t *sorted_ids, const double
     for (int c = 0; c < 6; ++c) box[c] = b[c];
     int link = parent_leaf[k];
     // The hierarchy is at most 64 levels deep (30-bit codes, index tie-break).
-    // Boxes pass between workgroups (possibly on different XCDs) through
-    // device-scope atomic stores and loads ordered by the counter's
-    // acquire-release, so no stale cache line is ever read.
+    // Box slots are written and read with atomic stores and loads ordered by
+    // the node counter's acquire-release, at the scope of the threads that
+    // can meet at the node, so no stale cache line is ever read.
+    const int chunk_lo = int(blockIdx.x * blockDim.x), chunk_hi = chunk_lo + int(blockDim.x);
     for (int level = 0; level < 128; ++level) {
         const int parent = link >> 1, side = link & 1;
         double *slot = nodes[parent].box + 6 * side;
-#pragma unroll
-        for (int c = 0; c < 6; ++c) __hip_atomic_store(slot + c, box[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (__hip_atomic_fetch_add(&counters[parent], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
         double both[12];
+        const bool local = nodes[parent].range[0] >= chunk_lo && nodes[parent].range[1] < chunk_hi;
+        if (local) {
 #pragma unroll
-        for (int c = 0; c < 12; ++c)
-            both[c] = __hip_atomic_load(nodes[parent].box + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (int c = 0; c < 6; ++c)
+                __hip_atomic_store(slot + c, box[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (__hip_atomic_fetch_add(&counters[parent], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
+                return;
+#pragma unroll
+            for (int c = 0; c < 12; ++c)
+                both[c] = __hip_atomic_load(nodes[parent].box + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else {
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+                __hip_atomic_store(slot + c, box[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (__hip_atomic_fetch_add(&counters[parent], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
+#pragma unroll
+            for (int c = 0; c < 12; ++c)
+                both[c] = __hip_atomic_load(nodes[parent].box + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             box[c] = fmin(both[c], both[6 + c]);

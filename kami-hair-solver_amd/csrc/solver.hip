@@ -111,6 +111,12 @@ struct DeviceState {
     Buffer<unsigned> counters;
     Buffer<unsigned char> sort_temp;
     size_t sort_temp_bytes = 0;
+    // per-element contact candidate lists (valid for the current substep)
+    Buffer<uint32_t> cache_count, cache_tris, cache_listed, cache_list, cache_list_count;
+    Buffer<double> cache_box;
+    double cache_skin = 0.0;
+    bool cache_valid = false;
+    Buffer<uint32_t> best_tri;  // closest triangle per element from the list pass
     // reductions and flags
     Buffer<Partial> partials, reduced;
     Buffer<unsigned> flag;
@@ -192,6 +198,11 @@ struct DeviceState {
                        strand_fixed.get()};
     }
     ColliderView collider() const { return ColliderView{V, T, col_cur.get(), col_old.get(), tris.get(), bvh.get()}; }
+    ContactCache cache() const
+    {
+        return ContactCache{cache_skin,        cache_count.get(), cache_tris.get(),       cache_box.get(),
+                            cache_listed.get(), cache_list.get(), cache_list_count.get(), tri_box.get()};
+    }
 
     size_t resident_bytes() const
     {
@@ -205,7 +216,8 @@ struct DeviceState {
                                           &tris, &keys, &keys_sorted, &ids, &ids_sorted})
             b += x->bytes();
         b += bvh.bytes() + parent_internal.bytes() + parent_leaf.bytes() + counters.bytes() + sort_temp.bytes() +
-             partials.bytes() + reduced.bytes();
+             partials.bytes() + reduced.bytes() + cache_count.bytes() + cache_tris.bytes() + cache_box.bytes() +
+             cache_listed.bytes() + cache_list.bytes() + cache_list_count.bytes() + best_tri.bytes();
         return b;
     }
 
@@ -482,7 +494,7 @@ KhsResult Solver::upload_model()
     d.rot_pend.upload(zero3.data(), zero3.size());
     d.out3.allocate(3 * std::max<size_t>(N, d.P));
     const uint32_t max_blocks = std::max(d.tiled_blocks(), d.node_blocks()) + 1;
-    d.partials.allocate(max_blocks + 16);
+    d.partials.allocate(max_blocks + kPairBlocks + 16);
     d.reduced.allocate(1);
     d.flag.allocate(1);
     d.key64.allocate(1);
@@ -516,6 +528,14 @@ KhsResult Solver::upload_model()
                                           d.ids_sorted.get(), T, 0, 30, d.stream));
         d.sort_temp.allocate(std::max<size_t>(temp, 1));
         d.sort_temp_bytes = temp;
+        d.cache_count.allocate(d.N);
+        HIP_TRY(hipMemset(d.cache_count.get(), 0, d.cache_count.bytes()));
+        d.cache_tris.allocate(size_t(kCacheCapacity) * d.N);
+        d.cache_box.allocate(6 * size_t(d.N));
+        d.cache_listed.allocate(d.N);
+        d.cache_list.allocate(d.N);
+        d.cache_list_count.allocate(1);
+        d.best_tri.allocate(d.N);
         rebuild_bvh(false);
     }
     trust_radius_ = std::max(0.25 * model_.maximum_rest_length,
@@ -529,6 +549,7 @@ void Solver::rebuild_bvh(bool swept)
 {
     DeviceState &d = *dev_;
     if (!d.T) return;
+    d.cache_valid = false;
     const ColliderView cv = d.collider();
     const unsigned tb = grid_for(d.T, kBlock);
     hipLaunchKernelGGL(dev::k_triangle_boxes, dim3(tb), dim3(kBlock), 0, d.stream, cv, swept ? 1 : 0, d.tri_box.get(),
@@ -551,6 +572,7 @@ void Solver::refit_bvh(bool swept)
 {
     DeviceState &d = *dev_;
     if (!d.T) return;
+    d.cache_valid = false;
     const unsigned tb = grid_for(d.T, kBlock);
     hipLaunchKernelGGL(dev::k_triangle_boxes, dim3(tb), dim3(kBlock), 0, d.stream, d.collider(), swept ? 1 : 0,
                        d.tri_box.get(), d.bound_partials.get());
@@ -558,6 +580,57 @@ void Solver::refit_bvh(bool swept)
                        d.T > 1 ? d.T - 1 : 1u);
     hipLaunchKernelGGL(dev::k_refit, dim3(tb), dim3(kBlock), 0, d.stream, d.T, d.ids_sorted.get(), d.tri_box.get(),
                        d.parent_internal.get(), d.parent_leaf.get(), d.bvh.get(), d.counters.get());
+    HIP_TRY(hipGetLastError());
+}
+
+// Brings the candidate lists up to date for a query pass (all of them
+// after the collider boxes changed): elements whose query box left their
+// cached box gather again.
+void Solver::refresh_contact_cache(int mode, bool full, double alpha)
+{
+    DeviceState &d = *dev_;
+    const double contact_pad = material_.radius + material_.collider_offset + material_.barrier_distance;
+    const double sweep_target = material_.radius + material_.collider_offset + desc_.minimum_gap;
+    const double pad = std::fmax(contact_pad, sweep_target);
+    d.cache_skin = pad;
+    const ContactCache cc = d.cache();
+    if (full) {
+        hipLaunchKernelGGL(dev::k_fill_u32, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.cache_listed.get(),
+                           size_t(d.N), 0u);
+        hipLaunchKernelGGL(dev::k_fill_u32, dim3(1), dim3(64), 0, d.stream, d.cache_list_count.get(), size_t(1), 0u);
+    }
+    const int f = full ? 1 : 0;
+    const dim3 grid(d.node_blocks()), block(kBlock);
+    if (mode == dev::kAtState)
+        hipLaunchKernelGGL(dev::k_contact_refresh<dev::kAtState>, grid, block, 0, d.stream, d.rods(), d.collider(), cc,
+                           d.q.get(), d.q_base.get(), d.dir.get(), alpha, pad, f);
+    else if (mode == dev::kAtTrial)
+        hipLaunchKernelGGL(dev::k_contact_refresh<dev::kAtTrial>, grid, block, 0, d.stream, d.rods(), d.collider(), cc,
+                           d.q.get(), d.q_base.get(), d.dir.get(), alpha, pad, f);
+    else
+        hipLaunchKernelGGL(dev::k_contact_refresh<dev::kSwept>, grid, block, 0, d.stream, d.rods(), d.collider(), cc,
+                           d.q.get(), d.q_base.get(), d.dir.get(), alpha, pad, f);
+    HIP_TRY(hipGetLastError());
+    d.cache_valid = true;
+}
+
+// Closest triangle of every listed element at q (or at alpha d + q_base),
+// one wavefront per element; partials (may be null) receive the candidate
+// count.
+void Solver::closest_pass(bool line_search, double alpha, Partial *partials)
+{
+    DeviceState &d = *dev_;
+    const double contact_pad = material_.radius + material_.collider_offset + material_.barrier_distance;
+    refresh_contact_cache(line_search ? dev::kAtTrial : dev::kAtState, false, alpha);
+    const ContactCache cc = d.cache();
+    if (line_search)
+        hipLaunchKernelGGL(dev::k_list_closest<dev::kAtTrial>, dim3(kPairBlocks), dim3(kBlock), 0, d.stream,
+                           d.flags.get(), d.collider(), cc, contact_pad, d.q.get(), d.q_base.get(), d.dir.get(), alpha,
+                           d.best_tri.get(), partials);
+    else
+        hipLaunchKernelGGL(dev::k_list_closest<dev::kAtState>, dim3(kPairBlocks), dim3(kBlock), 0, d.stream,
+                           d.flags.get(), d.collider(), cc, contact_pad, d.q.get(), d.q_base.get(), d.dir.get(), alpha,
+                           d.best_tri.get(), partials);
     HIP_TRY(hipGetLastError());
 }
 
@@ -773,23 +846,31 @@ Solver::Reduced Solver::evaluate(bool gradient, double dt, bool line_search, dou
     const unsigned blocks = d.tiled_blocks();
     const RodView rv = d.rods();
     const ColliderView cv = d.collider();
+    const ContactCache cc = d.cache();
     hipEvent_t t0 = d.tic();
+    // With valid candidate lists the pair pass finds the closest triangles
+    // first; its partials follow the evaluation's.
+    // With current candidate lists the list pass finds the closest
+    // triangles first; its partials follow the evaluation's.
+    const bool list_pass = d.cache_valid;
+    if (list_pass) closest_pass(line_search, alpha, d.partials.get() + blocks);
+    const uint32_t *best = list_pass ? d.best_tri.get() : nullptr;
     if (gradient)
-        hipLaunchKernelGGL((dev::k_evaluate<true, false>), dim3(blocks), dim3(kBlock), 0, d.stream, rv, cv, p,
+        hipLaunchKernelGGL((dev::k_evaluate<true, false>), dim3(blocks), dim3(kBlock), 0, d.stream, rv, cv, cc, best, p,
                            d.q.get(), d.q_old.get(), d.vel.get(), d.q_base.get(), d.dir.get(), 0.0, d.grad.get(),
                            d.Dblk.get(), d.Ublk.get(), d.partials.get());
     else if (line_search)
-        hipLaunchKernelGGL((dev::k_evaluate<false, true>), dim3(blocks), dim3(kBlock), 0, d.stream, rv, cv, p,
+        hipLaunchKernelGGL((dev::k_evaluate<false, true>), dim3(blocks), dim3(kBlock), 0, d.stream, rv, cv, cc, best, p,
                            d.q.get(), d.q_old.get(), d.vel.get(), d.q_base.get(), d.dir.get(), alpha, d.grad.get(),
                            d.Dblk.get(), d.Ublk.get(), d.partials.get());
     else
-        hipLaunchKernelGGL((dev::k_evaluate<false, false>), dim3(blocks), dim3(kBlock), 0, d.stream, rv, cv, p,
+        hipLaunchKernelGGL((dev::k_evaluate<false, false>), dim3(blocks), dim3(kBlock), 0, d.stream, rv, cv, cc, best, p,
                            d.q.get(), d.q_old.get(), d.vel.get(), d.q_base.get(), d.dir.get(), 0.0, d.grad.get(),
                            d.Dblk.get(), d.Ublk.get(), d.partials.get());
     HIP_TRY(hipGetLastError());
     if (gradient) d.toc(t0, d.totals.assembly_ms, d.totals.assembly_launches);
     else d.toc(t0, d.totals.energy_ms, d.totals.energy_launches);
-    const Partial r2 = d.reduce(blocks);
+    const Partial r2 = d.reduce(blocks + (list_pass ? kPairBlocks : 0));
     Reduced out{};
     out.objective = r2.sum[kObjective];
     out.elastic = r2.sum[kElastic];
@@ -911,11 +992,19 @@ KhsResult Solver::optimize_substep(double dt)
         for (int attempt = 0; attempt < 2; ++attempt) {
             phase_.store(KHS_PHASE_CCD);
             hipEvent_t t0 = d.tic();
+            const bool list_pass = d.cache_valid;
+            if (list_pass) {
+                refresh_contact_cache(dev::kSwept, false, 0.0);
+                hipLaunchKernelGGL(dev::k_list_ccd, dim3(kPairBlocks), dim3(kBlock), 0, d.stream, d.flags.get(),
+                                   d.collider(), d.cache(), cp, d.q.get(), d.dir.get(),
+                                   d.partials.get() + d.node_blocks());
+            }
             hipLaunchKernelGGL(dev::k_ccd_slope, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, d.collider(),
-                               cp, d.q.get(), d.grad.get(), d.dir.get(), d.partials.get());
+                               d.cache(), list_pass ? 1 : 0, cp, d.q.get(), d.grad.get(), d.dir.get(),
+                               d.partials.get());
             HIP_TRY(hipGetLastError());
             d.toc(t0, d.totals.collision_ms, d.totals.collision_launches);
-            const Partial cr = d.reduce(d.node_blocks());
+            const Partial cr = d.reduce(d.node_blocks() + (list_pass ? kPairBlocks : 0));
             const double slope = cr.sum[kSlope];
             alpha = 1.0;
             if (d.T) {
@@ -1070,6 +1159,7 @@ KhsResult Solver::run_frame(double frame_dt, const double *root_a, const double 
                 }
                 hipEvent_t t1 = d.tic();
                 refit_bvh(false);
+                refresh_contact_cache(dev::kAtState, true, 0.0);
                 d.toc(t1, d.totals.collision_ms, d.totals.collision_launches);
             }
             phase_.store(KHS_PHASE_ASSEMBLING);
@@ -1084,8 +1174,9 @@ KhsResult Solver::run_frame(double frame_dt, const double *root_a, const double 
                 break;
             }
             if (d.T && material_.friction > 0.0) {
+                if (d.cache_valid) closest_pass(false, 0.0, nullptr);
                 hipLaunchKernelGGL(dev::k_friction, dim3(d.tiled_blocks()), dim3(kBlock), 0, d.stream, rv,
-                                   d.collider(), sp, d.q.get(), d.vel.get(), d.dv.get());
+                                   d.collider(), d.cache(), d.cache_valid ? d.best_tri.get() : nullptr, sp, d.q.get(), d.vel.get(), d.dv.get());
                 hipLaunchKernelGGL(dev::k_apply_dv, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.N, d.dv.get(),
                                    d.vel.get());
                 HIP_TRY(hipGetLastError());

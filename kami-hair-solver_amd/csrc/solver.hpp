@@ -15,6 +15,7 @@
 namespace khs {
 
 struct DeviceState;  // device buffers, defined in solver.hip
+struct Partial;      // per-block reduction record of the kernels
 
 class Solver {
 public:
@@ -83,6 +84,8 @@ private:
     void release_device();
     void rebuild_bvh(bool swept);
     void refit_bvh(bool swept);
+    void refresh_contact_cache(int mode, bool full, double alpha);
+    void closest_pass(bool line_search, double alpha, Partial *partials);
     Reduced evaluate(bool gradient, double dt, bool line_search, double alpha);
     bool make_direction();
     void steepest_direction();

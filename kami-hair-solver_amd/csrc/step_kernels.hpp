@@ -125,14 +125,19 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const Partial *partials, uint
 }
 
 
-// Closest candidate triangle of a segment; ties in distance go to the
-// lower triangle index.
+__device__ __forceinline__ int internal_count(const ColliderView &c)
+{
+    return c.triangle_count > 1 ? int(c.triangle_count) - 1 : 1;
+}
+
+// Closest candidate triangle of a segment from the hierarchy; ties in
+// distance go to the lower triangle index.
 __device__ inline bool closest_triangle(const ColliderView &c, V3 p0, V3 p1, double pad, Closest &best,
                                         uint32_t &best_tri, unsigned long long &count)
 {
     best = Closest{1.0e300, 0.0, mk(1, 0, 0)};
-    best_tri = 0xffffffffu;
-    bvh_query(c.bvh, c.triangle_count > 1 ? int(c.triangle_count) - 1 : 1, vmin(p0, p1), vmax(p0, p1), pad, [&](uint32_t t) {
+    best_tri = kNoTriangle;
+    bvh_query(c.bvh, internal_count(c), vmin(p0, p1), vmax(p0, p1), pad, [&](uint32_t t) {
         ++count;
         const uint32_t *tri = c.triangles + 3 * size_t(t);
         const Closest cand = closest_segment_triangle(p0, p1, vertex(c, tri[0]), vertex(c, tri[1]), vertex(c, tri[2]));
@@ -141,7 +146,217 @@ __device__ inline bool closest_triangle(const ColliderView &c, V3 p0, V3 p1, dou
             best_tri = t;
         }
     });
-    return best_tri != 0xffffffffu;
+    return best_tri != kNoTriangle;
+}
+
+// Query points of element L in the three kinds of pass.
+enum QueryMode { kAtState = 0, kAtTrial = 1, kSwept = 2 };
+
+// Endpoints at q, or at alpha d + q_base in a line search (the expression
+// the evaluation uses, so the same rounding).
+template <int MODE>
+__device__ __forceinline__ void element_points(uint32_t L, const double *q, const double *q_base, const double *dir,
+                                               double alpha, V3 &p0, V3 &p1)
+{
+    if (MODE == kAtTrial) {
+        const double *b0 = q_base + 6 * size_t(L), *d0 = dir + 6 * size_t(L);
+        p0 = mk(alpha * d0[0] + b0[0], alpha * d0[1] + b0[1], alpha * d0[2] + b0[2]);
+        p1 = mk(alpha * d0[6] + b0[6], alpha * d0[7] + b0[7], alpha * d0[8] + b0[8]);
+    } else {
+        p0 = load3(q + 6 * size_t(L));
+        p1 = load3(q + 6 * size_t(L + 1));
+    }
+}
+
+// Unpadded query box of element L: the segment, or for kSwept the segment
+// at q and at q + d (the box the line-search bound queries).
+template <int MODE>
+__device__ __forceinline__ void query_box(uint32_t L, const double *q, const double *q_base, const double *dir,
+                                          double alpha, V3 &lo, V3 &hi)
+{
+    V3 p0, p1;
+    element_points<MODE>(L, q, q_base, dir, alpha, p0, p1);
+    if (MODE == kSwept) {
+        const V3 d0 = load3(dir + 6 * size_t(L)), d1 = load3(dir + 6 * size_t(L + 1));
+        lo = vmin(vmin(p0, p1), vmin(add(p0, d0), add(p1, d1)));
+        hi = vmax(vmax(p0, p1), vmax(add(p0, d0), add(p1, d1)));
+    } else {
+        lo = vmin(p0, p1);
+        hi = vmax(p0, p1);
+    }
+}
+
+// [lo - pad, hi + pad] inside the cached box with a margin far above the
+// rounding of either box test, so every triangle passing the query's test
+// passed the list's (false for NaN).
+__device__ __forceinline__ bool box_covered(const double *box, V3 lo, V3 hi, double pad)
+{
+    const double l[3] = {lo.x - pad, lo.y - pad, lo.z - pad}, h[3] = {hi.x + pad, hi.y + pad, hi.z + pad};
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double ml = 1.0e-12 + 1.0e-9 * fabs(l[k]), mh = 1.0e-12 + 1.0e-9 * fabs(h[k]);
+        ok = ok && box[k] <= l[k] - ml && box[3 + k] >= h[k] + mh;
+    }
+    return ok;
+}
+
+// Before a query pass: every contact element whose query box (grown by
+// pad, the largest pad of the passes) is not inside its cached box walks
+// the hierarchy for that box grown by the skin and stores the overlapping
+// triangles. With `full` (new collider boxes) every element does.
+template <int MODE>
+__global__ void k_contact_refresh(RodView m, ColliderView col, ContactCache cc, const double *q, const double *q_base,
+                                  const double *dir, double alpha, double pad, int full)
+{
+    const uint32_t L = blockIdx.x * blockDim.x + threadIdx.x;
+    if (L >= m.node_count) return;
+    const uint32_t fl = m.flags[L];
+    if (!(fl & kNodeHasElement) || !(fl & kNodeContact)) return;
+    V3 lo, hi;
+    query_box<MODE>(L, q, q_base, dir, alpha, lo, hi);
+    double *box = cc.box + 6 * size_t(L);
+    if (!full && box_covered(box, lo, hi, pad)) return;
+    const double grow = pad + cc.skin;
+    const V3 blo = mk(lo.x - grow, lo.y - grow, lo.z - grow), bhi = mk(hi.x + grow, hi.y + grow, hi.z + grow);
+    uint32_t n = 0;
+    uint32_t *tris = cc.tris + size_t(L) * kCacheCapacity;
+    bvh_query(col.bvh, internal_count(col), blo, bhi, 0.0, [&](uint32_t t) {
+        if (n < kCacheCapacity) tris[n] = t;
+        ++n;
+    });
+    cc.count[L] = n <= kCacheCapacity ? n : kCacheOverflow;
+    store3(box, blo);
+    store3(box + 3, bhi);
+    if (n > 0 && !cc.listed[L]) {
+        cc.listed[L] = 1;
+        cc.list[atomicAdd(cc.list_count, 1u)] = L;
+    }
+}
+
+// Minimum of (distance, triangle) in lexicographic order across the
+// wavefront; NaN distances never enter (callers skip them).
+__device__ __forceinline__ void wave_closest(double &d, uint32_t &t)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const double od = __shfl_xor(d, o, kWave);
+        const uint32_t ot = __shfl_xor(t, o, kWave);
+        if (od < d || (od == d && ot < t)) {
+            d = od;
+            t = ot;
+        }
+    }
+}
+
+// Narrow phase, one wavefront per listed element, its lanes striding the
+// list: the closest triangle among those whose leaf box overlaps the padded
+// segment box (ties to the lower index) goes to best_tri. Candidates of
+// elements with a free left node, the ones the evaluation queries, are
+// counted. Overflowed elements are left to the consumer's tree walk.
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, ColliderView col, ContactCache cc,
+                                                         double pad, const double *q, const double *q_base,
+                                                         const double *dir, double alpha, uint32_t *best_tri,
+                                                         Partial *partials)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const uint32_t waves = gridDim.x * blockDim.x / kWave;
+    const uint32_t listed = *cc.list_count;
+    unsigned long long candidates = 0;
+    for (uint32_t j = wave; j < listed; j += waves) {
+        const uint32_t L = cc.list[j];
+        const uint32_t c = cc.count[L];
+        if (c == 0 || c == kCacheOverflow) continue;
+        const bool counted = !(flags[L] & kNodeFixed);
+        V3 p0, p1;
+        element_points<MODE>(L, q, q_base, dir, alpha, p0, p1);
+        const V3 lo = vmin(p0, p1), hi = vmax(p0, p1);
+        const uint32_t *tris = cc.tris + size_t(L) * kCacheCapacity;
+        double bd = 1.0e300;
+        uint32_t bt = kNoTriangle;
+        for (uint32_t k = lane; k < c; k += kWave) {
+            const uint32_t t = tris[k];
+            const double *b = cc.tri_box + 6 * size_t(t);
+            if (!overlap(lo, hi, mk(b[0], b[1], b[2]), mk(b[3], b[4], b[5]), pad)) continue;
+            if (counted) ++candidates;
+            const uint32_t *t3 = col.triangles + 3 * size_t(t);
+            const double d =
+                closest_segment_triangle(p0, p1, vertex(col, t3[0]), vertex(col, t3[1]), vertex(col, t3[2])).distance;
+            if (d < bd || (d == bd && t < bt)) {
+                bd = d;
+                bt = t;
+            }
+        }
+        wave_closest(bd, bt);
+        if (lane == 0) best_tri[L] = bt;
+    }
+    if (partials) {
+        Partial acc = empty_partial();
+        acc.candidates = candidates;
+        block_reduce(acc, partials + blockIdx.x);
+    }
+}
+
+// Conservative advancement of one segment along the direction against one
+// static triangle (cuda_backend.cu:989-1000): the largest safe step, 1 when
+// the triangle does not limit it.
+__device__ inline double advancement_limit(V3 p0, V3 p1, V3 d0, V3 d1, double speed, V3 a, V3 b, V3 c, double target)
+{
+    double alpha = 0.0;
+    for (int it = 0; it < 80; ++it) {
+        const double dist = closest_segment_triangle(add(p0, mul(d0, alpha)), add(p1, mul(d1, alpha)), a, b, c).distance;
+        const double clear = dist - target;
+        if (clear <= 0) return fmax(0.0, alpha * 0.9);
+        const double advance = 0.8 * clear / speed;
+        if (alpha + advance >= 1) return 1.0;
+        if (advance < 1e-12) return alpha;
+        alpha += advance;
+    }
+    return alpha;
+}
+
+// Line-search step bound, one wavefront per listed element with a free left
+// node, its lanes striding the list.
+__global__ __launch_bounds__(kBlock) void k_list_ccd(const uint32_t *flags, ColliderView col, ContactCache cc,
+                                                     StepParams p, const double *q, const double *dir,
+                                                     Partial *partials)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const uint32_t waves = gridDim.x * blockDim.x / kWave;
+    const uint32_t listed = *cc.list_count;
+    double limit = 1.0;
+    for (uint32_t j = wave; j < listed; j += waves) {
+        const uint32_t L = cc.list[j];
+        const uint32_t c = cc.count[L];
+        if (c == 0 || c == kCacheOverflow || (flags[L] & kNodeFixed)) continue;
+        const V3 p0 = load3(q + 6 * size_t(L)), p1 = load3(q + 6 * size_t(L + 1));
+        const V3 d0 = load3(dir + 6 * size_t(L)), d1 = load3(dir + 6 * size_t(L + 1));
+        const double speed = fmax(norm(d0), norm(d1));
+        if (speed < 1e-30) continue;
+        const V3 lo = vmin(vmin(p0, p1), vmin(add(p0, d0), add(p1, d1)));
+        const V3 hi = vmax(vmax(p0, p1), vmax(add(p0, d0), add(p1, d1)));
+        const uint32_t *tris = cc.tris + size_t(L) * kCacheCapacity;
+        for (uint32_t k = lane; k < c; k += kWave) {
+            const uint32_t t = tris[k];
+            const double *b = cc.tri_box + 6 * size_t(t);
+            if (!overlap(lo, hi, mk(b[0], b[1], b[2]), mk(b[3], b[4], b[5]), p.sweep_target)) continue;
+            const uint32_t *t3 = col.triangles + 3 * size_t(t);
+            limit = fmin(limit, advancement_limit(p0, p1, d0, d1, speed, vertex(col, t3[0]), vertex(col, t3[1]),
+                                                  vertex(col, t3[2]), p.sweep_target));
+        }
+    }
+    Partial acc = empty_partial();
+    acc.alpha = limit;
+    block_reduce(acc, partials + blockIdx.x);
+}
+
+// Whether element L's query in this pass was answered by the list pass.
+__device__ __forceinline__ bool listed_query(const ContactCache &cc, const uint32_t *best_tri, uint32_t L)
+{
+    return best_tri && cc.count[L] != kCacheOverflow;
 }
 
 // ------------------------------------------------------------ evaluation
@@ -151,7 +366,8 @@ __device__ inline bool closest_triangle(const ColliderView &c, V3 p0, V3 p1, dou
 // diagonal block D and the upper coupling block U of each node). With LS the
 // state is q = alpha d + q_base, written back by the owners.
 template <bool GRAD, bool LS>
-__global__ __launch_bounds__(kBlock) void k_evaluate(RodView m, ColliderView col, StepParams p, double *q,
+__global__ __launch_bounds__(kBlock) void k_evaluate(RodView m, ColliderView col, ContactCache cc,
+                                                     const uint32_t *best_tri, StepParams p, double *q,
                                                      const double *q_old, const double *vel, const double *q_base,
                                                      const double *dir, double alpha, double *grad, double *Dblk,
                                                      double *Ublk, Partial *partials)
@@ -211,11 +427,22 @@ __global__ __launch_bounds__(kBlock) void k_evaluate(RodView m, ColliderView col
         const V3 p0 = mk(qi[0], qi[1], qi[2]), p1 = mk(qj[0], qj[1], qj[2]);
         Closest pair;
         uint32_t tri = 0;
-        unsigned long long count = 0;
-        const bool found = closest_triangle(col, p0, p1, p.contact_pad, pair, tri, count);
-        // Lane 0 recomputes the previous tile's element for its coupling terms
-        // only; statistics are counted by the element's owner lane.
-        if (owner) acc.candidates += count;
+        bool found;
+        if (listed_query(cc, best_tri, uint32_t(L))) {
+            // the list pass counted the candidates and chose the triangle
+            tri = cc.count[L] ? best_tri[L] : kNoTriangle;
+            found = tri != kNoTriangle;
+            if (found) {
+                const uint32_t *t3 = col.triangles + 3 * size_t(tri);
+                pair = closest_segment_triangle(p0, p1, vertex(col, t3[0]), vertex(col, t3[1]), vertex(col, t3[2]));
+            }
+        } else {
+            unsigned long long count = 0;
+            found = closest_triangle(col, p0, p1, p.contact_pad, pair, tri, count);
+            // Lane 0 recomputes the previous tile's element for its coupling
+            // terms only; statistics are counted by the element's owner lane.
+            if (owner) acc.candidates += count;
+        }
         if (found) {
             const double gap = pair.distance - p.radius_offset;
             if (owner) acc.min_gap = fmin(acc.min_gap, gap);
@@ -649,8 +876,9 @@ __global__ void k_copy(double *dst, const double *src, size_t count)
 // Line-search step bound: conservative advancement of each contact-enabled
 // segment along the direction against the static collider
 // (cuda_backend.cu:989-1000), plus the slope g.d.
-__global__ __launch_bounds__(kBlock) void k_ccd_slope(RodView m, ColliderView col, StepParams p, const double *q,
-                                                      const double *grad, const double *dir, Partial *partials)
+__global__ __launch_bounds__(kBlock) void k_ccd_slope(RodView m, ColliderView col, ContactCache cc, int list_pass,
+                                                      StepParams p, const double *q, const double *grad,
+                                                      const double *dir, Partial *partials)
 {
     const uint32_t L = blockIdx.x * blockDim.x + threadIdx.x;
     Partial acc = empty_partial();
@@ -660,30 +888,21 @@ __global__ __launch_bounds__(kBlock) void k_ccd_slope(RodView m, ColliderView co
         for (int k = 0; k < 6; ++k) gd += grad[6 * size_t(L) + k] * dir[6 * size_t(L) + k];
         acc.sum[kSlope] = gd;
         const uint32_t fl = m.flags[L];
-        if (p.has_collider && (fl & kNodeHasElement) && (fl & kNodeContact) && !(fl & kNodeFixed)) {
+        // elements with a list were bounded by the list pass
+        if (p.has_collider && (fl & kNodeHasElement) && (fl & kNodeContact) && !(fl & kNodeFixed) &&
+            !(list_pass && cc.count[L] != kCacheOverflow)) {
             const V3 p0 = load3(q + 6 * size_t(L)), p1 = load3(q + 6 * size_t(L + 1));
             const V3 d0 = load3(dir + 6 * size_t(L)), d1 = load3(dir + 6 * size_t(L + 1));
             const V3 lo = vmin(vmin(p0, p1), vmin(add(p0, d0), add(p1, d1)));
             const V3 hi = vmax(vmax(p0, p1), vmax(add(p0, d0), add(p1, d1)));
             const double speed = fmax(norm(d0), norm(d1));
             double limit = 1.0;
-            bvh_query(col.bvh, col.triangle_count > 1 ? int(col.triangle_count) - 1 : 1, lo, hi, p.sweep_target, [&](uint32_t t) {
-                if (speed < 1e-30) return;
-                const uint32_t *t3 = col.triangles + 3 * size_t(t);
-                const V3 a = vertex(col, t3[0]), b = vertex(col, t3[1]), c = vertex(col, t3[2]);
-                double alpha = 0.0;
-                for (int it = 0; it < 80; ++it) {
-                    const double dist =
-                        closest_segment_triangle(add(p0, mul(d0, alpha)), add(p1, mul(d1, alpha)), a, b, c).distance;
-                    const double clear = dist - p.sweep_target;
-                    if (clear <= 0) { limit = fmin(limit, fmax(0.0, alpha * 0.9)); return; }
-                    const double advance = 0.8 * clear / speed;
-                    if (alpha + advance >= 1) return;
-                    if (advance < 1e-12) { limit = fmin(limit, alpha); return; }
-                    alpha += advance;
-                }
-                limit = fmin(limit, alpha);
-            });
+            if (speed >= 1e-30)
+                bvh_query(col.bvh, internal_count(col), lo, hi, p.sweep_target, [&](uint32_t t) {
+                    const uint32_t *t3 = col.triangles + 3 * size_t(t);
+                    limit = fmin(limit, advancement_limit(p0, p1, d0, d1, speed, vertex(col, t3[0]), vertex(col, t3[1]),
+                                                          vertex(col, t3[2]), p.sweep_target));
+                });
             acc.alpha = limit;
         }
     }
@@ -703,7 +922,7 @@ __global__ void k_sweep(RodView m, ColliderView col, StepParams p, const double 
     const V3 p0 = load3(q + 6 * size_t(L)), p1 = load3(q + 6 * size_t(L + 1));
     const uint32_t element = L - m.strand_of[L];
     unsigned long long worst = ~0ull;
-    bvh_query(col.bvh, col.triangle_count > 1 ? int(col.triangle_count) - 1 : 1, vmin(p0, p1), vmax(p0, p1), p.sweep_target, [&](uint32_t t) {
+    bvh_query(col.bvh, internal_count(col), vmin(p0, p1), vmax(p0, p1), p.sweep_target, [&](uint32_t t) {
         const uint32_t *t3 = col.triangles + 3 * size_t(t);
         const V3 oa = vertex_prev(col, t3[0]), ob = vertex_prev(col, t3[1]), oc = vertex_prev(col, t3[2]);
         const V3 da = sub(vertex(col, t3[0]), oa), db = sub(vertex(col, t3[1]), ob), dc = sub(vertex(col, t3[2]), oc);
@@ -786,7 +1005,8 @@ __global__ void k_sweep_record(RodView m, ColliderView col, StepParams p, const 
 // Post-substep smoothed Coulomb impulse against the closest triangle
 // (cuda_backend.cu:941-964); the velocity change of each node is gathered
 // from its two elements in element order.
-__global__ __launch_bounds__(kBlock) void k_friction(RodView m, ColliderView col, StepParams p, const double *q,
+__global__ __launch_bounds__(kBlock) void k_friction(RodView m, ColliderView col, ContactCache cc,
+                                                     const uint32_t *best_tri, StepParams p, const double *q,
                                                      const double *vel, double *dv)
 {
     const int lane = threadIdx.x & (kWave - 1);
@@ -803,8 +1023,19 @@ __global__ __launch_bounds__(kBlock) void k_friction(RodView m, ColliderView col
             const V3 p0 = load3(q + 6 * L), p1 = load3(q + 6 * (L + 1));
             Closest pair;
             uint32_t tri;
-            unsigned long long count = 0;
-            if (closest_triangle(col, p0, p1, p.contact_pad, pair, tri, count)) {
+            bool found;
+            if (listed_query(cc, best_tri, uint32_t(L))) {
+                tri = cc.count[L] ? best_tri[L] : kNoTriangle;
+                found = tri != kNoTriangle;
+                if (found) {
+                    const uint32_t *t3 = col.triangles + 3 * size_t(tri);
+                    pair = closest_segment_triangle(p0, p1, vertex(col, t3[0]), vertex(col, t3[1]), vertex(col, t3[2]));
+                }
+            } else {
+                unsigned long long count = 0;
+                found = closest_triangle(col, p0, p1, p.contact_pad, pair, tri, count);
+            }
+            if (found) {
                 const double gap = pair.distance - p.radius_offset;
                 if (gap > 0.0 && gap < p.barrier) {
                     const uint32_t *t3 = col.triangles + 3 * size_t(tri);
@@ -873,7 +1104,7 @@ __global__ void k_point_inside(RodView m, ColliderView col, const double *pos, d
     const double reach = fmax(x_max - pt.x, 0.0) + 1.0e-6;
     const V3 lo = pt, hi = add(pt, mul(dirv, reach));
     unsigned hits = 0;
-    bvh_query(col.bvh, col.triangle_count > 1 ? int(col.triangle_count) - 1 : 1, vmin(lo, hi), vmax(lo, hi), 1.0e-9, [&](uint32_t t) {
+    bvh_query(col.bvh, internal_count(col), vmin(lo, hi), vmax(lo, hi), 1.0e-9, [&](uint32_t t) {
         const uint32_t *t3 = col.triangles + 3 * size_t(t);
         const V3 a = vertex(col, t3[0]), b = vertex(col, t3[1]), c = vertex(col, t3[2]);
         const V3 e1 = sub(b, a), e2 = sub(c, a);
@@ -903,7 +1134,7 @@ __global__ __launch_bounds__(kBlock) void k_initial_gap(RodView m, ColliderView 
         const uint32_t fl = m.flags[L];
         if ((fl & kNodeHasElement) && (fl & kNodeContact) && !(fl & kNodeFixed)) {
             const V3 p0 = load3(pos + 3 * size_t(L)), p1 = load3(pos + 3 * size_t(L + 1));
-            bvh_query(col.bvh, col.triangle_count > 1 ? int(col.triangle_count) - 1 : 1, vmin(p0, p1), vmax(p0, p1), p.contact_pad, [&](uint32_t t) {
+            bvh_query(col.bvh, internal_count(col), vmin(p0, p1), vmax(p0, p1), p.contact_pad, [&](uint32_t t) {
                 const uint32_t *t3 = col.triangles + 3 * size_t(t);
                 const Closest c = closest_segment_triangle(p0, p1, vertex(col, t3[0]), vertex(col, t3[1]),
                                                            vertex(col, t3[2]), true);
@@ -925,7 +1156,7 @@ __global__ void k_broadphase_pairs(RodView m, ColliderView col, StepParams p, co
     if (!(fl & kNodeHasElement) || !(fl & kNodeContact) || (fl & kNodeFixed)) return;
     const V3 p0 = load3(q + 6 * size_t(L)), p1 = load3(q + 6 * size_t(L + 1));
     const uint32_t element = L - m.strand_of[L];
-    bvh_query(col.bvh, col.triangle_count > 1 ? int(col.triangle_count) - 1 : 1, vmin(p0, p1), vmax(p0, p1), p.contact_pad, [&](uint32_t t) {
+    bvh_query(col.bvh, internal_count(col), vmin(p0, p1), vmax(p0, p1), p.contact_pad, [&](uint32_t t) {
         const unsigned long long slot = atomicAdd(count, 1ull);
         if (slot < capacity) {
             pairs[2 * slot] = element;

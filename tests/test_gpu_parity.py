@@ -147,9 +147,15 @@ def test_one_frame_positions_match_oracle(oracle_lib, gpu_lib):
     xo, xg = o.internal_positions(), g.internal_positions()
     err = float(np.max(np.abs(xo - xg)))
     so, sg = o.step_stats(), g.step_stats()
-    print(f"max position error {err:.3e} m; newton oracle {so.newton_iterations} gpu {sg.newton_iterations}")
+    print(f"max position error {err:.3e} m; newton oracle {so.newton_iterations} gpu {sg.newton_iterations}; "
+          f"candidates oracle {so.contact_candidate_count} gpu {sg.contact_candidate_count}; "
+          f"active oracle {so.active_contact_count} gpu {sg.active_contact_count}")
     assert err <= 1e-7
     assert sg.phase == kn.PHASE_FINISHED
+    # The candidate count sums the triangles every evaluation tested, so it
+    # also checks that per-element candidate lists report the full set.
+    assert sg.contact_candidate_count == so.contact_candidate_count
+    assert sg.active_contact_count == so.active_contact_count
     assert sg.minimum_gap > scene.desc.get("minimum_gap", 1e-7) or np.isinf(sg.minimum_gap)
 
 
@@ -185,3 +191,63 @@ def test_contact_frames_match_oracle(oracle_lib, gpu_lib):
     assert err <= 1e-7
     assert out[1][1].active_contact_count > 0
     assert out[1][1].minimum_gap > 1e-7
+
+
+def _grid_plane(n, h):
+    """n x n quads of side h centred on the origin in z = 0, two triangles each."""
+    xs = (np.arange(n + 1) - n / 2) * h
+    gx, gy = np.meshgrid(xs, xs, indexing="ij")
+    verts = np.stack([gx.ravel(), gy.ravel(), np.zeros(gx.size)], axis=1)
+    tris = []
+    for i in range(n):
+        for j in range(n):
+            a, b, c, d = i * (n + 1) + j, (i + 1) * (n + 1) + j, (i + 1) * (n + 1) + j + 1, i * (n + 1) + j + 1
+            tris += [[a, b, c], [a, c, d]]
+    return verts, np.array(tris, np.uint32)
+
+
+@pytest.mark.parametrize("quad", [5.0e-3, 1.0e-3])
+def test_draped_strand_on_grid_matches_oracle(oracle_lib, gpu_lib, quad):
+    """A strand falls onto a tessellated plane. With 1 mm quads an element's
+    padded box overlaps more triangles than a per-element candidate list
+    holds, so both the list path and the hierarchy path are exercised."""
+    print(f"[gpu] draped strand on grid, quad {quad}", flush=True)
+    verts, tris = _grid_plane(int(round(0.08 / quad)), quad)
+    pts = np.array([[-0.03 + 0.01 * k, 0.0013 * k, 0.006] for k in range(7)])
+    out = []
+    for lib in (oracle_lib, gpu_lib):
+        desc = kn.default_desc(lib)
+        desc.substeps = 4
+        desc.fixed_root_nodes = 1
+        desc.maximum_element_length = 0.01
+        desc.newton_iterations = 60
+        desc.relative_tolerance = 1e-6
+        desc.absolute_tolerance = 1e-30
+        desc.increment_tolerance = 1e-14
+        mat = kn.default_material(lib)
+        mat.radius = 1.0e-4
+        mat.young_modulus = 1.0e7
+        mat.contact_stiffness = 100.0
+        mat.barrier_distance = 2.0e-3
+        mat.mass_damping = 2.0
+        mat.friction = 0.3
+        s = kn.HairSolver(lib, desc)
+        s.check(s.set_hair(pts, [0, len(pts)], mat))
+        s.check(s.set_collider(verts, tris))
+        s.check(s.build())
+        s.check(s.set_convergence_mode(1))
+        cand, active = [], []
+        for _ in range(4):
+            s.check(s.step(1.0 / 30.0))
+            st = s.step_stats()
+            cand.append(st.contact_candidate_count)
+            active.append(st.active_contact_count)
+        out.append((s.internal_positions(), cand, active, s.step_stats()))
+    err = float(np.max(np.abs(out[0][0] - out[1][0])))
+    print(f"max position error {err:.3e} m; candidates oracle {out[0][1]} gpu {out[1][1]}; "
+          f"active oracle {out[0][2]} gpu {out[1][2]}")
+    assert err <= 1e-7
+    assert out[1][1] == out[0][1]
+    assert out[1][2] == out[0][2]
+    assert sum(out[1][2]) > 0
+    assert out[1][3].minimum_gap > 1e-7
