@@ -46,6 +46,7 @@ struct ColliderView {
     const double *previous;
     const uint32_t *triangles;
     const BvhNode *bvh;
+    const double *leaf_box;  // 6 per triangle at `current`, or null: leaves use the tree's boxes
 };
 
 // Per-element lists of the collider triangles whose leaf box overlaps a
@@ -67,7 +68,7 @@ struct ContactCache {
     uint32_t *listed;       // per element: 1 once appended to the list below
     uint32_t *list;         // elements whose list became non-empty
     uint32_t *list_count;
-    const double *tri_box;  // 6 per triangle, the leaf boxes of the hierarchy
+    const double *tri_box;  // 6 per triangle: the box at the current collider position
 };
 
 // Scalars of one substep.

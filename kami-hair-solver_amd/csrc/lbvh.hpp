@@ -25,8 +25,13 @@ constexpr int kBvhStack = 64;
 // Calls f(triangle) for every triangle whose box overlaps [lo, hi] grown by
 // pad. The Karras hierarchy over 30-bit codes with index tie-break has
 // depth below 64, so the stack never overflows.
+// With leaf_box (6 doubles per triangle) a leaf is tested against its own
+// entry there instead of the box stored in the tree, which must contain it:
+// a tree refit with boxes swept over a collider motion then answers queries
+// on the collider's current position exactly.
 template <class F>
-__device__ inline void bvh_query(const BvhNode *nodes, int internal_count, V3 lo, V3 hi, double pad, F &&f)
+__device__ inline void bvh_query(const BvhNode *nodes, int internal_count, const double *leaf_box, V3 lo, V3 hi,
+                                 double pad, F &&f)
 {
     int stack[kBvhStack];
     int top = 0, cur = 0;
@@ -43,7 +48,12 @@ __device__ inline void bvh_query(const BvhNode *nodes, int internal_count, V3 lo
             const int ch = n.child[c];
             if (ch >= internal_count) continue;  // never taken by a well-formed tree
             if (ch < 0) {
-                f(uint32_t(-1 - ch));
+                const uint32_t t = uint32_t(-1 - ch);
+                if (leaf_box) {
+                    const double *b = leaf_box + 6 * size_t(t);
+                    if (!overlap(lo, hi, mk(b[0], b[1], b[2]), mk(b[3], b[4], b[5]), pad)) continue;
+                }
+                f(t);
             } else if (next < 0) {
                 next = ch;
             } else if (top < kBvhStack) {

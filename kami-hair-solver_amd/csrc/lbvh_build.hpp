@@ -33,7 +33,7 @@ __device__ __forceinline__ uint32_t spread_bits(uint32_t v)
 // Triangle boxes (static, or swept over previous and current positions)
 // and per-block centroid bounds.
 __global__ __launch_bounds__(256) void k_triangle_boxes(ColliderView col, int swept, double *tri_box,
-                                                           double *bound_partials)
+                                                           double *static_box, double *bound_partials)
 {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     __shared__ double lo_s[256][3], hi_s[256][3];
@@ -48,7 +48,14 @@ __global__ __launch_bounds__(256) void k_triangle_boxes(ColliderView col, int sw
             lo = vmin(lo, v);
             hi = vmax(hi, v);
             centroid = add(centroid, v);
-            if (swept) {
+        }
+        if (static_box) {
+            store3(static_box + 6 * size_t(t), lo);
+            store3(static_box + 6 * size_t(t) + 3, hi);
+        }
+        if (swept) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
                 const V3 o = vertex_prev(col, t3[c]);
                 lo = vmin(lo, o);
                 hi = vmax(hi, o);

@@ -96,6 +96,7 @@ struct DeviceState {
     // rod model
     Buffer<double> frame, rest, mass, rmass;
     Buffer<uint32_t> flags, strand_of, strand_first, strand_count, strand_fixed, mapping;
+    Buffer<uint32_t> solve_order;  // strands by decreasing free node count, for the solve's dispatch
     // state and solver vectors
     Buffer<double> q, q_old, vel, q_base, snap_q, snap_v, grad, dir, precond, Dblk, Ublk, Cp, dp, dv;
     // root targets (3 per node)
@@ -104,7 +105,7 @@ struct DeviceState {
     Buffer<double> col_cur, col_old, col_start, col_pend, col_anim;
     Buffer<uint32_t> tris;
     // LBVH
-    Buffer<double> tri_box, bound_partials, bounds;
+    Buffer<double> tri_box, tri_box_static, bound_partials, bounds;
     Buffer<uint32_t> keys, keys_sorted, ids, ids_sorted;
     Buffer<BvhNode> bvh;
     Buffer<int> parent_internal, parent_leaf;
@@ -197,11 +198,21 @@ struct DeviceState {
                        rmass.get(), flags.get(), strand_of.get(), strand_first.get(), strand_count.get(),
                        strand_fixed.get()};
     }
-    ColliderView collider() const { return ColliderView{V, T, col_cur.get(), col_old.get(), tris.get(), bvh.get()}; }
+    // Queries test leaves against the boxes at the current collider position
+    // (the tree may be refit over the substep's motion); the sweep test
+    // alone uses the tree's own, swept, leaf boxes.
+    ColliderView collider() const
+    {
+        return ColliderView{V, T, col_cur.get(), col_old.get(), tris.get(), bvh.get(), tri_box_static.get()};
+    }
+    ColliderView swept_collider() const
+    {
+        return ColliderView{V, T, col_cur.get(), col_old.get(), tris.get(), bvh.get(), nullptr};
+    }
     ContactCache cache() const
     {
         return ContactCache{cache_skin,        cache_count.get(), cache_tris.get(),       cache_box.get(),
-                            cache_listed.get(), cache_list.get(), cache_list_count.get(), tri_box.get()};
+                            cache_listed.get(), cache_list.get(), cache_list_count.get(), tri_box_static.get()};
     }
 
     size_t resident_bytes() const
@@ -210,7 +221,7 @@ struct DeviceState {
         for (const Buffer<double> *x : {&frame, &rest, &mass, &rmass, &q, &q_old, &vel, &q_base, &snap_q, &snap_v,
                                         &grad, &dir, &precond, &Dblk, &Ublk, &Cp, &dp, &dv, &root_cur, &rot_cur,
                                         &root_pend, &rot_pend, &root_anim, &rot_anim, &col_cur, &col_old, &col_start,
-                                        &col_pend, &col_anim, &tri_box, &bound_partials, &bounds, &out3})
+                                        &col_pend, &col_anim, &tri_box, &tri_box_static, &bound_partials, &bounds, &out3})
             b += x->bytes();
         for (const Buffer<uint32_t> *x : {&flags, &strand_of, &strand_first, &strand_count, &strand_fixed, &mapping,
                                           &tris, &keys, &keys_sorted, &ids, &ids_sorted})
@@ -460,6 +471,17 @@ KhsResult Solver::upload_model()
     d.strand_count.upload(m.strand_count_of.data(), d.S);
     d.strand_fixed.allocate(d.S);
     d.strand_fixed.upload(m.strand_fixed.data(), d.S);
+    {
+        // The solve is sequential along a strand: dispatching the longest
+        // strands first keeps them off the tail of the launch.
+        std::vector<uint32_t> order(d.S);
+        for (uint32_t k = 0; k < d.S; ++k) order[k] = k;
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+            return m.strand_count_of[x] - m.strand_fixed[x] > m.strand_count_of[y] - m.strand_fixed[y];
+        });
+        d.solve_order.allocate(d.S);
+        d.solve_order.upload(order.data(), d.S);
+    }
     d.mapping.allocate(d.P);
     d.mapping.upload(m.original_to_internal.data(), d.P);
 
@@ -513,6 +535,7 @@ KhsResult Solver::upload_model()
         d.tris.upload(collider_.triangles.data(), collider_.triangles.size());
         const size_t T = d.T;
         d.tri_box.allocate(6 * T);
+        d.tri_box_static.allocate(6 * T);
         d.bound_partials.allocate(6 * size_t(grid_for(T, kBlock)));
         d.bounds.allocate(6);
         d.keys.allocate(T);
@@ -553,7 +576,7 @@ void Solver::rebuild_bvh(bool swept)
     const ColliderView cv = d.collider();
     const unsigned tb = grid_for(d.T, kBlock);
     hipLaunchKernelGGL(dev::k_triangle_boxes, dim3(tb), dim3(kBlock), 0, d.stream, cv, swept ? 1 : 0, d.tri_box.get(),
-                       d.bound_partials.get());
+                       d.tri_box_static.get(), d.bound_partials.get());
     hipLaunchKernelGGL(dev::k_bounds_final, dim3(1), dim3(256), 0, d.stream, d.bound_partials.get(), tb, d.bounds.get());
     hipLaunchKernelGGL(dev::k_morton, dim3(tb), dim3(kBlock), 0, d.stream, cv, d.bounds.get(), d.keys.get(),
                        d.ids.get());
@@ -575,7 +598,7 @@ void Solver::refit_bvh(bool swept)
     d.cache_valid = false;
     const unsigned tb = grid_for(d.T, kBlock);
     hipLaunchKernelGGL(dev::k_triangle_boxes, dim3(tb), dim3(kBlock), 0, d.stream, d.collider(), swept ? 1 : 0,
-                       d.tri_box.get(), d.bound_partials.get());
+                       d.tri_box.get(), d.tri_box_static.get(), d.bound_partials.get());
     hipLaunchKernelGGL(dev::k_zero_counters, dim3(tb), dim3(kBlock), 0, d.stream, d.counters.get(),
                        d.T > 1 ? d.T - 1 : 1u);
     hipLaunchKernelGGL(dev::k_refit, dim3(tb), dim3(kBlock), 0, d.stream, d.T, d.ids_sorted.get(), d.tri_box.get(),
@@ -918,7 +941,7 @@ bool Solver::make_direction()
     const RodView rv = d.rods();
     hipLaunchKernelGGL(dev::k_fill_u32, dim3(1), dim3(64), 0, d.stream, d.flag.get(), size_t(1), 0u);
     hipEvent_t t0 = d.tic();
-    hipLaunchKernelGGL(dev::k_block_solve, dim3(grid_for(d.S, 64)), dim3(64), 0, d.stream, rv, d.Dblk.get(),
+    hipLaunchKernelGGL(dev::k_block_solve, dim3(d.S), dim3(kWave), 0, d.stream, rv, d.solve_order.get(), d.Dblk.get(),
                        d.Ublk.get(), d.grad.get(), d.Cp.get(), d.dp.get(), d.dir.get(), d.flag.get());
     d.toc(t0, d.totals.solve_ms, d.totals.solve_launches);
     hipLaunchKernelGGL(dev::k_direction_stats, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, d.grad.get(),
@@ -1131,8 +1154,8 @@ KhsResult Solver::run_frame(double frame_dt, const double *root_a, const double 
                 phase_.store(KHS_PHASE_CCD);
                 const unsigned long long none = ~0ull;
                 hipLaunchKernelGGL(dev::k_fill_u64, dim3(1), dim3(64), 0, d.stream, d.key64.get(), size_t(1), none);
-                hipLaunchKernelGGL(dev::k_sweep, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, d.collider(),
-                                   sp, d.q.get(), d.key64.get());
+                hipLaunchKernelGGL(dev::k_sweep, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv,
+                                   d.swept_collider(), sp, d.q.get(), d.key64.get());
                 hipLaunchKernelGGL(dev::k_sweep_record, dim3(1), dim3(1), 0, d.stream, rv, d.collider(), sp,
                                    d.q.get(), d.key64.get(), d.sweep.get());
                 HIP_TRY(hipGetLastError());
@@ -1157,8 +1180,10 @@ KhsResult Solver::run_frame(double frame_dt, const double *root_a, const double 
                     crossed = true;
                     break;
                 }
+                // The swept tree serves the rest of the substep: its boxes
+                // contain the current ones, and queries test leaves against
+                // the current boxes.
                 hipEvent_t t1 = d.tic();
-                refit_bvh(false);
                 refresh_contact_cache(dev::kAtState, true, 0.0);
                 d.toc(t1, d.totals.collision_ms, d.totals.collision_launches);
             }
@@ -1495,7 +1520,7 @@ KhsResult Solver::test_evaluate(double dt, double *gradient, double *direction, 
     if (direction) {
         const RodView rv = d.rods();
         hipLaunchKernelGGL(dev::k_fill_u32, dim3(1), dim3(64), 0, d.stream, d.flag.get(), size_t(1), 0u);
-        hipLaunchKernelGGL(dev::k_block_solve, dim3(grid_for(d.S, 64)), dim3(64), 0, d.stream, rv, d.Dblk.get(),
+        hipLaunchKernelGGL(dev::k_block_solve, dim3(d.S), dim3(kWave), 0, d.stream, rv, d.solve_order.get(), d.Dblk.get(),
                            d.Ublk.get(), d.grad.get(), d.Cp.get(), d.dp.get(), d.dir.get(), d.flag.get());
         HIP_TRY(hipGetLastError());
         unsigned failed = 0;

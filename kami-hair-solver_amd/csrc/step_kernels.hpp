@@ -137,7 +137,7 @@ __device__ inline bool closest_triangle(const ColliderView &c, V3 p0, V3 p1, dou
 {
     best = Closest{1.0e300, 0.0, mk(1, 0, 0)};
     best_tri = kNoTriangle;
-    bvh_query(c.bvh, internal_count(c), vmin(p0, p1), vmax(p0, p1), pad, [&](uint32_t t) {
+    bvh_query(c.bvh, internal_count(c), c.leaf_box, vmin(p0, p1), vmax(p0, p1), pad, [&](uint32_t t) {
         ++count;
         const uint32_t *tri = c.triangles + 3 * size_t(t);
         const Closest cand = closest_segment_triangle(p0, p1, vertex(c, tri[0]), vertex(c, tri[1]), vertex(c, tri[2]));
@@ -221,7 +221,7 @@ __global__ void k_contact_refresh(RodView m, ColliderView col, ContactCache cc, 
     const V3 blo = mk(lo.x - grow, lo.y - grow, lo.z - grow), bhi = mk(hi.x + grow, hi.y + grow, hi.z + grow);
     uint32_t n = 0;
     uint32_t *tris = cc.tris + size_t(L) * kCacheCapacity;
-    bvh_query(col.bvh, internal_count(col), blo, bhi, 0.0, [&](uint32_t t) {
+    bvh_query(col.bvh, internal_count(col), col.leaf_box, blo, bhi, 0.0, [&](uint32_t t) {
         if (n < kCacheCapacity) tris[n] = t;
         ++n;
     });
@@ -652,97 +652,158 @@ __device__ inline bool inverse6(double *a, double *inv)
     return true;
 }
 
-// Block Thomas per strand on the free nodes; fixed root nodes are
-// eliminated and get a zero direction (cuda_backend.cu:866-881). The
-// previous node's C', d' and coupling block stay in registers between
-// iterations; C' and d' are written once for the back substitution.
-__global__ __launch_bounds__(64) void k_block_solve(RodView m, const double *Dblk, const double *Ublk,
-                                                    const double *grad, double *Cp, double *dp, double *dir,
-                                                    unsigned *failed)
+// Value of `v` in lane `lane`, a wave-uniform index.
+__device__ __forceinline__ double read_lane(double v, int lane)
 {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= m.strand_count) return;
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
+// Block Thomas per strand on the free nodes; fixed root nodes are
+// eliminated and get a zero direction (cuda_backend.cu:866-881).
+//
+// One wavefront per strand. Lane l < 36 owns entry (r, c) = (l / 6, l % 6)
+// of every 6x6 block and lane r < 6 entry r of every 6-vector; products
+// gather their operands with lane shuffles and sum k = 0..5 from zero, and
+// the Gauss-Jordan inverse (diagonal shift max(1e-12 max|a_ii|, 1e-24),
+// partial pivoting, pivot < 1e-30 fails, cuda_backend.cu:855-864) updates
+// all entries of a column step at once with the sequential algorithm's
+// operations, so every value is rounded as in a one-thread solve. The next
+// node's blocks are loaded while the current one is factored.
+__global__ __launch_bounds__(kWave, 7) void k_block_solve(RodView m, const uint32_t *order, const double *Dblk,
+                                                          const double *Ublk, const double *grad, double *Cp,
+                                                          double *dp, double *dir, unsigned *failed)
+{
+    if (blockIdx.x >= m.strand_count) return;
+    const uint32_t s = order[blockIdx.x];
+    const int l = threadIdx.x;
+    const bool entry = l < 36;
+    const int r = entry ? l / 6 : 0, c = entry ? l % 6 : 0;
     const uint32_t first = m.strand_first[s], begin = first + m.strand_fixed[s], end = first + m.strand_count_of[s];
     for (uint32_t n = first; n < begin; ++n)
-        for (int k = 0; k < 6; ++k) dir[6 * size_t(n) + k] = 0.0;
+        if (l < 6) dir[6 * size_t(n) + l] = 0.0;
     if (begin >= end) return;
-    double low[36], prevc[36], prevd[6];
+
+    double low = 0.0, prevc = 0.0, prevd = 0.0;  // U of the previous node, its C', d'
+    double nd = entry ? Dblk[36 * size_t(begin) + l] : 0.0;
+    double ng = l < 6 ? grad[6 * size_t(begin) + l] : 0.0;
+    double nu = (entry && begin + 1 < end) ? Ublk[36 * size_t(begin) + l] : 0.0;
     for (uint32_t n = begin; n < end; ++n) {
-        double b[36], inv[36], rhs[6];
-#pragma unroll
-        for (int i = 0; i < 36; ++i) b[i] = Dblk[36 * size_t(n) + i];
-#pragma unroll
-        for (int i = 0; i < 6; ++i) rhs[i] = -grad[6 * size_t(n) + i];
+        double a = nd, rhs = -ng;
+        const double u = nu;
+        if (n + 1 < end) {
+            nd = entry ? Dblk[36 * size_t(n + 1) + l] : 0.0;
+            ng = l < 6 ? grad[6 * size_t(n + 1) + l] : 0.0;
+            nu = (entry && n + 2 < end) ? Ublk[36 * size_t(n + 1) + l] : 0.0;
+        }
         if (n > begin) {
+            // a -= sum_k low[k][r] prevc[k][c]; rhs_r -= sum_k low[k][r] prevd[k]
+            double sum = 0.0, sumd = 0.0;
 #pragma unroll
-            for (int r = 0; r < 6; ++r) {
+            for (int k = 0; k < 6; ++k) {
+                const double lkr = __shfl(low, 6 * k + r, kWave);
+                sum += lkr * __shfl(prevc, 6 * k + c, kWave);
+                const double lkl = __shfl(low, 6 * k + (l < 6 ? l : 0), kWave);
+                sumd += lkl * __shfl(prevd, k, kWave);
+            }
+            a -= sum;
+            rhs -= sumd;
+        }
+        // Gauss-Jordan inverse of the entry-distributed block a.
+        double inv = (entry && r == c) ? 1.0 : 0.0;
+        double scale = 0.0;
 #pragma unroll
-                for (int c = 0; c < 6; ++c) {
-                    double sum = 0.0;
+        for (int i = 0; i < 6; ++i) scale = fmax(scale, fabs(read_lane(a, 7 * i)));
+        const double shift = fmax(1.0e-12 * scale, 1.0e-24);
+        if (entry && r == c) a += shift;
+        bool ok = true;
 #pragma unroll
-                    for (int k = 0; k < 6; ++k) sum += low[6 * k + r] * prevc[6 * k + c];
-                    b[6 * r + c] -= sum;
+        for (int col = 0; col < 6; ++col) {
+            double cv[6];
+#pragma unroll
+            for (int i = 0; i < 6; ++i) cv[i] = read_lane(a, 6 * i + col);
+            int pivot = col;
+            double best = fabs(cv[col]);
+#pragma unroll
+            for (int row = col + 1; row < 6; ++row) {
+                const double v = fabs(cv[row]);
+                if (v > best) { best = v; pivot = row; }
+            }
+            if (best < 1.0e-30 || !isfinite(best)) { ok = false; break; }
+            if (pivot != col) {
+                const int src = r == col ? 6 * pivot + c : (r == pivot ? 6 * col + c : l);
+                a = __shfl(a, src, kWave);
+                inv = __shfl(inv, src, kWave);
+#pragma unroll
+                for (int i = col + 1; i < 6; ++i) {
+                    const bool sw = i == pivot;
+                    const double x = cv[col], y = cv[i];
+                    cv[col] = sw ? y : x;
+                    cv[i] = sw ? x : y;
                 }
-                double sum = 0.0;
+            }
+            const double invp = 1.0 / cv[col];
+            if (r == col) {
+                a *= invp;
+                inv *= invp;
+            }
+            // f = a[r][col] before this step's elimination
+            double f = cv[0];
 #pragma unroll
-                for (int k = 0; k < 6; ++k) sum += low[6 * k + r] * prevd[k];
-                rhs[r] -= sum;
+            for (int i = 1; i < 6; ++i) f = r == i ? cv[i] : f;
+            const double pa = __shfl(a, 6 * col + c, kWave), pi = __shfl(inv, 6 * col + c, kWave);
+            if (r != col) {
+                a -= f * pa;
+                inv -= f * pi;
             }
         }
-        if (!inverse6(b, inv)) {
-            atomicOr(failed, 1u);
+        if (!ok) {
+            if (l == 0) atomicOr(failed, 1u);
             return;
         }
-        double *cc = Cp + 36 * size_t(n);
+        // C' = inv U (zero on the last node), d' = inv rhs
+        double cnew = 0.0, dnew = 0.0;
         if (n + 1 < end) {
 #pragma unroll
-            for (int i = 0; i < 36; ++i) low[i] = Ublk[36 * size_t(n) + i];
-#pragma unroll
-            for (int r = 0; r < 6; ++r)
-#pragma unroll
-                for (int c = 0; c < 6; ++c) {
-                    double sum = 0.0;
-#pragma unroll
-                    for (int k = 0; k < 6; ++k) sum += inv[6 * r + k] * low[6 * k + c];
-                    prevc[6 * r + c] = sum;
-                }
-        } else {
-#pragma unroll
-            for (int i = 0; i < 36; ++i) prevc[i] = 0.0;
+            for (int k = 0; k < 6; ++k) cnew += __shfl(inv, 6 * r + k, kWave) * __shfl(u, 6 * k + c, kWave);
         }
 #pragma unroll
-        for (int i = 0; i < 36; ++i) cc[i] = prevc[i];
-#pragma unroll
-        for (int r = 0; r < 6; ++r) {
-            double sum = 0.0;
-#pragma unroll
-            for (int k = 0; k < 6; ++k) sum += inv[6 * r + k] * rhs[k];
-            prevd[r] = sum;
-            dp[6 * size_t(n) + r] = sum;
-        }
+        for (int k = 0; k < 6; ++k)
+            dnew += __shfl(inv, 6 * (l < 6 ? l : 0) + k, kWave) * __shfl(rhs, k, kWave);
+        if (entry) Cp[36 * size_t(n) + l] = cnew;
+        if (l < 6) dp[6 * size_t(n) + l] = dnew;
+        low = u;
+        prevc = cnew;
+        prevd = dnew;
     }
-    double next[6] = {0, 0, 0, 0, 0, 0};
+    // Back substitution, lane r < 6 owning entry r; the node below's C' row
+    // and d' are loaded one step ahead.
+    double next = 0.0;
+    double cn[6], dn;
+    {
+        const uint32_t n = end - 1;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) cn[k] = l < 6 ? Cp[36 * size_t(n) + 6 * l + k] : 0.0;
+        dn = l < 6 ? dp[6 * size_t(n) + l] : 0.0;
+    }
     for (uint32_t n = end; n-- > begin;) {
-        double cur[6], cn[36], dn[6];
-        const double *cc = Cp + 36 * size_t(n);
+        double row[6];
 #pragma unroll
-        for (int i = 0; i < 36; ++i) cn[i] = cc[i];
+        for (int k = 0; k < 6; ++k) row[k] = cn[k];
+        const double d0 = dn;
+        if (n > begin) {
 #pragma unroll
-        for (int i = 0; i < 6; ++i) dn[i] = dp[6 * size_t(n) + i];
-#pragma unroll
-        for (int r = 0; r < 6; ++r) {
-            double value = dn[r];
-            if (n + 1 < end) {
-#pragma unroll
-                for (int k = 0; k < 6; ++k) value -= cn[6 * r + k] * next[k];
-            }
-            cur[r] = value;
+            for (int k = 0; k < 6; ++k) cn[k] = l < 6 ? Cp[36 * size_t(n - 1) + 6 * l + k] : 0.0;
+            dn = l < 6 ? dp[6 * size_t(n - 1) + l] : 0.0;
         }
+        double value = d0;
+        if (n + 1 < end) {
 #pragma unroll
-        for (int r = 0; r < 6; ++r) {
-            dir[6 * size_t(n) + r] = cur[r];
-            next[r] = cur[r];
+            for (int k = 0; k < 6; ++k) value -= row[k] * __shfl(next, k, kWave);
         }
+        if (l < 6) dir[6 * size_t(n) + l] = value;
+        next = value;
     }
 }
 
@@ -898,7 +959,7 @@ __global__ __launch_bounds__(kBlock) void k_ccd_slope(RodView m, ColliderView co
             const double speed = fmax(norm(d0), norm(d1));
             double limit = 1.0;
             if (speed >= 1e-30)
-                bvh_query(col.bvh, internal_count(col), lo, hi, p.sweep_target, [&](uint32_t t) {
+                bvh_query(col.bvh, internal_count(col), col.leaf_box, lo, hi, p.sweep_target, [&](uint32_t t) {
                     const uint32_t *t3 = col.triangles + 3 * size_t(t);
                     limit = fmin(limit, advancement_limit(p0, p1, d0, d1, speed, vertex(col, t3[0]), vertex(col, t3[1]),
                                                           vertex(col, t3[2]), p.sweep_target));
@@ -922,7 +983,7 @@ __global__ void k_sweep(RodView m, ColliderView col, StepParams p, const double 
     const V3 p0 = load3(q + 6 * size_t(L)), p1 = load3(q + 6 * size_t(L + 1));
     const uint32_t element = L - m.strand_of[L];
     unsigned long long worst = ~0ull;
-    bvh_query(col.bvh, internal_count(col), vmin(p0, p1), vmax(p0, p1), p.sweep_target, [&](uint32_t t) {
+    bvh_query(col.bvh, internal_count(col), col.leaf_box, vmin(p0, p1), vmax(p0, p1), p.sweep_target, [&](uint32_t t) {
         const uint32_t *t3 = col.triangles + 3 * size_t(t);
         const V3 oa = vertex_prev(col, t3[0]), ob = vertex_prev(col, t3[1]), oc = vertex_prev(col, t3[2]);
         const V3 da = sub(vertex(col, t3[0]), oa), db = sub(vertex(col, t3[1]), ob), dc = sub(vertex(col, t3[2]), oc);
@@ -1104,7 +1165,7 @@ __global__ void k_point_inside(RodView m, ColliderView col, const double *pos, d
     const double reach = fmax(x_max - pt.x, 0.0) + 1.0e-6;
     const V3 lo = pt, hi = add(pt, mul(dirv, reach));
     unsigned hits = 0;
-    bvh_query(col.bvh, internal_count(col), vmin(lo, hi), vmax(lo, hi), 1.0e-9, [&](uint32_t t) {
+    bvh_query(col.bvh, internal_count(col), col.leaf_box, vmin(lo, hi), vmax(lo, hi), 1.0e-9, [&](uint32_t t) {
         const uint32_t *t3 = col.triangles + 3 * size_t(t);
         const V3 a = vertex(col, t3[0]), b = vertex(col, t3[1]), c = vertex(col, t3[2]);
         const V3 e1 = sub(b, a), e2 = sub(c, a);
@@ -1134,7 +1195,7 @@ __global__ __launch_bounds__(kBlock) void k_initial_gap(RodView m, ColliderView 
         const uint32_t fl = m.flags[L];
         if ((fl & kNodeHasElement) && (fl & kNodeContact) && !(fl & kNodeFixed)) {
             const V3 p0 = load3(pos + 3 * size_t(L)), p1 = load3(pos + 3 * size_t(L + 1));
-            bvh_query(col.bvh, internal_count(col), vmin(p0, p1), vmax(p0, p1), p.contact_pad, [&](uint32_t t) {
+            bvh_query(col.bvh, internal_count(col), col.leaf_box, vmin(p0, p1), vmax(p0, p1), p.contact_pad, [&](uint32_t t) {
                 const uint32_t *t3 = col.triangles + 3 * size_t(t);
                 const Closest c = closest_segment_triangle(p0, p1, vertex(col, t3[0]), vertex(col, t3[1]),
                                                            vertex(col, t3[2]), true);
@@ -1156,7 +1217,7 @@ __global__ void k_broadphase_pairs(RodView m, ColliderView col, StepParams p, co
     if (!(fl & kNodeHasElement) || !(fl & kNodeContact) || (fl & kNodeFixed)) return;
     const V3 p0 = load3(q + 6 * size_t(L)), p1 = load3(q + 6 * size_t(L + 1));
     const uint32_t element = L - m.strand_of[L];
-    bvh_query(col.bvh, internal_count(col), vmin(p0, p1), vmax(p0, p1), p.contact_pad, [&](uint32_t t) {
+    bvh_query(col.bvh, internal_count(col), col.leaf_box, vmin(p0, p1), vmax(p0, p1), p.contact_pad, [&](uint32_t t) {
         const unsigned long long slot = atomicAdd(count, 1ull);
         if (slot < capacity) {
             pairs[2 * slot] = element;

@@ -38,7 +38,7 @@ int main()
     std::vector<double> queries(7 * size_t(Q));
     for (auto &x : queries) if (std::scanf("%lf", &x) != 1) return 1;
 
-    ColliderView col{V, T, verts.data(), verts.data(), tris.data(), nullptr};
+    ColliderView col{V, T, verts.data(), verts.data(), tris.data(), nullptr, nullptr};
     std::vector<double> tri_box(6 * size_t(T)), bounds(6);
     for (uint32_t t = 0; t < T; ++t) {  // boxes and centroid bounds on the host
         double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -83,22 +83,43 @@ int main()
         launch(T, [&] { k_check_bvh(T, keys_sorted.data(), ids_sorted.data(), tri_box.data(), parent_internal.data(), parent_leaf.data(), nodes.data(), v); });
         for (int k = 0; k < 6; ++k) worst[k] = std::max(worst[k], v[k]);
     }
-    unsigned long long mismatched = 0, total = 0;
+    unsigned long long mismatched = 0, total = 0, leaf_mismatched = 0;
     const int internal = T > 1 ? int(T) - 1 : 1;
+    std::vector<std::vector<uint32_t>> want_all(Q);
     for (uint32_t q = 0; q < Q; ++q) {
         const double *x = &queries[7 * size_t(q)];
         const V3 lo = mk(x[0], x[1], x[2]), hi = mk(x[3], x[4], x[5]);
-        std::vector<uint32_t> got, want;
-        bvh_query(nodes.data(), internal, lo, hi, x[6], [&](uint32_t t) { got.push_back(t); });
+        std::vector<uint32_t> got, &want = want_all[q];
+        bvh_query(nodes.data(), internal, nullptr, lo, hi, x[6], [&](uint32_t t) { got.push_back(t); });
         for (uint32_t t = 0; t < T; ++t)
             if (overlap(lo, hi, load3(&tri_box[6 * t]), load3(&tri_box[6 * t + 3]), x[6])) want.push_back(t);
         std::sort(got.begin(), got.end());
         total += want.size();
         if (got != want) ++mismatched;
     }
+    // The same tree refit with boxes grown by random amounts (as a refit
+    // swept over a collider motion is), queried with the ungrown boxes as
+    // leaf boxes, must report exactly the ungrown overlaps.
+    std::vector<double> grown(tri_box);
+    std::uniform_real_distribution<double> grow(0.0, 0.02);
+    for (uint32_t t = 0; t < T; ++t)
+        for (int k = 0; k < 3; ++k) {
+            grown[6 * t + k] -= grow(rng);
+            grown[6 * t + 3 + k] += grow(rng);
+        }
+    std::fill(counters.begin(), counters.end(), 0u);
+    launch(T, [&] { k_refit(T, ids_sorted.data(), grown.data(), parent_internal.data(), parent_leaf.data(), nodes.data(), counters.data()); });
+    for (uint32_t q = 0; q < Q; ++q) {
+        const double *x = &queries[7 * size_t(q)];
+        const V3 lo = mk(x[0], x[1], x[2]), hi = mk(x[3], x[4], x[5]);
+        std::vector<uint32_t> got;
+        bvh_query(nodes.data(), internal, tri_box.data(), lo, hi, x[6], [&](uint32_t t) { got.push_back(t); });
+        std::sort(got.begin(), got.end());
+        if (got != want_all[q]) ++leaf_mismatched;
+    }
     std::printf("{\"box_violations\": %llu, \"chain_violations\": %llu, \"order_violations\": %llu, "
                 "\"leaf_box_mismatches\": %llu, \"inverted_boxes\": %llu, \"depth\": %llu, "
-                "\"query_mismatches\": %llu, \"queries\": %u, \"pairs\": %llu}\n",
-                worst[0], worst[1], worst[2], worst[3], worst[4], worst[5], mismatched, Q, total);
+                "\"query_mismatches\": %llu, \"leaf_box_query_mismatches\": %llu, \"queries\": %u, \"pairs\": %llu}\n",
+                worst[0], worst[1], worst[2], worst[3], worst[4], worst[5], mismatched, leaf_mismatched, Q, total);
     return 0;
 }

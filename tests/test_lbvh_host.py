@@ -1,7 +1,8 @@
 """CPU run of the HIP library's LBVH construction kernels (lbvh_build.hpp
 compiled for the host, threads executed one by one with leaves arriving in
 three orders): the self-check must be clean and box queries must return
-exactly the brute-force overlap set."""
+exactly the brute-force overlap set, also on a tree refit with grown boxes
+when the leaves are tested against the ungrown ones."""
 import json
 import os
 import subprocess
@@ -42,6 +43,7 @@ def test_lbvh_kernels_on_icosphere():
     assert out["box_violations"] == out["chain_violations"] == out["order_violations"] == 0
     assert out["leaf_box_mismatches"] == out["inverted_boxes"] == 0
     assert out["query_mismatches"] == 0 and out["pairs"] > 1000
+    assert out["leaf_box_query_mismatches"] == 0
 
 
 def test_lbvh_kernels_on_duplicate_codes():
@@ -52,3 +54,4 @@ def test_lbvh_kernels_on_duplicate_codes():
     out = _run(v, t, _queries(500, 1, 1.0, 0.2, 1e-3))
     assert out["box_violations"] == out["chain_violations"] == out["order_violations"] == 0
     assert out["query_mismatches"] == 0
+    assert out["leaf_box_query_mismatches"] == 0
