@@ -22,15 +22,16 @@ namespace dev {
 
 constexpr int kBvhStack = 64;
 
+// The Karras hierarchy over 30-bit codes with index tie-break has depth
+// below 64, so the traversal stack never overflows.
+
 // Calls f(triangle) for every triangle whose box overlaps [lo, hi] grown by
-// pad. The Karras hierarchy over 30-bit codes with index tie-break has
-// depth below 64, so the stack never overflows.
-// With leaf_box (6 doubles per triangle) a leaf is tested against its own
-// entry there instead of the box stored in the tree, which must contain it:
-// a tree refit with boxes swept over a collider motion then answers queries
-// on the collider's current position exactly.
+// pad, until f returns false. With leaf_box (6 doubles per triangle) a leaf
+// is tested against its own entry there instead of the box stored in the
+// tree, which must contain it: a tree refit with boxes swept over a collider
+// motion then answers queries on the collider's current position exactly.
 template <class F>
-__device__ inline void bvh_query(const BvhNode *nodes, int internal_count, const double *leaf_box, V3 lo, V3 hi,
+__device__ inline void bvh_visit(const BvhNode *nodes, int internal_count, const double *leaf_box, V3 lo, V3 hi,
                                  double pad, F &&f)
 {
     int stack[kBvhStack];
@@ -53,7 +54,7 @@ __device__ inline void bvh_query(const BvhNode *nodes, int internal_count, const
                     const double *b = leaf_box + 6 * size_t(t);
                     if (!overlap(lo, hi, mk(b[0], b[1], b[2]), mk(b[3], b[4], b[5]), pad)) continue;
                 }
-                f(t);
+                if (!f(t)) return;
             } else if (next < 0) {
                 next = ch;
             } else if (top < kBvhStack) {
@@ -66,6 +67,17 @@ __device__ inline void bvh_query(const BvhNode *nodes, int internal_count, const
         }
         cur = next;
     }
+}
+
+// Calls f(triangle) for every overlapping triangle (see bvh_visit).
+template <class F>
+__device__ inline void bvh_query(const BvhNode *nodes, int internal_count, const double *leaf_box, V3 lo, V3 hi,
+                                 double pad, F &&f)
+{
+    bvh_visit(nodes, internal_count, leaf_box, lo, hi, pad, [&](uint32_t t) {
+        f(t);
+        return true;
+    });
 }
 
 }  // namespace dev

@@ -211,8 +211,8 @@ struct DeviceState {
     }
     ContactCache cache() const
     {
-        return ContactCache{cache_skin,        cache_count.get(), cache_tris.get(),       cache_box.get(),
-                            cache_listed.get(), cache_list.get(), cache_list_count.get(), tri_box_static.get()};
+        return ContactCache{cache_skin,         cache_count.get(), cache_tris.get(),       cache_box.get(),
+                            cache_listed.get(), cache_list.get(),  cache_list_count.get(), tri_box_static.get()};
     }
 
     size_t resident_bytes() const
@@ -941,7 +941,8 @@ bool Solver::make_direction()
     const RodView rv = d.rods();
     hipLaunchKernelGGL(dev::k_fill_u32, dim3(1), dim3(64), 0, d.stream, d.flag.get(), size_t(1), 0u);
     hipEvent_t t0 = d.tic();
-    hipLaunchKernelGGL(dev::k_block_solve, dim3(d.S), dim3(kWave), 0, d.stream, rv, d.solve_order.get(), d.Dblk.get(),
+    hipLaunchKernelGGL(dev::k_block_solve, dim3(grid_for(d.S, dev::kSolveGroups)), dim3(kWave), 0, d.stream, rv,
+                       d.solve_order.get(), d.Dblk.get(),
                        d.Ublk.get(), d.grad.get(), d.Cp.get(), d.dp.get(), d.dir.get(), d.flag.get());
     d.toc(t0, d.totals.solve_ms, d.totals.solve_launches);
     hipLaunchKernelGGL(dev::k_direction_stats, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, d.grad.get(),
@@ -1520,7 +1521,8 @@ KhsResult Solver::test_evaluate(double dt, double *gradient, double *direction, 
     if (direction) {
         const RodView rv = d.rods();
         hipLaunchKernelGGL(dev::k_fill_u32, dim3(1), dim3(64), 0, d.stream, d.flag.get(), size_t(1), 0u);
-        hipLaunchKernelGGL(dev::k_block_solve, dim3(d.S), dim3(kWave), 0, d.stream, rv, d.solve_order.get(), d.Dblk.get(),
+        hipLaunchKernelGGL(dev::k_block_solve, dim3(grid_for(d.S, dev::kSolveGroups)), dim3(kWave), 0, d.stream, rv,
+                       d.solve_order.get(), d.Dblk.get(),
                            d.Ublk.get(), d.grad.get(), d.Cp.get(), d.dp.get(), d.dir.get(), d.flag.get());
         HIP_TRY(hipGetLastError());
         unsigned failed = 0;

@@ -219,11 +219,12 @@ __global__ void k_contact_refresh(RodView m, ColliderView col, ContactCache cc, 
     if (!full && box_covered(box, lo, hi, pad)) return;
     const double grow = pad + cc.skin;
     const V3 blo = mk(lo.x - grow, lo.y - grow, lo.z - grow), bhi = mk(hi.x + grow, hi.y + grow, hi.z + grow);
-    uint32_t n = 0;
     uint32_t *tris = cc.tris + size_t(L) * kCacheCapacity;
-    bvh_query(col.bvh, internal_count(col), col.leaf_box, blo, bhi, 0.0, [&](uint32_t t) {
+    uint32_t n = 0;
+    bvh_visit(col.bvh, internal_count(col), col.leaf_box, blo, bhi, 0.0, [&](uint32_t t) {
         if (n < kCacheCapacity) tris[n] = t;
         ++n;
+        return n <= kCacheCapacity;  // an overflowed list is not used
     });
     cc.count[L] = n <= kCacheCapacity ? n : kCacheOverflow;
     store3(box, blo);
@@ -652,157 +653,219 @@ __device__ inline bool inverse6(double *a, double *inv)
     return true;
 }
 
-// Value of `v` in lane `lane`, a wave-uniform index.
-__device__ __forceinline__ double read_lane(double v, int lane)
-{
-    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-    return __hiloint2double(hi, lo);
-}
-
 // Block Thomas per strand on the free nodes; fixed root nodes are
 // eliminated and get a zero direction (cuda_backend.cu:866-881).
 //
-// One wavefront per strand. Lane l < 36 owns entry (r, c) = (l / 6, l % 6)
-// of every 6x6 block and lane r < 6 entry r of every 6-vector; products
-// gather their operands with lane shuffles and sum k = 0..5 from zero, and
-// the Gauss-Jordan inverse (diagonal shift max(1e-12 max|a_ii|, 1e-24),
-// partial pivoting, pivot < 1e-30 fails, cuda_backend.cu:855-864) updates
-// all entries of a column step at once with the sequential algorithm's
-// operations, so every value is rounded as in a one-thread solve. The next
-// node's blocks are loaded while the current one is factored.
-__global__ __launch_bounds__(kWave, 7) void k_block_solve(RodView m, const uint32_t *order, const double *Dblk,
-                                                          const double *Ublk, const double *grad, double *Cp,
-                                                          double *dp, double *dir, unsigned *failed)
-{
-    if (blockIdx.x >= m.strand_count) return;
-    const uint32_t s = order[blockIdx.x];
-    const int l = threadIdx.x;
-    const bool entry = l < 36;
-    const int r = entry ? l / 6 : 0, c = entry ? l % 6 : 0;
-    const uint32_t first = m.strand_first[s], begin = first + m.strand_fixed[s], end = first + m.strand_count_of[s];
-    for (uint32_t n = first; n < begin; ++n)
-        if (l < 6) dir[6 * size_t(n) + l] = 0.0;
-    if (begin >= end) return;
+// A group of 8 lanes solves one strand, lane r < 6 owning row r of every 6x6
+// block and entry r of every 6-vector; a wavefront holds 8 strands of
+// similar length (the launch order sorts them) and runs the longest one's
+// node count, the others idling. Rows travel between the group's lanes
+// through LDS; the block is one wavefront, so its barriers are cheap and
+// only order the lanes' LDS accesses. Each entry is computed with the
+// sequential algorithm's
+// operations in its order: products sum k = 0..5 from zero, and the
+// Gauss-Jordan inverse (diagonal shift max(1e-12 max|a_ii|, 1e-24), partial
+// pivoting, pivot below 1e-30 fails; cuda_backend.cu:855-864) swaps, scales
+// and eliminates whole rows per column, so the result is bit-identical to a
+// one-thread solve.
+constexpr int kSolveLanes = 8;
+constexpr int kSolveGroups = kWave / kSolveLanes;
 
-    double low = 0.0, prevc = 0.0, prevd = 0.0;  // U of the previous node, its C', d'
-    double nd = entry ? Dblk[36 * size_t(begin) + l] : 0.0;
-    double ng = l < 6 ? grad[6 * size_t(begin) + l] : 0.0;
-    double nu = (entry && begin + 1 < end) ? Ublk[36 * size_t(begin) + l] : 0.0;
-    for (uint32_t n = begin; n < end; ++n) {
-        double a = nd, rhs = -ng;
-        const double u = nu;
-        if (n + 1 < end) {
-            nd = entry ? Dblk[36 * size_t(n + 1) + l] : 0.0;
-            ng = l < 6 ? grad[6 * size_t(n + 1) + l] : 0.0;
-            nu = (entry && n + 2 < end) ? Ublk[36 * size_t(n + 1) + l] : 0.0;
-        }
-        if (n > begin) {
-            // a -= sum_k low[k][r] prevc[k][c]; rhs_r -= sum_k low[k][r] prevd[k]
-            double sum = 0.0, sumd = 0.0;
+__global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t *order, const double *Dblk,
+                                                       const double *Ublk, const double *grad, double *Cp,
+                                                       double *dp, double *dir, unsigned *failed)
+{
+    __shared__ double rows_s[kSolveGroups][6][12];  // a | inv rows
+    __shared__ double c_s[kSolveGroups][36];        // C' of the previous node
+    __shared__ double u_s[kSolveGroups][36];        // U of the previous, then the current node
+    __shared__ double v_s[kSolveGroups][6];         // d' of the previous node, then rhs
+    const int g = threadIdx.x / kSolveLanes, lane_r = threadIdx.x % kSolveLanes;
+    const bool row = lane_r < 6;
+    const int r = row ? lane_r : 0;  // idle lanes read as row 0 and never write
+    const uint32_t gi = blockIdx.x * kSolveGroups + g;
+    uint32_t begin = 0, end = 0;
+    if (gi < m.strand_count) {
+        const uint32_t s = order[gi];
+        const uint32_t first = m.strand_first[s];
+        begin = first + m.strand_fixed[s];
+        end = first + m.strand_count_of[s];
+        if (row)
+            for (uint32_t n = first; n < begin; ++n) dir[6 * size_t(n) + r] = 0.0;
+        if (begin > end) begin = end;
+    }
+    const uint32_t len = end - begin;
+    uint32_t steps = len;
 #pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                const double lkr = __shfl(low, 6 * k + r, kWave);
-                sum += lkr * __shfl(prevc, 6 * k + c, kWave);
-                const double lkl = __shfl(low, 6 * k + (l < 6 ? l : 0), kWave);
-                sumd += lkl * __shfl(prevd, k, kWave);
-            }
-            a -= sum;
-            rhs -= sumd;
+    for (int o = kWave / 2; o >= 1; o >>= 1) steps = max(steps, uint32_t(__shfl_xor(int(steps), o, kWave)));
+    double (*rows)[12] = rows_s[g];
+    double *cb = c_s[g], *ub = u_s[g], *vb = v_s[g];
+    bool ok = true;
+
+    // next node's D row, U row and gradient entry, loaded one node ahead
+    double nd[6], nu[6], ng = 0.0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) nd[k] = nu[k] = 0.0;
+    if (len > 0) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) nd[k] = Dblk[36 * size_t(begin) + 6 * r + k];
+        if (len > 1)
+#pragma unroll
+            for (int k = 0; k < 6; ++k) nu[k] = Ublk[36 * size_t(begin) + 6 * r + k];
+        ng = grad[6 * size_t(begin) + r];
+    }
+    for (uint32_t it = 0; it < steps; ++it) {
+        const uint32_t n = begin + (it < len ? it : 0);
+        double a[6], inv[6], u[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            a[k] = nd[k];
+            u[k] = nu[k];
         }
-        // Gauss-Jordan inverse of the entry-distributed block a.
-        double inv = (entry && r == c) ? 1.0 : 0.0;
+        double rhs = -ng;
+        if (it + 1 < len) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) nd[k] = Dblk[36 * size_t(n + 1) + 6 * r + k];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) nu[k] = it + 2 < len ? Ublk[36 * size_t(n + 1) + 6 * r + k] : 0.0;
+            ng = grad[6 * size_t(n + 1) + r];
+        }
+        if (it > 0) {
+            // a[c] -= sum_k low[k][r] prevc[k][c]; rhs -= sum_k low[k][r] prevd[k]
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                double sum = 0.0;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) sum += ub[6 * k + r] * cb[6 * k + c];
+                a[c] -= sum;
+            }
+            double sum = 0.0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) sum += ub[6 * k + r] * vb[k];
+            rhs -= sum;
+        }
+        // Gauss-Jordan inverse, row r per lane.
+#pragma unroll
+        for (int k = 0; k < 6; ++k) inv[k] = k == r ? 1.0 : 0.0;
+        __syncthreads();
+        if (row)
+#pragma unroll
+            for (int k = 0; k < 6; ++k) rows[r][k] = a[k];
+        __syncthreads();
         double scale = 0.0;
 #pragma unroll
-        for (int i = 0; i < 6; ++i) scale = fmax(scale, fabs(read_lane(a, 7 * i)));
+        for (int i = 0; i < 6; ++i) scale = fmax(scale, fabs(rows[i][i]));
         const double shift = fmax(1.0e-12 * scale, 1.0e-24);
-        if (entry && r == c) a += shift;
-        bool ok = true;
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            if (k == r) a[k] += shift;
 #pragma unroll
         for (int col = 0; col < 6; ++col) {
+            __syncthreads();
+            if (row)
+#pragma unroll
+                for (int k = 0; k < 6; ++k) {
+                    rows[r][k] = a[k];
+                    rows[r][6 + k] = inv[k];
+                }
+            __syncthreads();
             double cv[6];
 #pragma unroll
-            for (int i = 0; i < 6; ++i) cv[i] = read_lane(a, 6 * i + col);
+            for (int i = 0; i < 6; ++i) cv[i] = rows[i][col];
             int pivot = col;
             double best = fabs(cv[col]);
 #pragma unroll
-            for (int row = col + 1; row < 6; ++row) {
-                const double v = fabs(cv[row]);
-                if (v > best) { best = v; pivot = row; }
+            for (int i = col + 1; i < 6; ++i) {
+                const double v = fabs(cv[i]);
+                if (v > best) { best = v; pivot = i; }
             }
-            if (best < 1.0e-30 || !isfinite(best)) { ok = false; break; }
-            if (pivot != col) {
-                const int src = r == col ? 6 * pivot + c : (r == pivot ? 6 * col + c : l);
-                a = __shfl(a, src, kWave);
-                inv = __shfl(inv, src, kWave);
+            if (best < 1.0e-30 || !isfinite(best)) {
+                if (it < len) ok = false;
+                pivot = col;
+            }
+            // this lane's row after the exchange of rows col and pivot, and
+            // the pivot row (row col after it), scaled
+            const int src = r == col ? pivot : (r == pivot ? col : r);
+            double mine[12], prow[12];
 #pragma unroll
-                for (int i = col + 1; i < 6; ++i) {
-                    const bool sw = i == pivot;
-                    const double x = cv[col], y = cv[i];
-                    cv[col] = sw ? y : x;
-                    cv[i] = sw ? x : y;
-                }
+            for (int k = 0; k < 12; ++k) {
+                mine[k] = rows[src][k];
+                prow[k] = rows[pivot][k];
             }
-            const double invp = 1.0 / cv[col];
-            if (r == col) {
-                a *= invp;
-                inv *= invp;
-            }
-            // f = a[r][col] before this step's elimination
-            double f = cv[0];
+            const double invp = 1.0 / prow[col];
 #pragma unroll
-            for (int i = 1; i < 6; ++i) f = r == i ? cv[i] : f;
-            const double pa = __shfl(a, 6 * col + c, kWave), pi = __shfl(inv, 6 * col + c, kWave);
-            if (r != col) {
-                a -= f * pa;
-                inv -= f * pi;
+            for (int k = 0; k < 12; ++k) prow[k] *= invp;
+            const double f = mine[col];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                a[k] = r == col ? prow[k] : mine[k] - f * prow[k];
+                inv[k] = r == col ? prow[6 + k] : mine[6 + k] - f * prow[6 + k];
             }
         }
-        if (!ok) {
-            if (l == 0) atomicOr(failed, 1u);
-            return;
-        }
-        // C' = inv U (zero on the last node), d' = inv rhs
-        double cnew = 0.0, dnew = 0.0;
-        if (n + 1 < end) {
+        // C' = inv U (zero on the last node), d' = inv rhs; U and rhs pass
+        // through LDS after the products above have read the previous ones.
+        __syncthreads();
+        if (row) {
 #pragma unroll
-            for (int k = 0; k < 6; ++k) cnew += __shfl(inv, 6 * r + k, kWave) * __shfl(u, 6 * k + c, kWave);
+            for (int k = 0; k < 6; ++k) ub[6 * r + k] = u[k];
+            vb[r] = rhs;
         }
+        __syncthreads();
+        double cnew[6];
 #pragma unroll
-        for (int k = 0; k < 6; ++k)
-            dnew += __shfl(inv, 6 * (l < 6 ? l : 0) + k, kWave) * __shfl(rhs, k, kWave);
-        if (entry) Cp[36 * size_t(n) + l] = cnew;
-        if (l < 6) dp[6 * size_t(n) + l] = dnew;
-        low = u;
-        prevc = cnew;
-        prevd = dnew;
+        for (int c = 0; c < 6; ++c) {
+            double sum = 0.0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) sum += inv[k] * ub[6 * k + c];
+            cnew[c] = it + 1 < len ? sum : 0.0;
+        }
+        double dnew = 0.0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) dnew += inv[k] * vb[k];
+        __syncthreads();
+        if (row) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) cb[6 * r + c] = cnew[c];
+            vb[r] = dnew;
+        }
+        __syncthreads();
+        if (it < len && ok && row) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) Cp[36 * size_t(n) + 6 * r + c] = cnew[c];
+            dp[6 * size_t(n) + r] = dnew;
+        }
     }
-    // Back substitution, lane r < 6 owning entry r; the node below's C' row
-    // and d' are loaded one step ahead.
-    double next = 0.0;
-    double cn[6], dn;
-    {
-        const uint32_t n = end - 1;
-#pragma unroll
-        for (int k = 0; k < 6; ++k) cn[k] = l < 6 ? Cp[36 * size_t(n) + 6 * l + k] : 0.0;
-        dn = l < 6 ? dp[6 * size_t(n) + l] : 0.0;
+    if (!ok) {
+        if (lane_r == 0) atomicOr(failed, 1u);
+        return;
     }
-    for (uint32_t n = end; n-- > begin;) {
-        double row[6];
+    // Back substitution: x_n = d'_n - C'_n x_{n+1}, the node below loaded
+    // one step ahead.
+    double next = 0.0, cn[6], dn = 0.0;
 #pragma unroll
-        for (int k = 0; k < 6; ++k) row[k] = cn[k];
+    for (int k = 0; k < 6; ++k) cn[k] = 0.0;
+    if (len > 0) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) cn[k] = Cp[36 * size_t(end - 1) + 6 * r + k];
+        dn = dp[6 * size_t(end - 1) + r];
+    }
+    for (uint32_t it = 0; it < steps; ++it) {
+        const bool live = it < len;
+        const uint32_t n = live ? end - 1 - it : begin;
+        double crow[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) crow[k] = cn[k];
         const double d0 = dn;
-        if (n > begin) {
+        if (it + 1 < len) {
 #pragma unroll
-            for (int k = 0; k < 6; ++k) cn[k] = l < 6 ? Cp[36 * size_t(n - 1) + 6 * l + k] : 0.0;
-            dn = l < 6 ? dp[6 * size_t(n - 1) + l] : 0.0;
+            for (int k = 0; k < 6; ++k) cn[k] = Cp[36 * size_t(n - 1) + 6 * r + k];
+            dn = dp[6 * size_t(n - 1) + r];
         }
         double value = d0;
-        if (n + 1 < end) {
+        if (it > 0) {
 #pragma unroll
-            for (int k = 0; k < 6; ++k) value -= row[k] * __shfl(next, k, kWave);
+            for (int k = 0; k < 6; ++k) value -= crow[k] * __shfl(next, k, kSolveLanes);
         }
-        if (l < 6) dir[6 * size_t(n) + l] = value;
+        if (live && row) dir[6 * size_t(n) + r] = value;
         next = value;
     }
 }
