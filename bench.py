@@ -9,9 +9,10 @@ A step is one animation frame; inputs (every frame's root targets and
 collider) are resident in HBM before the timed region.
 
 Prints one JSON line: node-substeps/s of the whole job, p50/p99 substep
-latency, the roofline of the dominant kernel (assembly of gradient and
-Gauss-Newton blocks) from HIP events recorded on the library's stream, and
-the CPU oracle timed on a bounded sample of the same workload.
+latency, the roofline of the dominant kernel class (by device time, from HIP
+events recorded on the library's stream), the algorithmic-byte roofline of
+the whole substep, and the CPU oracle timed on a bounded sample of the same
+workload.
 """
 import argparse
 import json
@@ -30,10 +31,26 @@ import khs_native as kn  # noqa: E402
 
 LIFTED = dict(relative_tolerance=1e-6, absolute_tolerance=1e-30, increment_tolerance=1e-14)
 HBM_PEAK_GBPS = 8000.0
-# Algorithmic bytes of one assembly launch per node: read q, q_old, v
-# (144 B), node constants R_ref, m, m_rot (88 B) and element constants
-# L0, rest shear, rest curvature (56 B); write the gradient (48 B).
-ASSEMBLY_BYTES_PER_NODE = 336
+# Algorithmic bytes per node of one launch of each kernel class. Assembly
+# and a line-search energy evaluation read q (or q_base and d), q_old, v
+# (144 B), node constants R_ref, m, m_rot (88 B) and element constants L0,
+# rest shear, rest curvature (56 B) and write 48 B (gradient, or the trial
+# state). The block solve's only algorithmic traffic is the direction it
+# writes (48 B): the Gauss-Newton blocks are intermediates.
+KERNEL_CLASSES = {
+    "assembly": ("k_evaluate<gradient> with its contact list pass (gradient + Gauss-Newton blocks)", 336),
+    "energy": ("k_evaluate<line search> with its contact list pass (trial energy)", 336),
+    "solve": ("k_block_solve (block-tridiagonal Thomas per strand)", 48),
+}
+
+
+def substep_bytes(nodes, newton, line_search, vertices, triangles, pairs, substeps):
+    """Algorithmic HBM bytes of `substeps` substeps: per node 528 B of
+    substep begin/end and first assembly, 480 B per Newton iteration, 336 B
+    per line-search trial; 72 B per collider vertex (lerp), 132 B per
+    triangle (LBVH input), 84 B per candidate pair (indices and vertices)."""
+    return (nodes * (528 * substeps + 480 * newton + 336 * line_search)
+            + (72 * vertices + 132 * triangles) * substeps + 84 * pairs)
 
 
 def log(msg):
@@ -51,26 +68,49 @@ def setup_animation(lib, scene, frames, **desc):
     return s
 
 
-def cpu_baseline(scene, strands, threads):
+def cpu_baseline(scene, strands, frames, threads):
     oracle_path = os.path.join(REPO, "oracle", "_build", "libkhs_oracle.so")
     lib = kn.load_library(oracle_path)
-    sub = grooms.strand_subset(scene, strands)
-    s = setup_animation(lib, sub, 2, **LIFTED)
+    sub = grooms.strand_subset(scene, strands) if strands < scene.strand_count else scene
+    s = setup_animation(lib, sub, frames + 1, **LIFTED)
     s.check(s.set_threads(threads))
+    substeps = 0
     t0 = time.perf_counter()
-    s.check(s.step_frame(1, scene.frame_dt))
+    for f in range(1, frames + 1):
+        s.check(s.step_frame(f, scene.frame_dt))
+        substeps += s.step_stats().substeps
     elapsed = time.perf_counter() - t0
-    st = s.step_stats()
     nodes = s.node_count()
     return {
-        "value": nodes * st.substeps / elapsed,
+        "value": nodes * substeps / elapsed,
         "unit": "node-substeps/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"first {strands} strands ({nodes} nodes) of the same groom and full 225,184-vertex body, "
-                  f"1 frame x {st.substeps} substeps to a 1e-6 residual, {elapsed:.1f} s on {threads} threads "
-                  f"({platform.processor() or platform.machine()})",
+        "sample": f"{sub.strand_count} strands ({nodes} nodes) of the same groom and the full 225,184-vertex body, "
+                  f"frames 1-{frames} ({substeps} substeps) to a 1e-6 residual, {elapsed:.1f} s on {threads} "
+                  f"threads ({platform.processor() or platform.machine()}, {os.cpu_count()} logical CPUs)",
     }
+
+
+def copy_calibration(mib):
+    """One device-to-device copy of `mib` MiB (reads and writes mib MiB),
+    the known kernel against which the PMC traffic counters are scaled. It
+    goes through the HIP runtime the library itself links."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so.7")  # the soname the library links
+    size = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(size)) != 0:
+            raise RuntimeError("hipMalloc failed for the calibration copy")
+    ok = hip.hipMemset(a, 1, ctypes.c_size_t(size)) == 0
+    ok = ok and hip.hipMemcpy(b, a, ctypes.c_size_t(size), 3) == 0  # hipMemcpyDeviceToDevice
+    ok = ok and hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    if not ok:
+        raise RuntimeError("calibration copy failed")
+    log(f"[bench] calibration copy of {mib} MiB done")
 
 
 def main():
@@ -78,9 +118,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=29, help="timed frames")
     ap.add_argument("--warmup", type=int, default=1, help="untimed frames before the timed ones")
-    ap.add_argument("--cpu-strands", type=int, default=4096)
+    ap.add_argument("--cpu-strands", type=int, default=6757, help="strands of the CPU sample (all by default)")
+    ap.add_argument("--cpu-frames", type=int, default=2, help="frames of the CPU sample")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="run one device copy of this size first (for scaling PMC traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {args.gpus}).", file=sys.stderr)
@@ -93,6 +136,8 @@ def main():
     if not os.path.exists(kn.GPU_LIBRARY):
         raise SystemExit("HIP library missing; run __graft_entry__.build()")
     lib = kn.load_library(kn.GPU_LIBRARY)
+    if args.copy_calibration_mib:
+        copy_calibration(args.copy_calibration_mib)
     frames = args.warmup + args.steps + 1
     t = time.perf_counter()
     s = setup_animation(lib, scene, frames, **LIFTED)
@@ -104,7 +149,7 @@ def main():
         log(f"[bench] warmup frame {f}: {s.gpu_stats().last_frame_milliseconds:.1f} ms, "
             f"substeps {st.substeps}, newton {st.newton_iterations}")
     s.check(s.set_kernel_timing(True))
-    substep_ms, total_substeps, newton, ls_evals = [], 0, 0, 0
+    substep_ms, total_substeps, newton, ls_evals, pairs = [], 0, 0, 0, 0
     min_gap, penetrations = float("inf"), 0
     t0 = time.perf_counter()
     for f in range(args.warmup + 1, args.warmup + args.steps + 1):
@@ -113,6 +158,7 @@ def main():
         total_substeps += st.substeps
         newton += st.newton_iterations
         ls_evals += st.line_search_evaluations
+        pairs += st.contact_candidate_count
         min_gap = min(min_gap, st.minimum_gap)
         if st.minimum_gap <= 1e-7:
             penetrations += 1
@@ -122,12 +168,18 @@ def main():
     log(f"[bench] timed {args.steps} frames in {elapsed:.3f} s; newton {newton}, line-search evals {ls_evals}, "
         f"min gap {min_gap:.3e}")
     value = nodes * total_substeps / elapsed
-    asm_avg_ms = timing.assembly_ms / max(timing.assembly_launches, 1)
-    achieved = ASSEMBLY_BYTES_PER_NODE * nodes / (asm_avg_ms * 1e-3) / 1e9 if timing.assembly_launches else None
+    totals = {k: (getattr(timing, k + "_ms"), getattr(timing, k + "_launches")) for k in KERNEL_CLASSES}
+    dominant = max(totals, key=lambda k: totals[k][0])
+    dom_ms, dom_launches = totals[dominant]
+    dom_avg_ms = dom_ms / max(dom_launches, 1)
+    kernel_name, bytes_per_node = KERNEL_CLASSES[dominant]
+    achieved = bytes_per_node * nodes / (dom_avg_ms * 1e-3) / 1e9 if dom_launches else None
     traffic = None
-    traffic_file = os.path.join(REPO, "profiles", "assembly_hbm_traffic.json")
+    traffic_file = os.path.join(REPO, "profiles", f"{dominant}_hbm_traffic.json")
     if os.path.exists(traffic_file):
         traffic = json.load(open(traffic_file)).get("bytes_per_launch")
+    n_vertices, n_triangles = int(scene.vertices.shape[0]), int(scene.triangles.shape[0])
+    step_bytes = substep_bytes(nodes, newton, ls_evals, n_vertices, n_triangles, pairs, total_substeps)
     result = {
         "metric": "node-substeps/s FP64 vs mixed at 1e-6 residual; p50/p99 substep latency",
         "value": value,
@@ -159,29 +211,41 @@ def main():
         "minimum_gap_m": min_gap,
         "penetrating_frames": penetrations,
         "kernel_ms": {
-            "assembly_avg": asm_avg_ms,
+            "assembly_avg": timing.assembly_ms / max(timing.assembly_launches, 1),
             "assembly_launches": timing.assembly_launches,
             "energy_avg": timing.energy_ms / max(timing.energy_launches, 1),
+            "energy_launches": timing.energy_launches,
             "solve_avg": timing.solve_ms / max(timing.solve_launches, 1),
-            "collision_total": timing.collision_ms,
+            "solve_launches": timing.solve_launches,
             "assembly_total": timing.assembly_ms,
             "energy_total": timing.energy_ms,
             "solve_total": timing.solve_ms,
+            "collision_total": timing.collision_ms,
         },
         "roofline": {
-            "kernel": "k_evaluate<gradient> (gradient + Gauss-Newton block assembly)",
+            "kernel": kernel_name,
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBPS) if achieved else None,
             "traffic": traffic,
+            "bytes_per_node": bytes_per_node,
+            "nodes_per_launch": nodes,
+            "avg_launch_ms": dom_avg_ms,
+        },
+        "substep_roofline": {
+            "algorithmic_bytes": step_bytes,
+            "achieved": step_bytes / elapsed / 1e9,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": step_bytes / elapsed / 1e9 / HBM_PEAK_GBPS,
         },
         "reference_derived_node_substeps_per_s": 3.16e5,
     }
     if not args.no_cpu_baseline:
         t = time.perf_counter()
-        result["cpu_baseline"] = cpu_baseline(scene, args.cpu_strands, args.cpu_threads)
+        result["cpu_baseline"] = cpu_baseline(scene, args.cpu_strands, args.cpu_frames, args.cpu_threads)
         log(f"[bench] cpu baseline in {time.perf_counter() - t:.1f} s")
     print(json.dumps(result), flush=True)
 
