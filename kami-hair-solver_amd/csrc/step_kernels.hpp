@@ -14,6 +14,9 @@
 #include "lbvh_build.hpp"
 #include "model.hpp"
 
+#include <type_traits>
+#include <utility>
+
 namespace khs {
 namespace dev {
 
@@ -653,35 +656,52 @@ __device__ inline bool inverse6(double *a, double *inv)
     return true;
 }
 
+// Value of `v` in lane L of this lane's 16-lane row (DPP row_newbcast).
+template <int L>
+__device__ __forceinline__ double row_bcast(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x150 + L, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x150 + L, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// f(std::integral_constant<int, 0>), ..., f(std::integral_constant<int, N - 1>).
+template <class F, int... Is>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, Is...>)
+{
+    (f(std::integral_constant<int, Is>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
 // Block Thomas per strand on the free nodes; fixed root nodes are
 // eliminated and get a zero direction (cuda_backend.cu:866-881).
 //
-// A group of 8 lanes solves one strand, lane r < 6 owning row r of every 6x6
-// block and entry r of every 6-vector; a wavefront holds 8 strands of
-// similar length (the launch order sorts them) and runs the longest one's
-// node count, the others idling. Rows travel between the group's lanes
-// through LDS; the block is one wavefront, so its barriers are cheap and
-// only order the lanes' LDS accesses. Each entry is computed with the
-// sequential algorithm's
-// operations in its order: products sum k = 0..5 from zero, and the
-// Gauss-Jordan inverse (diagonal shift max(1e-12 max|a_ii|, 1e-24), partial
-// pivoting, pivot below 1e-30 fails; cuda_backend.cu:855-864) swaps, scales
-// and eliminates whole rows per column, so the result is bit-identical to a
-// one-thread solve.
-constexpr int kSolveLanes = 8;
+// One 16-lane DPP row per strand, four strands per wavefront (strands
+// dispatched longest first, so a wavefront's strands have similar lengths
+// and it runs the longest one's node count). In the forward sweep lane
+// j < 6 holds column j of the block being inverted and of U, lanes 6..11
+// the columns of its inverse, lane 12 the right-hand side; every value a
+// lane needs from another comes by a row broadcast from a fixed lane, so a
+// pivot row exchange is a local select. Each entry is computed with the
+// sequential algorithm's operations in its order: products sum k = 0..5
+// from zero, and the Gauss-Jordan inverse (diagonal shift
+// max(1e-12 max|a_ii|, 1e-24), partial pivoting, pivot below 1e-30 fails;
+// cuda_backend.cu:855-864) applies each column step's row operations to all
+// columns at once, so the result is bit-identical to a one-thread solve.
+// The back substitution runs row-wise on lanes 0..5.
+constexpr int kSolveLanes = 16;
 constexpr int kSolveGroups = kWave / kSolveLanes;
 
 __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t *order, const double *Dblk,
                                                        const double *Ublk, const double *grad, double *Cp,
                                                        double *dp, double *dir, unsigned *failed)
 {
-    __shared__ double rows_s[kSolveGroups][6][12];  // a | inv rows
-    __shared__ double c_s[kSolveGroups][36];        // C' of the previous node
-    __shared__ double u_s[kSolveGroups][36];        // U of the previous, then the current node
-    __shared__ double v_s[kSolveGroups][6];         // d' of the previous node, then rhs
-    const int g = threadIdx.x / kSolveLanes, lane_r = threadIdx.x % kSolveLanes;
-    const bool row = lane_r < 6;
-    const int r = row ? lane_r : 0;  // idle lanes read as row 0 and never write
+    const int g = threadIdx.x / kSolveLanes, j = threadIdx.x % kSolveLanes;
+    const bool acol = j < 6, icol = j >= 6 && j < 12, vlane = j == 12;
     const uint32_t gi = blockIdx.x * kSolveGroups + g;
     uint32_t begin = 0, end = 0;
     if (gi < m.strand_count) {
@@ -689,157 +709,136 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
         const uint32_t first = m.strand_first[s];
         begin = first + m.strand_fixed[s];
         end = first + m.strand_count_of[s];
-        if (row)
-            for (uint32_t n = first; n < begin; ++n) dir[6 * size_t(n) + r] = 0.0;
+        if (acol)
+            for (uint32_t n = first; n < begin; ++n) dir[6 * size_t(n) + j] = 0.0;
         if (begin > end) begin = end;
     }
     const uint32_t len = end - begin;
     uint32_t steps = len;
 #pragma unroll
     for (int o = kWave / 2; o >= 1; o >>= 1) steps = max(steps, uint32_t(__shfl_xor(int(steps), o, kWave)));
-    double (*rows)[12] = rows_s[g];
-    double *cb = c_s[g], *ub = u_s[g], *vb = v_s[g];
     bool ok = true;
 
-    // next node's D row, U row and gradient entry, loaded one node ahead
-    double nd[6], nu[6], ng = 0.0;
+    // next node's D column (gradient on the rhs lane) and U column
+    double nd[6], nu[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) nd[k] = nu[k] = 0.0;
+    for (int r = 0; r < 6; ++r) {
+        nd[r] = 0.0;
+        nu[r] = 0.0;
+    }
     if (len > 0) {
 #pragma unroll
-        for (int k = 0; k < 6; ++k) nd[k] = Dblk[36 * size_t(begin) + 6 * r + k];
-        if (len > 1)
+        for (int r = 0; r < 6; ++r) {
+            if (acol) nd[r] = Dblk[36 * size_t(begin) + 6 * r + j];
+            if (vlane) nd[r] = grad[6 * size_t(begin) + r];
+            if (acol && len > 1) nu[r] = Ublk[36 * size_t(begin) + 6 * r + j];
+        }
+    }
+    double p[6], low[6];  // previous C' column (d' on the rhs lane); previous U column
 #pragma unroll
-            for (int k = 0; k < 6; ++k) nu[k] = Ublk[36 * size_t(begin) + 6 * r + k];
-        ng = grad[6 * size_t(begin) + r];
+    for (int r = 0; r < 6; ++r) {
+        p[r] = 0.0;
+        low[r] = 0.0;
     }
     for (uint32_t it = 0; it < steps; ++it) {
         const uint32_t n = begin + (it < len ? it : 0);
-        double a[6], inv[6], u[6];
+        double col[6], w[6];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            a[k] = nd[k];
-            u[k] = nu[k];
+        for (int r = 0; r < 6; ++r) {
+            col[r] = acol ? nd[r] : (vlane ? -nd[r] : ((icol && r == j - 6) ? 1.0 : 0.0));
+            w[r] = acol ? nu[r] : 0.0;
         }
-        double rhs = -ng;
         if (it + 1 < len) {
 #pragma unroll
-            for (int k = 0; k < 6; ++k) nd[k] = Dblk[36 * size_t(n + 1) + 6 * r + k];
-#pragma unroll
-            for (int k = 0; k < 6; ++k) nu[k] = it + 2 < len ? Ublk[36 * size_t(n + 1) + 6 * r + k] : 0.0;
-            ng = grad[6 * size_t(n + 1) + r];
+            for (int r = 0; r < 6; ++r) {
+                if (acol) nd[r] = Dblk[36 * size_t(n + 1) + 6 * r + j];
+                if (vlane) nd[r] = grad[6 * size_t(n + 1) + r];
+                nu[r] = (acol && it + 2 < len) ? Ublk[36 * size_t(n + 1) + 6 * r + j] : 0.0;
+            }
         }
         if (it > 0) {
-            // a[c] -= sum_k low[k][r] prevc[k][c]; rhs -= sum_k low[k][r] prevd[k]
-#pragma unroll
-            for (int c = 0; c < 6; ++c) {
+            // col[r] -= sum_k low[k][r] p[k]: D - U'^T C' on the block columns,
+            // rhs - U'^T d' on the rhs lane; low[k][r] is lane r's low[k]
+            static_for<6>([&](auto R) {
                 double sum = 0.0;
 #pragma unroll
-                for (int k = 0; k < 6; ++k) sum += ub[6 * k + r] * cb[6 * k + c];
-                a[c] -= sum;
-            }
-            double sum = 0.0;
-#pragma unroll
-            for (int k = 0; k < 6; ++k) sum += ub[6 * k + r] * vb[k];
-            rhs -= sum;
+                for (int k = 0; k < 6; ++k) sum += row_bcast<R>(low[k]) * p[k];
+                col[R] -= sum;
+            });
         }
-        // Gauss-Jordan inverse, row r per lane.
+        double rhs[6];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) inv[k] = k == r ? 1.0 : 0.0;
-        __syncthreads();
-        if (row)
-#pragma unroll
-            for (int k = 0; k < 6; ++k) rows[r][k] = a[k];
-        __syncthreads();
+        for (int r = 0; r < 6; ++r) {
+            rhs[r] = col[r];
+            if (vlane) col[r] = 0.0;
+        }
+        // Gauss-Jordan inverse on the columns of [a | inv].
         double scale = 0.0;
-#pragma unroll
-        for (int i = 0; i < 6; ++i) scale = fmax(scale, fabs(rows[i][i]));
+        static_for<6>([&](auto I) { scale = fmax(scale, fabs(row_bcast<I>(col[I]))); });
         const double shift = fmax(1.0e-12 * scale, 1.0e-24);
 #pragma unroll
-        for (int k = 0; k < 6; ++k)
-            if (k == r) a[k] += shift;
-#pragma unroll
-        for (int col = 0; col < 6; ++col) {
-            __syncthreads();
-            if (row)
-#pragma unroll
-                for (int k = 0; k < 6; ++k) {
-                    rows[r][k] = a[k];
-                    rows[r][6 + k] = inv[k];
-                }
-            __syncthreads();
+        for (int r = 0; r < 6; ++r)
+            if (acol && r == j) col[r] += shift;
+        static_for<6>([&](auto C) {
             double cv[6];
 #pragma unroll
-            for (int i = 0; i < 6; ++i) cv[i] = rows[i][col];
-            int pivot = col;
-            double best = fabs(cv[col]);
+            for (int i = 0; i < 6; ++i) cv[i] = row_bcast<C>(col[i]);
+            int pivot = C;
+            double best = fabs(cv[C]);
 #pragma unroll
-            for (int i = col + 1; i < 6; ++i) {
+            for (int i = C + 1; i < 6; ++i) {
                 const double v = fabs(cv[i]);
                 if (v > best) { best = v; pivot = i; }
             }
             if (best < 1.0e-30 || !isfinite(best)) {
                 if (it < len) ok = false;
-                pivot = col;
+                pivot = C;
             }
-            // this lane's row after the exchange of rows col and pivot, and
-            // the pivot row (row col after it), scaled
-            const int src = r == col ? pivot : (r == pivot ? col : r);
-            double mine[12], prow[12];
 #pragma unroll
-            for (int k = 0; k < 12; ++k) {
-                mine[k] = rows[src][k];
-                prow[k] = rows[pivot][k];
+            for (int i = C + 1; i < 6; ++i) {
+                const bool sw = pivot == i;
+                const double x = col[C], y = col[i];
+                col[C] = sw ? y : x;
+                col[i] = sw ? x : y;
+                const double cx = cv[C], cy = cv[i];
+                cv[C] = sw ? cy : cx;
+                cv[i] = sw ? cx : cy;
             }
-            const double invp = 1.0 / prow[col];
+            const double invp = 1.0 / cv[C];
+            col[C] *= invp;
 #pragma unroll
-            for (int k = 0; k < 12; ++k) prow[k] *= invp;
-            const double f = mine[col];
+            for (int r = 0; r < 6; ++r)
+                if (r != C) col[r] -= cv[r] * col[C];
+        });
+        // res[r] = sum_k inv[r][k] w[k]: C' = inv U on the block columns,
+        // d' = inv rhs on the rhs lane; inv column k lives in lane 6 + k
 #pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                a[k] = r == col ? prow[k] : mine[k] - f * prow[k];
-                inv[k] = r == col ? prow[6 + k] : mine[6 + k] - f * prow[6 + k];
+        for (int r = 0; r < 6; ++r)
+            if (vlane) w[r] = rhs[r];
+        double res[6];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) res[r] = 0.0;
+        static_for<6>([&](auto K) {
+#pragma unroll
+            for (int r = 0; r < 6; ++r) res[r] += row_bcast<6 + K>(col[r]) * w[K];
+        });
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            p[r] = acol ? (it + 1 < len ? res[r] : 0.0) : (vlane ? res[r] : 0.0);
+            low[r] = acol ? w[r] : 0.0;
+        }
+        if (it < len && ok) {
+#pragma unroll
+            for (int r = 0; r < 6; ++r) {
+                if (acol) Cp[36 * size_t(n) + 6 * r + j] = p[r];
+                if (vlane) dp[6 * size_t(n) + r] = p[r];
             }
-        }
-        // C' = inv U (zero on the last node), d' = inv rhs; U and rhs pass
-        // through LDS after the products above have read the previous ones.
-        __syncthreads();
-        if (row) {
-#pragma unroll
-            for (int k = 0; k < 6; ++k) ub[6 * r + k] = u[k];
-            vb[r] = rhs;
-        }
-        __syncthreads();
-        double cnew[6];
-#pragma unroll
-        for (int c = 0; c < 6; ++c) {
-            double sum = 0.0;
-#pragma unroll
-            for (int k = 0; k < 6; ++k) sum += inv[k] * ub[6 * k + c];
-            cnew[c] = it + 1 < len ? sum : 0.0;
-        }
-        double dnew = 0.0;
-#pragma unroll
-        for (int k = 0; k < 6; ++k) dnew += inv[k] * vb[k];
-        __syncthreads();
-        if (row) {
-#pragma unroll
-            for (int c = 0; c < 6; ++c) cb[6 * r + c] = cnew[c];
-            vb[r] = dnew;
-        }
-        __syncthreads();
-        if (it < len && ok && row) {
-#pragma unroll
-            for (int c = 0; c < 6; ++c) Cp[36 * size_t(n) + 6 * r + c] = cnew[c];
-            dp[6 * size_t(n) + r] = dnew;
         }
     }
-    if (!ok) {
-        if (lane_r == 0) atomicOr(failed, 1u);
-        return;
-    }
-    // Back substitution: x_n = d'_n - C'_n x_{n+1}, the node below loaded
-    // one step ahead.
+    if (!ok && j == 0) atomicOr(failed, 1u);
+    // Back substitution x_n = d'_n - C'_n x_{n+1}, lane r < 6 owning row r;
+    // the node below is loaded one step ahead.
+    const int r = acol ? j : 0;
     double next = 0.0, cn[6], dn = 0.0;
 #pragma unroll
     for (int k = 0; k < 6; ++k) cn[k] = 0.0;
@@ -861,11 +860,8 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
             dn = dp[6 * size_t(n - 1) + r];
         }
         double value = d0;
-        if (it > 0) {
-#pragma unroll
-            for (int k = 0; k < 6; ++k) value -= crow[k] * __shfl(next, k, kSolveLanes);
-        }
-        if (live && row) dir[6 * size_t(n) + r] = value;
+        if (it > 0) static_for<6>([&](auto K) { value -= crow[K] * row_bcast<K>(next); });
+        if (live && ok && acol) dir[6 * size_t(n) + j] = value;
         next = value;
     }
 }
