@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstring>
 #define __device__
+#define __host__
 #define __forceinline__ inline
 using std::cos;
 using std::fabs;

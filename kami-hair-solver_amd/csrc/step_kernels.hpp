@@ -364,6 +364,25 @@ __device__ __forceinline__ bool listed_query(const ContactCache &cc, const uint3
 }
 
 // ------------------------------------------------------------ evaluation
+// LDS stride of one node's staged 36-entry block (odd in 8-byte words, so
+// lanes writing consecutive nodes fall in different banks).
+constexpr int kStageStride = 37;
+
+// Writes a tile's staged blocks (nodes first .. first + count - 1, 36
+// entries each) as one contiguous run, consecutive lanes on consecutive
+// words. Every wavefront of the block calls it.
+__device__ inline void store_tile_blocks(const double *st, double *out, long first, int count, int lane)
+{
+    __syncthreads();
+    const int total = 36 * count;
+    double *dst = out + 36 * size_t(first);
+    for (int k = lane; k < total; k += kWave) {
+        const int node = k / 36, e = k - 36 * node;
+        dst[k] = st[node * kStageStride + e];
+    }
+    __syncthreads();
+}
+
 // Incremental potential of one substep (reference cuda_backend.cu:883-939):
 // inertia and damping per node, Cosserat elasticity and log-barrier contact
 // per element; with GRAD also the gradient and the Gauss-Newton blocks (the
@@ -507,7 +526,7 @@ __global__ __launch_bounds__(kBlock) void k_evaluate(RodView m, ColliderView col
         acc.sum[kContact] += contact_energy;
     }
 
-    if (GRAD) {
+    if constexpr (GRAD) {
         const double L0 = elem ? m.rest[8 * L] : 0.0;
         double Lw[6], cw[6];
 #pragma unroll
@@ -570,30 +589,47 @@ __global__ __launch_bounds__(kBlock) void k_evaluate(RodView m, ColliderView col
             acc.sum[kDirNorm2] += dd;
         }
         const bool write_u = owner && elem && !fixed_i && !fixed_j;
+        // The blocks are staged per wavefront in LDS and written out as the
+        // tile's contiguous 36-entry records, whole lines at a time: first D
+        // (inertia, both elements' j- and i-parts, both contact terms), then
+        // U (this element's coupling and contact term).
+        __shared__ double stage[kTilesPerBlock][kTile * kStageStride];
+        double *st = stage[threadIdx.x / kWave];
+        const int slot = lane - 1;
+        const long first_node = tile * kTile;
+        const int tile_nodes = first_node < long(m.node_count) ? int(min(long(kTile), long(m.node_count) - first_node)) : 0;
 #pragma unroll
         for (int a = 0; a < 6; ++a) {
 #pragma unroll
             for (int b = 0; b < 6; ++b) {
-                // j-part of this element for the node above, i-part for this node
-                double hj = 0.0, hi = 0.0, hu = 0.0;
+                double hj = 0.0, hi = 0.0;
 #pragma unroll
                 for (int s = 0; s < 6; ++s) {
                     if (!structural_zero(s, 6 + a) && !structural_zero(s, 6 + b)) hj = hj + (Lw[s] * jac(s, 6 + a)) * jac(s, 6 + b);
                     if (!structural_zero(s, a) && !structural_zero(s, b)) hi = hi + (Lw[s] * jac(s, a)) * jac(s, b);
-                    if (!structural_zero(s, a) && !structural_zero(s, 6 + b)) hu = hu + (Lw[s] * jac(s, a)) * jac(s, 6 + b);
                 }
                 const double left = from_lane_below(hj);
                 double v = (a == b ? dnode[a] : 0.0) + left;
                 if (a < 3 && b < 3) v = v + (curv_left * dg_left[a]) * dg_left[b];
                 v = v + hi;
                 if (a < 3 && b < 3) v = v + (curvature * dg[a]) * dg[b];
-                if (a < 3 && b < 3) hu = hu + (curvature * dg[a]) * dg[3 + b];
-                if (owner) {
-                    Dblk[36 * L + 6 * a + b] = fixed_i ? 0.0 : v;
-                    Ublk[36 * L + 6 * a + b] = write_u ? hu : 0.0;
-                }
+                if (owner) st[slot * kStageStride + 6 * a + b] = fixed_i ? 0.0 : v;
             }
         }
+        store_tile_blocks(st, Dblk, first_node, tile_nodes, lane);
+#pragma unroll
+        for (int a = 0; a < 6; ++a) {
+#pragma unroll
+            for (int b = 0; b < 6; ++b) {
+                double hu = 0.0;
+#pragma unroll
+                for (int s = 0; s < 6; ++s)
+                    if (!structural_zero(s, a) && !structural_zero(s, 6 + b)) hu = hu + (Lw[s] * jac(s, a)) * jac(s, 6 + b);
+                if (a < 3 && b < 3) hu = hu + (curvature * dg[a]) * dg[3 + b];
+                if (owner) st[slot * kStageStride + 6 * a + b] = write_u ? hu : 0.0;
+            }
+        }
+        store_tile_blocks(st, Ublk, first_node, tile_nodes, lane);
     }
     block_reduce(acc, partials + blockIdx.x);
 }
