@@ -92,6 +92,57 @@ def cpu_baseline(scene, strands, frames, threads):
     }
 
 
+def run_batched(lib, scene, jobs, warmup, frames):
+    """`jobs` copies of the workload stepped concurrently, each solver on
+    its own host thread and HIP stream (the library call releases the
+    interpreter lock). Returns aggregate node-substeps/s over the timed
+    frames and the per-job substep latencies."""
+    import threading
+    solvers = [setup_animation(lib, scene, warmup + frames + 1, **LIFTED) for _ in range(jobs)]
+    for s in solvers:
+        for f in range(1, warmup + 1):
+            s.check(s.step_frame(f, scene.frame_dt))
+    counts = [0] * jobs
+    latencies = [[] for _ in range(jobs)]
+    errors = []
+    start = threading.Barrier(jobs + 1)
+
+    def work(k):
+        s = solvers[k]
+        start.wait()
+        try:
+            for f in range(warmup + 1, warmup + frames + 1):
+                s.check(s.step_frame(f, scene.frame_dt))
+                counts[k] += s.step_stats().substeps
+                latencies[k].extend(s.substep_times().tolist())
+        except Exception as e:  # reported after the join
+            errors.append(e)
+
+    threads = [threading.Thread(target=work, args=(k,)) for k in range(jobs)]
+    for t in threads:
+        t.start()
+    start.wait()
+    t0 = time.perf_counter()
+    for t in threads:
+        t.join()
+    elapsed = time.perf_counter() - t0
+    if errors:
+        raise errors[0]
+    nodes = solvers[0].node_count()
+    lat = np.concatenate([np.asarray(x) for x in latencies])
+    for s in solvers:
+        s.close()
+    return {
+        "jobs": jobs,
+        "frames_per_job": frames,
+        "value": nodes * sum(counts) / elapsed,
+        "unit": "node-substeps/s",
+        "elapsed_s": elapsed,
+        "substep_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "form": "independent jobs on concurrent streams, plain FP64",
+    }
+
+
 def copy_calibration(mib):
     """One device-to-device copy of `mib` MiB (reads and writes mib MiB),
     the known kernel against which the PMC traffic counters are scaled. It
@@ -124,6 +175,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one device copy of this size first (for scaling PMC traffic counters)")
+    ap.add_argument("--batch-jobs", type=int, default=4, help="concurrent jobs of the batched form (0: skip)")
+    ap.add_argument("--batch-frames", type=int, default=None,
+                    help="timed frames per job of the batched form (default: --steps)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {args.gpus}).", file=sys.stderr)
@@ -243,6 +297,12 @@ def main():
         },
         "reference_derived_node_substeps_per_s": 3.16e5,
     }
+    if args.batch_jobs > 0:
+        s.close()
+        t = time.perf_counter()
+        result["batched"] = run_batched(lib, scene, args.batch_jobs, args.warmup, args.batch_frames or args.steps)
+        log(f"[bench] batched form: {args.batch_jobs} jobs in {time.perf_counter() - t:.1f} s, "
+            f"{result['batched']['value']:.4g} node-substeps/s")
     if not args.no_cpu_baseline:
         t = time.perf_counter()
         result["cpu_baseline"] = cpu_baseline(scene, args.cpu_strands, args.cpu_frames, args.cpu_threads)

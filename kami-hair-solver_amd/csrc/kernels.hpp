@@ -71,6 +71,15 @@ struct ContactCache {
     const double *tri_box;  // 6 per triangle: the box at the current collider position
 };
 
+// The Newton direction's choice on the device: the block direction if it
+// was solved and descends, else steepest descent; and the factor that
+// brings its largest free-node translation within the trust radius.
+struct DirectionChoice {
+    int used;  // 1: block direction, 0: steepest descent
+    int pad;
+    double scale;
+};
+
 // Scalars of one substep.
 struct StepParams {
     double dt, inv_dt, inv_dt2;

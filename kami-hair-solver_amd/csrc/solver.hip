@@ -125,7 +125,8 @@ struct DeviceState {
     Buffer<SweepRecord> sweep;
     Buffer<double> out3;
     Buffer<uint32_t> pairs;
-    Partial *host_partial = nullptr;
+    Buffer<DirectionChoice> choice;
+    Buffer<double> alpha;  // line-search step read by the trial kernels
 
     // Event timing of kernel classes (enabled on request) and of substeps.
     struct Pending {
@@ -186,7 +187,7 @@ struct DeviceState {
         }
         for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
         for (hipEvent_t e : substep_events) (void)hipEventDestroy(e);
-        if (host_partial) (void)hipHostFree(host_partial);
+        if (host_view) (void)hipHostFree(host_view);
         if (frame_begin) (void)hipEventDestroy(frame_begin);
         if (frame_end) (void)hipEventDestroy(frame_end);
         if (stream) (void)hipStreamDestroy(stream);
@@ -234,13 +235,33 @@ struct DeviceState {
 
     Partial reduce(uint32_t blocks)
     {
-        hipLaunchKernelGGL(dev::k_reduce, dim3(1), dim3(kBlock), 0, stream, partials.get(), blocks, reduced.get());
+        reduce_async(blocks, kEvalSlot);
+        return read_back().eval;
+    }
+    // Folds the partials into reduced[slot] without waiting.
+    void reduce_async(uint32_t blocks, int slot)
+    {
+        hipLaunchKernelGGL(dev::k_reduce, dim3(1), dim3(kBlock), 0, stream, partials.get(), blocks,
+                           reduced.get() + slot);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(host_partial, reduced.get(), sizeof(Partial), hipMemcpyDeviceToHost, stream));
+    }
+    // One wait for everything the host reads after a batch of launches.
+    struct HostView {
+        Partial eval, steep, ccd;
+        DirectionChoice choice;
+    };
+    HostView read_back()
+    {
+        static_assert(sizeof(HostView) == 3 * sizeof(Partial) + sizeof(DirectionChoice), "packed readback");
+        HIP_TRY(hipMemcpyAsync(host_view, reduced.get(), 3 * sizeof(Partial), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipMemcpyAsync(&host_view->choice, choice.get(), sizeof(DirectionChoice), hipMemcpyDeviceToHost,
+                               stream));
         HIP_TRY(hipStreamSynchronize(stream));
         settle();
-        return *host_partial;
+        return *host_view;
     }
+    static constexpr int kEvalSlot = 0, kSteepSlot = 1, kCcdSlot = 2;
+    HostView *host_view = nullptr;  // pinned
 
     uint32_t tiled_blocks() const { return grid_for(size_t(N) + 1, kNodesPerBlock); }
     uint32_t node_blocks() const { return grid_for(N, kBlock); }
@@ -445,7 +466,9 @@ KhsResult Solver::upload_model()
     HIP_TRY(hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreate(&d.frame_begin));
     HIP_TRY(hipEventCreate(&d.frame_end));
-    HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&d.host_partial), sizeof(Partial)));
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&d.host_view), sizeof(DeviceState::HostView)));
+    d.choice.allocate(1);
+    d.alpha.allocate(1);
     const RodModel &m = model_;
     d.N = m.node_count;
     d.S = m.strand_count;
@@ -517,7 +540,7 @@ KhsResult Solver::upload_model()
     d.out3.allocate(3 * std::max<size_t>(N, d.P));
     const uint32_t max_blocks = std::max(d.tiled_blocks(), d.node_blocks()) + 1;
     d.partials.allocate(max_blocks + kPairBlocks + 16);
-    d.reduced.allocate(1);
+    d.reduced.allocate(3);
     d.flag.allocate(1);
     d.key64.allocate(1);
     d.sweep.allocate(1);
@@ -609,7 +632,7 @@ void Solver::refit_bvh(bool swept)
 // Brings the candidate lists up to date for a query pass (all of them
 // after the collider boxes changed): elements whose query box left their
 // cached box gather again.
-void Solver::refresh_contact_cache(int mode, bool full, double alpha)
+void Solver::refresh_contact_cache(int mode, bool full)
 {
     DeviceState &d = *dev_;
     const double contact_pad = material_.radius + material_.collider_offset + material_.barrier_distance;
@@ -626,34 +649,34 @@ void Solver::refresh_contact_cache(int mode, bool full, double alpha)
     const dim3 grid(d.node_blocks()), block(kBlock);
     if (mode == dev::kAtState)
         hipLaunchKernelGGL(dev::k_contact_refresh<dev::kAtState>, grid, block, 0, d.stream, d.rods(), d.collider(), cc,
-                           d.q.get(), d.q_base.get(), d.dir.get(), alpha, pad, f);
+                           d.q.get(), d.q_base.get(), d.dir.get(), d.alpha.get(), pad, f);
     else if (mode == dev::kAtTrial)
         hipLaunchKernelGGL(dev::k_contact_refresh<dev::kAtTrial>, grid, block, 0, d.stream, d.rods(), d.collider(), cc,
-                           d.q.get(), d.q_base.get(), d.dir.get(), alpha, pad, f);
+                           d.q.get(), d.q_base.get(), d.dir.get(), d.alpha.get(), pad, f);
     else
         hipLaunchKernelGGL(dev::k_contact_refresh<dev::kSwept>, grid, block, 0, d.stream, d.rods(), d.collider(), cc,
-                           d.q.get(), d.q_base.get(), d.dir.get(), alpha, pad, f);
+                           d.q.get(), d.q_base.get(), d.dir.get(), d.alpha.get(), pad, f);
     HIP_TRY(hipGetLastError());
     d.cache_valid = true;
 }
 
-// Closest triangle of every listed element at q (or at alpha d + q_base),
-// one wavefront per element; partials (may be null) receive the candidate
-// count.
-void Solver::closest_pass(bool line_search, double alpha, Partial *partials)
+// Closest triangle of every listed element at q (or at alpha d + q_base
+// with the device's alpha), one 16-lane group per element; partials (may
+// be null) receive the candidate count.
+void Solver::closest_pass(bool line_search, Partial *partials)
 {
     DeviceState &d = *dev_;
     const double contact_pad = material_.radius + material_.collider_offset + material_.barrier_distance;
-    refresh_contact_cache(line_search ? dev::kAtTrial : dev::kAtState, false, alpha);
+    refresh_contact_cache(line_search ? dev::kAtTrial : dev::kAtState, false);
     const ContactCache cc = d.cache();
     if (line_search)
         hipLaunchKernelGGL(dev::k_list_closest<dev::kAtTrial>, dim3(kPairBlocks), dim3(kBlock), 0, d.stream,
-                           d.flags.get(), d.collider(), cc, contact_pad, d.q.get(), d.q_base.get(), d.dir.get(), alpha,
-                           d.best_tri.get(), partials);
+                           d.flags.get(), d.collider(), cc, contact_pad, d.q.get(), d.q_base.get(), d.dir.get(),
+                           d.alpha.get(), d.best_tri.get(), partials);
     else
         hipLaunchKernelGGL(dev::k_list_closest<dev::kAtState>, dim3(kPairBlocks), dim3(kBlock), 0, d.stream,
-                           d.flags.get(), d.collider(), cc, contact_pad, d.q.get(), d.q_base.get(), d.dir.get(), alpha,
-                           d.best_tri.get(), partials);
+                           d.flags.get(), d.collider(), cc, contact_pad, d.q.get(), d.q_base.get(), d.dir.get(),
+                           d.alpha.get(), d.best_tri.get(), partials);
     HIP_TRY(hipGetLastError());
 }
 
@@ -837,7 +860,10 @@ KhsResult Solver::finalize_animation()
 }
 
 // ---------------------------------------------------------------- solving
-Solver::Reduced Solver::evaluate(bool gradient, double dt, bool line_search, double alpha)
+// Enqueues one evaluation of the incremental potential (the line-search
+// variant at alpha d + q_base, alpha read from the device) and its
+// reduction into reduced[kEvalSlot]; returns the partial count folded.
+void Solver::evaluate_async(bool gradient, double dt, bool line_search)
 {
     DeviceState &d = *dev_;
     StepParams p{};
@@ -871,29 +897,31 @@ Solver::Reduced Solver::evaluate(bool gradient, double dt, bool line_search, dou
     const ColliderView cv = d.collider();
     const ContactCache cc = d.cache();
     hipEvent_t t0 = d.tic();
-    // With valid candidate lists the pair pass finds the closest triangles
-    // first; its partials follow the evaluation's.
     // With current candidate lists the list pass finds the closest
     // triangles first; its partials follow the evaluation's.
     const bool list_pass = d.cache_valid;
-    if (list_pass) closest_pass(line_search, alpha, d.partials.get() + blocks);
+    if (list_pass) closest_pass(line_search, d.partials.get() + blocks);
     const uint32_t *best = list_pass ? d.best_tri.get() : nullptr;
     if (gradient)
         hipLaunchKernelGGL((dev::k_evaluate<true, false>), dim3(blocks), dim3(kBlock), 0, d.stream, rv, cv, cc, best, p,
-                           d.q.get(), d.q_old.get(), d.vel.get(), d.q_base.get(), d.dir.get(), 0.0, d.grad.get(),
-                           d.Dblk.get(), d.Ublk.get(), d.partials.get());
+                           d.q.get(), d.q_old.get(), d.vel.get(), d.q_base.get(), d.dir.get(), d.alpha.get(),
+                           d.grad.get(), d.Dblk.get(), d.Ublk.get(), d.partials.get());
     else if (line_search)
         hipLaunchKernelGGL((dev::k_evaluate<false, true>), dim3(blocks), dim3(kBlock), 0, d.stream, rv, cv, cc, best, p,
-                           d.q.get(), d.q_old.get(), d.vel.get(), d.q_base.get(), d.dir.get(), alpha, d.grad.get(),
-                           d.Dblk.get(), d.Ublk.get(), d.partials.get());
+                           d.q.get(), d.q_old.get(), d.vel.get(), d.q_base.get(), d.dir.get(), d.alpha.get(),
+                           d.grad.get(), d.Dblk.get(), d.Ublk.get(), d.partials.get());
     else
         hipLaunchKernelGGL((dev::k_evaluate<false, false>), dim3(blocks), dim3(kBlock), 0, d.stream, rv, cv, cc, best, p,
-                           d.q.get(), d.q_old.get(), d.vel.get(), d.q_base.get(), d.dir.get(), 0.0, d.grad.get(),
-                           d.Dblk.get(), d.Ublk.get(), d.partials.get());
+                           d.q.get(), d.q_old.get(), d.vel.get(), d.q_base.get(), d.dir.get(), d.alpha.get(),
+                           d.grad.get(), d.Dblk.get(), d.Ublk.get(), d.partials.get());
     HIP_TRY(hipGetLastError());
     if (gradient) d.toc(t0, d.totals.assembly_ms, d.totals.assembly_launches);
     else d.toc(t0, d.totals.energy_ms, d.totals.energy_launches);
-    const Partial r2 = d.reduce(blocks + (list_pass ? kPairBlocks : 0));
+    d.reduce_async(blocks + (list_pass ? kPairBlocks : 0), DeviceState::kEvalSlot);
+}
+
+Solver::Reduced Solver::unpack(const Partial &r2)
+{
     Reduced out{};
     out.objective = r2.sum[kObjective];
     out.elastic = r2.sum[kElastic];
@@ -908,55 +936,57 @@ Solver::Reduced Solver::evaluate(bool gradient, double dt, bool line_search, dou
     return out;
 }
 
-void Solver::limit_direction(double max_translation)
+Solver::Reduced Solver::evaluate(bool gradient, double dt, bool line_search)
 {
-    // reference cuda_backend.cu:623-630
-    if (max_translation > trust_radius_) {
-        DeviceState &d = *dev_;
-        const double s = trust_radius_ / max_translation;
-        hipLaunchKernelGGL(dev::k_scale, dim3(grid_for(6 * size_t(d.N), kBlock)), dim3(kBlock), 0, d.stream,
-                           d.dir.get(), 6 * size_t(d.N), s);
-        HIP_TRY(hipGetLastError());
-    }
+    evaluate_async(gradient, dt, line_search);
+    return unpack(dev_->read_back().eval);
 }
 
-void Solver::steepest_direction()
+// Steepest descent and its trust-radius factor when the choice is not the
+// block direction (reference cuda_backend.cu:623-638).
+void Solver::steepest_direction_async()
 {
-    // reference cuda_backend.cu:632-638
     DeviceState &d = *dev_;
     const RodView rv = d.rods();
-    hipLaunchKernelGGL(dev::k_steepest, dim3(grid_for(6 * size_t(d.N), kBlock)), dim3(kBlock), 0, d.stream, rv,
+    const size_t dofs = 6 * size_t(d.N);
+    hipLaunchKernelGGL(dev::k_steepest, dim3(grid_for(dofs, kBlock)), dim3(kBlock), 0, d.stream, rv, d.choice.get(),
                        d.grad.get(), d.precond.get(), d.dir.get());
     hipLaunchKernelGGL(dev::k_direction_stats, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, d.grad.get(),
-                       d.dir.get(), (const unsigned *)nullptr, d.partials.get());
+                       d.dir.get(), (const unsigned *)nullptr, (const DirectionChoice *)d.choice.get(),
+                       d.partials.get());
+    d.reduce_async(d.node_blocks(), DeviceState::kSteepSlot);
+    hipLaunchKernelGGL(dev::k_choose_steepest_scale, dim3(1), dim3(1), 0, d.stream,
+                       (const Partial *)(d.reduced.get() + DeviceState::kSteepSlot), trust_radius_, d.choice.get());
+    hipLaunchKernelGGL(dev::k_scale_direction, dim3(grid_for(dofs, kBlock)), dim3(kBlock), 0, d.stream,
+                       (const DirectionChoice *)d.choice.get(), 0, d.dir.get(), dofs);
     HIP_TRY(hipGetLastError());
-    const Partial r = d.reduce(d.node_blocks());
-    limit_direction(r.max_translation);
 }
 
-bool Solver::make_direction()
+// Block direction limited to the trust radius, with the choice whether it
+// is usable (solved and descending) made on the device (reference
+// cuda_backend.cu:623-648). The choice reaches the host with the next
+// read-back; when the direction is not usable the host replaces it by
+// steepest descent and redoes the step bound (optimize_substep).
+void Solver::make_direction_async()
 {
-    // reference cuda_backend.cu:640-648
     DeviceState &d = *dev_;
     const RodView rv = d.rods();
     hipLaunchKernelGGL(dev::k_fill_u32, dim3(1), dim3(64), 0, d.stream, d.flag.get(), size_t(1), 0u);
     hipEvent_t t0 = d.tic();
     hipLaunchKernelGGL(dev::k_block_solve, dim3(grid_for(d.S, dev::kSolveGroups)), dim3(kWave), 0, d.stream, rv,
-                       d.solve_order.get(), d.Dblk.get(),
-                       d.Ublk.get(), d.grad.get(), d.Cp.get(), d.dp.get(), d.dir.get(), d.flag.get());
+                       d.solve_order.get(), d.Dblk.get(), d.Ublk.get(), d.grad.get(), d.Cp.get(), d.dp.get(),
+                       d.dir.get(), d.flag.get());
     d.toc(t0, d.totals.solve_ms, d.totals.solve_launches);
     hipLaunchKernelGGL(dev::k_direction_stats, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, d.grad.get(),
-                       d.dir.get(), (const unsigned *)d.flag.get(), d.partials.get());
+                       d.dir.get(), (const unsigned *)d.flag.get(), (const DirectionChoice *)nullptr,
+                       d.partials.get());
+    d.reduce_async(d.node_blocks(), DeviceState::kEvalSlot);
+    hipLaunchKernelGGL(dev::k_choose_direction, dim3(1), dim3(1), 0, d.stream,
+                       (const Partial *)(d.reduced.get() + DeviceState::kEvalSlot), trust_radius_, d.choice.get());
+    const size_t dofs = 6 * size_t(d.N);
+    hipLaunchKernelGGL(dev::k_scale_direction, dim3(grid_for(dofs, kBlock)), dim3(kBlock), 0, d.stream,
+                       (const DirectionChoice *)d.choice.get(), 1, d.dir.get(), dofs);
     HIP_TRY(hipGetLastError());
-    const Partial r = d.reduce(d.node_blocks());
-    const double slope = r.sum[kSlope];
-    const bool used = !r.failed && slope < 0 && std::isfinite(slope);
-    if (!used) {
-        steepest_direction();
-        return false;
-    }
-    limit_direction(r.max_translation);
-    return true;
 }
 
 KhsResult Solver::optimize_substep(double dt)
@@ -974,7 +1004,7 @@ KhsResult Solver::optimize_substep(double dt)
         hipLaunchKernelGGL(dev::k_preconditioner, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, 1.0 / dt,
                            material_.mass_damping, tk, ra, rb, d.precond.get());
     }
-    Reduced current = evaluate(true, dt, false, 0.0);
+    Reduced current = evaluate(true, dt, false);
     if (!current.feasible) return fail(KHS_ERROR_NOT_CONVERGED, "State is outside the barrier feasible region.");
     const double initial = std::sqrt(current.grad_norm2);
     step_stats_.initial_residual_norm = std::fmax(step_stats_.initial_residual_norm, initial);
@@ -1008,17 +1038,17 @@ KhsResult Solver::optimize_substep(double dt)
         }
         if (it == desc_.newton_iterations) break;
         phase_.store(KHS_PHASE_LINEAR_SOLVE);
-        bool block_direction = make_direction();
+        make_direction_async();
         ++step_stats_.linear_solves;
         HIP_TRY(hipMemcpyAsync(d.q_base.get(), d.q.get(), d.q.bytes(), hipMemcpyDeviceToDevice, d.stream));
-        bool accepted = false;
+        bool accepted = false, block_direction = true;
         double alpha = 1.0;
-        for (int attempt = 0; attempt < 2; ++attempt) {
+        for (int attempt = 0; attempt < 2;) {
             phase_.store(KHS_PHASE_CCD);
             hipEvent_t t0 = d.tic();
             const bool list_pass = d.cache_valid;
             if (list_pass) {
-                refresh_contact_cache(dev::kSwept, false, 0.0);
+                refresh_contact_cache(dev::kSwept, false);
                 hipLaunchKernelGGL(dev::k_list_ccd, dim3(kPairBlocks), dim3(kBlock), 0, d.stream, d.flags.get(),
                                    d.collider(), d.cache(), cp, d.q.get(), d.dir.get(),
                                    d.partials.get() + d.node_blocks());
@@ -1028,7 +1058,26 @@ KhsResult Solver::optimize_substep(double dt)
                                d.partials.get());
             HIP_TRY(hipGetLastError());
             d.toc(t0, d.totals.collision_ms, d.totals.collision_launches);
-            const Partial cr = d.reduce(d.node_blocks() + (list_pass ? kPairBlocks : 0));
+            d.reduce_async(d.node_blocks() + (list_pass ? kPairBlocks : 0), DeviceState::kCcdSlot);
+            hipLaunchKernelGGL(dev::k_choose_alpha, dim3(1), dim3(1), 0, d.stream,
+                               (const Partial *)(d.reduced.get() + DeviceState::kCcdSlot), d.T ? 1 : 0,
+                               d.alpha.get());
+            // The first trial runs at the device's step before the host has
+            // seen the bound; one read-back returns direction choice, bound
+            // and trial. A trial the host would not have run is discarded,
+            // and every path that does not accept restores q from q_base.
+            phase_.store(KHS_PHASE_LINE_SEARCH);
+            evaluate_async(false, dt, true);
+            const DeviceState::HostView v = d.read_back();
+            if (attempt == 0 && block_direction && !v.choice.used) {
+                // The block direction failed or does not descend: the step
+                // starts from steepest descent instead, with no retry.
+                HIP_TRY(hipMemcpyAsync(d.q.get(), d.q_base.get(), d.q.bytes(), hipMemcpyDeviceToDevice, d.stream));
+                steepest_direction_async();
+                block_direction = false;
+                continue;
+            }
+            const Partial &cr = v.ccd;
             const double slope = cr.sum[kSlope];
             alpha = 1.0;
             if (d.T) {
@@ -1037,10 +1086,13 @@ KhsResult Solver::optimize_substep(double dt)
                 step_stats_.ccd_step_limit = std::fmin(step_stats_.ccd_step_limit, alpha);
             }
             accepted = false;
-            phase_.store(KHS_PHASE_LINE_SEARCH);
-            if (alpha >= desc_.minimum_line_search_step)
+            if (alpha >= desc_.minimum_line_search_step) {
+                Reduced trial = unpack(v.eval);
                 for (uint32_t search = 0; search < desc_.line_search_iterations; ++search) {
-                    const Reduced trial = evaluate(false, dt, true, alpha);
+                    if (search > 0) {
+                        hipLaunchKernelGGL(dev::k_set_double, dim3(1), dim3(1), 0, d.stream, d.alpha.get(), alpha);
+                        trial = evaluate(false, dt, true);
+                    }
                     ++step_stats_.line_search_evaluations;
                     if (trial.feasible && std::isfinite(trial.objective) &&
                         trial.objective <= current.objective + 1e-4 * alpha * slope) {
@@ -1050,17 +1102,20 @@ KhsResult Solver::optimize_substep(double dt)
                     alpha *= 0.5;
                     if (alpha < desc_.minimum_line_search_step) break;
                 }
+            }
             if (accepted) break;
             if (!block_direction) break;
             HIP_TRY(hipMemcpyAsync(d.q.get(), d.q_base.get(), d.q.bytes(), hipMemcpyDeviceToDevice, d.stream));
-            steepest_direction();
+            hipLaunchKernelGGL(dev::k_force_steepest, dim3(1), dim3(1), 0, d.stream, d.choice.get());
+            steepest_direction_async();
             block_direction = false;
+            ++attempt;
         }
         if (!accepted) {
             HIP_TRY(hipMemcpyAsync(d.q.get(), d.q_base.get(), d.q.bytes(), hipMemcpyDeviceToDevice, d.stream));
             return fail(KHS_ERROR_NOT_CONVERGED, "Gauss-Newton line search failed.");
         }
-        current = evaluate(true, dt, false, 0.0);
+        current = evaluate(true, dt, false);
         const double dnorm = std::sqrt(current.dir_norm2);
         step_stats_.increment_norm = alpha * dnorm;
         step_stats_.accepted_step_length = alpha;
@@ -1185,7 +1240,7 @@ KhsResult Solver::run_frame(double frame_dt, const double *root_a, const double 
                 // contain the current ones, and queries test leaves against
                 // the current boxes.
                 hipEvent_t t1 = d.tic();
-                refresh_contact_cache(dev::kAtState, true, 0.0);
+                refresh_contact_cache(dev::kAtState, true);
                 d.toc(t1, d.totals.collision_ms, d.totals.collision_launches);
             }
             phase_.store(KHS_PHASE_ASSEMBLING);
@@ -1200,7 +1255,7 @@ KhsResult Solver::run_frame(double frame_dt, const double *root_a, const double 
                 break;
             }
             if (d.T && material_.friction > 0.0) {
-                if (d.cache_valid) closest_pass(false, 0.0, nullptr);
+                if (d.cache_valid) closest_pass(false, nullptr);
                 hipLaunchKernelGGL(dev::k_friction, dim3(d.tiled_blocks()), dim3(kBlock), 0, d.stream, rv,
                                    d.collider(), d.cache(), d.cache_valid ? d.best_tri.get() : nullptr, sp, d.q.get(), d.vel.get(), d.dv.get());
                 hipLaunchKernelGGL(dev::k_apply_dv, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.N, d.dv.get(),
@@ -1506,7 +1561,7 @@ KhsResult Solver::test_evaluate(double dt, double *gradient, double *direction, 
     if (!built_ || !(dt > 0.0) || !out || out->struct_size < sizeof(KhsxEvaluation)) return KHS_ERROR_INVALID_ARGUMENT;
     DeviceState &d = *dev_;
     if (d.T) rebuild_bvh(false);
-    const Reduced r = evaluate(true, dt, false, 0.0);
+    const Reduced r = evaluate(true, dt, false);
     KhsxEvaluation e{};
     e.struct_size = sizeof(e);
     e.feasible = r.feasible ? 1u : 0u;

@@ -84,12 +84,13 @@ private:
     void release_device();
     void rebuild_bvh(bool swept);
     void refit_bvh(bool swept);
-    void refresh_contact_cache(int mode, bool full, double alpha);
-    void closest_pass(bool line_search, double alpha, Partial *partials);
-    Reduced evaluate(bool gradient, double dt, bool line_search, double alpha);
-    bool make_direction();
-    void steepest_direction();
-    void limit_direction(double max_translation);
+    void refresh_contact_cache(int mode, bool full);
+    void closest_pass(bool line_search, Partial *partials);
+    void evaluate_async(bool gradient, double dt, bool line_search);
+    static Reduced unpack(const Partial &r);
+    Reduced evaluate(bool gradient, double dt, bool line_search);
+    void make_direction_async();
+    void steepest_direction_async();
     KhsResult optimize_substep(double dt);
     KhsResult run_frame(double frame_dt, const double *root_a, const double *rot_a, const double *root_b,
                         const double *rot_b, const double *collider_a, const double *collider_b, uint32_t frame,

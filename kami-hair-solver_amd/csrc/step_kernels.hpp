@@ -210,8 +210,9 @@ __device__ __forceinline__ bool box_covered(const double *box, V3 lo, V3 hi, dou
 // triangles. With `full` (new collider boxes) every element does.
 template <int MODE>
 __global__ void k_contact_refresh(RodView m, ColliderView col, ContactCache cc, const double *q, const double *q_base,
-                                  const double *dir, double alpha, double pad, int full)
+                                  const double *dir, const double *alpha_at, double pad, int full)
 {
+    const double alpha = MODE == kAtTrial ? *alpha_at : 0.0;
     const uint32_t L = blockIdx.x * blockDim.x + threadIdx.x;
     if (L >= m.node_count) return;
     const uint32_t fl = m.flags[L];
@@ -238,12 +239,15 @@ __global__ void k_contact_refresh(RodView m, ColliderView col, ContactCache cc, 
     }
 }
 
-// Minimum of (distance, triangle) in lexicographic order across the
-// wavefront; NaN distances never enter (callers skip them).
-__device__ __forceinline__ void wave_closest(double &d, uint32_t &t)
+// Lanes per listed element in the list passes (one wavefront).
+constexpr int kListLanes = kWave;
+
+// Minimum of (distance, triangle) in lexicographic order across a list
+// pass's lane group; NaN distances never enter (callers skip them).
+__device__ __forceinline__ void group_closest(double &d, uint32_t &t)
 {
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
+    for (int o = kListLanes / 2; o >= 1; o >>= 1) {
         const double od = __shfl_xor(d, o, kWave);
         const uint32_t ot = __shfl_xor(t, o, kWave);
         if (od < d || (od == d && ot < t)) {
@@ -261,15 +265,16 @@ __device__ __forceinline__ void wave_closest(double &d, uint32_t &t)
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, ColliderView col, ContactCache cc,
                                                          double pad, const double *q, const double *q_base,
-                                                         const double *dir, double alpha, uint32_t *best_tri,
-                                                         Partial *partials)
+                                                         const double *dir, const double *alpha_at,
+                                                         uint32_t *best_tri, Partial *partials)
 {
-    const int lane = threadIdx.x & (kWave - 1);
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-    const uint32_t waves = gridDim.x * blockDim.x / kWave;
+    const double alpha = MODE == kAtTrial ? *alpha_at : 0.0;
+    const int lane = threadIdx.x & (kListLanes - 1);
+    const uint32_t group = (blockIdx.x * blockDim.x + threadIdx.x) / kListLanes;
+    const uint32_t groups = gridDim.x * blockDim.x / kListLanes;
     const uint32_t listed = *cc.list_count;
     unsigned long long candidates = 0;
-    for (uint32_t j = wave; j < listed; j += waves) {
+    for (uint32_t j = group; j < listed; j += groups) {
         const uint32_t L = cc.list[j];
         const uint32_t c = cc.count[L];
         if (c == 0 || c == kCacheOverflow) continue;
@@ -280,7 +285,7 @@ __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, 
         const uint32_t *tris = cc.tris + size_t(L) * kCacheCapacity;
         double bd = 1.0e300;
         uint32_t bt = kNoTriangle;
-        for (uint32_t k = lane; k < c; k += kWave) {
+        for (uint32_t k = lane; k < c; k += kListLanes) {
             const uint32_t t = tris[k];
             const double *b = cc.tri_box + 6 * size_t(t);
             if (!overlap(lo, hi, mk(b[0], b[1], b[2]), mk(b[3], b[4], b[5]), pad)) continue;
@@ -293,7 +298,7 @@ __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, 
                 bt = t;
             }
         }
-        wave_closest(bd, bt);
+        group_closest(bd, bt);
         if (lane == 0) best_tri[L] = bt;
     }
     if (partials) {
@@ -321,18 +326,18 @@ __device__ inline double advancement_limit(V3 p0, V3 p1, V3 d0, V3 d1, double sp
     return alpha;
 }
 
-// Line-search step bound, one wavefront per listed element with a free left
-// node, its lanes striding the list.
+// Line-search step bound, one wavefront per listed element with a free
+// left node, its lanes striding the list.
 __global__ __launch_bounds__(kBlock) void k_list_ccd(const uint32_t *flags, ColliderView col, ContactCache cc,
                                                      StepParams p, const double *q, const double *dir,
                                                      Partial *partials)
 {
-    const int lane = threadIdx.x & (kWave - 1);
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-    const uint32_t waves = gridDim.x * blockDim.x / kWave;
+    const int lane = threadIdx.x & (kListLanes - 1);
+    const uint32_t group = (blockIdx.x * blockDim.x + threadIdx.x) / kListLanes;
+    const uint32_t groups = gridDim.x * blockDim.x / kListLanes;
     const uint32_t listed = *cc.list_count;
     double limit = 1.0;
-    for (uint32_t j = wave; j < listed; j += waves) {
+    for (uint32_t j = group; j < listed; j += groups) {
         const uint32_t L = cc.list[j];
         const uint32_t c = cc.count[L];
         if (c == 0 || c == kCacheOverflow || (flags[L] & kNodeFixed)) continue;
@@ -343,7 +348,7 @@ __global__ __launch_bounds__(kBlock) void k_list_ccd(const uint32_t *flags, Coll
         const V3 lo = vmin(vmin(p0, p1), vmin(add(p0, d0), add(p1, d1)));
         const V3 hi = vmax(vmax(p0, p1), vmax(add(p0, d0), add(p1, d1)));
         const uint32_t *tris = cc.tris + size_t(L) * kCacheCapacity;
-        for (uint32_t k = lane; k < c; k += kWave) {
+        for (uint32_t k = lane; k < c; k += kListLanes) {
             const uint32_t t = tris[k];
             const double *b = cc.tri_box + 6 * size_t(t);
             if (!overlap(lo, hi, mk(b[0], b[1], b[2]), mk(b[3], b[4], b[5]), p.sweep_target)) continue;
@@ -392,9 +397,10 @@ template <bool GRAD, bool LS>
 __global__ __launch_bounds__(kBlock) void k_evaluate(RodView m, ColliderView col, ContactCache cc,
                                                      const uint32_t *best_tri, StepParams p, double *q,
                                                      const double *q_old, const double *vel, const double *q_base,
-                                                     const double *dir, double alpha, double *grad, double *Dblk,
-                                                     double *Ublk, Partial *partials)
+                                                     const double *dir, const double *alpha_at, double *grad,
+                                                     double *Dblk, double *Ublk, Partial *partials)
 {
+    const double alpha = LS ? *alpha_at : 0.0;
     const int lane = threadIdx.x & (kWave - 1);
     const long tile = long(blockIdx.x) * kTilesPerBlock + threadIdx.x / kWave;
     const long L = tile * kTile - 1 + lane;
@@ -905,8 +911,10 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
 // Slope g.d and the largest translation of a free node (host then limits
 // the direction by the trust radius, cuda_backend.cu:623-630).
 __global__ __launch_bounds__(kBlock) void k_direction_stats(RodView m, const double *grad, const double *dir,
-                                                            const unsigned *failed, Partial *partials)
+                                                            const unsigned *failed, const DirectionChoice *skip_if_used,
+                                                            Partial *partials)
 {
+    if (skip_if_used && skip_if_used->used) return;
     const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
     Partial acc = empty_partial();
     if (n < m.node_count) {
@@ -920,18 +928,70 @@ __global__ __launch_bounds__(kBlock) void k_direction_stats(RodView m, const dou
     block_reduce(acc, partials + blockIdx.x);
 }
 
-// Steepest descent preconditioned by the Jacobi diagonal (cuda_backend.cu:632-638).
-__global__ void k_steepest(RodView m, const double *grad, const double *precond, double *dir)
+// Steepest descent preconditioned by the Jacobi diagonal (cuda_backend.cu:632-638),
+// unless the choice kept the block direction.
+__global__ void k_steepest(RodView m, const DirectionChoice *choice, const double *grad, const double *precond,
+                           double *dir)
 {
+    if (choice->used) return;
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= size_t(m.node_count) * 6) return;
     dir[i] = (m.flags[i / 6] & kNodeFixed) ? 0.0 : (-grad[i]) * precond[i];
 }
 
-__global__ void k_scale(double *x, size_t count, double s)
+// The host decisions of the direction step (cuda_backend.cu:623-648) made
+// on the device from the reduced statistics: keep the block direction if
+// it was solved and g.d < 0; scale a direction whose largest free-node
+// translation exceeds the trust radius down to it.
+__global__ void k_choose_direction(const Partial *stats, double trust, DirectionChoice *choice)
 {
+    if (threadIdx.x || blockIdx.x) return;
+    const double slope = stats->sum[kSlope];
+    const int used = !stats->failed && slope < 0 && isfinite(slope);
+    choice->used = used;
+    choice->scale = (used && stats->max_translation > trust) ? trust / stats->max_translation : 1.0;
+}
+
+__global__ void k_force_steepest(DirectionChoice *choice)
+{
+    if (threadIdx.x || blockIdx.x) return;
+    choice->used = 0;
+    choice->scale = 1.0;
+}
+
+// Trust-radius factor of the steepest-descent direction.
+__global__ void k_choose_steepest_scale(const Partial *stats, double trust, DirectionChoice *choice)
+{
+    if (threadIdx.x || blockIdx.x || choice->used) return;
+    choice->scale = stats->max_translation > trust ? trust / stats->max_translation : 1.0;
+}
+
+// Applies the trust-radius factor to the direction of the given kind only
+// (used = 1 block, 0 steepest), so each direction is scaled once.
+__global__ void k_scale_direction(const DirectionChoice *choice, int used, double *x, size_t count)
+{
+    const double s = choice->scale;
+    if (choice->used != used || s == 1.0) return;
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < count) x[i] *= s;
+}
+
+// First line-search step: the CCD bound times 0.9 when it binds
+// (cuda_backend.cu:650-688).
+__global__ void k_choose_alpha(const Partial *ccd, int has_collider, double *alpha)
+{
+    if (threadIdx.x || blockIdx.x) return;
+    double a = 1.0;
+    if (has_collider) {
+        a = ccd->alpha;
+        if (a < 1.0) a *= 0.9;
+    }
+    *alpha = a;
+}
+
+__global__ void k_set_double(double *x, double v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *x = v;
 }
 
 __global__ void k_fill_u32(unsigned *x, size_t count, unsigned v)

@@ -251,3 +251,69 @@ def test_draped_strand_on_grid_matches_oracle(oracle_lib, gpu_lib, quad):
     assert out[1][2] == out[0][2]
     assert sum(out[1][2]) > 0
     assert out[1][3].minimum_gap > 1e-7
+
+
+def test_trust_radius_frames_match_oracle(oracle_lib, gpu_lib):
+    """A long frame with one substep: the free fall g dt^2 / 2 is far above
+    the trust radius, so every Newton direction is scaled down to it (and
+    the steepest-descent fallback may be taken). Positions and iteration
+    counts must follow the oracle."""
+    print("[gpu] trust-radius limited frames", flush=True)
+    pts = np.array([[0.01 * k, 0.0, 0.0] for k in range(8)])
+    out = []
+    for lib in (oracle_lib, gpu_lib):
+        desc = kn.default_desc(lib)
+        desc.substeps = 1
+        desc.maximum_substeps = 1
+        desc.fixed_root_nodes = 1
+        desc.maximum_element_length = 0.02
+        desc.newton_iterations = 12
+        mat = kn.default_material(lib)
+        mat.young_modulus = 1.0e7
+        mat.radius = 1.0e-4
+        s = kn.HairSolver(lib, desc)
+        s.check(s.set_hair(pts, [0, len(pts)], mat))
+        s.check(s.build())
+        s.check(s.set_convergence_mode(1))
+        stats = []
+        for _ in range(2):
+            s.check(s.step(0.25))
+            st = s.step_stats()
+            stats.append((st.newton_iterations, st.line_search_evaluations, st.linear_solves))
+        out.append((s.internal_positions(), stats))
+    err = float(np.max(np.abs(out[0][0] - out[1][0])))
+    print(f"max position error {err:.3e} m; oracle {out[0][1]} gpu {out[1][1]}")
+    assert out[1][1] == out[0][1]
+    assert err <= 1e-7
+    moved = float(np.max(np.abs(out[1][0] - pts)))
+    assert moved > 0.01  # the steps really were trust-limited
+
+
+def test_concurrent_solvers_match_alone(gpu_lib):
+    """Independent solvers stepped at once from several host threads (each
+    on its own stream) give bit-for-bit the positions of a solver run
+    alone: the batched form shares nothing between jobs."""
+    import threading
+    print("[gpu] concurrent solvers against one alone", flush=True)
+    scene = grooms.small_sphere_scene()
+
+    def run(s):
+        s.check(s.set_convergence_mode(1))
+        for _ in range(2):
+            s.check(s.step(scene.frame_dt))
+        return s.internal_positions()
+
+    alone = run(grooms.build_solver(gpu_lib, scene, **LIFTED))
+    solvers = [grooms.build_solver(gpu_lib, scene, **LIFTED) for _ in range(3)]
+    results = [None] * len(solvers)
+
+    def work(k):
+        results[k] = run(solvers[k])
+
+    threads = [threading.Thread(target=work, args=(k,)) for k in range(len(solvers))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    for r in results:
+        assert r is not None and np.array_equal(r, alone)
