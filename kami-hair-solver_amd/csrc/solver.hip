@@ -113,7 +113,8 @@ struct DeviceState {
     Buffer<unsigned char> sort_temp;
     size_t sort_temp_bytes = 0;
     // per-element contact candidate lists (valid for the current substep)
-    Buffer<uint32_t> cache_count, cache_tris, cache_listed, cache_list, cache_list_count;
+    Buffer<uint32_t> cache_count, cache_tris, cache_listed, cache_list, cache_list_count, cache_pending,
+        cache_pending_count;
     Buffer<double> cache_box;
     double cache_skin = 0.0;
     bool cache_valid = false;
@@ -212,8 +213,9 @@ struct DeviceState {
     }
     ContactCache cache() const
     {
-        return ContactCache{cache_skin,         cache_count.get(), cache_tris.get(),       cache_box.get(),
-                            cache_listed.get(), cache_list.get(),  cache_list_count.get(), tri_box_static.get()};
+        return ContactCache{cache_skin,       cache_count.get(),      cache_tris.get(),        cache_box.get(),
+                            cache_listed.get(), cache_list.get(),  cache_list_count.get(), cache_pending.get(),
+                            cache_pending_count.get(), tri_box_static.get()};
     }
 
     size_t resident_bytes() const
@@ -229,7 +231,8 @@ struct DeviceState {
             b += x->bytes();
         b += bvh.bytes() + parent_internal.bytes() + parent_leaf.bytes() + counters.bytes() + sort_temp.bytes() +
              partials.bytes() + reduced.bytes() + cache_count.bytes() + cache_tris.bytes() + cache_box.bytes() +
-             cache_listed.bytes() + cache_list.bytes() + cache_list_count.bytes() + best_tri.bytes();
+             cache_listed.bytes() + cache_list.bytes() + cache_list_count.bytes() + cache_pending.bytes() +
+             best_tri.bytes();
         return b;
     }
 
@@ -581,6 +584,8 @@ KhsResult Solver::upload_model()
         d.cache_listed.allocate(d.N);
         d.cache_list.allocate(d.N);
         d.cache_list_count.allocate(1);
+        d.cache_pending.allocate(d.N);
+        d.cache_pending_count.allocate(1);
         d.best_tri.allocate(d.N);
         rebuild_bvh(false);
     }
@@ -632,6 +637,7 @@ void Solver::refit_bvh(bool swept)
 // Brings the candidate lists up to date for a query pass (all of them
 // after the collider boxes changed): elements whose query box left their
 // cached box gather again.
+constexpr unsigned kGatherBlocks = 512;
 void Solver::refresh_contact_cache(int mode, bool full)
 {
     DeviceState &d = *dev_;
@@ -645,17 +651,26 @@ void Solver::refresh_contact_cache(int mode, bool full)
                            size_t(d.N), 0u);
         hipLaunchKernelGGL(dev::k_fill_u32, dim3(1), dim3(64), 0, d.stream, d.cache_list_count.get(), size_t(1), 0u);
     }
-    const int f = full ? 1 : 0;
     const dim3 grid(d.node_blocks()), block(kBlock);
-    if (mode == dev::kAtState)
+    if (full) {
+        // every element gathers: one lane each
         hipLaunchKernelGGL(dev::k_contact_refresh<dev::kAtState>, grid, block, 0, d.stream, d.rods(), d.collider(), cc,
-                           d.q.get(), d.q_base.get(), d.dir.get(), d.alpha.get(), pad, f);
-    else if (mode == dev::kAtTrial)
-        hipLaunchKernelGGL(dev::k_contact_refresh<dev::kAtTrial>, grid, block, 0, d.stream, d.rods(), d.collider(), cc,
-                           d.q.get(), d.q_base.get(), d.dir.get(), d.alpha.get(), pad, f);
-    else
-        hipLaunchKernelGGL(dev::k_contact_refresh<dev::kSwept>, grid, block, 0, d.stream, d.rods(), d.collider(), cc,
-                           d.q.get(), d.q_base.get(), d.dir.get(), d.alpha.get(), pad, f);
+                           d.q.get(), d.q_base.get(), d.dir.get(), d.alpha.get(), pad, 1);
+    } else {
+        // the few elements that left their box gather with a wavefront each
+        hipLaunchKernelGGL(dev::k_fill_u32, dim3(1), dim3(64), 0, d.stream, d.cache_pending_count.get(), size_t(1),
+                           0u);
+        if (mode == dev::kAtState)
+            hipLaunchKernelGGL(dev::k_contact_check<dev::kAtState>, grid, block, 0, d.stream, d.rods(), cc, d.q.get(),
+                               d.q_base.get(), d.dir.get(), d.alpha.get(), pad);
+        else if (mode == dev::kAtTrial)
+            hipLaunchKernelGGL(dev::k_contact_check<dev::kAtTrial>, grid, block, 0, d.stream, d.rods(), cc, d.q.get(),
+                               d.q_base.get(), d.dir.get(), d.alpha.get(), pad);
+        else
+            hipLaunchKernelGGL(dev::k_contact_check<dev::kSwept>, grid, block, 0, d.stream, d.rods(), cc, d.q.get(),
+                               d.q_base.get(), d.dir.get(), d.alpha.get(), pad);
+        hipLaunchKernelGGL(dev::k_contact_gather, dim3(kGatherBlocks), block, 0, d.stream, d.collider(), cc);
+    }
     HIP_TRY(hipGetLastError());
     d.cache_valid = true;
 }

@@ -222,7 +222,7 @@ __global__ void k_contact_refresh(RodView m, ColliderView col, ContactCache cc, 
     double *box = cc.box + 6 * size_t(L);
     if (!full && box_covered(box, lo, hi, pad)) return;
     const double grow = pad + cc.skin;
-    const V3 blo = mk(lo.x - grow, lo.y - grow, lo.z - grow), bhi = mk(hi.x + grow, hi.y + grow, hi.z + grow);
+    V3 blo = mk(lo.x - grow, lo.y - grow, lo.z - grow), bhi = mk(hi.x + grow, hi.y + grow, hi.z + grow);
     uint32_t *tris = cc.tris + size_t(L) * kCacheCapacity;
     uint32_t n = 0;
     bvh_visit(col.bvh, internal_count(col), col.leaf_box, blo, bhi, 0.0, [&](uint32_t t) {
@@ -230,6 +230,21 @@ __global__ void k_contact_refresh(RodView m, ColliderView col, ContactCache cc, 
         ++n;
         return n <= kCacheCapacity;  // an overflowed list is not used
     });
+    if (n == 0) {
+        // Nothing near: keep a box eight skins wider if it is empty too, so
+        // hair moving through free space rarely gathers again.
+        const double far = pad + 8.0 * cc.skin;
+        const V3 flo = mk(lo.x - far, lo.y - far, lo.z - far), fhi = mk(hi.x + far, hi.y + far, hi.z + far);
+        bool empty = true;
+        bvh_visit(col.bvh, internal_count(col), col.leaf_box, flo, fhi, 0.0, [&](uint32_t) {
+            empty = false;
+            return false;
+        });
+        if (empty) {
+            blo = flo;
+            bhi = fhi;
+        }
+    }
     cc.count[L] = n <= kCacheCapacity ? n : kCacheOverflow;
     store3(box, blo);
     store3(box + 3, bhi);
@@ -241,6 +256,128 @@ __global__ void k_contact_refresh(RodView m, ColliderView col, ContactCache cc, 
 
 // Lanes per listed element in the list passes (one wavefront).
 constexpr int kListLanes = kWave;
+
+// Entries of a wavefront's breadth-first traversal queue (LDS).
+constexpr uint32_t kGatherQueue = 1024;
+
+// Wavefront-cooperative walk of the hierarchy for one box (pad 0, leaves
+// tested against col.leaf_box): each step takes up to 64 queued nodes, one
+// per lane, tests both children, and compacts the overlapping internal
+// children onto the queue and the overlapping triangles into out (first
+// `cap` kept). Returns the count of triangles found, or cap + 1 once it
+// exceeds cap or the queue would overflow (the list is then not used).
+// With stop_at_first it stops at the first triangle. All lanes call it with
+// the same box; every quantity steering the loop is wave-uniform.
+__device__ inline uint32_t wave_gather(const ColliderView &col, V3 lo, V3 hi, uint32_t *out, uint32_t cap,
+                                       volatile int *queue, bool stop_at_first)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const unsigned long long below = __lanemask_lt();
+    const int ic = internal_count(col);
+    uint32_t head = 0, tail = 1, n = 0;
+    if (lane == 0) queue[0] = 0;
+    // A tree of ic internal nodes queues each at most once; the bound only
+    // ends the walk should the tree be malformed.
+    while (head < tail && head <= uint32_t(ic) + 1u) {
+        const uint32_t take = (tail - head) < uint32_t(kWave) ? (tail - head) : uint32_t(kWave);
+        const int node = uint32_t(lane) < take ? queue[(head + lane) % kGatherQueue] : -1;
+        head += take;
+        bool push[2] = {false, false}, emit[2] = {false, false};
+        int child[2] = {-1, -1};
+        if (node >= 0) {
+            const BvhNode &nd = col.bvh[node];
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const V3 bl = mk(nd.box[6 * c], nd.box[6 * c + 1], nd.box[6 * c + 2]);
+                const V3 bu = mk(nd.box[6 * c + 3], nd.box[6 * c + 4], nd.box[6 * c + 5]);
+                const int ch = nd.child[c];
+                child[c] = ch;
+                if (!overlap(lo, hi, bl, bu, 0.0) || ch >= ic) continue;
+                if (ch >= 0) {
+                    push[c] = true;
+                } else if (col.leaf_box) {
+                    const double *b = col.leaf_box + 6 * size_t(-1 - ch);
+                    emit[c] = overlap(lo, hi, mk(b[0], b[1], b[2]), mk(b[3], b[4], b[5]), 0.0);
+                } else {
+                    emit[c] = true;
+                }
+            }
+        }
+        const unsigned long long p0 = __ballot(push[0]), p1 = __ballot(push[1]);
+        const uint32_t pushes = __popcll(p0) + __popcll(p1);
+        if (tail - head + pushes > kGatherQueue) return cap + 1;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const unsigned long long pm = c == 0 ? p0 : p1;
+            if (push[c]) queue[(tail + __popcll(pm & below)) % kGatherQueue] = child[c];
+            tail += __popcll(pm);
+            const unsigned long long em = __ballot(emit[c]);
+            const uint32_t at = n + __popcll(em & below);
+            if (emit[c] && at < cap) out[at] = uint32_t(-1 - child[c]);
+            n += __popcll(em);
+        }
+        if (n > cap || (stop_at_first && n > 0)) return n > cap ? cap + 1 : n;
+    }
+    return n;
+}
+
+// Partial refresh, first half: every contact element whose query box
+// (grown by pad plus the rounding margin) left its cached box gets the new
+// box (query box grown by pad + skin) and is queued for gathering.
+template <int MODE>
+__global__ void k_contact_check(RodView m, ContactCache cc, const double *q, const double *q_base, const double *dir,
+                                const double *alpha_at, double pad)
+{
+    const double alpha = MODE == kAtTrial ? *alpha_at : 0.0;
+    const uint32_t L = blockIdx.x * blockDim.x + threadIdx.x;
+    if (L >= m.node_count) return;
+    const uint32_t fl = m.flags[L];
+    if (!(fl & kNodeHasElement) || !(fl & kNodeContact)) return;
+    V3 lo, hi;
+    query_box<MODE>(L, q, q_base, dir, alpha, lo, hi);
+    double *box = cc.box + 6 * size_t(L);
+    if (box_covered(box, lo, hi, pad)) return;
+    const double grow = pad + cc.skin;
+    store3(box, mk(lo.x - grow, lo.y - grow, lo.z - grow));
+    store3(box + 3, mk(hi.x + grow, hi.y + grow, hi.z + grow));
+    cc.pending[atomicAdd(cc.pending_count, 1u)] = L;
+}
+
+// Partial refresh, second half: one wavefront per queued element gathers
+// the triangles overlapping its new box; an empty result keeps a box seven
+// skins wider still when that is empty too.
+__global__ __launch_bounds__(kBlock) void k_contact_gather(ColliderView col, ContactCache cc)
+{
+    __shared__ int queues[kTilesPerBlock][kGatherQueue];
+    volatile int *queue = queues[threadIdx.x / kWave];
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const uint32_t waves = gridDim.x * blockDim.x / kWave;
+    const uint32_t pending = *cc.pending_count;
+    for (uint32_t j = wave; j < pending; j += waves) {
+        const uint32_t L = cc.pending[j];
+        double *box = cc.box + 6 * size_t(L);
+        const V3 lo = mk(box[0], box[1], box[2]), hi = mk(box[3], box[4], box[5]);
+        uint32_t *tris = cc.tris + size_t(L) * kCacheCapacity;
+        const uint32_t n = wave_gather(col, lo, hi, tris, kCacheCapacity, queue, false);
+        if (n == 0) {
+            const double far = 7.0 * cc.skin;
+            const V3 flo = mk(lo.x - far, lo.y - far, lo.z - far), fhi = mk(hi.x + far, hi.y + far, hi.z + far);
+            uint32_t none;
+            if (wave_gather(col, flo, fhi, &none, 0, queue, true) == 0 && lane == 0) {
+                store3(box, flo);
+                store3(box + 3, fhi);
+            }
+        }
+        if (lane == 0) {
+            cc.count[L] = n <= kCacheCapacity ? n : kCacheOverflow;
+            if (n > 0 && !cc.listed[L]) {
+                cc.listed[L] = 1;
+                cc.list[atomicAdd(cc.list_count, 1u)] = L;
+            }
+        }
+    }
+}
 
 // Minimum of (distance, triangle) in lexicographic order across a list
 // pass's lane group; NaN distances never enter (callers skip them).
