@@ -638,7 +638,7 @@ void Solver::refit_bvh(bool swept)
 // after the collider boxes changed): elements whose query box left their
 // cached box gather again.
 constexpr unsigned kGatherBlocks = 512;
-void Solver::refresh_contact_cache(int mode, bool full)
+void Solver::refresh_contact_cache(int mode, bool full, bool swept_leaves)
 {
     DeviceState &d = *dev_;
     const double contact_pad = material_.radius + material_.collider_offset + material_.barrier_distance;
@@ -654,8 +654,9 @@ void Solver::refresh_contact_cache(int mode, bool full)
     const dim3 grid(d.node_blocks()), block(kBlock);
     if (full) {
         // every element gathers: one lane each
-        hipLaunchKernelGGL(dev::k_contact_refresh<dev::kAtState>, grid, block, 0, d.stream, d.rods(), d.collider(), cc,
-                           d.q.get(), d.q_base.get(), d.dir.get(), d.alpha.get(), pad, 1);
+        hipLaunchKernelGGL(dev::k_contact_refresh<dev::kAtState>, grid, block, 0, d.stream, d.rods(),
+                           swept_leaves ? d.swept_collider() : d.collider(), cc, d.q.get(), d.q_base.get(),
+                           d.dir.get(), d.alpha.get(), pad, 1);
     } else {
         // the few elements that left their box gather with a wavefront each
         hipLaunchKernelGGL(dev::k_fill_u32, dim3(1), dim3(64), 0, d.stream, d.cache_pending_count.get(), size_t(1),
@@ -1222,11 +1223,20 @@ KhsResult Solver::run_frame(double frame_dt, const double *root_a, const double 
                                    d.stream, d.V, d.col_cur.get(), d.col_old.get(), collider_a, collider_b, f);
                 hipEvent_t t0 = d.tic();
                 rebuild_bvh(true);
+                // Every list is gathered against the swept triangle boxes:
+                // the sweep below queries those, and since they contain the
+                // current boxes the lists also serve the substep's other
+                // queries, which filter with the current boxes.
+                refresh_contact_cache(dev::kAtState, true, true);
                 phase_.store(KHS_PHASE_CCD);
                 const unsigned long long none = ~0ull;
                 hipLaunchKernelGGL(dev::k_fill_u64, dim3(1), dim3(64), 0, d.stream, d.key64.get(), size_t(1), none);
+                hipLaunchKernelGGL(dev::k_list_sweep, dim3(kPairBlocks), dim3(kBlock), 0, d.stream, d.flags.get(),
+                                   d.strand_of.get(), d.collider(), d.cache(), d.tri_box.get(), sp, d.q.get(),
+                                   d.key64.get());
                 hipLaunchKernelGGL(dev::k_sweep, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv,
-                                   d.swept_collider(), sp, d.q.get(), d.key64.get());
+                                   d.swept_collider(), sp, d.q.get(), (const uint32_t *)d.cache_count.get(),
+                                   d.key64.get());
                 hipLaunchKernelGGL(dev::k_sweep_record, dim3(1), dim3(1), 0, d.stream, rv, d.collider(), sp,
                                    d.q.get(), d.key64.get(), d.sweep.get());
                 HIP_TRY(hipGetLastError());
@@ -1254,9 +1264,6 @@ KhsResult Solver::run_frame(double frame_dt, const double *root_a, const double 
                 // The swept tree serves the rest of the substep: its boxes
                 // contain the current ones, and queries test leaves against
                 // the current boxes.
-                hipEvent_t t1 = d.tic();
-                refresh_contact_cache(dev::kAtState, true);
-                d.toc(t1, d.totals.collision_ms, d.totals.collision_launches);
             }
             phase_.store(KHS_PHASE_ASSEMBLING);
             result = optimize_substep(dt);

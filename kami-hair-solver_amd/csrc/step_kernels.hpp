@@ -1266,39 +1266,90 @@ __global__ __launch_bounds__(kBlock) void k_ccd_slope(RodView m, ColliderView co
 // Conservative advancement of each candidate triangle from its previous to
 // its current position against the stationary segment
 // (cuda_backend.cu:975-987); the lowest failing (element, triangle) wins.
-__global__ void k_sweep(RodView m, ColliderView col, StepParams p, const double *q, unsigned long long *first_failure)
+// Whether triangle t, moving from its previous to its current position,
+// cannot be advanced to the end of the substep against the stationary
+// segment while keeping the target clearance (conservative advancement).
+__device__ inline bool sweep_fails(const ColliderView &col, V3 p0, V3 p1, uint32_t t, double target)
+{
+    const uint32_t *t3 = col.triangles + 3 * size_t(t);
+    const V3 oa = vertex_prev(col, t3[0]), ob = vertex_prev(col, t3[1]), oc = vertex_prev(col, t3[2]);
+    const V3 da = sub(vertex(col, t3[0]), oa), db = sub(vertex(col, t3[1]), ob), dc = sub(vertex(col, t3[2]), oc);
+    const double speed = fmax(norm(da), fmax(norm(db), norm(dc)));
+    if (speed < 1e-30) return false;
+    double alpha = 0.0;
+    for (int it = 0; it < 80; ++it) {
+        const Closest pair =
+            closest_segment_triangle(p0, p1, add(oa, mul(da, alpha)), add(ob, mul(db, alpha)), add(oc, mul(dc, alpha)));
+        const double clearance = pair.distance - target;
+        if (clearance <= 0) return true;
+        const double advance = 0.8 * clearance / speed;
+        if (alpha + advance >= 1) return false;
+        if (advance < 1e-12) return true;
+        alpha += advance;
+    }
+    return true;
+}
+
+// Collider sweep by walking the swept tree, one lane per element; with
+// only_overflow (the list counts) just the elements whose list overflowed.
+__global__ void k_sweep(RodView m, ColliderView col, StepParams p, const double *q, const uint32_t *only_overflow,
+                        unsigned long long *first_failure)
 {
     const uint32_t L = blockIdx.x * blockDim.x + threadIdx.x;
     if (L >= m.node_count) return;
     const uint32_t fl = m.flags[L];
     if (!(fl & kNodeHasElement) || !(fl & kNodeContact) || (fl & kNodeFixed)) return;
+    if (only_overflow && only_overflow[L] != kCacheOverflow) return;
     const V3 p0 = load3(q + 6 * size_t(L)), p1 = load3(q + 6 * size_t(L + 1));
     const uint32_t element = L - m.strand_of[L];
     unsigned long long worst = ~0ull;
     bvh_query(col.bvh, internal_count(col), col.leaf_box, vmin(p0, p1), vmax(p0, p1), p.sweep_target, [&](uint32_t t) {
-        const uint32_t *t3 = col.triangles + 3 * size_t(t);
-        const V3 oa = vertex_prev(col, t3[0]), ob = vertex_prev(col, t3[1]), oc = vertex_prev(col, t3[2]);
-        const V3 da = sub(vertex(col, t3[0]), oa), db = sub(vertex(col, t3[1]), ob), dc = sub(vertex(col, t3[2]), oc);
-        const double speed = fmax(norm(da), fmax(norm(db), norm(dc)));
-        if (speed < 1e-30) return;
-        double alpha = 0.0;
-        bool bad = true;
-        for (int it = 0; it < 80; ++it) {
-            const Closest pair = closest_segment_triangle(p0, p1, add(oa, mul(da, alpha)), add(ob, mul(db, alpha)),
-                                                          add(oc, mul(dc, alpha)));
-            const double clearance = pair.distance - p.sweep_target;
-            if (clearance <= 0) break;
-            const double advance = 0.8 * clearance / speed;
-            if (alpha + advance >= 1) { bad = false; break; }
-            if (advance < 1e-12) break;
-            alpha += advance;
-        }
-        if (bad) {
+        if (sweep_fails(col, p0, p1, t, p.sweep_target)) {
             const unsigned long long key = (static_cast<unsigned long long>(element) << 32) | t;
             worst = key < worst ? key : worst;
         }
     });
     if (worst != ~0ull) atomicMin(first_failure, worst);
+}
+
+// Collider sweep from the candidate lists, one wavefront per listed element
+// with a free left node, its lanes striding the list. The lists were
+// gathered against the swept triangle boxes (`swept_box`), which the sweep
+// queries too; the lowest failing (element, triangle) is kept.
+__global__ __launch_bounds__(kBlock) void k_list_sweep(const uint32_t *flags, const uint32_t *strand_of,
+                                                       ColliderView col, ContactCache cc, const double *swept_box,
+                                                       StepParams p, const double *q,
+                                                       unsigned long long *first_failure)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const uint32_t waves = gridDim.x * blockDim.x / kWave;
+    const uint32_t listed = *cc.list_count;
+    for (uint32_t j = wave; j < listed; j += waves) {
+        const uint32_t L = cc.list[j];
+        const uint32_t c = cc.count[L];
+        if (c == 0 || c == kCacheOverflow || (flags[L] & kNodeFixed)) continue;
+        const V3 p0 = load3(q + 6 * size_t(L)), p1 = load3(q + 6 * size_t(L + 1));
+        const V3 lo = vmin(p0, p1), hi = vmax(p0, p1);
+        const uint32_t element = L - strand_of[L];
+        const uint32_t *tris = cc.tris + size_t(L) * kCacheCapacity;
+        unsigned long long worst = ~0ull;
+        for (uint32_t k = lane; k < c; k += kWave) {
+            const uint32_t t = tris[k];
+            const double *b = swept_box + 6 * size_t(t);
+            if (!overlap(lo, hi, mk(b[0], b[1], b[2]), mk(b[3], b[4], b[5]), p.sweep_target)) continue;
+            if (sweep_fails(col, p0, p1, t, p.sweep_target)) {
+                const unsigned long long key = (static_cast<unsigned long long>(element) << 32) | t;
+                worst = key < worst ? key : worst;
+            }
+        }
+#pragma unroll
+        for (int o = kWave / 2; o >= 1; o >>= 1) {
+            const unsigned long long other = __shfl_xor(worst, o, kWave);
+            worst = other < worst ? other : worst;
+        }
+        if (lane == 0 && worst != ~0ull) atomicMin(first_failure, worst);
+    }
 }
 
 // Recomputes the advancement of the reported pair to fill the diagnostics
