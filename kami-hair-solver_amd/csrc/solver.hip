@@ -126,8 +126,9 @@ struct DeviceState {
     Buffer<SweepRecord> sweep;
     Buffer<double> out3;
     Buffer<uint32_t> pairs;
-    Buffer<DirectionChoice> choice;
     Buffer<double> alpha;  // line-search step read by the trial kernels
+    uint32_t pending_slot = 0;  // which pending counter the next partial refresh uses
+    DirectionChoice *choice() const { return reinterpret_cast<DirectionChoice *>(reduced.get() + 3); }
 
     // Event timing of kernel classes (enabled on request) and of substeps.
     struct Pending {
@@ -215,7 +216,7 @@ struct DeviceState {
     {
         return ContactCache{cache_skin,       cache_count.get(),      cache_tris.get(),        cache_box.get(),
                             cache_listed.get(), cache_list.get(),  cache_list_count.get(), cache_pending.get(),
-                            cache_pending_count.get(), tri_box_static.get()};
+                            cache_pending_count.get() + pending_slot, tri_box_static.get()};
     }
 
     size_t resident_bytes() const
@@ -256,9 +257,8 @@ struct DeviceState {
     HostView read_back()
     {
         static_assert(sizeof(HostView) == 3 * sizeof(Partial) + sizeof(DirectionChoice), "packed readback");
-        HIP_TRY(hipMemcpyAsync(host_view, reduced.get(), 3 * sizeof(Partial), hipMemcpyDeviceToHost, stream));
-        HIP_TRY(hipMemcpyAsync(&host_view->choice, choice.get(), sizeof(DirectionChoice), hipMemcpyDeviceToHost,
-                               stream));
+        // reduced holds the three Partials followed by the choice
+        HIP_TRY(hipMemcpyAsync(host_view, reduced.get(), sizeof(HostView), hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
         settle();
         return *host_view;
@@ -470,7 +470,6 @@ KhsResult Solver::upload_model()
     HIP_TRY(hipEventCreate(&d.frame_begin));
     HIP_TRY(hipEventCreate(&d.frame_end));
     HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&d.host_view), sizeof(DeviceState::HostView)));
-    d.choice.allocate(1);
     d.alpha.allocate(1);
     const RodModel &m = model_;
     d.N = m.node_count;
@@ -543,7 +542,8 @@ KhsResult Solver::upload_model()
     d.out3.allocate(3 * std::max<size_t>(N, d.P));
     const uint32_t max_blocks = std::max(d.tiled_blocks(), d.node_blocks()) + 1;
     d.partials.allocate(max_blocks + kPairBlocks + 16);
-    d.reduced.allocate(3);
+    d.reduced.allocate(4);  // three reductions, then the direction choice (at most one Partial)
+    static_assert(sizeof(DirectionChoice) <= sizeof(Partial), "choice slot");
     d.flag.allocate(1);
     d.key64.allocate(1);
     d.sweep.allocate(1);
@@ -585,7 +585,8 @@ KhsResult Solver::upload_model()
         d.cache_list.allocate(d.N);
         d.cache_list_count.allocate(1);
         d.cache_pending.allocate(d.N);
-        d.cache_pending_count.allocate(1);
+        d.cache_pending_count.allocate(2);
+        HIP_TRY(hipMemset(d.cache_pending_count.get(), 0, d.cache_pending_count.bytes()));
         d.best_tri.allocate(d.N);
         rebuild_bvh(false);
     }
@@ -658,9 +659,9 @@ void Solver::refresh_contact_cache(int mode, bool full, bool swept_leaves)
                            swept_leaves ? d.swept_collider() : d.collider(), cc, d.q.get(), d.q_base.get(),
                            d.dir.get(), d.alpha.get(), pad, 1);
     } else {
-        // the few elements that left their box gather with a wavefront each
-        hipLaunchKernelGGL(dev::k_fill_u32, dim3(1), dim3(64), 0, d.stream, d.cache_pending_count.get(), size_t(1),
-                           0u);
+        // the few elements that left their box gather with a wavefront each;
+        // the two pending counters alternate, each cleared by the gather
+        // before its next use
         if (mode == dev::kAtState)
             hipLaunchKernelGGL(dev::k_contact_check<dev::kAtState>, grid, block, 0, d.stream, d.rods(), cc, d.q.get(),
                                d.q_base.get(), d.dir.get(), d.alpha.get(), pad);
@@ -670,7 +671,9 @@ void Solver::refresh_contact_cache(int mode, bool full, bool swept_leaves)
         else
             hipLaunchKernelGGL(dev::k_contact_check<dev::kSwept>, grid, block, 0, d.stream, d.rods(), cc, d.q.get(),
                                d.q_base.get(), d.dir.get(), d.alpha.get(), pad);
-        hipLaunchKernelGGL(dev::k_contact_gather, dim3(kGatherBlocks), block, 0, d.stream, d.collider(), cc);
+        hipLaunchKernelGGL(dev::k_contact_gather, dim3(kGatherBlocks), block, 0, d.stream, d.collider(), cc,
+                           d.cache_pending_count.get() + (d.pending_slot ^ 1u));
+        d.pending_slot ^= 1u;
     }
     HIP_TRY(hipGetLastError());
     d.cache_valid = true;
@@ -965,16 +968,16 @@ void Solver::steepest_direction_async()
     DeviceState &d = *dev_;
     const RodView rv = d.rods();
     const size_t dofs = 6 * size_t(d.N);
-    hipLaunchKernelGGL(dev::k_steepest, dim3(grid_for(dofs, kBlock)), dim3(kBlock), 0, d.stream, rv, d.choice.get(),
+    hipLaunchKernelGGL(dev::k_steepest, dim3(grid_for(dofs, kBlock)), dim3(kBlock), 0, d.stream, rv, d.choice(),
                        d.grad.get(), d.precond.get(), d.dir.get());
     hipLaunchKernelGGL(dev::k_direction_stats, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, d.grad.get(),
-                       d.dir.get(), (const unsigned *)nullptr, (const DirectionChoice *)d.choice.get(),
+                       d.dir.get(), (const unsigned *)nullptr, (const DirectionChoice *)d.choice(),
                        d.partials.get());
     d.reduce_async(d.node_blocks(), DeviceState::kSteepSlot);
     hipLaunchKernelGGL(dev::k_choose_steepest_scale, dim3(1), dim3(1), 0, d.stream,
-                       (const Partial *)(d.reduced.get() + DeviceState::kSteepSlot), trust_radius_, d.choice.get());
+                       (const Partial *)(d.reduced.get() + DeviceState::kSteepSlot), trust_radius_, d.choice());
     hipLaunchKernelGGL(dev::k_scale_direction, dim3(grid_for(dofs, kBlock)), dim3(kBlock), 0, d.stream,
-                       (const DirectionChoice *)d.choice.get(), 0, d.dir.get(), dofs);
+                       (const DirectionChoice *)d.choice(), 0, d.dir.get(), dofs);
     HIP_TRY(hipGetLastError());
 }
 
@@ -997,11 +1000,10 @@ void Solver::make_direction_async()
                        d.dir.get(), (const unsigned *)d.flag.get(), (const DirectionChoice *)nullptr,
                        d.partials.get());
     d.reduce_async(d.node_blocks(), DeviceState::kEvalSlot);
-    hipLaunchKernelGGL(dev::k_choose_direction, dim3(1), dim3(1), 0, d.stream,
-                       (const Partial *)(d.reduced.get() + DeviceState::kEvalSlot), trust_radius_, d.choice.get());
     const size_t dofs = 6 * size_t(d.N);
-    hipLaunchKernelGGL(dev::k_scale_direction, dim3(grid_for(dofs, kBlock)), dim3(kBlock), 0, d.stream,
-                       (const DirectionChoice *)d.choice.get(), 1, d.dir.get(), dofs);
+    hipLaunchKernelGGL(dev::k_apply_direction_choice, dim3(grid_for(dofs, kBlock)), dim3(kBlock), 0, d.stream,
+                       (const Partial *)(d.reduced.get() + DeviceState::kEvalSlot), trust_radius_, d.choice(),
+                       d.dir.get(), dofs);
     HIP_TRY(hipGetLastError());
 }
 
@@ -1122,7 +1124,7 @@ KhsResult Solver::optimize_substep(double dt)
             if (accepted) break;
             if (!block_direction) break;
             HIP_TRY(hipMemcpyAsync(d.q.get(), d.q_base.get(), d.q.bytes(), hipMemcpyDeviceToDevice, d.stream));
-            hipLaunchKernelGGL(dev::k_force_steepest, dim3(1), dim3(1), 0, d.stream, d.choice.get());
+            hipLaunchKernelGGL(dev::k_force_steepest, dim3(1), dim3(1), 0, d.stream, d.choice());
             steepest_direction_async();
             block_direction = false;
             ++attempt;

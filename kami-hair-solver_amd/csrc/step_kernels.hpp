@@ -345,8 +345,9 @@ __global__ void k_contact_check(RodView m, ContactCache cc, const double *q, con
 
 // Partial refresh, second half: one wavefront per queued element gathers
 // the triangles overlapping its new box; an empty result keeps a box seven
-// skins wider still when that is empty too.
-__global__ __launch_bounds__(kBlock) void k_contact_gather(ColliderView col, ContactCache cc)
+// skins wider still when that is empty too. It also clears the other
+// pending counter, which the next partial refresh appends to.
+__global__ __launch_bounds__(kBlock) void k_contact_gather(ColliderView col, ContactCache cc, uint32_t *next_count)
 {
     __shared__ int queues[kTilesPerBlock][kGatherQueue];
     volatile int *queue = queues[threadIdx.x / kWave];
@@ -354,6 +355,7 @@ __global__ __launch_bounds__(kBlock) void k_contact_gather(ColliderView col, Con
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const uint32_t waves = gridDim.x * blockDim.x / kWave;
     const uint32_t pending = *cc.pending_count;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *next_count = 0;
     for (uint32_t j = wave; j < pending; j += waves) {
         const uint32_t L = cc.pending[j];
         double *box = cc.box + 6 * size_t(L);
@@ -1087,6 +1089,22 @@ __global__ void k_choose_direction(const Partial *stats, double trust, Direction
     const int used = !stats->failed && slope < 0 && isfinite(slope);
     choice->used = used;
     choice->scale = (used && stats->max_translation > trust) ? trust / stats->max_translation : 1.0;
+}
+
+// k_choose_direction and the block direction's scaling in one pass: every
+// thread derives the choice from the reduced statistics, one records it.
+__global__ void k_apply_direction_choice(const Partial *stats, double trust, DirectionChoice *choice, double *x,
+                                         size_t count)
+{
+    const double slope = stats->sum[kSlope];
+    const int used = !stats->failed && slope < 0 && isfinite(slope);
+    const double scale = (used && stats->max_translation > trust) ? trust / stats->max_translation : 1.0;
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        choice->used = used;
+        choice->scale = scale;
+    }
+    if (used && scale != 1.0 && i < count) x[i] *= scale;
 }
 
 __global__ void k_force_steepest(DirectionChoice *choice)
