@@ -18,6 +18,12 @@ struct ShimIndex {
 inline thread_local ShimIndex threadIdx, blockIdx, blockDim;
 inline void __syncthreads() {}
 inline int __clz(uint32_t v) { return v ? __builtin_clz(v) : 32; }
+inline unsigned atomicAdd(unsigned *p, unsigned v)
+{
+    const unsigned o = *p;
+    *p += v;
+    return o;
+}
 inline unsigned long long atomicAdd(unsigned long long *p, unsigned long long v)
 {
     const unsigned long long o = *p;

@@ -272,6 +272,117 @@ __global__ void k_refit(uint32_t count, const uint32_t *sorted_ids, const double
     }
 }
 
+// Two-phase bottom-up boxes. Workgroup w (256 threads) starts at sorted
+// leaves [256 w, 256 w + 256); every node whose leaf range lies inside that
+// chunk is reached only by the workgroup's threads, so phase 1 combines
+// their boxes through LDS slots and counters. A node whose range crosses a
+// chunk boundary receives its child boxes in global memory and an arrival on
+// its global counter; the arrival that completes both children queues it.
+// Phase 2, one workgroup after the kernel boundary, climbs from the queued
+// nodes to the root. No acquire or release beyond the workgroup is needed.
+constexpr int kRefitChunk = 256;
+
+struct RefitLocal {
+    double slot[kRefitChunk][12];  // child boxes of the chunk's nodes
+    unsigned count[kRefitChunk];
+};
+
+__device__ inline void refit_local_init(RefitLocal &s) { s.count[threadIdx.x] = 0; }
+
+__device__ inline void refit_local_climb(RefitLocal &s, uint32_t count, const uint32_t *sorted_ids,
+                                         const double *tri_box, const int *parent_internal, const int *parent_leaf,
+                                         BvhNode *nodes, unsigned *counters, uint32_t *ready, unsigned *ready_count)
+{
+    const uint32_t base = blockIdx.x * kRefitChunk;
+    const uint32_t k = base + threadIdx.x;
+    if (k >= count) return;
+    // six separate 64-bit loads (see k_refit)
+    double box[6];
+    const volatile double *b = tri_box + 6 * size_t(sorted_ids[k]);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) box[c] = b[c];
+    int link = parent_leaf[k];
+    for (int level = 0; level < 128; ++level) {
+        const int parent = link >> 1, side = link & 1;
+        const bool local = nodes[parent].range[0] >= int(base) && nodes[parent].range[1] < int(base) + kRefitChunk;
+        if (!local) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) nodes[parent].box[6 * side + c] = box[c];
+            if (__hip_atomic_fetch_add(&counters[parent], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 1)
+                ready[__hip_atomic_fetch_add(ready_count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)] =
+                    uint32_t(parent);
+            return;
+        }
+        const int li = parent - int(base);
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+            __hip_atomic_store(&s.slot[li][6 * side + c], box[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (__hip_atomic_fetch_add(&s.count[li], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0) return;
+        double both[12];
+#pragma unroll
+        for (int c = 0; c < 12; ++c)
+            both[c] = __hip_atomic_load(&s.slot[li][c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+#pragma unroll
+        for (int c = 0; c < 12; ++c) nodes[parent].box[c] = both[c];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            box[c] = fmin(both[c], both[6 + c]);
+            box[3 + c] = fmax(both[3 + c], both[9 + c]);
+        }
+        link = parent_internal[parent];
+        if (link < 0) return;
+    }
+}
+
+__global__ __launch_bounds__(kRefitChunk) void k_refit_local(uint32_t count, const uint32_t *sorted_ids,
+                                                             const double *tri_box, const int *parent_internal,
+                                                             const int *parent_leaf, BvhNode *nodes,
+                                                             unsigned *counters, uint32_t *ready,
+                                                             unsigned *ready_count)
+{
+    __shared__ RefitLocal s;
+    refit_local_init(s);
+    __syncthreads();
+    refit_local_climb(s, count, sorted_ids, tri_box, parent_internal, parent_leaf, nodes, counters, ready,
+                      ready_count);
+}
+
+// Phase 2 (one workgroup): from each queued node (both children's boxes in
+// place) combine its box and climb, the second arrival at a node going on.
+__global__ __launch_bounds__(1024) void k_refit_top(const int *parent_internal, BvhNode *nodes, unsigned *counters,
+                                                    const uint32_t *ready, const unsigned *ready_count)
+{
+    const unsigned n = *ready_count;
+    for (unsigned r = threadIdx.x; r < n; r += blockDim.x) {
+        int node = int(ready[r]);
+        double both[12];
+#pragma unroll
+        for (int c = 0; c < 12; ++c)
+            both[c] = __hip_atomic_load(nodes[node].box + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        for (int level = 0; level < 128; ++level) {
+            double box[6];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                box[c] = fmin(both[c], both[6 + c]);
+                box[3 + c] = fmax(both[3 + c], both[9 + c]);
+            }
+            const int link = parent_internal[node];
+            if (link < 0) break;
+            const int parent = link >> 1, side = link & 1;
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+                __hip_atomic_store(nodes[parent].box + 6 * side + c, box[c], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (__hip_atomic_fetch_add(&counters[parent], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
+                break;
+#pragma unroll
+            for (int c = 0; c < 12; ++c)
+                both[c] = __hip_atomic_load(nodes[parent].box + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            node = parent;
+        }
+    }
+}
+
 // Tree check used by tests. Counts into out[]: 0 triangle boxes not inside
 // the box stored for an ancestor, 1 broken parent chains, 2 out-of-order
 // sort keys, 3 leaves whose own stored box differs from their triangle box,

@@ -109,7 +109,8 @@ struct DeviceState {
     Buffer<uint32_t> keys, keys_sorted, ids, ids_sorted;
     Buffer<BvhNode> bvh;
     Buffer<int> parent_internal, parent_leaf;
-    Buffer<unsigned> counters;
+    Buffer<unsigned> counters, refit_ready_count;
+    Buffer<uint32_t> refit_ready;  // chunk-spanning nodes whose children are done
     Buffer<unsigned char> sort_temp;
     size_t sort_temp_bytes = 0;
     // per-element contact candidate lists (valid for the current substep)
@@ -572,6 +573,8 @@ KhsResult Solver::upload_model()
         d.parent_internal.allocate(std::max<size_t>(1, T - 1));
         d.parent_leaf.allocate(T);
         d.counters.allocate(std::max<size_t>(1, T - 1));
+        d.refit_ready.allocate(std::max<size_t>(1, T - 1));
+        d.refit_ready_count.allocate(1);
         size_t temp = 0;
         HIP_TRY(rocprim::radix_sort_pairs(nullptr, temp, d.keys.get(), d.keys_sorted.get(), d.ids.get(),
                                           d.ids_sorted.get(), T, 0, 30, d.stream));
@@ -614,8 +617,25 @@ void Solver::rebuild_bvh(bool swept)
                                       d.ids_sorted.get(), d.T, 0, 30, d.stream));
     hipLaunchKernelGGL(dev::k_hierarchy, dim3(tb), dim3(kBlock), 0, d.stream, d.T, d.keys_sorted.get(),
                        d.ids_sorted.get(), d.bvh.get(), d.parent_internal.get(), d.parent_leaf.get(), d.counters.get());
-    hipLaunchKernelGGL(dev::k_refit, dim3(tb), dim3(kBlock), 0, d.stream, d.T, d.ids_sorted.get(), d.tri_box.get(),
-                       d.parent_internal.get(), d.parent_leaf.get(), d.bvh.get(), d.counters.get());
+    refit_boxes();
+}
+
+// Bottom-up boxes on the current hierarchy (counters zeroed by the caller).
+void Solver::refit_boxes()
+{
+    DeviceState &d = *dev_;
+    if (d.T == 1) {
+        hipLaunchKernelGGL(dev::k_refit, dim3(1), dim3(kBlock), 0, d.stream, d.T, d.ids_sorted.get(), d.tri_box.get(),
+                           d.parent_internal.get(), d.parent_leaf.get(), d.bvh.get(), d.counters.get());
+    } else {
+        hipLaunchKernelGGL(dev::k_fill_u32, dim3(1), dim3(64), 0, d.stream, d.refit_ready_count.get(), size_t(1), 0u);
+        hipLaunchKernelGGL(dev::k_refit_local, dim3(grid_for(d.T, dev::kRefitChunk)), dim3(dev::kRefitChunk), 0,
+                           d.stream, d.T, d.ids_sorted.get(), d.tri_box.get(), d.parent_internal.get(),
+                           d.parent_leaf.get(), d.bvh.get(), d.counters.get(), d.refit_ready.get(),
+                           d.refit_ready_count.get());
+        hipLaunchKernelGGL(dev::k_refit_top, dim3(1), dim3(1024), 0, d.stream, d.parent_internal.get(), d.bvh.get(),
+                           d.counters.get(), d.refit_ready.get(), d.refit_ready_count.get());
+    }
     HIP_TRY(hipGetLastError());
 }
 
@@ -630,9 +650,7 @@ void Solver::refit_bvh(bool swept)
                        d.tri_box.get(), d.tri_box_static.get(), d.bound_partials.get());
     hipLaunchKernelGGL(dev::k_zero_counters, dim3(tb), dim3(kBlock), 0, d.stream, d.counters.get(),
                        d.T > 1 ? d.T - 1 : 1u);
-    hipLaunchKernelGGL(dev::k_refit, dim3(tb), dim3(kBlock), 0, d.stream, d.T, d.ids_sorted.get(), d.tri_box.get(),
-                       d.parent_internal.get(), d.parent_leaf.get(), d.bvh.get(), d.counters.get());
-    HIP_TRY(hipGetLastError());
+    refit_boxes();
 }
 
 // Brings the candidate lists up to date for a query pass (all of them

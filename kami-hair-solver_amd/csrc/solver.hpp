@@ -84,6 +84,7 @@ private:
     void release_device();
     void rebuild_bvh(bool swept);
     void refit_bvh(bool swept);
+    void refit_boxes();
     void refresh_contact_cache(int mode, bool full, bool swept_leaves = false);
     void closest_pass(bool line_search, Partial *partials);
     void evaluate_async(bool gradient, double dt, bool line_search);

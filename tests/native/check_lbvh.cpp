@@ -25,6 +25,39 @@ static void launch(uint32_t threads, F &&f)
     }
 }
 
+// The two-launch refit as the GPU runs it: phase 1 (one workgroup per chunk
+// of kRefitChunk leaves, each with its own LDS record) with threads arriving
+// in the given global order, then phase 2 (one workgroup of 1024) with its
+// threads in reverse order.
+static void refit_two_phase(uint32_t T, const std::vector<uint32_t> &arrival, const uint32_t *ids_sorted,
+                            const double *boxes, const int *parent_internal, const int *parent_leaf, BvhNode *nodes,
+                            unsigned *counters)
+{
+    const uint32_t chunks = (T + kRefitChunk - 1) / kRefitChunk;
+    std::vector<RefitLocal> local(chunks);
+    std::vector<uint32_t> ready(std::max<uint32_t>(1, T - 1));
+    unsigned ready_count = 0;
+    blockDim.x = kRefitChunk;
+    for (uint32_t b = 0; b < chunks; ++b)
+        for (uint32_t t = 0; t < uint32_t(kRefitChunk); ++t) {
+            threadIdx.x = t;
+            blockIdx.x = b;
+            refit_local_init(local[b]);
+        }
+    for (uint32_t k : arrival) {
+        threadIdx.x = k % kRefitChunk;
+        blockIdx.x = k / kRefitChunk;
+        refit_local_climb(local[blockIdx.x], T, ids_sorted, boxes, parent_internal, parent_leaf, nodes, counters,
+                          ready.data(), &ready_count);
+    }
+    blockDim.x = 1024;
+    blockIdx.x = 0;
+    for (int t = 1023; t >= 0; --t) {
+        threadIdx.x = unsigned(t);
+        k_refit_top(parent_internal, nodes, counters, ready.data(), &ready_count);
+    }
+}
+
 int main()
 {
     uint32_t V = 0, T = 0;
@@ -68,16 +101,22 @@ int main()
     std::vector<unsigned> counters(std::max<uint32_t>(1, T - 1));
     unsigned long long worst[6] = {0, 0, 0, 0, 0, 0};
     std::mt19937 rng(7);
-    for (int trial = 0; trial < 3; ++trial) {
+    // trials 0-2: the single-launch refit; 3-5: the two-launch refit
+    for (int trial = 0; trial < 6; ++trial) {
         for (auto &n : nodes) for (double &b : n.box) b = 12345.0;  // garbage
         launch(T, [&] { k_hierarchy(T, keys_sorted.data(), ids_sorted.data(), nodes.data(), parent_internal.data(), parent_leaf.data(), counters.data()); });
         std::vector<uint32_t> arrival(T);
         std::iota(arrival.begin(), arrival.end(), 0u);
-        if (trial == 1) std::reverse(arrival.begin(), arrival.end());
-        if (trial == 2) std::shuffle(arrival.begin(), arrival.end(), rng);
-        for (uint32_t k : arrival) {
-            threadIdx.x = k % 256; blockIdx.x = k / 256; blockDim.x = 256;
-            k_refit(T, ids_sorted.data(), tri_box.data(), parent_internal.data(), parent_leaf.data(), nodes.data(), counters.data());
+        if (trial % 3 == 1) std::reverse(arrival.begin(), arrival.end());
+        if (trial % 3 == 2) std::shuffle(arrival.begin(), arrival.end(), rng);
+        if (trial < 3 || T < 2) {
+            for (uint32_t k : arrival) {
+                threadIdx.x = k % 256; blockIdx.x = k / 256; blockDim.x = 256;
+                k_refit(T, ids_sorted.data(), tri_box.data(), parent_internal.data(), parent_leaf.data(), nodes.data(), counters.data());
+            }
+        } else {
+            refit_two_phase(T, arrival, ids_sorted.data(), tri_box.data(), parent_internal.data(), parent_leaf.data(),
+                            nodes.data(), counters.data());
         }
         unsigned long long v[6] = {0, 0, 0, 0, 0, 0};
         launch(T, [&] { k_check_bvh(T, keys_sorted.data(), ids_sorted.data(), tri_box.data(), parent_internal.data(), parent_leaf.data(), nodes.data(), v); });
@@ -108,7 +147,15 @@ int main()
             grown[6 * t + 3 + k] += grow(rng);
         }
     std::fill(counters.begin(), counters.end(), 0u);
-    launch(T, [&] { k_refit(T, ids_sorted.data(), grown.data(), parent_internal.data(), parent_leaf.data(), nodes.data(), counters.data()); });
+    if (T < 2) {
+        launch(T, [&] { k_refit(T, ids_sorted.data(), grown.data(), parent_internal.data(), parent_leaf.data(), nodes.data(), counters.data()); });
+    } else {
+        std::vector<uint32_t> arrival(T);
+        std::iota(arrival.begin(), arrival.end(), 0u);
+        std::shuffle(arrival.begin(), arrival.end(), rng);
+        refit_two_phase(T, arrival, ids_sorted.data(), grown.data(), parent_internal.data(), parent_leaf.data(),
+                        nodes.data(), counters.data());
+    }
     for (uint32_t q = 0; q < Q; ++q) {
         const double *x = &queries[7 * size_t(q)];
         const V3 lo = mk(x[0], x[1], x[2]), hi = mk(x[3], x[4], x[5]);
