@@ -464,6 +464,11 @@ KhsResult Solver::upload_model()
         error_ = "No HIP GPU is available.";
         return KHS_ERROR_INVALID_STATE;
     }
+    if (uint64_t(model_.node_count) > dev::kMaxSolveNodes) {
+        error_ = "The model has " + std::to_string(model_.node_count) + " nodes; the block solve addresses at most " +
+                 std::to_string(dev::kMaxSolveNodes) + " per solver (32-bit byte offsets into the Hessian blocks).";
+        return KHS_ERROR_OUT_OF_MEMORY;
+    }
     HIP_TRY(hipSetDevice(0));
     dev_ = std::make_unique<DeviceState>();
     DeviceState &d = *dev_;

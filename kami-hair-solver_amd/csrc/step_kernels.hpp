@@ -838,12 +838,11 @@ __device__ inline bool inverse6(double *a, double *inv)
 }
 
 // Value of `v` in lane L of this lane's 16-lane row (DPP row_newbcast).
+// (one 64-bit DPP move: every lane of the row has a valid source lane)
 template <int L>
 __device__ __forceinline__ double row_bcast(double v)
 {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x150 + L, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x150 + L, 0xf, 0xf, false);
-    return __hiloint2double(hi, lo);
+    return __builtin_amdgcn_mov_dpp(v, 0x150 + L, 0xf, 0xf, false);
 }
 
 // f(std::integral_constant<int, 0>), ..., f(std::integral_constant<int, N - 1>).
@@ -858,6 +857,33 @@ __device__ __forceinline__ void static_for(F &&f)
     static_for_impl(f, std::make_integer_sequence<int, N>{});
 }
 
+// Raw buffer access over one array with a wave-uniform resource. A lane
+// whose byte offset is kNoAccess (plus at most 248) reads zero and writes
+// nothing (the hardware's range check drops it), so predicated loads and
+// stores need no branches and every pass of a loop issues the same memory
+// operations, which keeps the compiler's wait counts exact across
+// iterations.
+constexpr uint32_t kNoAccess = 0xffffff00u;
+constexpr uint64_t kMaxBufferBytes = kNoAccess;  // arrays read this way stay below this size
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_of(const void *p, uint64_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, int(uint32_t(bytes)), 0x00020000);
+}
+__device__ __forceinline__ double buffer_load(__amdgpu_buffer_rsrc_t r, uint32_t offset)
+{
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, int(offset), 0, 0);
+    return __hiloint2double(int(v[1]), int(v[0]));
+}
+__device__ __forceinline__ void buffer_store(__amdgpu_buffer_rsrc_t r, uint32_t offset, double x)
+{
+    typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+    u2 v;
+    v[0] = unsigned(__double2loint(x));
+    v[1] = unsigned(__double2hiint(x));
+    __builtin_amdgcn_raw_buffer_store_b64(v, r, int(offset), 0, 0);
+}
+
 // Block Thomas per strand on the free nodes; fixed root nodes are
 // eliminated and get a zero direction (cuda_backend.cu:866-881).
 //
@@ -865,22 +891,44 @@ __device__ __forceinline__ void static_for(F &&f)
 // dispatched longest first, so a wavefront's strands have similar lengths
 // and it runs the longest one's node count). In the forward sweep lane
 // j < 6 holds column j of the block being inverted and of U, lanes 6..11
-// the columns of its inverse, lane 12 the right-hand side; every value a
-// lane needs from another comes by a row broadcast from a fixed lane, so a
-// pivot row exchange is a local select. Each entry is computed with the
-// sequential algorithm's operations in its order: products sum k = 0..5
-// from zero, and the Gauss-Jordan inverse (diagonal shift
-// max(1e-12 max|a_ii|, 1e-24), partial pivoting, pivot below 1e-30 fails;
-// cuda_backend.cu:855-864) applies each column step's row operations to all
-// columns at once, so the result is bit-identical to a one-thread solve.
+// the columns of its inverse, lane 12 the right-hand side. The pivot
+// column of each Gauss-Jordan step comes by row broadcasts from its lane;
+// the previous node's U (for the Schur complement) and the finished inverse
+// (for C' = inv U and d' = inv rhs) are shared through LDS. Each entry is
+// computed with the sequential algorithm's operations in its order:
+// products sum k = 0..5 from zero, and the Gauss-Jordan inverse (diagonal
+// shift max(1e-12 max|a_ii|, 1e-24), partial pivoting, pivot below 1e-30
+// fails; cuda_backend.cu:855-864) applies each column step's row operations
+// to all columns at once, so the result is bit-identical to a one-thread
+// solve. A row exchange is skipped when no strand of the wavefront pivots.
 // The back substitution runs row-wise on lanes 0..5.
+//
+// Memory: a ring of kSolveAhead (forward) or kBackAhead (back substitution)
+// nodes' inputs; a slot is refilled with the node that many steps later as
+// soon as its step is done. Loads and stores are raw buffer operations
+// predicated by offset, so no step waits for the stores of the one before.
 constexpr int kSolveLanes = 16;
 constexpr int kSolveGroups = kWave / kSolveLanes;
+constexpr int kSolveAhead = 3;
+constexpr int kBackAhead = 8;
+constexpr uint64_t kMaxSolveNodes = kMaxBufferBytes / 288;  // 36 doubles per node block
+
+struct SolveShared {
+    double upper[kSolveGroups][36];    // previous node's U, column c at [6c]
+    double inverse[kSolveGroups][36];  // inverse of the current pivot block, column c at [6c]
+};
+
+struct ForwardSlot {
+    double d[6];  // D column
+    double g[6];  // gradient (rhs lane)
+    double u[6];  // U column
+};
 
 __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t *order, const double *Dblk,
                                                        const double *Ublk, const double *grad, double *Cp,
                                                        double *dp, double *dir, unsigned *failed)
 {
+    __shared__ SolveShared sh;
     const int g = threadIdx.x / kSolveLanes, j = threadIdx.x % kSolveLanes;
     const bool acol = j < 6, icol = j >= 6 && j < 12, vlane = j == 12;
     const uint32_t gi = blockIdx.x * kSolveGroups + g;
@@ -898,54 +946,60 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
     uint32_t steps = len;
 #pragma unroll
     for (int o = kWave / 2; o >= 1; o >>= 1) steps = max(steps, uint32_t(__shfl_xor(int(steps), o, kWave)));
-    bool ok = true;
+    steps = __builtin_amdgcn_readfirstlane(steps);  // equal in every lane: keep the loops scalar
 
-    // next node's D column (gradient on the rhs lane) and U column
-    double nd[6], nu[6];
-#pragma unroll
-    for (int r = 0; r < 6; ++r) {
-        nd[r] = 0.0;
-        nu[r] = 0.0;
-    }
-    if (len > 0) {
+    const uint64_t block_bytes = 288 * uint64_t(m.node_count), vec_bytes = 48 * uint64_t(m.node_count);
+    const __amdgpu_buffer_rsrc_t rD = buffer_of(Dblk, block_bytes), rU = buffer_of(Ublk, block_bytes);
+    const __amdgpu_buffer_rsrc_t rG = buffer_of(grad, vec_bytes), rC = buffer_of(Cp, block_bytes);
+    const __amdgpu_buffer_rsrc_t rP = buffer_of(dp, vec_bytes), rX = buffer_of(dir, vec_bytes);
+
+    // node t's inputs: D column j (acol), gradient (rhs lane), U column j
+    // (acol, when a node follows)
+    auto fetch = [&](ForwardSlot &f, uint32_t t) __attribute__((always_inline)) {
+        const uint32_t n = begin + t;
+        const uint32_t od = (acol && t < len) ? 288 * n + 8 * j : kNoAccess;
+        const uint32_t og = (vlane && t < len) ? 48 * n : kNoAccess;
+        const uint32_t ou = (acol && t + 1 < len) ? 288 * n + 8 * j : kNoAccess;
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
-            if (acol) nd[r] = Dblk[36 * size_t(begin) + 6 * r + j];
-            if (vlane) nd[r] = grad[6 * size_t(begin) + r];
-            if (acol && len > 1) nu[r] = Ublk[36 * size_t(begin) + 6 * r + j];
+            f.d[r] = buffer_load(rD, od + 48 * r);
+            f.g[r] = buffer_load(rG, og + 8 * r);
+            f.u[r] = buffer_load(rU, ou + 48 * r);
         }
-    }
-    double p[6], low[6];  // previous C' column (d' on the rhs lane); previous U column
+    };
+
+    bool ok = true;
+    double p[6];  // previous C' column (d' on the rhs lane)
 #pragma unroll
-    for (int r = 0; r < 6; ++r) {
-        p[r] = 0.0;
-        low[r] = 0.0;
-    }
-    for (uint32_t it = 0; it < steps; ++it) {
-        const uint32_t n = begin + (it < len ? it : 0);
+    for (int r = 0; r < 6; ++r) p[r] = 0.0;
+    ForwardSlot ring[kSolveAhead];
+#pragma unroll
+    for (int a = 0; a < kSolveAhead; ++a) fetch(ring[a], a);
+
+    auto forward = [&](ForwardSlot &slot, uint32_t t) __attribute__((always_inline)) {
+        const uint32_t n = begin + t;
         double col[6], w[6];
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
-            col[r] = acol ? nd[r] : (vlane ? -nd[r] : ((icol && r == j - 6) ? 1.0 : 0.0));
-            w[r] = acol ? nu[r] : 0.0;
+            col[r] = acol ? slot.d[r] : (vlane ? -slot.g[r] : ((icol && r == j - 6) ? 1.0 : 0.0));
+            w[r] = slot.u[r];
         }
-        if (it + 1 < len) {
+        if (t > 0) {
+            // col[r] -= sum_k U'[k][r] p[k]: D - U'^T C' on the block
+            // columns, rhs - U'^T d' on the rhs lane
+            const double *up = sh.upper[g];
 #pragma unroll
             for (int r = 0; r < 6; ++r) {
-                if (acol) nd[r] = Dblk[36 * size_t(n + 1) + 6 * r + j];
-                if (vlane) nd[r] = grad[6 * size_t(n + 1) + r];
-                nu[r] = (acol && it + 2 < len) ? Ublk[36 * size_t(n + 1) + 6 * r + j] : 0.0;
-            }
-        }
-        if (it > 0) {
-            // col[r] -= sum_k low[k][r] p[k]: D - U'^T C' on the block columns,
-            // rhs - U'^T d' on the rhs lane; low[k][r] is lane r's low[k]
-            static_for<6>([&](auto R) {
                 double sum = 0.0;
 #pragma unroll
-                for (int k = 0; k < 6; ++k) sum += row_bcast<R>(low[k]) * p[k];
-                col[R] -= sum;
-            });
+                for (int k = 0; k < 6; ++k) sum += up[6 * r + k] * p[k];
+                col[r] -= sum;
+            }
+        }
+        __syncthreads();
+        if (acol) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) sh.upper[g][6 * j + k] = w[k];
         }
         double rhs[6];
 #pragma unroll
@@ -972,18 +1026,20 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
                 if (v > best) { best = v; pivot = i; }
             }
             if (best < 1.0e-30 || !isfinite(best)) {
-                if (it < len) ok = false;
+                if (t < len) ok = false;
                 pivot = C;
             }
+            if (__builtin_amdgcn_ballot_w64(pivot != C)) {
 #pragma unroll
-            for (int i = C + 1; i < 6; ++i) {
-                const bool sw = pivot == i;
-                const double x = col[C], y = col[i];
-                col[C] = sw ? y : x;
-                col[i] = sw ? x : y;
-                const double cx = cv[C], cy = cv[i];
-                cv[C] = sw ? cy : cx;
-                cv[i] = sw ? cx : cy;
+                for (int i = C + 1; i < 6; ++i) {
+                    const bool sw = pivot == i;
+                    const double x = col[C], y = col[i];
+                    col[C] = sw ? y : x;
+                    col[i] = sw ? x : y;
+                    const double cx = cv[C], cy = cv[i];
+                    cv[C] = sw ? cy : cx;
+                    cv[i] = sw ? cx : cy;
+                }
             }
             const double invp = 1.0 / cv[C];
             col[C] *= invp;
@@ -991,60 +1047,71 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
             for (int r = 0; r < 6; ++r)
                 if (r != C) col[r] -= cv[r] * col[C];
         });
+        if (icol) {
+#pragma unroll
+            for (int r = 0; r < 6; ++r) sh.inverse[g][6 * (j - 6) + r] = col[r];
+        }
+        __syncthreads();
         // res[r] = sum_k inv[r][k] w[k]: C' = inv U on the block columns,
-        // d' = inv rhs on the rhs lane; inv column k lives in lane 6 + k
+        // d' = inv rhs on the rhs lane
 #pragma unroll
         for (int r = 0; r < 6; ++r)
             if (vlane) w[r] = rhs[r];
+        const double *inv = sh.inverse[g];
         double res[6];
 #pragma unroll
         for (int r = 0; r < 6; ++r) res[r] = 0.0;
-        static_for<6>([&](auto K) {
 #pragma unroll
-            for (int r = 0; r < 6; ++r) res[r] += row_bcast<6 + K>(col[r]) * w[K];
-        });
+        for (int k = 0; k < 6; ++k)
+#pragma unroll
+            for (int r = 0; r < 6; ++r) res[r] += inv[6 * k + r] * w[k];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) p[r] = acol ? (t + 1 < len ? res[r] : 0.0) : (vlane ? res[r] : 0.0);
+        const bool keep = t < len && ok;
+        const uint32_t oc = (acol && keep) ? 288 * n + 8 * j : kNoAccess;
+        const uint32_t op = (vlane && keep) ? 48 * n : kNoAccess;
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
-            p[r] = acol ? (it + 1 < len ? res[r] : 0.0) : (vlane ? res[r] : 0.0);
-            low[r] = acol ? w[r] : 0.0;
+            buffer_store(rC, oc + 48 * r, p[r]);
+            buffer_store(rP, op + 8 * r, p[r]);
         }
-        if (it < len && ok) {
-#pragma unroll
-            for (int r = 0; r < 6; ++r) {
-                if (acol) Cp[36 * size_t(n) + 6 * r + j] = p[r];
-                if (vlane) dp[6 * size_t(n) + r] = p[r];
-            }
-        }
-    }
+        // refilled once its values are dead, so it keeps its registers
+        fetch(slot, t + kSolveAhead);
+    };
+    for (uint32_t t0 = 0; t0 < steps; t0 += kSolveAhead)
+        static_for<kSolveAhead>([&](auto A) { forward(ring[A], t0 + A); });
     if (!ok && j == 0) atomicOr(failed, 1u);
-    // Back substitution x_n = d'_n - C'_n x_{n+1}, lane r < 6 owning row r;
-    // the node below is loaded one step ahead.
+
+    // Back substitution x_n = d'_n - C'_n x_{n+1}, lane r < 6 owning row r.
     const int r = acol ? j : 0;
-    double next = 0.0, cn[6], dn = 0.0;
+    struct BackSlot {
+        double c[6], d;
+    };
+    auto fetch_back = [&](BackSlot &b, uint32_t t) __attribute__((always_inline)) {
+        const uint32_t n = end - 1 - t;
+        const bool live = acol && t < len;
+        const uint32_t oc = live ? 288 * n + 48 * r : kNoAccess;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) cn[k] = 0.0;
-    if (len > 0) {
+        for (int k = 0; k < 6; ++k) b.c[k] = buffer_load(rC, oc + 8 * k);
+        b.d = buffer_load(rP, live ? 48 * n + 8 * r : kNoAccess);
+    };
+    BackSlot back[kBackAhead];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) cn[k] = Cp[36 * size_t(end - 1) + 6 * r + k];
-        dn = dp[6 * size_t(end - 1) + r];
-    }
-    for (uint32_t it = 0; it < steps; ++it) {
-        const bool live = it < len;
-        const uint32_t n = live ? end - 1 - it : begin;
+    for (int a = 0; a < kBackAhead; ++a) fetch_back(back[a], a);
+    double next = 0.0;
+    auto backward = [&](BackSlot &b, uint32_t t) __attribute__((always_inline)) {
+        const uint32_t n = end - 1 - t;
         double crow[6];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) crow[k] = cn[k];
-        const double d0 = dn;
-        if (it + 1 < len) {
-#pragma unroll
-            for (int k = 0; k < 6; ++k) cn[k] = Cp[36 * size_t(n - 1) + 6 * r + k];
-            dn = dp[6 * size_t(n - 1) + r];
-        }
-        double value = d0;
-        if (it > 0) static_for<6>([&](auto K) { value -= crow[K] * row_bcast<K>(next); });
-        if (live && ok && acol) dir[6 * size_t(n) + j] = value;
+        for (int k = 0; k < 6; ++k) crow[k] = b.c[k];
+        double value = b.d;
+        if (t > 0) static_for<6>([&](auto K) { value -= crow[K] * row_bcast<K>(next); });
+        buffer_store(rX, (t < len && ok && acol) ? 48 * n + 8 * j : kNoAccess, value);
         next = value;
-    }
+        fetch_back(b, t + kBackAhead);
+    };
+    for (uint32_t t0 = 0; t0 < steps; t0 += kBackAhead)
+        static_for<kBackAhead>([&](auto A) { backward(back[A], t0 + A); });
 }
 
 // Slope g.d and the largest translation of a free node (host then limits
