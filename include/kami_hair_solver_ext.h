@@ -124,6 +124,25 @@ KHS_API KhsResult khsxGetKernelTiming(const KhsSolver *solver, KhsxKernelTiming 
 KHS_API KhsResult khsxGetSubstepTimes(const KhsSolver *solver, double *milliseconds,
                                       uint32_t capacity, uint32_t *count);
 
+/* Per-element contact lists of the device step (collider triangles whose
+   box overlaps a cached box around each hair element), as left by the last
+   substep: how many elements hold one, their lengths and overflows. A
+   diagnostic that waits for the device; the CPU oracle keeps no lists and
+   reports zeros. */
+typedef struct KhsxContactListStats {
+    uint32_t struct_size;
+    uint32_t capacity;                /* entries an element's list holds */
+    uint64_t elements;                /* internal nodes (one element each) */
+    uint64_t listed;                  /* elements queued for the list passes */
+    uint64_t nonempty;                /* elements whose list is not empty */
+    uint64_t overflowed;              /* lists over capacity (queries walk the tree) */
+    uint64_t entries;                 /* triangles over all lists within capacity */
+    uint64_t longest;                 /* longest list within capacity */
+    uint64_t length_histogram[9];     /* lengths 1, 2-3, 4-7, ..., 128-255, 256 */
+} KhsxContactListStats;
+
+KHS_API KhsResult khsxGetContactListStats(KhsSolver *solver, KhsxContactListStats *stats);
+
 /* Worker threads the CPU oracle uses over strands (oracle only; the GPU
    library ignores it and returns KHS_OK). */
 KHS_API KhsResult khsxSetThreadCount(KhsSolver *solver, uint32_t threads);

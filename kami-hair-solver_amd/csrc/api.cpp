@@ -313,6 +313,11 @@ KhsResult khsxGetSubstepTimes(const KhsSolver *s, double *ms, uint32_t capacity,
     return guarded_const(s, [&] { return s->impl.substep_times(ms, capacity, count); });
 }
 
+KhsResult khsxGetContactListStats(KhsSolver *s, KhsxContactListStats *st)
+{
+    return guarded(s, [&] { return s->impl.contact_list_stats(st); });
+}
+
 KhsResult khsxSetThreadCount(KhsSolver *s, uint32_t threads)
 {
     (void)threads;

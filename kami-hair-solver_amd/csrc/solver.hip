@@ -703,9 +703,10 @@ void Solver::refresh_contact_cache(int mode, bool full, bool swept_leaves)
 }
 
 // Closest triangle of every listed element at q (or at alpha d + q_base
-// with the device's alpha), one 16-lane group per element; partials (may
-// be null) receive the candidate count.
-void Solver::closest_pass(bool line_search, Partial *partials)
+// with the device's alpha), one wavefront per element; partials (may be
+// null) receive the candidate count. Elements with a fixed left node are
+// included only for the friction pass (`with_roots`).
+void Solver::closest_pass(bool line_search, Partial *partials, bool with_roots)
 {
     DeviceState &d = *dev_;
     const double contact_pad = material_.radius + material_.collider_offset + material_.barrier_distance;
@@ -714,11 +715,11 @@ void Solver::closest_pass(bool line_search, Partial *partials)
     if (line_search)
         hipLaunchKernelGGL(dev::k_list_closest<dev::kAtTrial>, dim3(kPairBlocks), dim3(kBlock), 0, d.stream,
                            d.flags.get(), d.collider(), cc, contact_pad, d.q.get(), d.q_base.get(), d.dir.get(),
-                           d.alpha.get(), d.best_tri.get(), partials);
+                           d.alpha.get(), d.best_tri.get(), partials, with_roots ? 0 : 1);
     else
         hipLaunchKernelGGL(dev::k_list_closest<dev::kAtState>, dim3(kPairBlocks), dim3(kBlock), 0, d.stream,
                            d.flags.get(), d.collider(), cc, contact_pad, d.q.get(), d.q_base.get(), d.dir.get(),
-                           d.alpha.get(), d.best_tri.get(), partials);
+                           d.alpha.get(), d.best_tri.get(), partials, with_roots ? 0 : 1);
     HIP_TRY(hipGetLastError());
 }
 
@@ -1302,7 +1303,7 @@ KhsResult Solver::run_frame(double frame_dt, const double *root_a, const double 
                 break;
             }
             if (d.T && material_.friction > 0.0) {
-                if (d.cache_valid) closest_pass(false, nullptr);
+                if (d.cache_valid) closest_pass(false, nullptr, true);
                 hipLaunchKernelGGL(dev::k_friction, dim3(d.tiled_blocks()), dim3(kBlock), 0, d.stream, rv,
                                    d.collider(), d.cache(), d.cache_valid ? d.best_tri.get() : nullptr, sp, d.q.get(), d.vel.get(), d.dv.get());
                 hipLaunchKernelGGL(dev::k_apply_dv, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.N, d.dv.get(),
@@ -1745,6 +1746,38 @@ KhsResult Solver::substep_times(double *ms, uint32_t capacity, uint32_t *count) 
     *count = uint32_t(v.size());
     if (ms)
         for (uint32_t i = 0; i < std::min<uint32_t>(capacity, uint32_t(v.size())); ++i) ms[i] = v[i];
+    return KHS_OK;
+}
+
+KhsResult Solver::contact_list_stats(KhsxContactListStats *out)
+{
+    if (!out || out->struct_size < sizeof(KhsxContactListStats)) return KHS_ERROR_INVALID_ARGUMENT;
+    if (!built_) return fail(KHS_ERROR_INVALID_STATE, "The solver is not built.");
+    DeviceState &d = *dev_;
+    KhsxContactListStats r{};
+    r.struct_size = sizeof(r);
+    r.capacity = kCacheCapacity;
+    r.elements = d.N;
+    std::vector<uint32_t> count(d.N);
+    uint32_t listed = 0;
+    HIP_TRY(hipMemcpyAsync(count.data(), d.cache_count.get(), d.cache_count.bytes(), hipMemcpyDeviceToHost, d.stream));
+    HIP_TRY(hipMemcpyAsync(&listed, d.cache_list_count.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, d.stream));
+    HIP_TRY(hipStreamSynchronize(d.stream));
+    r.listed = listed;
+    for (uint32_t c : count) {
+        if (c == 0) continue;
+        ++r.nonempty;
+        if (c == kCacheOverflow) {
+            ++r.overflowed;
+            continue;
+        }
+        r.entries += c;
+        r.longest = std::max<uint64_t>(r.longest, c);
+        int bin = 0;
+        while (bin < 8 && (2u << bin) <= c) ++bin;
+        ++r.length_histogram[bin];
+    }
+    *out = r;
     return KHS_OK;
 }
 

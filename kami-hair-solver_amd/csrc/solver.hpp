@@ -69,6 +69,7 @@ public:
     KhsResult set_kernel_timing(uint32_t enable);
     KhsResult kernel_timing(KhsxKernelTiming *out) const;
     KhsResult substep_times(double *ms, uint32_t capacity, uint32_t *count) const;
+    KhsResult contact_list_stats(KhsxContactListStats *out);
 
     const std::string &last_error() const { return error_; }
     void set_error(std::string message) { error_ = std::move(message); }
@@ -86,7 +87,7 @@ private:
     void refit_bvh(bool swept);
     void refit_boxes();
     void refresh_contact_cache(int mode, bool full, bool swept_leaves = false);
-    void closest_pass(bool line_search, Partial *partials);
+    void closest_pass(bool line_search, Partial *partials, bool with_roots = false);
     void evaluate_async(bool gradient, double dt, bool line_search);
     static Reduced unpack(const Partial &r);
     Reduced evaluate(bool gradient, double dt, bool line_search);

@@ -400,12 +400,14 @@ __device__ __forceinline__ void group_closest(double &d, uint32_t &t)
 // list: the closest triangle among those whose leaf box overlaps the padded
 // segment box (ties to the lower index) goes to best_tri. Candidates of
 // elements with a free left node, the ones the evaluation queries, are
-// counted. Overflowed elements are left to the consumer's tree walk.
+// counted; with `free_only` the others (root elements, whose barrier the
+// energy leaves out and only the friction pass reads) are skipped.
+// Overflowed elements are left to the consumer's tree walk.
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, ColliderView col, ContactCache cc,
                                                          double pad, const double *q, const double *q_base,
                                                          const double *dir, const double *alpha_at,
-                                                         uint32_t *best_tri, Partial *partials)
+                                                         uint32_t *best_tri, Partial *partials, int free_only)
 {
     const double alpha = MODE == kAtTrial ? *alpha_at : 0.0;
     const int lane = threadIdx.x & (kListLanes - 1);
@@ -418,6 +420,7 @@ __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, 
         const uint32_t c = cc.count[L];
         if (c == 0 || c == kCacheOverflow) continue;
         const bool counted = !(flags[L] & kNodeFixed);
+        if (free_only && !counted) continue;
         V3 p0, p1;
         element_points<MODE>(L, q, q_base, dir, alpha, p0, p1);
         const V3 lo = vmin(p0, p1), hi = vmax(p0, p1);
@@ -975,6 +978,8 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
     ForwardSlot ring[kSolveAhead];
 #pragma unroll
     for (int a = 0; a < kSolveAhead; ++a) fetch(ring[a], a);
+    for (int e = j; e < 36; e += kSolveLanes) sh.upper[g][e] = 0.0;
+    __syncthreads();
 
     auto forward = [&](ForwardSlot &slot, uint32_t t) __attribute__((always_inline)) {
         const uint32_t n = begin + t;
@@ -984,9 +989,11 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
             col[r] = acol ? slot.d[r] : (vlane ? -slot.g[r] : ((icol && r == j - 6) ? 1.0 : 0.0));
             w[r] = slot.u[r];
         }
-        if (t > 0) {
+        {
             // col[r] -= sum_k U'[k][r] p[k]: D - U'^T C' on the block
-            // columns, rhs - U'^T d' on the rhs lane
+            // columns, rhs - U'^T d' on the rhs lane. At the first node U'
+            // and p are zero and this subtracts +0, which leaves every
+            // value (signed zeros, infinities and NaNs included) unchanged.
             const double *up = sh.upper[g];
 #pragma unroll
             for (int r = 0; r < 6; ++r) {
@@ -1020,10 +1027,17 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
             for (int i = 0; i < 6; ++i) cv[i] = row_bcast<C>(col[i]);
             int pivot = C;
             double best = fabs(cv[C]);
+            // the sequential search runs only when some strand of the
+            // wavefront has an entry larger than its diagonal one
+            bool larger = false;
 #pragma unroll
-            for (int i = C + 1; i < 6; ++i) {
-                const double v = fabs(cv[i]);
-                if (v > best) { best = v; pivot = i; }
+            for (int i = C + 1; i < 6; ++i) larger |= fabs(cv[i]) > best;
+            if (__builtin_amdgcn_ballot_w64(larger)) {
+#pragma unroll
+                for (int i = C + 1; i < 6; ++i) {
+                    const double v = fabs(cv[i]);
+                    if (v > best) { best = v; pivot = i; }
+                }
             }
             if (best < 1.0e-30 || !isfinite(best)) {
                 if (t < len) ok = false;

@@ -233,6 +233,20 @@ class KhsxKernelTiming(C.Structure):
     ]
 
 
+class KhsxContactListStats(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_uint32),
+        ("capacity", C.c_uint32),
+        ("elements", C.c_uint64),
+        ("listed", C.c_uint64),
+        ("nonempty", C.c_uint64),
+        ("overflowed", C.c_uint64),
+        ("entries", C.c_uint64),
+        ("longest", C.c_uint64),
+        ("length_histogram", C.c_uint64 * 9),
+    ]
+
+
 # Every exported function with its ctypes signature. The first 33 are the
 # reference ABI (include/kami_hair_solver.h:206-298); khsx* are additive.
 _P = C.c_void_p
@@ -288,6 +302,7 @@ EXT_FUNCTIONS = {
     "khsxSetKernelTiming": (C.c_int, [_P, C.c_uint32]),
     "khsxGetKernelTiming": (C.c_int, [_P, _D(KhsxKernelTiming)]),
     "khsxGetSubstepTimes": (C.c_int, [_P, _P, C.c_uint32, _D(C.c_uint32)]),
+    "khsxGetContactListStats": (C.c_int, [_P, _D(KhsxContactListStats)]),
 }
 
 
@@ -539,6 +554,9 @@ class HairSolver:
 
     def kernel_timing(self) -> KhsxKernelTiming:
         return self._get(self.lib.khsxGetKernelTiming, KhsxKernelTiming)
+
+    def contact_list_stats(self) -> KhsxContactListStats:
+        return self._get(self.lib.khsxGetContactListStats, KhsxContactListStats)
 
     def substep_times(self) -> np.ndarray:
         count = C.c_uint32(0)

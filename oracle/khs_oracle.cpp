@@ -2234,6 +2234,16 @@ KhsResult khsxGetSubstepTimes(const KhsSolver *s, double *ms, uint32_t capacity,
     return KHS_OK;
 }
 
+KhsResult khsxGetContactListStats(KhsSolver *s, KhsxContactListStats *st)
+{
+    if (!s || !st || st->struct_size < sizeof(KhsxContactListStats)) return KHS_ERROR_INVALID_ARGUMENT;
+    KhsxContactListStats r{};
+    r.struct_size = sizeof(r);
+    r.elements = s->impl.node_count();
+    *st = r;
+    return KHS_OK;
+}
+
 KhsResult khsxSetThreadCount(KhsSolver *s, uint32_t threads)
 {
     if (!s) return KHS_ERROR_INVALID_ARGUMENT;
