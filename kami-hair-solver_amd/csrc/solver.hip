@@ -97,6 +97,8 @@ struct DeviceState {
     Buffer<double> frame, rest, mass, rmass;
     Buffer<uint32_t> flags, strand_of, strand_first, strand_count, strand_fixed, mapping;
     Buffer<uint32_t> solve_order;  // strands by decreasing free node count, for the solve's dispatch
+    Buffer<uint32_t> root_elements;  // contact elements with a fixed left and a free right node
+    uint32_t root_element_count = 0;
     // state and solver vectors
     Buffer<double> q, q_old, vel, q_base, snap_q, snap_v, grad, dir, precond, Dblk, Ublk, Cp, dp, dv;
     // root targets (3 per node)
@@ -513,6 +515,18 @@ KhsResult Solver::upload_model()
         d.solve_order.allocate(d.S);
         d.solve_order.upload(order.data(), d.S);
     }
+    {
+        // Friction reads the closest triangle of these; the energy does not.
+        std::vector<uint32_t> roots;
+        for (uint32_t n = 0; n + 1 < d.N; ++n) {
+            const uint32_t f = m.flags[n];
+            if ((f & kNodeFixed) && (f & kNodeHasElement) && (f & kNodeContact) && !(m.flags[n + 1] & kNodeFixed))
+                roots.push_back(n);
+        }
+        d.root_element_count = uint32_t(roots.size());
+        d.root_elements.allocate(std::max<size_t>(1, roots.size()));
+        if (!roots.empty()) d.root_elements.upload(roots.data(), roots.size());
+    }
     d.mapping.allocate(d.P);
     d.mapping.upload(m.original_to_internal.data(), d.P);
 
@@ -662,7 +676,7 @@ void Solver::refit_bvh(bool swept)
 // after the collider boxes changed): elements whose query box left their
 // cached box gather again.
 constexpr unsigned kGatherBlocks = 512;
-void Solver::refresh_contact_cache(int mode, bool full, bool swept_leaves)
+void Solver::refresh_contact_cache(int mode, bool full, bool swept_leaves, bool with_roots)
 {
     DeviceState &d = *dev_;
     const double contact_pad = material_.radius + material_.collider_offset + material_.barrier_distance;
@@ -687,14 +701,14 @@ void Solver::refresh_contact_cache(int mode, bool full, bool swept_leaves)
         // before its next use
         if (mode == dev::kAtState)
             hipLaunchKernelGGL(dev::k_contact_check<dev::kAtState>, grid, block, 0, d.stream, d.rods(), cc, d.q.get(),
-                               d.q_base.get(), d.dir.get(), d.alpha.get(), pad);
+                               d.q_base.get(), d.dir.get(), d.alpha.get(), pad, with_roots ? 1 : 0);
         else if (mode == dev::kAtTrial)
             hipLaunchKernelGGL(dev::k_contact_check<dev::kAtTrial>, grid, block, 0, d.stream, d.rods(), cc, d.q.get(),
-                               d.q_base.get(), d.dir.get(), d.alpha.get(), pad);
+                               d.q_base.get(), d.dir.get(), d.alpha.get(), pad, with_roots ? 1 : 0);
         else
             hipLaunchKernelGGL(dev::k_contact_check<dev::kSwept>, grid, block, 0, d.stream, d.rods(), cc, d.q.get(),
-                               d.q_base.get(), d.dir.get(), d.alpha.get(), pad);
-        hipLaunchKernelGGL(dev::k_contact_gather, dim3(kGatherBlocks), block, 0, d.stream, d.collider(), cc,
+                               d.q_base.get(), d.dir.get(), d.alpha.get(), pad, with_roots ? 1 : 0);
+        hipLaunchKernelGGL(dev::k_contact_gather, dim3(kGatherBlocks), block, 0, d.stream, d.flags.get(), d.collider(), cc,
                            d.cache_pending_count.get() + (d.pending_slot ^ 1u));
         d.pending_slot ^= 1u;
     }
@@ -705,21 +719,23 @@ void Solver::refresh_contact_cache(int mode, bool full, bool swept_leaves)
 // Closest triangle of every listed element at q (or at alpha d + q_base
 // with the device's alpha), one wavefront per element; partials (may be
 // null) receive the candidate count. Elements with a fixed left node are
-// included only for the friction pass (`with_roots`).
+// refreshed and included only for the friction pass (`with_roots`).
 void Solver::closest_pass(bool line_search, Partial *partials, bool with_roots)
 {
     DeviceState &d = *dev_;
     const double contact_pad = material_.radius + material_.collider_offset + material_.barrier_distance;
-    refresh_contact_cache(line_search ? dev::kAtTrial : dev::kAtState, false);
+    refresh_contact_cache(line_search ? dev::kAtTrial : dev::kAtState, false, false, with_roots);
     const ContactCache cc = d.cache();
     if (line_search)
         hipLaunchKernelGGL(dev::k_list_closest<dev::kAtTrial>, dim3(kPairBlocks), dim3(kBlock), 0, d.stream,
                            d.flags.get(), d.collider(), cc, contact_pad, d.q.get(), d.q_base.get(), d.dir.get(),
-                           d.alpha.get(), d.best_tri.get(), partials, with_roots ? 0 : 1);
+                           d.alpha.get(), d.best_tri.get(), partials, with_roots ? d.root_elements.get() : nullptr,
+                           with_roots ? d.root_element_count : 0u);
     else
         hipLaunchKernelGGL(dev::k_list_closest<dev::kAtState>, dim3(kPairBlocks), dim3(kBlock), 0, d.stream,
                            d.flags.get(), d.collider(), cc, contact_pad, d.q.get(), d.q_base.get(), d.dir.get(),
-                           d.alpha.get(), d.best_tri.get(), partials, with_roots ? 0 : 1);
+                           d.alpha.get(), d.best_tri.get(), partials, with_roots ? d.root_elements.get() : nullptr,
+                           with_roots ? d.root_element_count : 0u);
     HIP_TRY(hipGetLastError());
 }
 
@@ -1761,17 +1777,22 @@ KhsResult Solver::contact_list_stats(KhsxContactListStats *out)
     std::vector<uint32_t> count(d.N);
     uint32_t listed = 0;
     HIP_TRY(hipMemcpyAsync(count.data(), d.cache_count.get(), d.cache_count.bytes(), hipMemcpyDeviceToHost, d.stream));
+    const std::vector<uint32_t> &flags = model_.flags;
     HIP_TRY(hipMemcpyAsync(&listed, d.cache_list_count.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, d.stream));
     HIP_TRY(hipStreamSynchronize(d.stream));
     r.listed = listed;
-    for (uint32_t c : count) {
+    for (uint32_t e = 0; e < d.N; ++e) {
+        const uint32_t c = count[e];
         if (c == 0) continue;
         ++r.nonempty;
+        const bool root = flags[e] & kNodeFixed;
+        r.root_nonempty += root ? 1 : 0;
         if (c == kCacheOverflow) {
             ++r.overflowed;
             continue;
         }
         r.entries += c;
+        r.root_entries += root ? c : 0;
         r.longest = std::max<uint64_t>(r.longest, c);
         int bin = 0;
         while (bin < 8 && (2u << bin) <= c) ++bin;

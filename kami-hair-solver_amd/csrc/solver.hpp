@@ -86,7 +86,7 @@ private:
     void rebuild_bvh(bool swept);
     void refit_bvh(bool swept);
     void refit_boxes();
-    void refresh_contact_cache(int mode, bool full, bool swept_leaves = false);
+    void refresh_contact_cache(int mode, bool full, bool swept_leaves = false, bool with_roots = false);
     void closest_pass(bool line_search, Partial *partials, bool with_roots = false);
     void evaluate_async(bool gradient, double dt, bool line_search);
     static Reduced unpack(const Partial &r);

@@ -248,7 +248,7 @@ __global__ void k_contact_refresh(RodView m, ColliderView col, ContactCache cc, 
     cc.count[L] = n <= kCacheCapacity ? n : kCacheOverflow;
     store3(box, blo);
     store3(box + 3, bhi);
-    if (n > 0 && !cc.listed[L]) {
+    if (n > 0 && !(fl & kNodeFixed) && !cc.listed[L]) {
         cc.listed[L] = 1;
         cc.list[atomicAdd(cc.list_count, 1u)] = L;
     }
@@ -324,15 +324,18 @@ __device__ inline uint32_t wave_gather(const ColliderView &col, V3 lo, V3 hi, ui
 // Partial refresh, first half: every contact element whose query box
 // (grown by pad plus the rounding margin) left its cached box gets the new
 // box (query box grown by pad + skin) and is queued for gathering.
+// Elements with a fixed left node take part only `with_roots` (the friction
+// pass, the one reader of their lists).
 template <int MODE>
 __global__ void k_contact_check(RodView m, ContactCache cc, const double *q, const double *q_base, const double *dir,
-                                const double *alpha_at, double pad)
+                                const double *alpha_at, double pad, int with_roots)
 {
     const double alpha = MODE == kAtTrial ? *alpha_at : 0.0;
     const uint32_t L = blockIdx.x * blockDim.x + threadIdx.x;
     if (L >= m.node_count) return;
     const uint32_t fl = m.flags[L];
     if (!(fl & kNodeHasElement) || !(fl & kNodeContact)) return;
+    if ((fl & kNodeFixed) && !with_roots) return;
     V3 lo, hi;
     query_box<MODE>(L, q, q_base, dir, alpha, lo, hi);
     double *box = cc.box + 6 * size_t(L);
@@ -346,8 +349,10 @@ __global__ void k_contact_check(RodView m, ContactCache cc, const double *q, con
 // Partial refresh, second half: one wavefront per queued element gathers
 // the triangles overlapping its new box; an empty result keeps a box seven
 // skins wider still when that is empty too. It also clears the other
-// pending counter, which the next partial refresh appends to.
-__global__ __launch_bounds__(kBlock) void k_contact_gather(ColliderView col, ContactCache cc, uint32_t *next_count)
+// pending counter, which the next partial refresh appends to. Elements
+// with a fixed left node keep lists but stay off the listed queue.
+__global__ __launch_bounds__(kBlock) void k_contact_gather(const uint32_t *flags, ColliderView col, ContactCache cc,
+                                                           uint32_t *next_count)
 {
     __shared__ int queues[kTilesPerBlock][kGatherQueue];
     volatile int *queue = queues[threadIdx.x / kWave];
@@ -373,7 +378,7 @@ __global__ __launch_bounds__(kBlock) void k_contact_gather(ColliderView col, Con
         }
         if (lane == 0) {
             cc.count[L] = n <= kCacheCapacity ? n : kCacheOverflow;
-            if (n > 0 && !cc.listed[L]) {
+            if (n > 0 && !(flags[L] & kNodeFixed) && !cc.listed[L]) {
                 cc.listed[L] = 1;
                 cc.list[atomicAdd(cc.list_count, 1u)] = L;
             }
@@ -398,16 +403,17 @@ __device__ __forceinline__ void group_closest(double &d, uint32_t &t)
 
 // Narrow phase, one wavefront per listed element, its lanes striding the
 // list: the closest triangle among those whose leaf box overlaps the padded
-// segment box (ties to the lower index) goes to best_tri. Candidates of
-// elements with a free left node, the ones the evaluation queries, are
-// counted; with `free_only` the others (root elements, whose barrier the
-// energy leaves out and only the friction pass reads) are skipped.
-// Overflowed elements are left to the consumer's tree walk.
+// segment box (ties to the lower index) goes to best_tri. The listed
+// queue holds elements with a free left node, the ones the energy queries,
+// and their candidates are counted; `extra` (the friction pass) appends the
+// root elements (fixed left node, free right node), whose lists only
+// friction reads. Overflowed elements are left to the consumer's tree walk.
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, ColliderView col, ContactCache cc,
                                                          double pad, const double *q, const double *q_base,
                                                          const double *dir, const double *alpha_at,
-                                                         uint32_t *best_tri, Partial *partials, int free_only)
+                                                         uint32_t *best_tri, Partial *partials,
+                                                         const uint32_t *extra, uint32_t extra_count)
 {
     const double alpha = MODE == kAtTrial ? *alpha_at : 0.0;
     const int lane = threadIdx.x & (kListLanes - 1);
@@ -415,12 +421,11 @@ __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, 
     const uint32_t groups = gridDim.x * blockDim.x / kListLanes;
     const uint32_t listed = *cc.list_count;
     unsigned long long candidates = 0;
-    for (uint32_t j = group; j < listed; j += groups) {
-        const uint32_t L = cc.list[j];
+    for (uint32_t j = group; j < listed + extra_count; j += groups) {
+        const uint32_t L = j < listed ? cc.list[j] : extra[j - listed];
         const uint32_t c = cc.count[L];
         if (c == 0 || c == kCacheOverflow) continue;
         const bool counted = !(flags[L] & kNodeFixed);
-        if (free_only && !counted) continue;
         V3 p0, p1;
         element_points<MODE>(L, q, q_base, dir, alpha, p0, p1);
         const V3 lo = vmin(p0, p1), hi = vmax(p0, p1);
@@ -903,7 +908,9 @@ __device__ __forceinline__ void buffer_store(__amdgpu_buffer_rsrc_t r, uint32_t 
 // shift max(1e-12 max|a_ii|, 1e-24), partial pivoting, pivot below 1e-30
 // fails; cuda_backend.cu:855-864) applies each column step's row operations
 // to all columns at once, so the result is bit-identical to a one-thread
-// solve. A row exchange is skipped when no strand of the wavefront pivots.
+// solve. A first pass without row exchanges stands whenever partial pivoting
+// would exchange nothing in any strand of the wavefront (the usual case for
+// these diagonally dominant blocks); otherwise the pivoting pass reruns.
 // The back substitution runs row-wise on lanes 0..5.
 //
 // Memory: a ring of kSolveAhead (forward) or kBackAhead (back substitution)
@@ -1008,12 +1015,12 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
 #pragma unroll
             for (int k = 0; k < 6; ++k) sh.upper[g][6 * j + k] = w[k];
         }
+        // The rhs lane keeps its column (the right-hand side) in rhs; the
+        // Gauss-Jordan steps below also transform its col, which nothing
+        // reads (lane 12 is never a pivot column).
         double rhs[6];
 #pragma unroll
-        for (int r = 0; r < 6; ++r) {
-            rhs[r] = col[r];
-            if (vlane) col[r] = 0.0;
-        }
+        for (int r = 0; r < 6; ++r) rhs[r] = col[r];
         // Gauss-Jordan inverse on the columns of [a | inv].
         double scale = 0.0;
         static_for<6>([&](auto I) { scale = fmax(scale, fabs(row_bcast<I>(col[I]))); });
@@ -1021,29 +1028,52 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
 #pragma unroll
         for (int r = 0; r < 6; ++r)
             if (acol && r == j) col[r] += shift;
+        // Fast pass without row exchanges. Where no column step finds an
+        // entry below the diagonal larger in magnitude than the diagonal one,
+        // partial pivoting would exchange nothing and this pass performs
+        // exactly its operations; otherwise (some strand of the wavefront)
+        // the pivoting pass reruns from the saved columns.
+        double start[6];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) start[r] = col[r];
+        bool exchange = false, singular = false;
         static_for<6>([&](auto C) {
             double cv[6];
 #pragma unroll
             for (int i = 0; i < 6; ++i) cv[i] = row_bcast<C>(col[i]);
-            int pivot = C;
-            double best = fabs(cv[C]);
-            // the sequential search runs only when some strand of the
-            // wavefront has an entry larger than its diagonal one
-            bool larger = false;
+            const double diag = fabs(cv[C]);
+            if constexpr (C < 5) {
+                double below = fabs(cv[C + 1]);
 #pragma unroll
-            for (int i = C + 1; i < 6; ++i) larger |= fabs(cv[i]) > best;
-            if (__builtin_amdgcn_ballot_w64(larger)) {
+                for (int i = C + 2; i < 6; ++i) below = fmax(below, fabs(cv[i]));
+                exchange |= below > diag;
+            }
+            singular |= diag < 1.0e-30 || !isfinite(diag);
+            const double invp = 1.0 / cv[C];
+            col[C] *= invp;
+#pragma unroll
+            for (int r = 0; r < 6; ++r)
+                if (r != C) col[r] -= cv[r] * col[C];
+        });
+        if (__builtin_amdgcn_ballot_w64(exchange)) {
+            singular = false;
+#pragma unroll
+            for (int r = 0; r < 6; ++r) col[r] = start[r];
+            static_for<6>([&](auto C) {
+                double cv[6];
+#pragma unroll
+                for (int i = 0; i < 6; ++i) cv[i] = row_bcast<C>(col[i]);
+                int pivot = C;
+                double best = fabs(cv[C]);
 #pragma unroll
                 for (int i = C + 1; i < 6; ++i) {
                     const double v = fabs(cv[i]);
                     if (v > best) { best = v; pivot = i; }
                 }
-            }
-            if (best < 1.0e-30 || !isfinite(best)) {
-                if (t < len) ok = false;
-                pivot = C;
-            }
-            if (__builtin_amdgcn_ballot_w64(pivot != C)) {
+                if (best < 1.0e-30 || !isfinite(best)) {
+                    singular = true;
+                    pivot = C;
+                }
 #pragma unroll
                 for (int i = C + 1; i < 6; ++i) {
                     const bool sw = pivot == i;
@@ -1054,13 +1084,14 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
                     cv[C] = sw ? cy : cx;
                     cv[i] = sw ? cx : cy;
                 }
-            }
-            const double invp = 1.0 / cv[C];
-            col[C] *= invp;
+                const double invp = 1.0 / cv[C];
+                col[C] *= invp;
 #pragma unroll
-            for (int r = 0; r < 6; ++r)
-                if (r != C) col[r] -= cv[r] * col[C];
-        });
+                for (int r = 0; r < 6; ++r)
+                    if (r != C) col[r] -= cv[r] * col[C];
+            });
+        }
+        if (singular && t < len) ok = false;
         if (icol) {
 #pragma unroll
             for (int r = 0; r < 6; ++r) sh.inverse[g][6 * (j - 6) + r] = col[r];

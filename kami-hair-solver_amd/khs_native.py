@@ -244,6 +244,8 @@ class KhsxContactListStats(C.Structure):
         ("entries", C.c_uint64),
         ("longest", C.c_uint64),
         ("length_histogram", C.c_uint64 * 9),
+        ("root_nonempty", C.c_uint64),
+        ("root_entries", C.c_uint64),
     ]
 
 
