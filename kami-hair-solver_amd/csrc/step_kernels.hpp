@@ -540,12 +540,13 @@ __device__ inline void store_tile_blocks(const double *st, double *out, long fir
 // per element; with GRAD also the gradient and the Gauss-Newton blocks (the
 // diagonal block D and the upper coupling block U of each node). With LS the
 // state is q = alpha d + q_base, written back by the owners.
+// The assembly is held to 256 registers so two wavefronts share a SIMD
+// (a few values spill); at one wavefront per SIMD its loads are exposed.
 template <bool GRAD, bool LS>
-__global__ __launch_bounds__(kBlock) void k_evaluate(RodView m, ColliderView col, ContactCache cc,
-                                                     const uint32_t *best_tri, StepParams p, double *q,
-                                                     const double *q_old, const double *vel, const double *q_base,
-                                                     const double *dir, const double *alpha_at, double *grad,
-                                                     double *Dblk, double *Ublk, Partial *partials)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GRAD ? 2 : 1))) void k_evaluate(
+    RodView m, ColliderView col, ContactCache cc, const uint32_t *best_tri, StepParams p, double *q,
+    const double *q_old, const double *vel, const double *q_base, const double *dir, const double *alpha_at,
+    double *grad, double *Dblk, double *Ublk, Partial *partials)
 {
     const double alpha = LS ? *alpha_at : 0.0;
     const int lane = threadIdx.x & (kWave - 1);
