@@ -386,12 +386,13 @@ __global__ __launch_bounds__(kBlock) void k_contact_gather(const uint32_t *flags
     }
 }
 
-// Minimum of (distance, triangle) in lexicographic order across a list
-// pass's lane group; NaN distances never enter (callers skip them).
+// Minimum of (distance, triangle) in lexicographic order across a group of
+// LANES lanes; NaN distances never enter (callers skip them).
+template <int LANES = kListLanes>
 __device__ __forceinline__ void group_closest(double &d, uint32_t &t)
 {
 #pragma unroll
-    for (int o = kListLanes / 2; o >= 1; o >>= 1) {
+    for (int o = LANES / 2; o >= 1; o >>= 1) {
         const double od = __shfl_xor(d, o, kWave);
         const uint32_t ot = __shfl_xor(t, o, kWave);
         if (od < d || (od == d && ot < t)) {
@@ -401,13 +402,16 @@ __device__ __forceinline__ void group_closest(double &d, uint32_t &t)
     }
 }
 
-// Narrow phase, one wavefront per listed element, its lanes striding the
-// list: the closest triangle among those whose leaf box overlaps the padded
-// segment box (ties to the lower index) goes to best_tri. The listed
-// queue holds elements with a free left node, the ones the energy queries,
-// and their candidates are counted; `extra` (the friction pass) appends the
-// root elements (fixed left node, free right node), whose lists only
-// friction reads. Overflowed elements are left to the consumer's tree walk.
+// Narrow phase, a group of kClosestLanes lanes per listed element: the
+// group box-tests its list kClosestLanes entries at a time, packs the
+// triangles whose leaf box overlaps the padded segment box into LDS, then
+// spreads their distance computations over its lanes; the closest (ties to
+// the lower index) goes to best_tri. The listed queue holds elements with a
+// free left node, the ones the energy queries, and their candidates are
+// counted; `extra` (the friction pass) appends the root elements (fixed left
+// node, free right node), whose lists only friction reads. Overflowed
+// elements are left to the consumer's tree walk.
+constexpr int kClosestLanes = 16;
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, ColliderView col, ContactCache cc,
                                                          double pad, const double *q, const double *q_base,
@@ -415,10 +419,16 @@ __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, 
                                                          uint32_t *best_tri, Partial *partials,
                                                          const uint32_t *extra, uint32_t extra_count)
 {
+    constexpr int G = kClosestLanes;
+    __shared__ uint32_t packed[kBlock / G][kCacheCapacity];
     const double alpha = MODE == kAtTrial ? *alpha_at : 0.0;
-    const int lane = threadIdx.x & (kListLanes - 1);
-    const uint32_t group = (blockIdx.x * blockDim.x + threadIdx.x) / kListLanes;
-    const uint32_t groups = gridDim.x * blockDim.x / kListLanes;
+    const int lane = threadIdx.x & (G - 1);
+    const int slot = threadIdx.x / G;
+    uint32_t *cand = packed[slot];
+    const int shift = (threadIdx.x & (kWave - 1)) & ~(G - 1);  // the group's first lane in the wavefront
+    const unsigned long long below = (1ull << lane) - 1;
+    const uint32_t group = (blockIdx.x * blockDim.x + threadIdx.x) / G;
+    const uint32_t groups = gridDim.x * blockDim.x / G;
     const uint32_t listed = *cc.list_count;
     unsigned long long candidates = 0;
     for (uint32_t j = group; j < listed + extra_count; j += groups) {
@@ -430,13 +440,25 @@ __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, 
         element_points<MODE>(L, q, q_base, dir, alpha, p0, p1);
         const V3 lo = vmin(p0, p1), hi = vmax(p0, p1);
         const uint32_t *tris = cc.tris + size_t(L) * kCacheCapacity;
+        uint32_t n = 0;
+        for (uint32_t k0 = 0; k0 < c; k0 += G) {
+            const uint32_t k = k0 + lane;
+            uint32_t t = 0;
+            bool hit = false;
+            if (k < c) {
+                t = tris[k];
+                const double *b = cc.tri_box + 6 * size_t(t);
+                hit = overlap(lo, hi, mk(b[0], b[1], b[2]), mk(b[3], b[4], b[5]), pad);
+            }
+            const unsigned long long mask = (__ballot(hit) >> shift) & ((G == 64) ? ~0ull : ((1ull << G) - 1));
+            if (hit) cand[n + __popcll(mask & below)] = t;
+            n += __popcll(mask);
+        }
+        __builtin_amdgcn_wave_barrier();
         double bd = 1.0e300;
         uint32_t bt = kNoTriangle;
-        for (uint32_t k = lane; k < c; k += kListLanes) {
-            const uint32_t t = tris[k];
-            const double *b = cc.tri_box + 6 * size_t(t);
-            if (!overlap(lo, hi, mk(b[0], b[1], b[2]), mk(b[3], b[4], b[5]), pad)) continue;
-            if (counted) ++candidates;
+        for (uint32_t i = lane; i < n; i += G) {
+            const uint32_t t = cand[i];
             const uint32_t *t3 = col.triangles + 3 * size_t(t);
             const double d =
                 closest_segment_triangle(p0, p1, vertex(col, t3[0]), vertex(col, t3[1]), vertex(col, t3[2])).distance;
@@ -445,8 +467,12 @@ __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, 
                 bt = t;
             }
         }
-        group_closest(bd, bt);
-        if (lane == 0) best_tri[L] = bt;
+        __builtin_amdgcn_wave_barrier();
+        group_closest<G>(bd, bt);
+        if (lane == 0) {
+            best_tri[L] = bt;
+            if (counted) candidates += n;
+        }
     }
     if (partials) {
         Partial acc = empty_partial();
