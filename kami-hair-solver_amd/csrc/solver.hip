@@ -99,6 +99,8 @@ struct DeviceState {
     Buffer<uint32_t> solve_order;  // strands by decreasing free node count, for the solve's dispatch
     Buffer<uint32_t> root_elements;  // contact elements with a fixed left and a free right node
     uint32_t root_element_count = 0;
+    Buffer<unsigned long long> ccd_bound;  // two alternating launch-wide step bounds of the list CCD
+    uint32_t ccd_slot = 0;
     // state and solver vectors
     Buffer<double> q, q_old, vel, q_base, snap_q, snap_v, grad, dir, precond, Dblk, Ublk, Cp, dp, dv;
     // root targets (3 per node)
@@ -524,6 +526,12 @@ KhsResult Solver::upload_model()
                 roots.push_back(n);
         }
         d.root_element_count = uint32_t(roots.size());
+        const double one = 1.0;
+        unsigned long long bits[2];
+        std::memcpy(&bits[0], &one, sizeof(one));
+        bits[1] = bits[0];
+        d.ccd_bound.allocate(2);
+        d.ccd_bound.upload(bits, 2);
         d.root_elements.allocate(std::max<size_t>(1, roots.size()));
         if (!roots.empty()) d.root_elements.upload(roots.data(), roots.size());
     }
@@ -1109,7 +1117,8 @@ KhsResult Solver::optimize_substep(double dt)
                 refresh_contact_cache(dev::kSwept, false);
                 hipLaunchKernelGGL(dev::k_list_ccd, dim3(kPairBlocks), dim3(kBlock), 0, d.stream, d.flags.get(),
                                    d.collider(), d.cache(), cp, d.q.get(), d.dir.get(),
-                                   d.partials.get() + d.node_blocks());
+                                   d.partials.get() + d.node_blocks(), d.ccd_bound.get(), int(d.ccd_slot));
+                d.ccd_slot ^= 1u;
             }
             hipLaunchKernelGGL(dev::k_ccd_slope, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, d.collider(),
                                d.cache(), list_pass ? 1 : 0, cp, d.q.get(), d.grad.get(), d.dir.get(),

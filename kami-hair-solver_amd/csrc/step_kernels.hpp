@@ -483,8 +483,12 @@ __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, 
 
 // Conservative advancement of one segment along the direction against one
 // static triangle (cuda_backend.cu:989-1000): the largest safe step, 1 when
-// the triangle does not limit it.
-__device__ inline double advancement_limit(V3 p0, V3 p1, V3 d0, V3 d1, double speed, V3 a, V3 b, V3 c, double target)
+// the triangle does not limit it. Once alpha reaches `stop_at` every later
+// return would be at least 0.9 alpha, so the loop returns alpha: callers
+// that only take a minimum pass stop_at = (a value some candidate already
+// returned) / 0.9 and get the same minimum.
+__device__ inline double advancement_limit(V3 p0, V3 p1, V3 d0, V3 d1, double speed, V3 a, V3 b, V3 c, double target,
+                                           double stop_at = 2.0)
 {
     double alpha = 0.0;
     for (int it = 0; it < 80; ++it) {
@@ -495,40 +499,101 @@ __device__ inline double advancement_limit(V3 p0, V3 p1, V3 d0, V3 d1, double sp
         if (alpha + advance >= 1) return 1.0;
         if (advance < 1e-12) return alpha;
         alpha += advance;
+        if (alpha >= stop_at) return alpha;
     }
     return alpha;
 }
 
-// Line-search step bound, one wavefront per listed element with a free
-// left node, its lanes striding the list.
+// Line-search step bound over the listed elements with a free left node.
+// A block takes the listed elements kBlock / kClosestLanes at a time: each
+// group of kClosestLanes lanes box-tests one element's list against its
+// swept segment box and appends the overlapping (element, triangle) pairs
+// to the block's LDS queue; then every lane of the block takes pairs from
+// the queue, so an element with many candidates spreads over the block.
+// Only the minimum step matters, so each advancement loop stops once it
+// cannot go below the smallest step found so far: the lane's own, or the
+// launch-wide one in `bound[slot]` (bits of a non-negative double, lowered
+// with atomicMin and read without ordering; any value read is some
+// candidate's step or 1, so the minimum is unchanged). The kernel resets
+// the other slot to 1 for the next launch.
+constexpr int kCcdBatch = kBlock / kClosestLanes;
 __global__ __launch_bounds__(kBlock) void k_list_ccd(const uint32_t *flags, ColliderView col, ContactCache cc,
                                                      StepParams p, const double *q, const double *dir,
-                                                     Partial *partials)
+                                                     Partial *partials, unsigned long long *bound, int slot)
 {
-    const int lane = threadIdx.x & (kListLanes - 1);
-    const uint32_t group = (blockIdx.x * blockDim.x + threadIdx.x) / kListLanes;
-    const uint32_t groups = gridDim.x * blockDim.x / kListLanes;
+    constexpr int G = kClosestLanes;
+    __shared__ uint32_t pair_tri[kCcdBatch * kCacheCapacity];
+    __shared__ uint8_t pair_elem[kCcdBatch * kCacheCapacity];
+    __shared__ double elem[kCcdBatch][13];  // p0, p1, d0, d1, speed
+    __shared__ uint32_t pair_count;
+    const int lane = threadIdx.x & (G - 1), my = threadIdx.x / G;
+    const int shift = (threadIdx.x & (kWave - 1)) & ~(G - 1);
+    const unsigned long long below = (1ull << lane) - 1;
     const uint32_t listed = *cc.list_count;
+    if (blockIdx.x == 0 && threadIdx.x == 0) bound[slot ^ 1] = __double_as_longlong(1.0);
     double limit = 1.0;
-    for (uint32_t j = group; j < listed; j += groups) {
-        const uint32_t L = cc.list[j];
-        const uint32_t c = cc.count[L];
-        if (c == 0 || c == kCacheOverflow || (flags[L] & kNodeFixed)) continue;
-        const V3 p0 = load3(q + 6 * size_t(L)), p1 = load3(q + 6 * size_t(L + 1));
-        const V3 d0 = load3(dir + 6 * size_t(L)), d1 = load3(dir + 6 * size_t(L + 1));
-        const double speed = fmax(norm(d0), norm(d1));
-        if (speed < 1e-30) continue;
+    for (uint32_t first = blockIdx.x * kCcdBatch; first < listed; first += gridDim.x * kCcdBatch) {
+        if (threadIdx.x == 0) pair_count = 0;
+        __syncthreads();
+        const uint32_t j = first + my;
+        uint32_t L = 0, c = 0;
+        if (j < listed) {
+            L = cc.list[j];
+            c = cc.count[L];
+            if (c == kCacheOverflow || (flags[L] & kNodeFixed)) c = 0;
+        }
+        V3 p0 = mk(0, 0, 0), p1 = p0, d0 = p0, d1 = p0;
+        double speed = 0.0;
+        if (c > 0) {
+            p0 = load3(q + 6 * size_t(L));
+            p1 = load3(q + 6 * size_t(L + 1));
+            d0 = load3(dir + 6 * size_t(L));
+            d1 = load3(dir + 6 * size_t(L + 1));
+            speed = fmax(norm(d0), norm(d1));
+            if (speed < 1e-30) c = 0;
+        }
+        if (c > 0 && lane == 0) {
+            const double v[13] = {p0.x, p0.y, p0.z, p1.x, p1.y, p1.z, d0.x, d0.y, d0.z, d1.x, d1.y, d1.z, speed};
+#pragma unroll
+            for (int k = 0; k < 13; ++k) elem[my][k] = v[k];
+        }
         const V3 lo = vmin(vmin(p0, p1), vmin(add(p0, d0), add(p1, d1)));
         const V3 hi = vmax(vmax(p0, p1), vmax(add(p0, d0), add(p1, d1)));
         const uint32_t *tris = cc.tris + size_t(L) * kCacheCapacity;
-        for (uint32_t k = lane; k < c; k += kListLanes) {
-            const uint32_t t = tris[k];
-            const double *b = cc.tri_box + 6 * size_t(t);
-            if (!overlap(lo, hi, mk(b[0], b[1], b[2]), mk(b[3], b[4], b[5]), p.sweep_target)) continue;
-            const uint32_t *t3 = col.triangles + 3 * size_t(t);
-            limit = fmin(limit, advancement_limit(p0, p1, d0, d1, speed, vertex(col, t3[0]), vertex(col, t3[1]),
-                                                  vertex(col, t3[2]), p.sweep_target));
+        for (uint32_t k0 = 0; k0 < c; k0 += G) {
+            const uint32_t k = k0 + lane;
+            uint32_t t = 0;
+            bool hit = false;
+            if (k < c) {
+                t = tris[k];
+                const double *b = cc.tri_box + 6 * size_t(t);
+                hit = overlap(lo, hi, mk(b[0], b[1], b[2]), mk(b[3], b[4], b[5]), p.sweep_target);
+            }
+            const unsigned long long mask = (__ballot(hit) >> shift) & ((1ull << G) - 1);
+            uint32_t base = 0;
+            if (lane == 0 && mask) base = atomicAdd(&pair_count, uint32_t(__popcll(mask)));
+            base = __shfl(base, shift, kWave);
+            if (hit) {
+                const uint32_t at = base + __popcll(mask & below);
+                pair_tri[at] = t;
+                pair_elem[at] = uint8_t(my);
+            }
         }
+        __syncthreads();
+        const uint32_t pairs = pair_count;
+        for (uint32_t i = threadIdx.x; i < pairs; i += kBlock) {
+            const uint32_t t = pair_tri[i];
+            const double *e = elem[pair_elem[i]];
+            const uint32_t *t3 = col.triangles + 3 * size_t(t);
+            const double seen = fmin(limit, __longlong_as_double(__hip_atomic_load(
+                                                bound + slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
+            const double r = advancement_limit(mk(e[0], e[1], e[2]), mk(e[3], e[4], e[5]), mk(e[6], e[7], e[8]),
+                                               mk(e[9], e[10], e[11]), e[12], vertex(col, t3[0]), vertex(col, t3[1]),
+                                               vertex(col, t3[2]), p.sweep_target, seen / 0.9);
+            if (r < seen) atomicMin(bound + slot, (unsigned long long)__double_as_longlong(r));
+            limit = fmin(limit, r);
+        }
+        __syncthreads();
     }
     Partial acc = empty_partial();
     acc.alpha = limit;
