@@ -94,10 +94,12 @@ KHS_API KhsResult khsxCheckBroadphase(KhsSolver *solver, uint64_t *violations);
    so the desc tolerances apply as given (used to run to a 1e-6 residual). */
 KHS_API KhsResult khsxSetConvergenceMode(KhsSolver *solver, uint32_t mode);
 
-/* Arithmetic mode of the GPU step. 0 (default): FP64 throughout.
-   1: mixed precision (FP32 assembly and block solve with FP64 residual and
-   refinement; FP64 barrier, CCD and convergence test). The oracle accepts
-   only 0. */
+/* Arithmetic mode of the step. 0 (default): FP64 throughout, the only mode
+   either library accepts. 1 is reserved for a mixed-precision step (FP32
+   assembly and block solve with FP64 refinement) and is rejected with
+   KHS_ERROR_INVALID_ARGUMENT: the per-strand systems have condition numbers
+   near 3e12, an FP32 block solve leaves relative residuals near 1e-3, and
+   non-packed FP32 vector instructions issue no faster than FP64 ones. */
 KHS_API KhsResult khsxSetPrecisionMode(KhsSolver *solver, uint32_t mode);
 
 /* Device-side timing of the step, from HIP events recorded on the

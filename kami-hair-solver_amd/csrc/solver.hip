@@ -1739,7 +1739,8 @@ KhsResult Solver::set_convergence_mode(uint32_t mode)
 KhsResult Solver::set_precision_mode(uint32_t mode)
 {
     if (mode > 0) {
-        error_ = "Only the FP64 step (mode 0) is available.";
+        error_ = "Only the FP64 step (mode 0) is available: an FP32 block solve of these strand systems "
+                 "(condition numbers near 3e12) leaves relative residuals near 1e-3.";
         return KHS_ERROR_INVALID_ARGUMENT;
     }
     precision_mode_ = mode;
