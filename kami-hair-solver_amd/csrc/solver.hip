@@ -106,6 +106,9 @@ struct DeviceState {
     uint32_t root_element_count = 0;
     Buffer<unsigned long long> ccd_bound;  // two alternating launch-wide step bounds of the list CCD
     uint32_t ccd_slot = 0;
+    Buffer<unsigned> solve_failed;  // two alternating failure flags of the block solve
+    uint32_t failed_slot = 0;
+    dev::TrialStep trial_step{};     // how the next trial's first kernel sets alpha
     // state and solver vectors
     Buffer<double> q, q_old, vel, q_base, snap_q, snap_v, grad, dir, precond, Dblk, Ublk, Cp, dp, dv;
     // root targets (3 per node)
@@ -542,6 +545,9 @@ KhsResult Solver::upload_model()
         bits[1] = bits[0];
         d.ccd_bound.allocate(2);
         d.ccd_bound.upload(bits, 2);
+        const unsigned zeros[2] = {0u, 0u};
+        d.solve_failed.allocate(2);
+        d.solve_failed.upload(zeros, 2);
         d.root_elements.allocate(std::max<size_t>(1, roots.size()));
         if (!roots.empty()) d.root_elements.upload(roots.data(), roots.size());
     }
@@ -719,13 +725,17 @@ void Solver::refresh_contact_cache(int mode, bool full, bool swept_leaves, bool 
         // before its next use
         if (mode == dev::kAtState)
             hipLaunchKernelGGL(dev::k_contact_check<dev::kAtState>, grid, block, 0, d.stream, d.rods(), cc, d.q.get(),
-                               d.q_base.get(), d.dir.get(), d.alpha.get(), pad, with_roots ? 1 : 0);
-        else if (mode == dev::kAtTrial)
+                               d.q_base.get(), d.dir.get(), d.alpha.get(), dev::TrialStep{}, pad,
+                               with_roots ? 1 : 0);
+        else if (mode == dev::kAtTrial) {
             hipLaunchKernelGGL(dev::k_contact_check<dev::kAtTrial>, grid, block, 0, d.stream, d.rods(), cc, d.q.get(),
-                               d.q_base.get(), d.dir.get(), d.alpha.get(), pad, with_roots ? 1 : 0);
+                               d.q_base.get(), d.dir.get(), d.alpha.get(), d.trial_step, pad, with_roots ? 1 : 0);
+            d.trial_step.source = dev::kAlphaStored;
+        }
         else
             hipLaunchKernelGGL(dev::k_contact_check<dev::kSwept>, grid, block, 0, d.stream, d.rods(), cc, d.q.get(),
-                               d.q_base.get(), d.dir.get(), d.alpha.get(), pad, with_roots ? 1 : 0);
+                               d.q_base.get(), d.dir.get(), d.alpha.get(), dev::TrialStep{}, pad,
+                               with_roots ? 1 : 0);
         hipLaunchKernelGGL(dev::k_contact_gather, dim3(kGatherBlocks), block, 0, d.stream, d.flags.get(), d.collider(), cc,
                            d.cache_pending_count.get() + (d.pending_slot ^ 1u));
         d.pending_slot ^= 1u;
@@ -1039,6 +1049,24 @@ void Solver::steepest_direction_async()
     HIP_TRY(hipGetLastError());
 }
 
+// Sets the next trial's step: the CCD bound of the last reduction or a
+// host value. With contact lists the trial's first kernel (the list check)
+// resolves it; otherwise a one-thread kernel does.
+void Solver::set_trial_step(int source, double value)
+{
+    DeviceState &d = *dev_;
+    const Partial *ccd = (const Partial *)(d.reduced.get() + DeviceState::kCcdSlot);
+    if (d.cache_valid) {
+        d.trial_step = dev::TrialStep{ccd, value, source, d.T ? 1 : 0};
+        return;
+    }
+    if (source == dev::kAlphaFromCcd)
+        hipLaunchKernelGGL(dev::k_choose_alpha, dim3(1), dim3(1), 0, d.stream, ccd, d.T ? 1 : 0, d.alpha.get());
+    else
+        hipLaunchKernelGGL(dev::k_set_double, dim3(1), dim3(1), 0, d.stream, d.alpha.get(), value);
+    HIP_TRY(hipGetLastError());
+}
+
 // Block direction limited to the trust radius, with the choice whether it
 // is usable (solved and descending) made on the device (reference
 // cuda_backend.cu:623-648). The choice reaches the host with the next
@@ -1048,20 +1076,21 @@ void Solver::make_direction_async()
 {
     DeviceState &d = *dev_;
     const RodView rv = d.rods();
-    hipLaunchKernelGGL(dev::k_fill_u32, dim3(1), dim3(64), 0, d.stream, d.flag.get(), size_t(1), 0u);
+    unsigned *failed = d.solve_failed.get() + d.failed_slot;
     hipEvent_t t0 = d.tic();
     hipLaunchKernelGGL(dev::k_block_solve, dim3(grid_for(d.S, dev::kSolveGroups)), dim3(kWave), kSolveLdsReserve,
                        d.stream, rv, d.solve_order.get(), d.Dblk.get(), d.Ublk.get(), d.grad.get(), d.Cp.get(),
-                       d.dp.get(), d.dir.get(), d.flag.get());
+                       d.dp.get(), d.dir.get(), failed, d.solve_failed.get() + (d.failed_slot ^ 1u));
+    d.failed_slot ^= 1u;
     d.toc(t0, d.totals.solve_ms, d.totals.solve_launches);
     hipLaunchKernelGGL(dev::k_direction_stats, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, d.grad.get(),
-                       d.dir.get(), (const unsigned *)d.flag.get(), (const DirectionChoice *)nullptr,
-                       d.partials.get());
+                       d.dir.get(), (const unsigned *)failed, (const DirectionChoice *)nullptr, d.partials.get());
     d.reduce_async(d.node_blocks(), DeviceState::kEvalSlot);
+    // the choice, and the line search's base state q_base = q
     const size_t dofs = 6 * size_t(d.N);
     hipLaunchKernelGGL(dev::k_apply_direction_choice, dim3(grid_for(dofs, kBlock)), dim3(kBlock), 0, d.stream,
                        (const Partial *)(d.reduced.get() + DeviceState::kEvalSlot), trust_radius_, d.choice(),
-                       d.dir.get(), dofs);
+                       d.dir.get(), dofs, (const double *)d.q.get(), d.q_base.get());
     HIP_TRY(hipGetLastError());
 }
 
@@ -1114,9 +1143,8 @@ KhsResult Solver::optimize_substep(double dt)
         }
         if (it == desc_.newton_iterations) break;
         phase_.store(KHS_PHASE_LINEAR_SOLVE);
-        make_direction_async();
+        make_direction_async();  // also q_base = q
         ++step_stats_.linear_solves;
-        HIP_TRY(hipMemcpyAsync(d.q_base.get(), d.q.get(), d.q.bytes(), hipMemcpyDeviceToDevice, d.stream));
         bool accepted = false, block_direction = true;
         double alpha = 1.0;
         for (int attempt = 0; attempt < 2;) {
@@ -1136,9 +1164,7 @@ KhsResult Solver::optimize_substep(double dt)
             HIP_TRY(hipGetLastError());
             d.toc(t0, d.totals.collision_ms, d.totals.collision_launches);
             d.reduce_async(d.node_blocks() + (list_pass ? kPairBlocks : 0), DeviceState::kCcdSlot);
-            hipLaunchKernelGGL(dev::k_choose_alpha, dim3(1), dim3(1), 0, d.stream,
-                               (const Partial *)(d.reduced.get() + DeviceState::kCcdSlot), d.T ? 1 : 0,
-                               d.alpha.get());
+            set_trial_step(dev::kAlphaFromCcd, 0.0);
             // The first trial runs at the device's step before the host has
             // seen the bound; one read-back returns direction choice, bound
             // and trial. A trial the host would not have run is discarded,
@@ -1167,7 +1193,7 @@ KhsResult Solver::optimize_substep(double dt)
                 Reduced trial = unpack(v.eval);
                 for (uint32_t search = 0; search < desc_.line_search_iterations; ++search) {
                     if (search > 0) {
-                        hipLaunchKernelGGL(dev::k_set_double, dim3(1), dim3(1), 0, d.stream, d.alpha.get(), alpha);
+                        set_trial_step(dev::kAlphaGiven, alpha);
                         trial = evaluate(false, dt, true);
                     }
                     ++step_stats_.line_search_evaluations;
@@ -1660,7 +1686,7 @@ KhsResult Solver::test_evaluate(double dt, double *gradient, double *direction, 
         hipLaunchKernelGGL(dev::k_fill_u32, dim3(1), dim3(64), 0, d.stream, d.flag.get(), size_t(1), 0u);
         hipLaunchKernelGGL(dev::k_block_solve, dim3(grid_for(d.S, dev::kSolveGroups)), dim3(kWave),
                            kSolveLdsReserve, d.stream, rv, d.solve_order.get(), d.Dblk.get(), d.Ublk.get(),
-                           d.grad.get(), d.Cp.get(), d.dp.get(), d.dir.get(), d.flag.get());
+                           d.grad.get(), d.Cp.get(), d.dp.get(), d.dir.get(), d.flag.get(), (unsigned *)nullptr);
         HIP_TRY(hipGetLastError());
         unsigned failed = 0;
         HIP_TRY(hipMemcpyAsync(&failed, d.flag.get(), sizeof(unsigned), hipMemcpyDeviceToHost, d.stream));

@@ -92,6 +92,7 @@ private:
     static Reduced unpack(const Partial &r);
     Reduced evaluate(bool gradient, double dt, bool line_search);
     void make_direction_async();
+    void set_trial_step(int source, double value);
     void steepest_direction_async();
     KhsResult optimize_substep(double dt);
     KhsResult run_frame(double frame_dt, const double *root_a, const double *rot_a, const double *root_b,

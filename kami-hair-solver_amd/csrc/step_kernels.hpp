@@ -321,6 +321,37 @@ __device__ inline uint32_t wave_gather(const ColliderView &col, V3 lo, V3 hi, ui
     return n;
 }
 
+// Where a trial's step comes from: already in alpha_at (kAlphaStored),
+// the CCD bound of the last reduction (kAlphaFromCcd), or a value the host
+// chose (kAlphaGiven). The first kernel of a trial evaluation resolves it
+// and stores it in alpha_at for the kernels after it.
+enum AlphaSource { kAlphaStored = 0, kAlphaFromCcd = 1, kAlphaGiven = 2 };
+struct TrialStep {
+    const Partial *ccd;
+    double value;
+    int source;
+    int has_collider;
+};
+
+// The CCD bound, scaled by 0.9 when it limits the step (cuda_backend.cu:623-648).
+__device__ __forceinline__ double step_from_ccd(const Partial *ccd, int has_collider)
+{
+    double a = 1.0;
+    if (has_collider) {
+        a = ccd->alpha;
+        if (a < 1.0) a *= 0.9;
+    }
+    return a;
+}
+
+__device__ __forceinline__ double resolve_step(const TrialStep &s, double *alpha_at)
+{
+    const double a = s.source == kAlphaFromCcd ? step_from_ccd(s.ccd, s.has_collider)
+                                               : (s.source == kAlphaGiven ? s.value : *alpha_at);
+    if (s.source != kAlphaStored && blockIdx.x == 0 && threadIdx.x == 0) *alpha_at = a;
+    return a;
+}
+
 // Partial refresh, first half: every contact element whose query box
 // (grown by pad plus the rounding margin) left its cached box gets the new
 // box (query box grown by pad + skin) and is queued for gathering.
@@ -328,9 +359,9 @@ __device__ inline uint32_t wave_gather(const ColliderView &col, V3 lo, V3 hi, ui
 // pass, the one reader of their lists).
 template <int MODE>
 __global__ void k_contact_check(RodView m, ContactCache cc, const double *q, const double *q_base, const double *dir,
-                                const double *alpha_at, double pad, int with_roots)
+                                double *alpha_at, TrialStep step, double pad, int with_roots)
 {
-    const double alpha = MODE == kAtTrial ? *alpha_at : 0.0;
+    const double alpha = MODE == kAtTrial ? resolve_step(step, alpha_at) : 0.0;
     const uint32_t L = blockIdx.x * blockDim.x + threadIdx.x;
     if (L >= m.node_count) return;
     const uint32_t fl = m.flags[L];
@@ -1028,8 +1059,11 @@ struct ForwardSlot {
 
 __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t *order, const double *Dblk,
                                                        const double *Ublk, const double *grad, double *Cp,
-                                                       double *dp, double *dir, unsigned *failed)
+                                                       double *dp, double *dir, unsigned *failed,
+                                                       unsigned *next_failed)
 {
+    // the next solve's flag starts at zero (the two alternate)
+    if (blockIdx.x == 0 && threadIdx.x == 0 && next_failed) *next_failed = 0;
     __shared__ SolveShared sh;
     const int g = threadIdx.x / kSolveLanes, j = threadIdx.x % kSolveLanes;
     const bool acol = j < 6, icol = j >= 6 && j < 12, vlane = j == 12;
@@ -1297,8 +1331,9 @@ __global__ void k_choose_direction(const Partial *stats, double trust, Direction
 
 // k_choose_direction and the block direction's scaling in one pass: every
 // thread derives the choice from the reduced statistics, one records it.
+// Also saves the line search's base state, q_base = q.
 __global__ void k_apply_direction_choice(const Partial *stats, double trust, DirectionChoice *choice, double *x,
-                                         size_t count)
+                                         size_t count, const double *q, double *q_base)
 {
     const double slope = stats->sum[kSlope];
     const int used = !stats->failed && slope < 0 && isfinite(slope);
@@ -1308,7 +1343,10 @@ __global__ void k_apply_direction_choice(const Partial *stats, double trust, Dir
         choice->used = used;
         choice->scale = scale;
     }
-    if (used && scale != 1.0 && i < count) x[i] *= scale;
+    if (i < count) {
+        if (used && scale != 1.0) x[i] *= scale;
+        if (q_base) q_base[i] = q[i];
+    }
 }
 
 __global__ void k_force_steepest(DirectionChoice *choice)
@@ -1340,12 +1378,7 @@ __global__ void k_scale_direction(const DirectionChoice *choice, int used, doubl
 __global__ void k_choose_alpha(const Partial *ccd, int has_collider, double *alpha)
 {
     if (threadIdx.x || blockIdx.x) return;
-    double a = 1.0;
-    if (has_collider) {
-        a = ccd->alpha;
-        if (a < 1.0) a *= 0.9;
-    }
-    *alpha = a;
+    *alpha = step_from_ccd(ccd, has_collider);
 }
 
 __global__ void k_set_double(double *x, double v)
