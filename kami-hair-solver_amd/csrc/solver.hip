@@ -260,7 +260,7 @@ struct DeviceState {
     void reduce_async(uint32_t blocks, int slot)
     {
         hipLaunchKernelGGL(dev::k_reduce, dim3(1), dim3(kBlock), 0, stream, partials.get(), blocks,
-                           reduced.get() + slot);
+                           reduced.get() + slot, mirror_slot(slot));
         HIP_TRY(hipGetLastError());
     }
     // One wait for everything the host reads after a batch of launches.
@@ -268,13 +268,19 @@ struct DeviceState {
         Partial eval, steep, ccd;
         DirectionChoice choice;
     };
+    // The reductions and choice kernels write host_view as they run (a
+    // mirror of `reduced` in coherent pinned memory), so a read-back is a
+    // wait and no copy.
     HostView read_back()
     {
         static_assert(sizeof(HostView) == 3 * sizeof(Partial) + sizeof(DirectionChoice), "packed readback");
-        // reduced holds the three Partials followed by the choice
-        HIP_TRY(hipMemcpyAsync(host_view, reduced.get(), sizeof(HostView), hipMemcpyDeviceToHost, stream));
         wait();
         return *host_view;
+    }
+    Partial *mirror_slot(int slot) const { return reinterpret_cast<Partial *>(host_view_device) + slot; }
+    DirectionChoice *mirror_choice() const
+    {
+        return reinterpret_cast<DirectionChoice *>(reinterpret_cast<Partial *>(host_view_device) + 3);
     }
     void wait()
     {
@@ -282,7 +288,8 @@ struct DeviceState {
         settle();
     }
     static constexpr int kEvalSlot = 0, kSteepSlot = 1, kCcdSlot = 2;
-    HostView *host_view = nullptr;  // pinned
+    HostView *host_view = nullptr;  // pinned, coherent
+    void *host_view_device = nullptr;  // its device address
 
     uint32_t tiled_blocks() const { return grid_for(size_t(N) + 1, kNodesPerBlock); }
     uint32_t node_blocks() const { return grid_for(N, kBlock); }
@@ -492,7 +499,10 @@ KhsResult Solver::upload_model()
     HIP_TRY(hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreate(&d.frame_begin));
     HIP_TRY(hipEventCreate(&d.frame_end));
-    HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&d.host_view), sizeof(DeviceState::HostView)));
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&d.host_view), sizeof(DeviceState::HostView),
+                          hipHostMallocMapped | hipHostMallocCoherent));
+    std::memset(d.host_view, 0, sizeof(DeviceState::HostView));
+    HIP_TRY(hipHostGetDevicePointer(&d.host_view_device, d.host_view, 0));
     d.alpha.allocate(1);
     const RodModel &m = model_;
     d.N = m.node_count;
@@ -1043,7 +1053,8 @@ void Solver::steepest_direction_async()
                        d.partials.get());
     d.reduce_async(d.node_blocks(), DeviceState::kSteepSlot);
     hipLaunchKernelGGL(dev::k_choose_steepest_scale, dim3(1), dim3(1), 0, d.stream,
-                       (const Partial *)(d.reduced.get() + DeviceState::kSteepSlot), trust_radius_, d.choice());
+                       (const Partial *)(d.reduced.get() + DeviceState::kSteepSlot), trust_radius_, d.choice(),
+                       d.mirror_choice());
     hipLaunchKernelGGL(dev::k_scale_direction, dim3(grid_for(dofs, kBlock)), dim3(kBlock), 0, d.stream,
                        (const DirectionChoice *)d.choice(), 0, d.dir.get(), dofs);
     HIP_TRY(hipGetLastError());
@@ -1090,7 +1101,7 @@ void Solver::make_direction_async()
     const size_t dofs = 6 * size_t(d.N);
     hipLaunchKernelGGL(dev::k_apply_direction_choice, dim3(grid_for(dofs, kBlock)), dim3(kBlock), 0, d.stream,
                        (const Partial *)(d.reduced.get() + DeviceState::kEvalSlot), trust_radius_, d.choice(),
-                       d.dir.get(), dofs, (const double *)d.q.get(), d.q_base.get());
+                       d.mirror_choice(), d.dir.get(), dofs, (const double *)d.q.get(), d.q_base.get());
     HIP_TRY(hipGetLastError());
 }
 
@@ -1209,7 +1220,7 @@ KhsResult Solver::optimize_substep(double dt)
             if (accepted) break;
             if (!block_direction) break;
             HIP_TRY(hipMemcpyAsync(d.q.get(), d.q_base.get(), d.q.bytes(), hipMemcpyDeviceToDevice, d.stream));
-            hipLaunchKernelGGL(dev::k_force_steepest, dim3(1), dim3(1), 0, d.stream, d.choice());
+            hipLaunchKernelGGL(dev::k_force_steepest, dim3(1), dim3(1), 0, d.stream, d.choice(), d.mirror_choice());
             steepest_direction_async();
             block_direction = false;
             ++attempt;

@@ -76,8 +76,9 @@ __device__ inline Partial empty_partial()
 }
 
 // Block-wide fixed-order reduction of one Partial per thread; thread 0
-// writes the result.
-__device__ inline void block_reduce(Partial p, Partial *out)
+// writes the result (also to `mirror`, host memory the host reads after
+// its wait, when given).
+__device__ inline void block_reduce(Partial p, Partial *out, Partial *mirror = nullptr)
 {
     __shared__ Partial waves[kBlock / kWave];
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
@@ -106,11 +107,13 @@ __device__ inline void block_reduce(Partial p, Partial *out)
             r.failed |= o.failed;
         }
         *out = r;
+        if (mirror) *mirror = r;
     }
 }
 
 // Folds per-block partials in index order (one block of kBlock threads).
-__global__ __launch_bounds__(kBlock) void k_reduce(const Partial *partials, uint32_t count, Partial *out)
+__global__ __launch_bounds__(kBlock) void k_reduce(const Partial *partials, uint32_t count, Partial *out,
+                                                   Partial *mirror)
 {
     Partial acc = empty_partial();
     for (uint32_t i = threadIdx.x; i < count; i += kBlock) {
@@ -124,7 +127,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const Partial *partials, uint
         acc.infeasible |= o.infeasible;
         acc.failed |= o.failed;
     }
-    block_reduce(acc, out);
+    block_reduce(acc, out, mirror);
 }
 
 
@@ -1331,9 +1334,11 @@ __global__ void k_choose_direction(const Partial *stats, double trust, Direction
 
 // k_choose_direction and the block direction's scaling in one pass: every
 // thread derives the choice from the reduced statistics, one records it.
-// Also saves the line search's base state, q_base = q.
-__global__ void k_apply_direction_choice(const Partial *stats, double trust, DirectionChoice *choice, double *x,
-                                         size_t count, const double *q, double *q_base)
+// Also saves the line search's base state, q_base = q. The choice is
+// written to the device copy and to its host mirror.
+__global__ void k_apply_direction_choice(const Partial *stats, double trust, DirectionChoice *choice,
+                                         DirectionChoice *mirror, double *x, size_t count, const double *q,
+                                         double *q_base)
 {
     const double slope = stats->sum[kSlope];
     const int used = !stats->failed && slope < 0 && isfinite(slope);
@@ -1342,6 +1347,8 @@ __global__ void k_apply_direction_choice(const Partial *stats, double trust, Dir
     if (i == 0) {
         choice->used = used;
         choice->scale = scale;
+        mirror->used = used;
+        mirror->scale = scale;
     }
     if (i < count) {
         if (used && scale != 1.0) x[i] *= scale;
@@ -1349,18 +1356,19 @@ __global__ void k_apply_direction_choice(const Partial *stats, double trust, Dir
     }
 }
 
-__global__ void k_force_steepest(DirectionChoice *choice)
+__global__ void k_force_steepest(DirectionChoice *choice, DirectionChoice *mirror)
 {
     if (threadIdx.x || blockIdx.x) return;
-    choice->used = 0;
-    choice->scale = 1.0;
+    choice->used = mirror->used = 0;
+    choice->scale = mirror->scale = 1.0;
 }
 
 // Trust-radius factor of the steepest-descent direction.
-__global__ void k_choose_steepest_scale(const Partial *stats, double trust, DirectionChoice *choice)
+__global__ void k_choose_steepest_scale(const Partial *stats, double trust, DirectionChoice *choice,
+                                        DirectionChoice *mirror)
 {
     if (threadIdx.x || blockIdx.x || choice->used) return;
-    choice->scale = stats->max_translation > trust ? trust / stats->max_translation : 1.0;
+    choice->scale = mirror->scale = stats->max_translation > trust ? trust / stats->max_translation : 1.0;
 }
 
 // Applies the trust-radius factor to the direction of the given kind only
