@@ -1,27 +1,35 @@
-// Host-side FP64 vector and frame helpers used while building the rod model.
+// FP64 vector and frame helpers of the rod model build. They also compile
+// for the device (the root targets of uploaded frames are computed there
+// with the same operations).
 #pragma once
 
 #include <algorithm>
 #include <cmath>
 
+#if defined(__HIP__)
+#define KHS_HD __host__ __device__
+#else
+#define KHS_HD
+#endif
+
 namespace khs {
 
 struct Vec3 {
     double x = 0.0, y = 0.0, z = 0.0;
-    double operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+    KHS_HD double operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
 };
 
-inline Vec3 operator+(Vec3 a, Vec3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
-inline Vec3 operator-(Vec3 a, Vec3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-inline Vec3 operator*(double s, Vec3 a) { return {s * a.x, s * a.y, s * a.z}; }
-inline Vec3 operator*(Vec3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
-inline Vec3 over(Vec3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
-inline double dot(Vec3 a, Vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-inline Vec3 cross(Vec3 a, Vec3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
-inline double length2(Vec3 a) { return dot(a, a); }
-inline double length(Vec3 a) { return std::sqrt(dot(a, a)); }
-inline bool is_finite(Vec3 a) { return std::isfinite(a.x) && std::isfinite(a.y) && std::isfinite(a.z); }
-inline Vec3 unit(Vec3 a)
+KHS_HD inline Vec3 operator+(Vec3 a, Vec3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+KHS_HD inline Vec3 operator-(Vec3 a, Vec3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+KHS_HD inline Vec3 operator*(double s, Vec3 a) { return {s * a.x, s * a.y, s * a.z}; }
+KHS_HD inline Vec3 operator*(Vec3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+KHS_HD inline Vec3 over(Vec3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+KHS_HD inline double dot(Vec3 a, Vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+KHS_HD inline Vec3 cross(Vec3 a, Vec3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+KHS_HD inline double length2(Vec3 a) { return dot(a, a); }
+KHS_HD inline double length(Vec3 a) { return std::sqrt(dot(a, a)); }
+KHS_HD inline bool is_finite(Vec3 a) { return std::isfinite(a.x) && std::isfinite(a.y) && std::isfinite(a.z); }
+KHS_HD inline Vec3 unit(Vec3 a)
 {
     const double n2 = length2(a);
     return n2 > 0.0 ? over(a, std::sqrt(n2)) : a;
@@ -30,12 +38,12 @@ inline Vec3 unit(Vec3 a)
 // Row-major 3x3; the columns of a rod frame are its directors d1, d2, d3.
 struct Mat3 {
     double m[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-    double operator()(int r, int c) const { return m[3 * r + c]; }
-    double &operator()(int r, int c) { return m[3 * r + c]; }
-    Vec3 column(int c) const { return {m[c], m[3 + c], m[6 + c]}; }
+    KHS_HD double operator()(int r, int c) const { return m[3 * r + c]; }
+    KHS_HD double &operator()(int r, int c) { return m[3 * r + c]; }
+    KHS_HD Vec3 column(int c) const { return {m[c], m[3 + c], m[6 + c]}; }
 };
 
-inline Mat3 with_columns(Vec3 a, Vec3 b, Vec3 c)
+KHS_HD inline Mat3 with_columns(Vec3 a, Vec3 b, Vec3 c)
 {
     Mat3 r;
     const Vec3 cols[3] = {a, b, c};
@@ -47,7 +55,7 @@ inline Mat3 with_columns(Vec3 a, Vec3 b, Vec3 c)
     return r;
 }
 
-inline Mat3 transposed(const Mat3 &a)
+KHS_HD inline Mat3 transposed(const Mat3 &a)
 {
     Mat3 r;
     for (int i = 0; i < 3; ++i)
@@ -55,7 +63,7 @@ inline Mat3 transposed(const Mat3 &a)
     return r;
 }
 
-inline Mat3 product(const Mat3 &a, const Mat3 &b)
+KHS_HD inline Mat3 product(const Mat3 &a, const Mat3 &b)
 {
     Mat3 r;
     for (int i = 0; i < 3; ++i)
@@ -67,7 +75,7 @@ inline Mat3 product(const Mat3 &a, const Mat3 &b)
     return r;
 }
 
-inline Vec3 apply(const Mat3 &a, Vec3 v)
+KHS_HD inline Vec3 apply(const Mat3 &a, Vec3 v)
 {
     return {a(0, 0) * v.x + a(0, 1) * v.y + a(0, 2) * v.z, a(1, 0) * v.x + a(1, 1) * v.y + a(1, 2) * v.z,
             a(2, 0) * v.x + a(2, 1) * v.y + a(2, 2) * v.z};
@@ -75,7 +83,7 @@ inline Vec3 apply(const Mat3 &a, Vec3 v)
 
 // 2 tan(theta/2) axis of a rotation from its skew part and trace
 // (reference src/math.hpp:14-24).
-inline Vec3 cayley_vector(const Mat3 &q)
+KHS_HD inline Vec3 cayley_vector(const Mat3 &q)
 {
     const Vec3 v{(q(2, 1) - q(1, 2)) * 0.5, (q(0, 2) - q(2, 0)) * 0.5, (q(1, 0) - q(0, 1)) * 0.5};
     const double c = (q(0, 0) + q(1, 1) + q(2, 2) - 1.0) * 0.5;
@@ -84,7 +92,7 @@ inline Vec3 cayley_vector(const Mat3 &q)
 
 // Orthonormal frame with d3 along the tangent and d1 from the hint
 // (reference src/math.hpp:26-42).
-inline Mat3 tangent_frame(Vec3 tangent, Vec3 hint)
+KHS_HD inline Mat3 tangent_frame(Vec3 tangent, Vec3 hint)
 {
     const Vec3 d3 = unit(tangent);
     Vec3 d1 = hint - d3 * dot(hint, d3);
@@ -99,7 +107,7 @@ inline Mat3 tangent_frame(Vec3 tangent, Vec3 hint)
 }
 
 // A unit vector orthogonal to s, as Eigen 3.4's unitOrthogonal picks it.
-inline Vec3 orthogonal_unit(Vec3 s)
+KHS_HD inline Vec3 orthogonal_unit(Vec3 s)
 {
     const double eps = 1.0e-12;
     if (!(std::fabs(s.x) <= std::fabs(s.z) * eps) || !(std::fabs(s.y) <= std::fabs(s.z) * eps)) {
@@ -112,7 +120,7 @@ inline Vec3 orthogonal_unit(Vec3 s)
 
 // Minimal rotation of a director from one tangent to the next
 // (reference src/math.hpp:44-57).
-inline Vec3 transport(Vec3 director, Vec3 from, Vec3 to)
+KHS_HD inline Vec3 transport(Vec3 director, Vec3 from, Vec3 to)
 {
     const Vec3 a = unit(from), b = unit(to);
     const Vec3 axis = cross(a, b);
@@ -130,7 +138,7 @@ inline Vec3 transport(Vec3 director, Vec3 from, Vec3 to)
 // Axis-angle vector of a rotation through its quaternion (largest-pivot
 // extraction, angle 2 atan2(|v|, |w|)), as Eigen 3.4's AngleAxisd computes
 // it (reference src/solver.cpp:19-26).
-inline Vec3 axis_angle(const Mat3 &r)
+KHS_HD inline Vec3 axis_angle(const Mat3 &r)
 {
     double q[4];  // x y z w
     const double tr = r(0, 0) + r(1, 1) + r(2, 2);

@@ -3,6 +3,7 @@
 // (reference src/cuda_backend.cu:341-723, src/solver.cpp:449-893).
 #include "solver.hpp"
 
+#include "root_kernels.hpp"
 #include "step_kernels.hpp"
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -106,6 +107,9 @@ struct DeviceState {
     uint32_t root_element_count = 0;
     Buffer<unsigned long long> ccd_bound;  // two alternating launch-wide step bounds of the list CCD
     uint32_t ccd_slot = 0;
+    Buffer<uint32_t> bind_left, bind_right;  // per node: the original points its position lerps
+    Buffer<double> bind_t, root_points;      // lerp weight; staging for one frame of original points
+    Buffer<unsigned> root_flag;              // set when a frame's targets have a degenerate tangent
     Buffer<unsigned> solve_failed;  // two alternating failure flags of the block solve
     uint32_t failed_slot = 0;
     dev::TrialStep trial_step{};     // how the next trial's first kernel sets alpha
@@ -558,6 +562,14 @@ KhsResult Solver::upload_model()
         const unsigned zeros[2] = {0u, 0u};
         d.solve_failed.allocate(2);
         d.solve_failed.upload(zeros, 2);
+        d.root_flag.allocate(1);
+        d.bind_left.allocate(N);
+        d.bind_left.upload(m.bind_left.data(), N);
+        d.bind_right.allocate(N);
+        d.bind_right.upload(m.bind_right.data(), N);
+        d.bind_t.allocate(N);
+        d.bind_t.upload(m.bind_t.data(), N);
+        d.root_points.allocate(3 * size_t(d.P));
         d.root_elements.allocate(std::max<size_t>(1, roots.size()));
         if (!roots.empty()) d.root_elements.upload(roots.data(), roots.size());
     }
@@ -880,15 +892,31 @@ KhsResult Solver::update_roots(const KhsVec3 *points, uint32_t point_count)
             error_ = "Root target is not finite.";
             return KHS_ERROR_INVALID_HAIR;
         }
-    std::vector<Vec3> tp, tr;
-    if (!root_targets(model_, p, tp, tr, std::max(1u, std::thread::hardware_concurrency()))) {
+    if (!device_root_targets(p, dev_->root_pend.get(), dev_->rot_pend.get())) {
         error_ = "Cannot build root orientation from the targets.";
         return KHS_ERROR_INVALID_HAIR;
     }
-    const std::vector<double> fp = flatten(tp), fr = flatten(tr);
-    dev_->root_pend.upload(fp.data(), fp.size());
-    dev_->rot_pend.upload(fr.data(), fr.size());
     return KHS_OK;
+}
+
+// Root targets of one frame of original points on the device, into pos and
+// rot (3N each, device); false when a tangent is degenerate.
+bool Solver::device_root_targets(const std::vector<Vec3> &points, double *pos, double *rot)
+{
+    DeviceState &d = *dev_;
+    const std::vector<double> flat = flatten(points);
+    d.root_points.upload(flat.data(), flat.size());
+    hipLaunchKernelGGL(dev::k_fill_u32, dim3(1), dim3(64), 0, d.stream, d.root_flag.get(), size_t(1), 0u);
+    hipLaunchKernelGGL(dev::k_root_positions, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.N,
+                       (const uint32_t *)d.bind_left.get(), (const uint32_t *)d.bind_right.get(),
+                       (const double *)d.bind_t.get(), (const double *)d.root_points.get(), pos);
+    hipLaunchKernelGGL(dev::k_root_rotations, dim3(grid_for(d.S, kBlock)), dim3(kBlock), 0, d.stream, d.rods(),
+                       (const double *)pos, rot, d.root_flag.get());
+    HIP_TRY(hipGetLastError());
+    unsigned bad = 0;
+    HIP_TRY(hipMemcpyAsync(&bad, d.root_flag.get(), sizeof(bad), hipMemcpyDeviceToHost, d.stream));
+    HIP_TRY(hipStreamSynchronize(d.stream));
+    return bad == 0;
 }
 
 // -------------------------------------------------------------- animation
@@ -918,13 +946,10 @@ KhsResult Solver::set_root_animation_frame(uint32_t frame, const KhsVec3 *points
     std::vector<Vec3> p = to_vec3(points, point_count);
     for (const Vec3 &x : p)
         if (!is_finite(x)) return KHS_ERROR_INVALID_HAIR;
-    std::vector<Vec3> tp, tr;
-    if (!root_targets(model_, p, tp, tr, std::max(1u, std::thread::hardware_concurrency()))) return KHS_ERROR_INVALID_HAIR;
     if (frame >= animation_frames_) return fail(KHS_ERROR_INVALID_ARGUMENT, "Invalid root animation frame.");
-    const std::vector<double> fp = flatten(tp), fr = flatten(tr);
     const size_t n3 = 3 * size_t(dev_->N);
-    dev_->root_anim.upload(fp.data(), n3, size_t(frame) * n3);
-    dev_->rot_anim.upload(fr.data(), n3, size_t(frame) * n3);
+    if (!device_root_targets(p, dev_->root_anim.get() + size_t(frame) * n3, dev_->rot_anim.get() + size_t(frame) * n3))
+        return KHS_ERROR_INVALID_HAIR;
     root_frame_set_[frame] = true;
     return KHS_OK;
 }

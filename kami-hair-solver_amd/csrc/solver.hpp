@@ -81,6 +81,7 @@ private:
     void set_failure_context(uint32_t kind, uint32_t frame, uint32_t substep, uint32_t attempted, uint32_t maximum,
                              uint32_t attempts);
     KhsResult upload_model();
+    bool device_root_targets(const std::vector<Vec3> &points, double *pos, double *rot);
     KhsResult initial_feasibility();
     void release_device();
     void rebuild_bvh(bool swept);

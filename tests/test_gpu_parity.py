@@ -317,3 +317,42 @@ def test_concurrent_solvers_match_alone(gpu_lib):
         t.join()
     for r in results:
         assert r is not None and np.array_equal(r, alone)
+
+
+def _axis_rotation(axis, angle):
+    a = np.asarray(axis, float) / np.linalg.norm(axis)
+    K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    return np.eye(3) + np.sin(angle) * K + (1 - np.cos(angle)) * K @ K
+
+
+@pytest.mark.parametrize("degrees", [35.0, 130.0])
+def test_device_root_targets_match_oracle(oracle_lib, gpu_lib, degrees):
+    """Root targets of uploaded frames are computed on the device (binding
+    lerp, transported frames, rotation vectors of the fixed nodes). After one
+    frame the fixed nodes sit exactly on the last frame's targets, so their
+    state compares the device targets with the oracle's host ones (whose
+    rigid-rotation closed forms test_oracle_pins checks); two fixed nodes per
+    strand exercise the transport between them, and 130 degrees about a
+    tilted axis the half-turn branch of the rotation vector."""
+    print(f"[gpu] device root targets, rotation {degrees} deg", flush=True)
+    scene = grooms.small_sphere_scene()
+    phi = np.radians(degrees)
+    R = _axis_rotation([0.0, 0.0, 1.0], phi) if degrees < 90 else _axis_rotation([0.3, -0.2, 1.0], phi)
+    states = []
+    for lib in (oracle_lib, gpu_lib):
+        s = grooms.build_solver(lib, scene, collider=False, fixed_root_nodes=2)
+        s.check(s.allocate_animation(2))
+        s.check(s.set_root_frame(0, scene.points))
+        s.check(s.set_root_frame(1, scene.points @ R.T))
+        s.check(s.finalize_animation())
+        s.check(s.step_frame(1, scene.frame_dt))
+        q, _ = s.get_state()
+        _, _, _, flags = s.node_model()
+        states.append((q.reshape(-1, 6), flags))
+    (qo, fo), (qg, fg) = states
+    assert np.array_equal(fo, fg)
+    fixed = (fo & 1) == 1
+    assert fixed.sum() == 2 * 64
+    err = np.max(np.abs(qg[fixed] - qo[fixed]))
+    print(f"fixed-node target difference {err:.3e}")
+    assert err <= 1e-13
