@@ -270,14 +270,18 @@ struct DeviceState {
     // One wait for everything the host reads after a batch of launches.
     struct HostView {
         Partial eval, steep, ccd;
-        DirectionChoice choice;
+        union {
+            DirectionChoice choice;
+            Partial choice_space;
+        };
+        Partial grad, dir;
     };
     // The reductions and choice kernels write host_view as they run (a
     // mirror of `reduced` in coherent pinned memory), so a read-back is a
     // wait and no copy.
     HostView read_back()
     {
-        static_assert(sizeof(HostView) == 3 * sizeof(Partial) + sizeof(DirectionChoice), "packed readback");
+        static_assert(sizeof(HostView) == 6 * sizeof(Partial), "one Partial per slot of reduced");
         wait();
         return *host_view;
     }
@@ -291,7 +295,11 @@ struct DeviceState {
         HIP_TRY(hipStreamSynchronize(stream));
         settle();
     }
-    static constexpr int kEvalSlot = 0, kSteepSlot = 1, kCcdSlot = 2;
+    // slots of reduced (3 holds the direction choice)
+    static constexpr int kEvalSlot = 0, kSteepSlot = 1, kCcdSlot = 2, kGradSlot = 4, kDirSlot = 5;
+    Buffer<unsigned> stop;  // set by k_check_converged: work enqueued ahead skips itself
+    bool skipping = false;  // inside the Newton loop, where that skip applies
+    const unsigned *skip() const { return skipping ? stop.get() : nullptr; }
     HostView *host_view = nullptr;  // pinned, coherent
     void *host_view_device = nullptr;  // its device address
 
@@ -608,7 +616,12 @@ KhsResult Solver::upload_model()
     d.out3.allocate(3 * std::max<size_t>(N, d.P));
     const uint32_t max_blocks = std::max(d.tiled_blocks(), d.node_blocks()) + 1;
     d.partials.allocate(max_blocks + kPairBlocks + 16);
-    d.reduced.allocate(4);  // three reductions, then the direction choice (at most one Partial)
+    d.reduced.allocate(6);  // reductions, with the direction choice in slot 3 (at most one Partial)
+    {
+        const unsigned zero = 0;
+        d.stop.allocate(1);
+        d.stop.upload(&zero, 1);
+    }
     static_assert(sizeof(DirectionChoice) <= sizeof(Partial), "choice slot");
     d.flag.allocate(1);
     d.key64.allocate(1);
@@ -780,12 +793,12 @@ void Solver::closest_pass(bool line_search, Partial *partials, bool with_roots)
         hipLaunchKernelGGL(dev::k_list_closest<dev::kAtTrial>, dim3(kPairBlocks), dim3(kBlock), 0, d.stream,
                            d.flags.get(), d.collider(), cc, contact_pad, d.q.get(), d.q_base.get(), d.dir.get(),
                            d.alpha.get(), d.best_tri.get(), partials, with_roots ? d.root_elements.get() : nullptr,
-                           with_roots ? d.root_element_count : 0u);
+                           with_roots ? d.root_element_count : 0u, d.skip());
     else
         hipLaunchKernelGGL(dev::k_list_closest<dev::kAtState>, dim3(kPairBlocks), dim3(kBlock), 0, d.stream,
                            d.flags.get(), d.collider(), cc, contact_pad, d.q.get(), d.q_base.get(), d.dir.get(),
                            d.alpha.get(), d.best_tri.get(), partials, with_roots ? d.root_elements.get() : nullptr,
-                           with_roots ? d.root_element_count : 0u);
+                           with_roots ? d.root_element_count : 0u, (const unsigned *)nullptr);
     HIP_TRY(hipGetLastError());
 }
 
@@ -984,7 +997,7 @@ KhsResult Solver::finalize_animation()
 // ---------------------------------------------------------------- solving
 // Enqueues one evaluation of the incremental potential (the line-search
 // variant at alpha d + q_base, alpha read from the device) and its
-// reduction into reduced[kEvalSlot]; returns the partial count folded.
+// reduction into reduced[kGradSlot] (with the gradient) or reduced[kEvalSlot].
 void Solver::evaluate_async(bool gradient, double dt, bool line_search)
 {
     DeviceState &d = *dev_;
@@ -1027,19 +1040,19 @@ void Solver::evaluate_async(bool gradient, double dt, bool line_search)
     if (gradient)
         hipLaunchKernelGGL((dev::k_evaluate<true, false>), dim3(blocks), dim3(kBlock), 0, d.stream, rv, cv, cc, best, p,
                            d.q.get(), d.q_old.get(), d.vel.get(), d.q_base.get(), d.dir.get(), d.alpha.get(),
-                           d.grad.get(), d.Dblk.get(), d.Ublk.get(), d.partials.get());
+                           d.grad.get(), d.Dblk.get(), d.Ublk.get(), d.partials.get(), (const unsigned *)nullptr);
     else if (line_search)
         hipLaunchKernelGGL((dev::k_evaluate<false, true>), dim3(blocks), dim3(kBlock), 0, d.stream, rv, cv, cc, best, p,
                            d.q.get(), d.q_old.get(), d.vel.get(), d.q_base.get(), d.dir.get(), d.alpha.get(),
-                           d.grad.get(), d.Dblk.get(), d.Ublk.get(), d.partials.get());
+                           d.grad.get(), d.Dblk.get(), d.Ublk.get(), d.partials.get(), d.skip());
     else
         hipLaunchKernelGGL((dev::k_evaluate<false, false>), dim3(blocks), dim3(kBlock), 0, d.stream, rv, cv, cc, best, p,
                            d.q.get(), d.q_old.get(), d.vel.get(), d.q_base.get(), d.dir.get(), d.alpha.get(),
-                           d.grad.get(), d.Dblk.get(), d.Ublk.get(), d.partials.get());
+                           d.grad.get(), d.Dblk.get(), d.Ublk.get(), d.partials.get(), (const unsigned *)nullptr);
     HIP_TRY(hipGetLastError());
     if (gradient) d.toc(t0, d.totals.assembly_ms, d.totals.assembly_launches);
     else d.toc(t0, d.totals.energy_ms, d.totals.energy_launches);
-    d.reduce_async(blocks + (list_pass ? kPairBlocks : 0), DeviceState::kEvalSlot);
+    d.reduce_async(blocks + (list_pass ? kPairBlocks : 0), gradient ? DeviceState::kGradSlot : DeviceState::kEvalSlot);
 }
 
 Solver::Reduced Solver::unpack(const Partial &r2)
@@ -1061,7 +1074,8 @@ Solver::Reduced Solver::unpack(const Partial &r2)
 Solver::Reduced Solver::evaluate(bool gradient, double dt, bool line_search)
 {
     evaluate_async(gradient, dt, line_search);
-    return unpack(dev_->read_back().eval);
+    const DeviceState::HostView v = dev_->read_back();
+    return unpack(gradient ? v.grad : v.eval);
 }
 
 // Steepest descent and its trust-radius factor when the choice is not the
@@ -1116,16 +1130,16 @@ void Solver::make_direction_async()
     hipEvent_t t0 = d.tic();
     hipLaunchKernelGGL(dev::k_block_solve, dim3(grid_for(d.S, dev::kSolveGroups)), dim3(kWave), kSolveLdsReserve,
                        d.stream, rv, d.solve_order.get(), d.Dblk.get(), d.Ublk.get(), d.grad.get(), d.Cp.get(),
-                       d.dp.get(), d.dir.get(), failed, d.solve_failed.get() + (d.failed_slot ^ 1u));
+                       d.dp.get(), d.dir.get(), failed, d.solve_failed.get() + (d.failed_slot ^ 1u), d.skip());
     d.failed_slot ^= 1u;
     d.toc(t0, d.totals.solve_ms, d.totals.solve_launches);
     hipLaunchKernelGGL(dev::k_direction_stats, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, d.grad.get(),
                        d.dir.get(), (const unsigned *)failed, (const DirectionChoice *)nullptr, d.partials.get());
-    d.reduce_async(d.node_blocks(), DeviceState::kEvalSlot);
+    d.reduce_async(d.node_blocks(), DeviceState::kDirSlot);
     // the choice, and the line search's base state q_base = q
     const size_t dofs = 6 * size_t(d.N);
     hipLaunchKernelGGL(dev::k_apply_direction_choice, dim3(grid_for(dofs, kBlock)), dim3(kBlock), 0, d.stream,
-                       (const Partial *)(d.reduced.get() + DeviceState::kEvalSlot), trust_radius_, d.choice(),
+                       (const Partial *)(d.reduced.get() + DeviceState::kDirSlot), trust_radius_, d.choice(),
                        d.mirror_choice(), d.dir.get(), dofs, (const double *)d.q.get(), d.q_base.get());
     HIP_TRY(hipGetLastError());
 }
@@ -1145,15 +1159,56 @@ KhsResult Solver::optimize_substep(double dt)
         hipLaunchKernelGGL(dev::k_preconditioner, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, 1.0 / dt,
                            material_.mass_damping, tk, ra, rb, d.precond.get());
     }
-    Reduced current = evaluate(true, dt, false);
+    const double dofs = double(6 * size_t(d.N));
+    const double abs_tol = convergence_mode_ ? desc_.absolute_tolerance
+                                             : std::fmax(desc_.absolute_tolerance, 1.0e-5 * std::sqrt(dofs));
+    const double rel_tol = convergence_mode_ ? desc_.relative_tolerance : std::fmax(desc_.relative_tolerance, 1.0e-4);
+    // Work enqueued ahead for the next Newton iteration skips itself on the
+    // device when the gradient before it has converged.
+    d.skipping = true;
+    struct SkipOff {
+        DeviceState &d;
+        ~SkipOff() { d.skipping = false; }
+    } skip_off{d};
+    evaluate_async(true, dt, false);
+    hipLaunchKernelGGL(dev::k_check_converged, dim3(1), dim3(1), 0, d.stream,
+                       (const Partial *)(d.reduced.get() + DeviceState::kGradSlot), abs_tol, rel_tol, -1.0,
+                       d.stop.get());
+    Reduced current = unpack(d.read_back().grad);
     if (!current.feasible) return fail(KHS_ERROR_NOT_CONVERGED, "State is outside the barrier feasible region.");
     const double initial = std::sqrt(current.grad_norm2);
     step_stats_.initial_residual_norm = std::fmax(step_stats_.initial_residual_norm, initial);
-    const double dofs = double(6 * size_t(d.N));
     StepParams cp{};
     cp.radius_offset = material_.radius + material_.collider_offset;
     cp.sweep_target = material_.radius + material_.collider_offset + desc_.minimum_gap;
     cp.has_collider = d.T > 0;
+    // The step bound along the current direction and the first trial at it.
+    auto bound_and_trial = [&]() {
+        phase_.store(KHS_PHASE_CCD);
+        hipEvent_t t0 = d.tic();
+        const bool list_pass = d.cache_valid;
+        if (list_pass) {
+            refresh_contact_cache(dev::kSwept, false);
+            hipLaunchKernelGGL(dev::k_list_ccd, dim3(kPairBlocks), dim3(kBlock), 0, d.stream, d.flags.get(),
+                               d.collider(), d.cache(), cp, d.q.get(), d.dir.get(),
+                               d.partials.get() + d.node_blocks(), d.ccd_bound.get(), int(d.ccd_slot), d.skip());
+            d.ccd_slot ^= 1u;
+        }
+        hipLaunchKernelGGL(dev::k_ccd_slope, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, d.collider(),
+                           d.cache(), list_pass ? 1 : 0, cp, d.q.get(), d.grad.get(), d.dir.get(), d.partials.get());
+        HIP_TRY(hipGetLastError());
+        d.toc(t0, d.totals.collision_ms, d.totals.collision_launches);
+        d.reduce_async(d.node_blocks() + (list_pass ? kPairBlocks : 0), DeviceState::kCcdSlot);
+        set_trial_step(dev::kAlphaFromCcd, 0.0);
+        // The first trial runs at the device's step before the host has
+        // seen the bound; one read-back returns direction choice, bound and
+        // trial. A trial the host would not have run is discarded, and every
+        // path that does not accept restores q from q_base.
+        phase_.store(KHS_PHASE_LINE_SEARCH);
+        evaluate_async(false, dt, true);
+    };
+    bool head_ready = false;  // direction, bound and first trial already enqueued
+    DeviceState::HostView ahead{};
     for (uint32_t it = 0; it <= desc_.newton_iterations; ++it) {
         iteration_.store(it);
         if (cancel_.load()) return fail(KHS_ERROR_INVALID_STATE, "Solve cancelled.");
@@ -1168,10 +1223,9 @@ KhsResult Solver::optimize_substep(double dt)
         step_stats_.elastic_energy = current.elastic;
         step_stats_.contact_energy = current.contact;
         step_stats_.friction_energy = 0.0;
-        const double abs_tol = convergence_mode_ ? desc_.absolute_tolerance
-                                                 : std::fmax(desc_.absolute_tolerance, 1.0e-5 * std::sqrt(dofs));
-        const double rel_tol = convergence_mode_ ? desc_.relative_tolerance : std::fmax(desc_.relative_tolerance, 1.0e-4);
         if (residual <= abs_tol || relative <= rel_tol) {
+            // work enqueued ahead skipped itself (the device made the same
+            // test), so q is still this iterate
             hipLaunchKernelGGL(dev::k_update_velocity, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv,
                                d.q.get(), d.q_old.get(), d.vel.get(), 1.0 / dt);
             HIP_TRY(hipGetLastError());
@@ -1179,35 +1233,24 @@ KhsResult Solver::optimize_substep(double dt)
         }
         if (it == desc_.newton_iterations) break;
         phase_.store(KHS_PHASE_LINEAR_SOLVE);
-        make_direction_async();  // also q_base = q
+        DeviceState::HostView v{};
+        bool have_view = false;
+        if (head_ready) {
+            v = ahead;
+            have_view = true;
+            head_ready = false;
+        } else {
+            make_direction_async();  // also q_base = q
+        }
         ++step_stats_.linear_solves;
         bool accepted = false, block_direction = true;
         double alpha = 1.0;
         for (int attempt = 0; attempt < 2;) {
-            phase_.store(KHS_PHASE_CCD);
-            hipEvent_t t0 = d.tic();
-            const bool list_pass = d.cache_valid;
-            if (list_pass) {
-                refresh_contact_cache(dev::kSwept, false);
-                hipLaunchKernelGGL(dev::k_list_ccd, dim3(kPairBlocks), dim3(kBlock), 0, d.stream, d.flags.get(),
-                                   d.collider(), d.cache(), cp, d.q.get(), d.dir.get(),
-                                   d.partials.get() + d.node_blocks(), d.ccd_bound.get(), int(d.ccd_slot));
-                d.ccd_slot ^= 1u;
+            if (!have_view) {
+                bound_and_trial();
+                v = d.read_back();
             }
-            hipLaunchKernelGGL(dev::k_ccd_slope, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, d.collider(),
-                               d.cache(), list_pass ? 1 : 0, cp, d.q.get(), d.grad.get(), d.dir.get(),
-                               d.partials.get());
-            HIP_TRY(hipGetLastError());
-            d.toc(t0, d.totals.collision_ms, d.totals.collision_launches);
-            d.reduce_async(d.node_blocks() + (list_pass ? kPairBlocks : 0), DeviceState::kCcdSlot);
-            set_trial_step(dev::kAlphaFromCcd, 0.0);
-            // The first trial runs at the device's step before the host has
-            // seen the bound; one read-back returns direction choice, bound
-            // and trial. A trial the host would not have run is discarded,
-            // and every path that does not accept restores q from q_base.
-            phase_.store(KHS_PHASE_LINE_SEARCH);
-            evaluate_async(false, dt, true);
-            const DeviceState::HostView v = d.read_back();
+            have_view = false;
             if (attempt == 0 && block_direction && !v.choice.used) {
                 // The block direction failed or does not descend: the step
                 // starts from steepest descent instead, with no retry.
@@ -1254,13 +1297,29 @@ KhsResult Solver::optimize_substep(double dt)
             HIP_TRY(hipMemcpyAsync(d.q.get(), d.q_base.get(), d.q.bytes(), hipMemcpyDeviceToDevice, d.stream));
             return fail(KHS_ERROR_NOT_CONVERGED, "Gauss-Newton line search failed.");
         }
-        current = evaluate(true, dt, false);
+        // The gradient at the accepted iterate; unless the budget ends here,
+        // the next iteration's direction, bound and first trial follow before
+        // the one wait (they skip themselves if this gradient converged).
+        evaluate_async(true, dt, false);
+        hipLaunchKernelGGL(dev::k_check_converged, dim3(1), dim3(1), 0, d.stream,
+                           (const Partial *)(d.reduced.get() + DeviceState::kGradSlot), abs_tol, rel_tol, initial,
+                           d.stop.get());
+        if (it + 1 < desc_.newton_iterations) {
+            make_direction_async();
+            bound_and_trial();
+            head_ready = true;
+        }
+        ahead = d.read_back();
+        current = unpack(ahead.grad);
         const double dnorm = std::sqrt(current.dir_norm2);
         step_stats_.increment_norm = alpha * dnorm;
         step_stats_.accepted_step_length = alpha;
         ++step_stats_.newton_iterations;
         phase_.store(KHS_PHASE_ASSEMBLING);
         if (step_stats_.increment_norm / std::sqrt(dofs) <= 2.0 * desc_.increment_tolerance) {
+            // a trial enqueued ahead may have moved q: back to the iterate
+            if (head_ready)
+                HIP_TRY(hipMemcpyAsync(d.q.get(), d.q_base.get(), d.q.bytes(), hipMemcpyDeviceToDevice, d.stream));
             hipLaunchKernelGGL(dev::k_update_velocity, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv,
                                d.q.get(), d.q_old.get(), d.vel.get(), 1.0 / dt);
             HIP_TRY(hipGetLastError());
@@ -1722,7 +1781,8 @@ KhsResult Solver::test_evaluate(double dt, double *gradient, double *direction, 
         hipLaunchKernelGGL(dev::k_fill_u32, dim3(1), dim3(64), 0, d.stream, d.flag.get(), size_t(1), 0u);
         hipLaunchKernelGGL(dev::k_block_solve, dim3(grid_for(d.S, dev::kSolveGroups)), dim3(kWave),
                            kSolveLdsReserve, d.stream, rv, d.solve_order.get(), d.Dblk.get(), d.Ublk.get(),
-                           d.grad.get(), d.Cp.get(), d.dp.get(), d.dir.get(), d.flag.get(), (unsigned *)nullptr);
+                           d.grad.get(), d.Cp.get(), d.dp.get(), d.dir.get(), d.flag.get(), (unsigned *)nullptr,
+                           (const unsigned *)nullptr);
         HIP_TRY(hipGetLastError());
         unsigned failed = 0;
         HIP_TRY(hipMemcpyAsync(&failed, d.flag.get(), sizeof(unsigned), hipMemcpyDeviceToHost, d.stream));

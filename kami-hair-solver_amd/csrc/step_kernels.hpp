@@ -451,8 +451,10 @@ __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, 
                                                          double pad, const double *q, const double *q_base,
                                                          const double *dir, const double *alpha_at,
                                                          uint32_t *best_tri, Partial *partials,
-                                                         const uint32_t *extra, uint32_t extra_count)
+                                                         const uint32_t *extra, uint32_t extra_count,
+                                                         const unsigned *stop)
 {
+    if (stop && *stop) return;
     constexpr int G = kClosestLanes;
     __shared__ uint32_t packed[kBlock / G][kCacheCapacity];
     const double alpha = MODE == kAtTrial ? *alpha_at : 0.0;
@@ -553,7 +555,8 @@ __device__ inline double advancement_limit(V3 p0, V3 p1, V3 d0, V3 d1, double sp
 constexpr int kCcdBatch = kBlock / kClosestLanes;
 __global__ __launch_bounds__(kBlock) void k_list_ccd(const uint32_t *flags, ColliderView col, ContactCache cc,
                                                      StepParams p, const double *q, const double *dir,
-                                                     Partial *partials, unsigned long long *bound, int slot)
+                                                     Partial *partials, unsigned long long *bound, int slot,
+                                                     const unsigned *stop)
 {
     constexpr int G = kClosestLanes;
     __shared__ uint32_t pair_tri[kCcdBatch * kCacheCapacity];
@@ -565,6 +568,7 @@ __global__ __launch_bounds__(kBlock) void k_list_ccd(const uint32_t *flags, Coll
     const unsigned long long below = (1ull << lane) - 1;
     const uint32_t listed = *cc.list_count;
     if (blockIdx.x == 0 && threadIdx.x == 0) bound[slot ^ 1] = __double_as_longlong(1.0);
+    if (stop && *stop) return;
     double limit = 1.0;
     for (uint32_t first = blockIdx.x * kCcdBatch; first < listed; first += gridDim.x * kCcdBatch) {
         if (threadIdx.x == 0) pair_count = 0;
@@ -671,8 +675,9 @@ template <bool GRAD, bool LS>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GRAD ? 2 : 1))) void k_evaluate(
     RodView m, ColliderView col, ContactCache cc, const uint32_t *best_tri, StepParams p, double *q,
     const double *q_old, const double *vel, const double *q_base, const double *dir, const double *alpha_at,
-    double *grad, double *Dblk, double *Ublk, Partial *partials)
+    double *grad, double *Dblk, double *Ublk, Partial *partials, const unsigned *stop)
 {
+    if (stop && *stop) return;
     const double alpha = LS ? *alpha_at : 0.0;
     const int lane = threadIdx.x & (kWave - 1);
     const long tile = long(blockIdx.x) * kTilesPerBlock + threadIdx.x / kWave;
@@ -1063,10 +1068,11 @@ struct ForwardSlot {
 __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t *order, const double *Dblk,
                                                        const double *Ublk, const double *grad, double *Cp,
                                                        double *dp, double *dir, unsigned *failed,
-                                                       unsigned *next_failed)
+                                                       unsigned *next_failed, const unsigned *stop)
 {
     // the next solve's flag starts at zero (the two alternate)
     if (blockIdx.x == 0 && threadIdx.x == 0 && next_failed) *next_failed = 0;
+    if (stop && *stop) return;
     __shared__ SolveShared sh;
     const int g = threadIdx.x / kSolveLanes, j = threadIdx.x % kSolveLanes;
     const bool acol = j < 6, icol = j >= 6 && j < 12, vlane = j == 12;
@@ -1387,6 +1393,20 @@ __global__ void k_choose_alpha(const Partial *ccd, int has_collider, double *alp
 {
     if (threadIdx.x || blockIdx.x) return;
     *alpha = step_from_ccd(ccd, has_collider);
+}
+
+// The Newton convergence test on the device, the same comparisons as the
+// host's (cuda_backend.cu:662-672): stop = |g| <= abs_tol or |g| / initial
+// <= rel_tol, with initial < 0 meaning this gradient is the substep's first.
+// Work enqueued ahead for the next Newton iteration skips itself on stop.
+__global__ void k_check_converged(const Partial *grad, double abs_tol, double rel_tol, double initial,
+                                  unsigned *stop)
+{
+    if (threadIdx.x || blockIdx.x) return;
+    const double residual = sqrt(grad->sum[kGradNorm2]);
+    const double first = initial < 0.0 ? residual : initial;
+    const double relative = residual / fmax(first, 1e-30);
+    *stop = (residual <= abs_tol || relative <= rel_tol) ? 1u : 0u;
 }
 
 __global__ void k_set_double(double *x, double v)
