@@ -68,7 +68,7 @@ struct ContactCache {
     uint32_t *listed;       // per element: 1 once appended to the list below
     uint32_t *list;         // elements whose list became non-empty
     uint32_t *list_count;
-    uint32_t *pending;      // elements to gather in this pass
+    uint32_t *pending;      // elements to gather (the swept pass)
     uint32_t *pending_count;
     const double *tri_box;  // 6 per triangle: the box at the current collider position
 };

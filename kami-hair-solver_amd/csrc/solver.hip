@@ -132,6 +132,7 @@ struct DeviceState {
     // per-element contact candidate lists (valid for the current substep)
     Buffer<uint32_t> cache_count, cache_tris, cache_listed, cache_list, cache_list_count, cache_pending,
         cache_pending_count;
+    uint32_t pending_slot = 0;  // which pending counter the next swept pass uses
     Buffer<double> cache_box;
     double cache_skin = 0.0;
     bool cache_valid = false;
@@ -144,7 +145,6 @@ struct DeviceState {
     Buffer<double> out3;
     Buffer<uint32_t> pairs;
     Buffer<double> alpha;  // line-search step read by the trial kernels
-    uint32_t pending_slot = 0;  // which pending counter the next partial refresh uses
     DirectionChoice *choice() const { return reinterpret_cast<DirectionChoice *>(reduced.get() + 3); }
 
     // Event timing of kernel classes (enabled on request) and of substeps.
@@ -662,12 +662,12 @@ KhsResult Solver::upload_model()
         HIP_TRY(hipMemset(d.cache_count.get(), 0, d.cache_count.bytes()));
         d.cache_tris.allocate(size_t(kCacheCapacity) * d.N);
         d.cache_box.allocate(6 * size_t(d.N));
-        d.cache_listed.allocate(d.N);
-        d.cache_list.allocate(d.N);
-        d.cache_list_count.allocate(1);
         d.cache_pending.allocate(d.N);
         d.cache_pending_count.allocate(2);
         HIP_TRY(hipMemset(d.cache_pending_count.get(), 0, d.cache_pending_count.bytes()));
+        d.cache_listed.allocate(d.N);
+        d.cache_list.allocate(d.N);
+        d.cache_list_count.allocate(1);
         d.best_tri.allocate(d.N);
         rebuild_bvh(false);
     }
@@ -755,25 +755,29 @@ void Solver::refresh_contact_cache(int mode, bool full, bool swept_leaves, bool 
                            swept_leaves ? d.swept_collider() : d.collider(), cc, d.q.get(), d.q_base.get(),
                            d.dir.get(), d.alpha.get(), pad, 1);
     } else {
-        // the few elements that left their box gather with a wavefront each;
-        // the two pending counters alternate, each cleared by the gather
-        // before its next use
+        // the few elements that left their box gather, each with its
+        // wavefront
+        const ColliderView col = d.collider();
         if (mode == dev::kAtState)
-            hipLaunchKernelGGL(dev::k_contact_check<dev::kAtState>, grid, block, 0, d.stream, d.rods(), cc, d.q.get(),
-                               d.q_base.get(), d.dir.get(), d.alpha.get(), dev::TrialStep{}, pad,
+            hipLaunchKernelGGL(dev::k_contact_check<dev::kAtState>, grid, block, 0, d.stream, d.rods(), col, cc,
+                               d.q.get(), d.q_base.get(), d.dir.get(), d.alpha.get(), dev::TrialStep{}, pad,
                                with_roots ? 1 : 0);
         else if (mode == dev::kAtTrial) {
-            hipLaunchKernelGGL(dev::k_contact_check<dev::kAtTrial>, grid, block, 0, d.stream, d.rods(), cc, d.q.get(),
-                               d.q_base.get(), d.dir.get(), d.alpha.get(), d.trial_step, pad, with_roots ? 1 : 0);
+            hipLaunchKernelGGL(dev::k_contact_check<dev::kAtTrial>, grid, block, 0, d.stream, d.rods(), col, cc,
+                               d.q.get(), d.q_base.get(), d.dir.get(), d.alpha.get(), d.trial_step, pad,
+                               with_roots ? 1 : 0);
             d.trial_step.source = dev::kAlphaStored;
         }
-        else
-            hipLaunchKernelGGL(dev::k_contact_check<dev::kSwept>, grid, block, 0, d.stream, d.rods(), cc, d.q.get(),
-                               d.q_base.get(), d.dir.get(), d.alpha.get(), dev::TrialStep{}, pad,
+        else {
+            // the swept pass queues its elements; the two pending counters
+            // alternate, each cleared by the gather before its next use
+            hipLaunchKernelGGL(dev::k_contact_check<dev::kSwept>, grid, block, 0, d.stream, d.rods(), col, cc,
+                               d.q.get(), d.q_base.get(), d.dir.get(), d.alpha.get(), dev::TrialStep{}, pad,
                                with_roots ? 1 : 0);
-        hipLaunchKernelGGL(dev::k_contact_gather, dim3(kGatherBlocks), block, 0, d.stream, d.flags.get(), d.collider(), cc,
-                           d.cache_pending_count.get() + (d.pending_slot ^ 1u));
-        d.pending_slot ^= 1u;
+            hipLaunchKernelGGL(dev::k_contact_gather, dim3(kGatherBlocks), block, 0, d.stream, d.flags.get(), col, cc,
+                               d.cache_pending_count.get() + (d.pending_slot ^ 1u));
+            d.pending_slot ^= 1u;
+        }
     }
     HIP_TRY(hipGetLastError());
     d.cache_valid = true;

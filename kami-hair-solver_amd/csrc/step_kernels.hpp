@@ -355,36 +355,83 @@ __device__ __forceinline__ double resolve_step(const TrialStep &s, double *alpha
     return a;
 }
 
-// Partial refresh, first half: every contact element whose query box
-// (grown by pad plus the rounding margin) left its cached box gets the new
-// box (query box grown by pad + skin) and is queued for gathering.
-// Elements with a fixed left node take part only `with_roots` (the friction
-// pass, the one reader of their lists).
+// Partial refresh: every contact element whose query box (grown by pad plus
+// the rounding margin) left its cached box gets a new box (query box grown
+// by pad + skin), and its wavefront gathers the triangles overlapping each
+// such box in turn, breadth-first through an LDS queue; an empty result
+// keeps a box seven skins wider still when that is empty too. Elements with
+// a fixed left node take part only `with_roots` (the friction pass, the one
+// reader of their lists), and keep lists but stay off the listed queue.
+// The swept pass (boxes over a whole step, where many neighbouring elements
+// leave their boxes at once) stores the new box and queues the element for
+// k_contact_gather instead, which spreads the gathers over all wavefronts.
 template <int MODE>
-__global__ void k_contact_check(RodView m, ContactCache cc, const double *q, const double *q_base, const double *dir,
-                                double *alpha_at, TrialStep step, double pad, int with_roots)
+__global__ __launch_bounds__(kBlock) void k_contact_check(RodView m, ColliderView col, ContactCache cc,
+                                                          const double *q, const double *q_base, const double *dir,
+                                                          double *alpha_at, TrialStep step, double pad,
+                                                          int with_roots)
 {
+    __shared__ int queues[kTilesPerBlock][kGatherQueue];
+    volatile int *queue = queues[threadIdx.x / kWave];
+    const int lane = threadIdx.x & (kWave - 1);
     const double alpha = MODE == kAtTrial ? resolve_step(step, alpha_at) : 0.0;
     const uint32_t L = blockIdx.x * blockDim.x + threadIdx.x;
-    if (L >= m.node_count) return;
-    const uint32_t fl = m.flags[L];
-    if (!(fl & kNodeHasElement) || !(fl & kNodeContact)) return;
-    if ((fl & kNodeFixed) && !with_roots) return;
-    V3 lo, hi;
-    query_box<MODE>(L, q, q_base, dir, alpha, lo, hi);
-    double *box = cc.box + 6 * size_t(L);
-    if (box_covered(box, lo, hi, pad)) return;
-    const double grow = pad + cc.skin;
-    store3(box, mk(lo.x - grow, lo.y - grow, lo.z - grow));
-    store3(box + 3, mk(hi.x + grow, hi.y + grow, hi.z + grow));
-    cc.pending[atomicAdd(cc.pending_count, 1u)] = L;
+    // every lane stays to the end: the gathers use the whole wavefront
+    bool need = false;
+    V3 nlo = mk(0, 0, 0), nhi = nlo;
+    if (L < m.node_count) {
+        const uint32_t fl = m.flags[L];
+        if ((fl & kNodeHasElement) && (fl & kNodeContact) && (!(fl & kNodeFixed) || with_roots)) {
+            V3 lo, hi;
+            query_box<MODE>(L, q, q_base, dir, alpha, lo, hi);
+            if (!box_covered(cc.box + 6 * size_t(L), lo, hi, pad)) {
+                const double grow = pad + cc.skin;
+                need = true;
+                nlo = mk(lo.x - grow, lo.y - grow, lo.z - grow);
+                nhi = mk(hi.x + grow, hi.y + grow, hi.z + grow);
+                if constexpr (MODE == kSwept) {
+                    store3(cc.box + 6 * size_t(L), nlo);
+                    store3(cc.box + 6 * size_t(L) + 3, nhi);
+                    cc.pending[atomicAdd(cc.pending_count, 1u)] = L;
+                }
+            }
+        }
+    }
+    if constexpr (MODE == kSwept) return;
+    for (unsigned long long mask = __ballot(need); mask; mask &= mask - 1) {
+        const int src = __ffsll((long long)mask) - 1;
+        const uint32_t E = uint32_t(__shfl(int(L), src, kWave));
+        const V3 lo = mk(__shfl(nlo.x, src, kWave), __shfl(nlo.y, src, kWave), __shfl(nlo.z, src, kWave));
+        const V3 hi = mk(__shfl(nhi.x, src, kWave), __shfl(nhi.y, src, kWave), __shfl(nhi.z, src, kWave));
+        uint32_t *tris = cc.tris + size_t(E) * kCacheCapacity;
+        const uint32_t n = wave_gather(col, lo, hi, tris, kCacheCapacity, queue, false);
+        V3 blo = lo, bhi = hi;
+        if (n == 0) {
+            const double far = 7.0 * cc.skin;
+            const V3 flo = mk(lo.x - far, lo.y - far, lo.z - far), fhi = mk(hi.x + far, hi.y + far, hi.z + far);
+            uint32_t none;
+            if (wave_gather(col, flo, fhi, &none, 0, queue, true) == 0) {
+                blo = flo;
+                bhi = fhi;
+            }
+        }
+        if (lane == 0) {
+            double *box = cc.box + 6 * size_t(E);
+            store3(box, blo);
+            store3(box + 3, bhi);
+            cc.count[E] = n <= kCacheCapacity ? n : kCacheOverflow;
+            if (n > 0 && !(m.flags[E] & kNodeFixed) && !cc.listed[E]) {
+                cc.listed[E] = 1;
+                cc.list[atomicAdd(cc.list_count, 1u)] = E;
+            }
+        }
+    }
 }
 
-// Partial refresh, second half: one wavefront per queued element gathers
-// the triangles overlapping its new box; an empty result keeps a box seven
-// skins wider still when that is empty too. It also clears the other
-// pending counter, which the next partial refresh appends to. Elements
-// with a fixed left node keep lists but stay off the listed queue.
+// Second half of the swept pass: one wavefront per queued element gathers
+// the triangles overlapping its new box (stored by the check); an empty
+// result keeps a box seven skins wider still when that is empty too. It also
+// clears the other pending counter, which the next swept pass appends to.
 __global__ __launch_bounds__(kBlock) void k_contact_gather(const uint32_t *flags, ColliderView col, ContactCache cc,
                                                            uint32_t *next_count)
 {
