@@ -287,6 +287,10 @@ def main():
             "bytes_per_node": bytes_per_node,
             "nodes_per_launch": nodes,
             "avg_launch_ms": dom_avg_ms,
+            # the measured HBM traffic over the same launch time: what the
+            # kernel actually moves (blocks, factors), against the peak
+            "traffic_gbps": traffic / (dom_avg_ms * 1e-3) / 1e9 if traffic and dom_launches else None,
+            "traffic_frac": traffic / (dom_avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS if traffic and dom_launches else None,
         },
         "substep_roofline": {
             "algorithmic_bytes": step_bytes,
