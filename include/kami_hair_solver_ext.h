@@ -147,6 +147,20 @@ typedef struct KhsxContactListStats {
 
 KHS_API KhsResult khsxGetContactListStats(KhsSolver *solver, KhsxContactListStats *stats);
 
+/* Rollback ring of animation states kept on the device: with `frames` > 0
+   the state (q and v) after each stepped animation frame is kept in slot
+   frame % frames (0 frees the ring). khsxRollback(frame) restores the state
+   kept after `frame`, as khsRestoreAnimationCheckpoint restores a checkpoint
+   written after that frame, without any host copy; the next frame to step
+   is frame + 1. It fails with KHS_ERROR_INVALID_ARGUMENT when the slot no
+   longer holds that frame. A rebuild drops the ring; a new animation
+   empties it. It replaces the host ring of checkpoints the Blender
+   add-on keeps for resuming a bake (blender_extension/kami_hair_solver/
+   addon.py:100-103, 642-701), whose every save and restore crosses the host.
+   The CPU oracle keeps its ring in host memory. */
+KHS_API KhsResult khsxSetRollbackRing(KhsSolver *solver, uint32_t frames);
+KHS_API KhsResult khsxRollback(KhsSolver *solver, uint32_t frame);
+
 /* Worker threads the CPU oracle uses over strands (oracle only; the GPU
    library ignores it and returns KHS_OK). */
 KHS_API KhsResult khsxSetThreadCount(KhsSolver *solver, uint32_t threads);

@@ -318,6 +318,16 @@ KhsResult khsxGetContactListStats(KhsSolver *s, KhsxContactListStats *st)
     return guarded(s, [&] { return s->impl.contact_list_stats(st); });
 }
 
+KhsResult khsxSetRollbackRing(KhsSolver *s, uint32_t frames)
+{
+    return guarded(s, [&] { return s->impl.set_rollback_ring(frames); });
+}
+
+KhsResult khsxRollback(KhsSolver *s, uint32_t frame)
+{
+    return guarded(s, [&] { return s->impl.rollback(frame); });
+}
+
 KhsResult khsxSetThreadCount(KhsSolver *s, uint32_t threads)
 {
     (void)threads;

@@ -117,6 +117,7 @@ struct DeviceState {
     Buffer<double> q, q_old, vel, q_base, snap_q, snap_v, grad, dir, precond, Dblk, Ublk, Cp, dp, dv;
     // root targets (3 per node)
     Buffer<double> root_cur, rot_cur, root_pend, rot_pend, root_anim, rot_anim;
+    Buffer<double> ring_q, ring_v;  // rollback ring: 6N doubles per slot each
     // collider (3 per vertex)
     Buffer<double> col_cur, col_old, col_start, col_pend, col_anim;
     Buffer<uint32_t> tris;
@@ -242,7 +243,8 @@ struct DeviceState {
         size_t b = 0;
         for (const Buffer<double> *x : {&frame, &rest, &mass, &rmass, &q, &q_old, &vel, &q_base, &snap_q, &snap_v,
                                         &grad, &dir, &precond, &Dblk, &Ublk, &Cp, &dp, &dv, &root_cur, &rot_cur,
-                                        &root_pend, &rot_pend, &root_anim, &rot_anim, &col_cur, &col_old, &col_start,
+                                        &root_pend, &rot_pend, &root_anim, &rot_anim, &ring_q, &ring_v, &col_cur,
+                                        &col_old, &col_start,
                                         &col_pend, &col_anim, &tri_box, &tri_box_static, &bound_partials, &bounds, &out3})
             b += x->bytes();
         for (const Buffer<uint32_t> *x : {&flags, &strand_of, &strand_first, &strand_count, &strand_fixed, &mapping,
@@ -384,6 +386,7 @@ KhsResult Solver::set_hair(const KhsVec3 *points, uint32_t point_count, const ui
         }
     built_ = false;
     animation_ready_ = false;
+    ring_frame_.clear();
     release_device();
     points_ = std::move(pts);
     offsets_.assign(offsets, offsets + strand_count + 1);
@@ -427,6 +430,7 @@ KhsResult Solver::build()
     }
     built_ = false;
     animation_ready_ = false;
+    ring_frame_.clear();
     release_device();
     step_stats_.phase = KHS_PHASE_PREPARING;
     if (!build_rod_model(points_, offsets_, desc_, model_, error_)) {
@@ -950,6 +954,7 @@ KhsResult Solver::allocate_animation(uint32_t frames)
         return fail(KHS_ERROR_OUT_OF_MEMORY, e.what());
     }
     animation_frames_ = frames;
+    std::fill(ring_frame_.begin(), ring_frame_.end(), ~0u);
     root_frame_set_.assign(frames, false);
     collider_frame_set_.assign(frames, !has_collider_);
     gpu_stats_.animation_frame_count = frames;
@@ -1540,8 +1545,73 @@ KhsResult Solver::step_animation_frame(uint32_t frame, double frame_dt)
                                   d.root_anim.get() + b * n3, d.rot_anim.get() + b * n3,
                                   d.T ? d.col_anim.get() + a * v3 : nullptr, d.T ? d.col_anim.get() + b * v3 : nullptr,
                                   frame, animation_frames_);
-    if (r == KHS_OK) current_frame_ = frame;
+    if (r == KHS_OK) {
+        current_frame_ = frame;
+        if (!ring_frame_.empty()) {
+            // keep the state after this frame in its rollback slot
+            const size_t dofs = 6 * size_t(d.N), slot = frame % ring_frame_.size();
+            HIP_TRY(hipMemcpyAsync(d.ring_q.get() + slot * dofs, d.q.get(), dofs * sizeof(double),
+                                   hipMemcpyDeviceToDevice, d.stream));
+            HIP_TRY(hipMemcpyAsync(d.ring_v.get() + slot * dofs, d.vel.get(), dofs * sizeof(double),
+                                   hipMemcpyDeviceToDevice, d.stream));
+            HIP_TRY(hipStreamSynchronize(d.stream));
+            ring_frame_[slot] = frame;
+        }
+    }
     return r;
+}
+
+// ------------------------------------------------------------ rollback ring
+KhsResult Solver::set_rollback_ring(uint32_t frames)
+{
+    if (!built_) return KHS_ERROR_INVALID_STATE;
+    DeviceState &d = *dev_;
+    try {
+        const size_t dofs = 6 * size_t(d.N);
+        d.ring_q.allocate(size_t(frames) * dofs);
+        d.ring_v.allocate(size_t(frames) * dofs);
+    } catch (const std::exception &e) {
+        ring_frame_.clear();
+        return fail(KHS_ERROR_OUT_OF_MEMORY, e.what());
+    }
+    ring_frame_.assign(frames, ~0u);
+    gpu_stats_.resident_bytes = d.resident_bytes();
+    return KHS_OK;
+}
+
+KhsResult Solver::rollback(uint32_t frame)
+{
+    if (!built_ || !animation_ready_) return KHS_ERROR_INVALID_STATE;
+    if (ring_frame_.empty() || ring_frame_[frame % ring_frame_.size()] != frame)
+        return fail(KHS_ERROR_INVALID_ARGUMENT, "The rollback ring does not hold that frame.");
+    DeviceState &d = *dev_;
+    const size_t dofs = 6 * size_t(d.N), slot = frame % ring_frame_.size();
+    const size_t bytes = dofs * sizeof(double);
+    const double *q = d.ring_q.get() + slot * dofs, *v = d.ring_v.get() + slot * dofs;
+    for (double *dst : {d.q.get(), d.q_old.get(), d.snap_q.get()})
+        HIP_TRY(hipMemcpyAsync(dst, q, bytes, hipMemcpyDeviceToDevice, d.stream));
+    for (double *dst : {d.vel.get(), d.snap_v.get()})
+        HIP_TRY(hipMemcpyAsync(dst, v, bytes, hipMemcpyDeviceToDevice, d.stream));
+    HIP_TRY(hipStreamSynchronize(d.stream));
+    restore_state(frame);
+    return KHS_OK;
+}
+
+// Solver bookkeeping of a restored state (checkpoint or rollback slot).
+void Solver::restore_state(uint32_t frame)
+{
+    current_frame_ = frame;
+    cancel_.store(false);
+    frame_.store(frame);
+    substep_.store(0);
+    substep_count_.store(0);
+    iteration_.store(0);
+    phase_.store(KHS_PHASE_FINISHED);
+    step_stats_ = {};
+    step_stats_.struct_size = sizeof(step_stats_);
+    step_stats_.phase = KHS_PHASE_FINISHED;
+    clear_failure();
+    error_.clear();
 }
 
 // ------------------------------------------------------------ checkpoints
@@ -1594,18 +1664,7 @@ KhsResult Solver::restore_checkpoint(const void *data, uint64_t size)
     d.q_old.upload(q.data(), dof);
     d.snap_q.upload(q.data(), dof);
     d.snap_v.upload(v.data(), dof);
-    current_frame_ = frame;
-    cancel_.store(false);
-    frame_.store(frame);
-    substep_.store(0);
-    substep_count_.store(0);
-    iteration_.store(0);
-    phase_.store(KHS_PHASE_FINISHED);
-    step_stats_ = {};
-    step_stats_.struct_size = sizeof(step_stats_);
-    step_stats_.phase = KHS_PHASE_FINISHED;
-    clear_failure();
-    error_.clear();
+    restore_state(frame);
     return KHS_OK;
 }
 

@@ -70,6 +70,8 @@ public:
     KhsResult kernel_timing(KhsxKernelTiming *out) const;
     KhsResult substep_times(double *ms, uint32_t capacity, uint32_t *count) const;
     KhsResult contact_list_stats(KhsxContactListStats *out);
+    KhsResult set_rollback_ring(uint32_t frames);
+    KhsResult rollback(uint32_t frame);
 
     const std::string &last_error() const { return error_; }
     void set_error(std::string message) { error_ = std::move(message); }
@@ -84,6 +86,7 @@ private:
     bool device_root_targets(const std::vector<Vec3> &points, double *pos, double *rot);
     KhsResult initial_feasibility();
     void release_device();
+    void restore_state(uint32_t frame);
     void rebuild_bvh(bool swept);
     void refit_bvh(bool swept);
     void refit_boxes();
@@ -118,6 +121,7 @@ private:
     uint32_t precision_mode_ = 0;
     double trust_radius_ = 0.0;
     uint32_t animation_frames_ = 0, current_frame_ = 0;
+    std::vector<uint32_t> ring_frame_;  // animation frame held by each rollback slot (~0u: empty)
     bool animation_ready_ = false;
     std::vector<bool> root_frame_set_, collider_frame_set_;
     std::atomic<bool> cancel_{false};

@@ -305,6 +305,8 @@ EXT_FUNCTIONS = {
     "khsxGetKernelTiming": (C.c_int, [_P, _D(KhsxKernelTiming)]),
     "khsxGetSubstepTimes": (C.c_int, [_P, _P, C.c_uint32, _D(C.c_uint32)]),
     "khsxGetContactListStats": (C.c_int, [_P, _D(KhsxContactListStats)]),
+    "khsxSetRollbackRing": (C.c_int, [_P, C.c_uint32]),
+    "khsxRollback": (C.c_int, [_P, C.c_uint32]),
 }
 
 
@@ -431,6 +433,12 @@ class HairSolver:
     def restore_checkpoint(self, buf: np.ndarray) -> int:
         b = np.ascontiguousarray(buf, dtype=np.uint8)
         return self.lib.khsRestoreAnimationCheckpoint(self.handle, _ptr(b), b.size)
+
+    def set_rollback_ring(self, frames: int) -> int:
+        return self.lib.khsxSetRollbackRing(self.handle, frames)
+
+    def rollback(self, frame: int) -> int:
+        return self.lib.khsxRollback(self.handle, frame)
 
     # ------------------------------------------------------------- output
     def original_count(self) -> int:

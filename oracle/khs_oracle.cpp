@@ -413,6 +413,7 @@ public:
             return KHS_ERROR_INVALID_ARGUMENT;
         }
         anim_frames_ = frames;
+        std::fill(ring_frame_.begin(), ring_frame_.end(), ~0u);
         anim_root_.assign(size_t(frames) * nodes_.size(), V3{});
         anim_rot_.assign(size_t(frames) * nodes_.size(), V3{});
         anim_collider_.assign(has_collider_ ? size_t(frames) * collider_input_.size() : 0, V3{});
@@ -502,8 +503,39 @@ public:
         KhsResult r = run_frame(frame_dt, &anim_root_[a * n], &anim_rot_[a * n], &anim_root_[b * n], &anim_rot_[b * n],
                                 vc ? &anim_collider_[a * vc] : nullptr, vc ? &anim_collider_[b * vc] : nullptr, frame,
                                 anim_frames_, vc);
-        if (r == KHS_OK) anim_current_ = frame;
+        if (r == KHS_OK) {
+            anim_current_ = frame;
+            if (!ring_frame_.empty()) {
+                const size_t slot = frame % ring_frame_.size();
+                ring_q_[slot] = q_;
+                ring_v_[slot] = v_;
+                ring_frame_[slot] = frame;
+            }
+        }
         return r;
+    }
+
+    // Rollback ring (host memory here): the state after each stepped frame
+    // in slot frame % size.
+    KhsResult set_rollback_ring(uint32_t frames)
+    {
+        if (!built_) return KHS_ERROR_INVALID_STATE;
+        ring_frame_.assign(frames, ~0u);
+        ring_q_.assign(frames, {});
+        ring_v_.assign(frames, {});
+        return KHS_OK;
+    }
+
+    KhsResult rollback(uint32_t frame)
+    {
+        if (!built_ || !anim_ready_) return KHS_ERROR_INVALID_STATE;
+        if (ring_frame_.empty() || ring_frame_[frame % ring_frame_.size()] != frame)
+            return fail(KHS_ERROR_INVALID_ARGUMENT, "The rollback ring does not hold that frame.");
+        const size_t slot = frame % ring_frame_.size();
+        q_ = ring_q_[slot];
+        v_ = ring_v_[slot];
+        restored(frame);
+        return KHS_OK;
     }
 
     uint64_t checkpoint_size() const
@@ -548,6 +580,13 @@ public:
             return fail(KHS_ERROR_INVALID_ARGUMENT, "Checkpoint does not match this solver.");
         std::memcpy(q_.data(), in + 24, q_.size() * 8);
         std::memcpy(v_.data(), in + 24 + q_.size() * 8, v_.size() * 8);
+        restored(frame);
+        return KHS_OK;
+    }
+
+    // Bookkeeping of a restored state (checkpoint or rollback slot).
+    void restored(uint32_t frame)
+    {
         q_old_ = q_;
         snap_q_ = q_;
         snap_v_ = v_;
@@ -563,7 +602,6 @@ public:
         step_stats_.phase = KHS_PHASE_FINISHED;
         clear_failure();
         error.clear();
-        return KHS_OK;
     }
 
     // --------------------------------------------------------------- output
@@ -1136,6 +1174,9 @@ private:
         anim_ready_ = false;
         anim_frames_ = 0;
         anim_current_ = 0;
+        ring_frame_.clear();
+        ring_q_.clear();
+        ring_v_.clear();
         clear_failure();
     }
 
@@ -1906,6 +1947,8 @@ private:
     std::vector<V3> root_cur_, rot_cur_, root_pending_, rot_pending_;
     uint32_t anim_frames_ = 0, anim_current_ = 0;
     bool anim_ready_ = false;
+    std::vector<uint32_t> ring_frame_;
+    std::vector<std::vector<double>> ring_q_, ring_v_;
     std::vector<V3> anim_root_, anim_rot_, anim_collider_;
     std::vector<bool> root_set_, collider_set_;
     KhsBuildStats build_stats_{};
@@ -2242,6 +2285,18 @@ KhsResult khsxGetContactListStats(KhsSolver *s, KhsxContactListStats *st)
     r.elements = s->impl.node_count();
     *st = r;
     return KHS_OK;
+}
+
+KhsResult khsxSetRollbackRing(KhsSolver *s, uint32_t frames)
+{
+    if (!s) return KHS_ERROR_INVALID_ARGUMENT;
+    return s->impl.set_rollback_ring(frames);
+}
+
+KhsResult khsxRollback(KhsSolver *s, uint32_t frame)
+{
+    if (!s) return KHS_ERROR_INVALID_ARGUMENT;
+    return s->impl.rollback(frame);
 }
 
 KhsResult khsxSetThreadCount(KhsSolver *s, uint32_t threads)

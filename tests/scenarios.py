@@ -3,8 +3,9 @@
 Each function takes a loaded library (the CPU oracle or the HIP library) and
 asserts what the matching reference test asserts
 (tests/test_solver.cpp:18-343 and tests/test_c_api.c:11-30 of the
-reference). The test modules run them against the oracle on the CPU and
-against the HIP library on the GPU.
+reference); EXTENSIONS holds scenarios of the extension API, which has no
+reference counterpart. The test modules run them against the oracle on the
+CPU and against the HIP library on the GPU.
 """
 import ctypes as C
 import math
@@ -240,6 +241,53 @@ def resident_animation_and_checkpoint(lib):
     assert s.progress().phase == kn.PHASE_FINISHED
     s.close()
 
+
+def rollback_ring(lib):
+    """The rollback ring of the extension API (no counterpart in the
+    reference): the state after each stepped frame is kept in slot
+    frame % size; rolling back to a kept frame and stepping on reproduces the
+    straight run bit for bit, and a frame the ring no longer holds is
+    refused."""
+    s, _ = _solver(lib, gravity=(0, -9.81, 0), fixed_root_nodes=1)
+    mat = kn.default_material(lib)
+    strand = np.array([[0, 0, 0], [0, -0.02, 0], [0, -0.04, 0], [0, -0.06, 0]], float)
+    points = np.concatenate([strand, strand + [0.01, 0, 0.003]])
+    s.check(s.set_hair(points, [0, 4, 8], mat))
+    s.check(s.build())
+    assert s.rollback(1) == kn.KHS_ERROR_INVALID_STATE
+    frames, dt = 7, 1.0 / 24.0
+    s.check(s.allocate_animation(frames))
+    for k in range(frames):
+        s.check(s.set_root_frame(k, points + [0.004 * math.sin(0.7 * k), 0.0, 0.003 * (1.0 - math.cos(0.7 * k))]))
+        s.check(s.set_collider_frame(k, None))
+    s.check(s.finalize_animation())
+    s.check(s.set_rollback_ring(3))
+    straight = {}
+    for k in range(1, frames):
+        s.check(s.step_frame(k, dt))
+        straight[k] = s.original_positions()
+    # frames 4, 5 and 6 are held; 3 has been evicted by 6
+    assert s.rollback(3) == kn.KHS_ERROR_INVALID_ARGUMENT
+    assert s.rollback(frames + 1) == kn.KHS_ERROR_INVALID_ARGUMENT
+    s.check(s.rollback(4))
+    assert np.array_equal(s.original_positions(), straight[4])
+    assert s.progress().phase == kn.PHASE_FINISHED
+    assert s.step_frame(6, dt) == kn.KHS_ERROR_INVALID_STATE
+    for k in (5, 6):
+        s.check(s.step_frame(k, dt))
+        assert np.array_equal(s.original_positions(), straight[k]), k
+    # rolling back further than the last rollback, then stepping on again
+    s.check(s.rollback(5))
+    s.check(s.step_frame(6, dt))
+    assert np.array_equal(s.original_positions(), straight[6])
+    assert not np.array_equal(straight[4], straight[6])
+    # a zero-sized ring keeps nothing
+    s.check(s.set_rollback_ring(0))
+    assert s.rollback(6) == kn.KHS_ERROR_INVALID_ARGUMENT
+    s.close()
+
+
+EXTENSIONS = [rollback_ring]
 
 ALL = [
     c_api_defaults,

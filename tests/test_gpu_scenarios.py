@@ -14,6 +14,12 @@ def test_reference_scenario_on_gpu(gpu_lib, case):
     case(gpu_lib)
 
 
+@pytest.mark.parametrize("case", scenarios.EXTENSIONS, ids=[f.__name__ for f in scenarios.EXTENSIONS])
+def test_extension_scenario_on_gpu(gpu_lib, case):
+    print(f"[gpu] extension scenario {case.__name__}", flush=True)
+    case(gpu_lib)
+
+
 def test_gpu_info(gpu_lib):
     """tests/test_c_api.c:26-29 asks for compute capability >= 12 (an NVIDIA
     sm_120 part); on MI355X the device reports gfx950, i.e. major 9, minor 5."""

@@ -12,3 +12,8 @@ import scenarios
 @pytest.mark.parametrize("case", scenarios.ALL, ids=[f.__name__ for f in scenarios.ALL])
 def test_reference_scenario_on_oracle(oracle_lib, case):
     case(oracle_lib)
+
+
+@pytest.mark.parametrize("case", scenarios.EXTENSIONS, ids=[f.__name__ for f in scenarios.EXTENSIONS])
+def test_extension_scenario_on_oracle(oracle_lib, case):
+    case(oracle_lib)
