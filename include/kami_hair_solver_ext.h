@@ -161,6 +161,15 @@ KHS_API KhsResult khsxGetContactListStats(KhsSolver *solver, KhsxContactListStat
 KHS_API KhsResult khsxSetRollbackRing(KhsSolver *solver, uint32_t frames);
 KHS_API KhsResult khsxRollback(KhsSolver *solver, uint32_t frame);
 
+/* Appends the current positions of the original points to an open .khc hair
+   cache: 3 * khsGetOriginalPointCount little-endian doubles, the frame
+   record the Blender add-on writes after khsCopyOriginalPositions
+   (blender_extension/kami_hair_solver/addon.py:580-584, header :23-24,
+   :704-724). The gather runs on the device and the bytes go to `fd` from
+   pinned staging, with no per-point host conversion. A failed write returns
+   KHS_ERROR_INTERNAL with the system's message in khsGetLastError. */
+KHS_API KhsResult khsxWriteCacheFrame(KhsSolver *solver, int fd);
+
 /* Worker threads the CPU oracle uses over strands (oracle only; the GPU
    library ignores it and returns KHS_OK). */
 KHS_API KhsResult khsxSetThreadCount(KhsSolver *solver, uint32_t threads);

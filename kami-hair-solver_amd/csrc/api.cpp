@@ -328,6 +328,11 @@ KhsResult khsxRollback(KhsSolver *s, uint32_t frame)
     return guarded(s, [&] { return s->impl.rollback(frame); });
 }
 
+KhsResult khsxWriteCacheFrame(KhsSolver *s, int fd)
+{
+    return guarded(s, [&] { return s->impl.write_cache_frame(fd); });
+}
+
 KhsResult khsxSetThreadCount(KhsSolver *s, uint32_t threads)
 {
     (void)threads;

@@ -8,12 +8,15 @@
 
 #include <rocprim/device/device_radix_sort.hpp>
 
+#include <cerrno>
 #include <cmath>
 #include <cstring>
 #include <limits>
 #include <sstream>
 #include <stdexcept>
 #include <thread>
+
+#include <unistd.h>
 
 namespace khs {
 
@@ -209,6 +212,7 @@ struct DeviceState {
         for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
         for (hipEvent_t e : substep_events) (void)hipEventDestroy(e);
         if (host_view) (void)hipHostFree(host_view);
+        if (cache_host) (void)hipHostFree(cache_host);
         if (frame_begin) (void)hipEventDestroy(frame_begin);
         if (frame_end) (void)hipEventDestroy(frame_end);
         if (stream) (void)hipStreamDestroy(stream);
@@ -304,6 +308,7 @@ struct DeviceState {
     const unsigned *skip() const { return skipping ? stop.get() : nullptr; }
     HostView *host_view = nullptr;  // pinned, coherent
     void *host_view_device = nullptr;  // its device address
+    double *cache_host = nullptr;      // pinned staging of one cache frame (3P doubles)
 
     uint32_t tiled_blocks() const { return grid_for(size_t(N) + 1, kNodesPerBlock); }
     uint32_t node_blocks() const { return grid_for(N, kBlock); }
@@ -1535,6 +1540,7 @@ KhsResult Solver::step_animation_frame(uint32_t frame, double frame_dt)
     if (!animation_ready_ || frame >= animation_frames_) return fail(KHS_ERROR_INVALID_ARGUMENT, "Invalid animation frame index.");
     if (frame == 0) {
         current_frame_ = 0;
+        keep_in_ring(0);
         return KHS_OK;
     }
     if (frame != current_frame_ + 1) return fail(KHS_ERROR_INVALID_STATE, "Animation frames must be stepped in order.");
@@ -1547,18 +1553,23 @@ KhsResult Solver::step_animation_frame(uint32_t frame, double frame_dt)
                                   frame, animation_frames_);
     if (r == KHS_OK) {
         current_frame_ = frame;
-        if (!ring_frame_.empty()) {
-            // keep the state after this frame in its rollback slot
-            const size_t dofs = 6 * size_t(d.N), slot = frame % ring_frame_.size();
-            HIP_TRY(hipMemcpyAsync(d.ring_q.get() + slot * dofs, d.q.get(), dofs * sizeof(double),
-                                   hipMemcpyDeviceToDevice, d.stream));
-            HIP_TRY(hipMemcpyAsync(d.ring_v.get() + slot * dofs, d.vel.get(), dofs * sizeof(double),
-                                   hipMemcpyDeviceToDevice, d.stream));
-            HIP_TRY(hipStreamSynchronize(d.stream));
-            ring_frame_[slot] = frame;
-        }
+        keep_in_ring(frame);
     }
     return r;
+}
+
+// Keeps the current state as that of `frame` in its rollback slot.
+void Solver::keep_in_ring(uint32_t frame)
+{
+    if (ring_frame_.empty()) return;
+    DeviceState &d = *dev_;
+    const size_t dofs = 6 * size_t(d.N), slot = frame % ring_frame_.size();
+    HIP_TRY(hipMemcpyAsync(d.ring_q.get() + slot * dofs, d.q.get(), dofs * sizeof(double), hipMemcpyDeviceToDevice,
+                           d.stream));
+    HIP_TRY(hipMemcpyAsync(d.ring_v.get() + slot * dofs, d.vel.get(), dofs * sizeof(double), hipMemcpyDeviceToDevice,
+                           d.stream));
+    HIP_TRY(hipStreamSynchronize(d.stream));
+    ring_frame_[slot] = frame;
 }
 
 // ------------------------------------------------------------ rollback ring
@@ -1683,6 +1694,36 @@ KhsResult Solver::copy_original_positions(KhsVec3 *positions, uint32_t capacity)
     } catch (...) {
         return KHS_ERROR_INTERNAL;
     }
+}
+
+// Appends the current original-point positions to a .khc cache file: the
+// gather runs on the device, one copy lands in pinned staging, and the
+// bytes go to the descriptor as they are (3P little-endian doubles, the
+// frame record of blender_extension/kami_hair_solver/addon.py:580-584).
+KhsResult Solver::write_cache_frame(int fd)
+{
+    if (!built_) return KHS_ERROR_INVALID_STATE;
+    if (fd < 0) return KHS_ERROR_INVALID_ARGUMENT;
+    DeviceState &d = *dev_;
+    const size_t bytes = 3 * size_t(d.P) * sizeof(double);
+    if (!d.cache_host) HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&d.cache_host), bytes, hipHostMallocDefault));
+    hipLaunchKernelGGL(dev::k_gather_positions, dim3(grid_for(d.P, kBlock)), dim3(kBlock), 0, d.stream, d.q.get(),
+                       d.mapping.get(), d.P, d.out3.get());
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(d.cache_host, d.out3.get(), bytes, hipMemcpyDeviceToHost, d.stream));
+    HIP_TRY(hipStreamSynchronize(d.stream));
+    const char *p = reinterpret_cast<const char *>(d.cache_host);
+    for (size_t left = bytes; left;) {
+        const ssize_t w = ::write(fd, p, left);
+        if (w < 0 && errno == EINTR) continue;
+        if (w <= 0) {
+            error_ = std::string("Writing the cache frame failed: ") + std::strerror(w < 0 ? errno : EIO);
+            return KHS_ERROR_INTERNAL;
+        }
+        p += w;
+        left -= size_t(w);
+    }
+    return KHS_OK;
 }
 
 KhsResult Solver::copy_internal_positions(KhsVec3 *positions, uint32_t capacity) const

@@ -72,6 +72,7 @@ public:
     KhsResult contact_list_stats(KhsxContactListStats *out);
     KhsResult set_rollback_ring(uint32_t frames);
     KhsResult rollback(uint32_t frame);
+    KhsResult write_cache_frame(int fd);
 
     const std::string &last_error() const { return error_; }
     void set_error(std::string message) { error_ = std::move(message); }
@@ -87,6 +88,7 @@ private:
     KhsResult initial_feasibility();
     void release_device();
     void restore_state(uint32_t frame);
+    void keep_in_ring(uint32_t frame);
     void rebuild_bvh(bool swept);
     void refit_bvh(bool swept);
     void refit_boxes();

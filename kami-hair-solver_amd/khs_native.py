@@ -307,6 +307,7 @@ EXT_FUNCTIONS = {
     "khsxGetContactListStats": (C.c_int, [_P, _D(KhsxContactListStats)]),
     "khsxSetRollbackRing": (C.c_int, [_P, C.c_uint32]),
     "khsxRollback": (C.c_int, [_P, C.c_uint32]),
+    "khsxWriteCacheFrame": (C.c_int, [_P, C.c_int]),
 }
 
 
@@ -439,6 +440,9 @@ class HairSolver:
 
     def rollback(self, frame: int) -> int:
         return self.lib.khsxRollback(self.handle, frame)
+
+    def write_cache_frame(self, fd: int) -> int:
+        return self.lib.khsxWriteCacheFrame(self.handle, fd)
 
     # ------------------------------------------------------------- output
     def original_count(self) -> int:

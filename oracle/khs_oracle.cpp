@@ -20,6 +20,7 @@
 #include "oracle_math.hpp"
 
 #include <atomic>
+#include <cerrno>
 #include <chrono>
 #include <cstring>
 #include <limits>
@@ -30,6 +31,8 @@
 #include <thread>
 #include <unordered_map>
 #include <vector>
+
+#include <unistd.h>
 
 namespace oracle {
 namespace {
@@ -495,6 +498,7 @@ public:
         }
         if (frame == 0) {
             anim_current_ = 0;
+            keep_in_ring(0);
             return KHS_OK;
         }
         if (frame != anim_current_ + 1) return fail(KHS_ERROR_INVALID_STATE, "Animation frames must be stepped in order.");
@@ -505,14 +509,39 @@ public:
                                 anim_frames_, vc);
         if (r == KHS_OK) {
             anim_current_ = frame;
-            if (!ring_frame_.empty()) {
-                const size_t slot = frame % ring_frame_.size();
-                ring_q_[slot] = q_;
-                ring_v_[slot] = v_;
-                ring_frame_[slot] = frame;
-            }
+            keep_in_ring(frame);
         }
         return r;
+    }
+
+    void keep_in_ring(uint32_t frame)
+    {
+        if (ring_frame_.empty()) return;
+        const size_t slot = frame % ring_frame_.size();
+        ring_q_[slot] = q_;
+        ring_v_[slot] = v_;
+        ring_frame_[slot] = frame;
+    }
+
+    // One .khc frame record: 3 doubles per original point, native order.
+    KhsResult write_cache_frame(int fd)
+    {
+        if (!built_) return KHS_ERROR_INVALID_STATE;
+        if (fd < 0) return KHS_ERROR_INVALID_ARGUMENT;
+        std::vector<KhsVec3> out(mapping_.size());
+        copy_original(out.data(), uint32_t(out.size()));
+        const char *p = reinterpret_cast<const char *>(out.data());
+        for (size_t left = out.size() * sizeof(KhsVec3); left;) {
+            const ssize_t w = ::write(fd, p, left);
+            if (w < 0 && errno == EINTR) continue;
+            if (w <= 0) {
+                error = std::string("Writing the cache frame failed: ") + std::strerror(w < 0 ? errno : EIO);
+                return KHS_ERROR_INTERNAL;
+            }
+            p += w;
+            left -= size_t(w);
+        }
+        return KHS_OK;
     }
 
     // Rollback ring (host memory here): the state after each stepped frame
@@ -2297,6 +2326,12 @@ KhsResult khsxRollback(KhsSolver *s, uint32_t frame)
 {
     if (!s) return KHS_ERROR_INVALID_ARGUMENT;
     return s->impl.rollback(frame);
+}
+
+KhsResult khsxWriteCacheFrame(KhsSolver *s, int fd)
+{
+    if (!s) return KHS_ERROR_INVALID_ARGUMENT;
+    return s->impl.write_cache_frame(fd);
 }
 
 KhsResult khsxSetThreadCount(KhsSolver *s, uint32_t threads)
