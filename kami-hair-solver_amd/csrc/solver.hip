@@ -3,6 +3,7 @@
 // (reference src/cuda_backend.cu:341-723, src/solver.cpp:449-893).
 #include "solver.hpp"
 
+#include "model_kernels.hpp"
 #include "root_kernels.hpp"
 #include "step_kernels.hpp"
 
@@ -438,18 +439,6 @@ KhsResult Solver::build()
     ring_frame_.clear();
     release_device();
     step_stats_.phase = KHS_PHASE_PREPARING;
-    if (!build_rod_model(points_, offsets_, desc_, model_, error_)) {
-        step_stats_.phase = KHS_PHASE_FAILED;
-        return KHS_ERROR_INVALID_HAIR;
-    }
-    assign_masses(material_, model_);
-    std::vector<Vec3> tp, tr;
-    const unsigned threads = std::max(1u, std::thread::hardware_concurrency());
-    if (!root_targets(model_, points_, tp, tr, threads)) {
-        error_ = "Cannot build the initial root orientation.";
-        step_stats_.phase = KHS_PHASE_FAILED;
-        return KHS_ERROR_INVALID_HAIR;
-    }
     if (has_collider_) validate_collider(collider_);
     KhsResult r = upload_model();
     if (r != KHS_OK) {
@@ -474,9 +463,7 @@ KhsResult Solver::build()
     build_stats_.original_point_count = uint32_t(points_.size());
     build_stats_.internal_node_count = model_.node_count;
     build_stats_.element_count = model_.element_count;
-    uint32_t fixed = 0;
-    for (uint32_t f : model_.flags) fixed += (f & kNodeFixed) ? 1 : 0;
-    build_stats_.fixed_node_count = fixed;
+    build_stats_.fixed_node_count = model_.fixed_nodes;
     build_stats_.collider_vertex_count = uint32_t(collider_.vertices.size());
     build_stats_.collider_triangle_count = collider_.triangle_count();
     build_stats_.excluded_collider_triangle_count = collider_.excluded;
@@ -502,17 +489,113 @@ KhsResult Solver::build()
     return KHS_OK;
 }
 
+// The rod model on the device (reference solver.cpp:227-373): a counting
+// pass per strand, the node layout scanned on the host from the per-strand
+// counts (S values), then a pass per strand that writes nodes, bindings,
+// frames, rest strains and masses into the buffers the step reads.
+KhsResult Solver::build_model_on_device()
+{
+    DeviceState &d = *dev_;
+    RodModel &m = model_;
+    m = RodModel{};
+    const uint32_t S = uint32_t(offsets_.size() - 1);
+    d.S = S;
+    d.P = uint32_t(points_.size());
+    const std::vector<double> flat = flatten(points_);
+    d.root_points.allocate(flat.size());
+    d.root_points.upload(flat.data(), flat.size());
+    Buffer<uint32_t> offsets;
+    offsets.allocate(S + 1);
+    offsets.upload(offsets_.data(), S + 1);
+    Buffer<dev::StrandShape> shape;
+    shape.allocate(S);
+    const double hmax = desc_.maximum_element_length, minimum = desc_.minimum_dynamic_length;
+    hipLaunchKernelGGL(dev::k_model_count, dim3(grid_for(S, kBlock)), dim3(kBlock), 0, d.stream, S,
+                       (const double *)d.root_points.get(), (const uint32_t *)offsets.get(), hmax, minimum,
+                       shape.get());
+    HIP_TRY(hipGetLastError());
+    std::vector<dev::StrandShape> sh(S);
+    HIP_TRY(hipMemcpyAsync(sh.data(), shape.get(), shape.bytes(), hipMemcpyDeviceToHost, d.stream));
+    HIP_TRY(hipStreamSynchronize(d.stream));
+
+    m.strand_count = S;
+    m.strand_first.resize(S);
+    m.strand_count_of.resize(S);
+    m.strand_fixed.resize(S);
+    m.strand_real.resize(S);
+    uint64_t total = 0;
+    for (uint32_t s = 0; s < S; ++s) {
+        const dev::StrandShape &x = sh[s];
+        if (x.degenerate) {
+            error_ = "Strand " + std::to_string(s) + " has all its points at one position.";
+            return KHS_ERROR_INVALID_HAIR;
+        }
+        m.strand_first[s] = uint32_t(std::min<uint64_t>(total, UINT32_MAX));
+        m.strand_count_of[s] = x.nodes;
+        m.strand_real[s] = x.real_nodes;
+        m.strand_fixed[s] = std::min<uint32_t>(desc_.fixed_root_nodes, x.nodes);
+        m.merged_segments += x.merged;
+        if (x.extension_nodes) {
+            ++m.extension_strands;
+            m.extension_nodes += x.extension_nodes;
+            m.extension_length += x.extension_length;
+        }
+        m.fixed_nodes += m.strand_fixed[s];
+        m.free_dofs += 6 * uint64_t(x.nodes - m.strand_fixed[s]);
+        total += x.nodes;
+    }
+    if (total > dev::kMaxSolveNodes) {
+        error_ = "The model has " + std::to_string(total) + " nodes; the block solve addresses at most " +
+                 std::to_string(dev::kMaxSolveNodes) + " per solver (32-bit byte offsets into the Hessian blocks).";
+        return KHS_ERROR_OUT_OF_MEMORY;
+    }
+    m.node_count = uint32_t(total);
+    m.element_count = m.node_count - S;
+    d.N = m.node_count;
+    const size_t N = d.N;
+
+    d.q.allocate(6 * N);
+    d.frame.allocate(9 * N);
+    d.rest.allocate(8 * N);
+    d.mass.allocate(N);
+    d.rmass.allocate(N);
+    d.flags.allocate(N);
+    d.strand_of.allocate(N);
+    d.bind_left.allocate(N);
+    d.bind_right.allocate(N);
+    d.bind_t.allocate(N);
+    d.mapping.allocate(d.P);
+    d.root_flag.allocate(1);
+    d.strand_first.allocate(S);
+    d.strand_first.upload(m.strand_first.data(), S);
+    d.strand_count.allocate(S);
+    d.strand_count.upload(m.strand_count_of.data(), S);
+    d.strand_fixed.allocate(S);
+    d.strand_fixed.upload(m.strand_fixed.data(), S);
+    Buffer<double> strand_max;
+    strand_max.allocate(S);
+    const MassDensity rho = mass_density(material_);
+    dev::ModelOut o{d.q.get(),        d.frame.get(),    d.rest.get(),       d.mass.get(),   d.rmass.get(),
+                    d.flags.get(),    d.strand_of.get(), d.bind_left.get(), d.bind_right.get(), d.bind_t.get(),
+                    d.mapping.get(),  strand_max.get()};
+    hipLaunchKernelGGL(dev::k_model_nodes, dim3(grid_for(S, kBlock)), dim3(kBlock), 0, d.stream, S,
+                       (const double *)d.root_points.get(), (const uint32_t *)offsets.get(), hmax, minimum,
+                       desc_.fixed_root_nodes, (const uint32_t *)d.strand_first.get(), rho.translational,
+                       rho.rotational, o);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> maxima(S);
+    HIP_TRY(hipMemcpyAsync(maxima.data(), strand_max.get(), strand_max.bytes(), hipMemcpyDeviceToHost, d.stream));
+    HIP_TRY(hipStreamSynchronize(d.stream));
+    for (double v : maxima) m.maximum_rest_length = std::max(m.maximum_rest_length, v);
+    return KHS_OK;
+}
+
 KhsResult Solver::upload_model()
 {
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count < 1) {
         error_ = "No HIP GPU is available.";
         return KHS_ERROR_INVALID_STATE;
-    }
-    if (uint64_t(model_.node_count) > dev::kMaxSolveNodes) {
-        error_ = "The model has " + std::to_string(model_.node_count) + " nodes; the block solve addresses at most " +
-                 std::to_string(dev::kMaxSolveNodes) + " per solver (32-bit byte offsets into the Hessian blocks).";
-        return KHS_ERROR_OUT_OF_MEMORY;
     }
     HIP_TRY(hipSetDevice(0));
     dev_ = std::make_unique<DeviceState>();
@@ -525,31 +608,10 @@ KhsResult Solver::upload_model()
     std::memset(d.host_view, 0, sizeof(DeviceState::HostView));
     HIP_TRY(hipHostGetDevicePointer(&d.host_view_device, d.host_view, 0));
     d.alpha.allocate(1);
+    KhsResult built = build_model_on_device();
+    if (built != KHS_OK) return built;
     const RodModel &m = model_;
-    d.N = m.node_count;
-    d.S = m.strand_count;
-    d.P = uint32_t(points_.size());
     const size_t N = d.N;
-    std::vector<double> frames(9 * N);
-    for (size_t n = 0; n < N; ++n) std::memcpy(&frames[9 * n], m.frame[n].m, 9 * sizeof(double));
-    d.frame.allocate(9 * N);
-    d.frame.upload(frames.data(), frames.size());
-    d.rest.allocate(8 * N);
-    d.rest.upload(m.rest.data(), m.rest.size());
-    d.mass.allocate(N);
-    d.mass.upload(m.mass.data(), N);
-    d.rmass.allocate(N);
-    d.rmass.upload(m.rotational_mass.data(), N);
-    d.flags.allocate(N);
-    d.flags.upload(m.flags.data(), N);
-    d.strand_of.allocate(N);
-    d.strand_of.upload(m.strand_of.data(), N);
-    d.strand_first.allocate(d.S);
-    d.strand_first.upload(m.strand_first.data(), d.S);
-    d.strand_count.allocate(d.S);
-    d.strand_count.upload(m.strand_count_of.data(), d.S);
-    d.strand_fixed.allocate(d.S);
-    d.strand_fixed.upload(m.strand_fixed.data(), d.S);
     {
         // The solve is sequential along a strand: dispatching the longest
         // strands first keeps them off the tail of the launch.
@@ -563,11 +625,13 @@ KhsResult Solver::upload_model()
     }
     {
         // Friction reads the closest triangle of these; the energy does not.
+        // Per strand it is the element of the last fixed node, when the node
+        // after it is free and not virtual.
         std::vector<uint32_t> roots;
-        for (uint32_t n = 0; n + 1 < d.N; ++n) {
-            const uint32_t f = m.flags[n];
-            if ((f & kNodeFixed) && (f & kNodeHasElement) && (f & kNodeContact) && !(m.flags[n + 1] & kNodeFixed))
-                roots.push_back(n);
+        for (uint32_t s = 0; s < d.S; ++s) {
+            const uint32_t fixed = m.strand_fixed[s];
+            if (fixed >= 1 && fixed < m.strand_count_of[s] && fixed < m.strand_real[s])
+                roots.push_back(m.strand_first[s] + fixed - 1);
         }
         d.root_element_count = uint32_t(roots.size());
         const double one = 1.0;
@@ -579,34 +643,17 @@ KhsResult Solver::upload_model()
         const unsigned zeros[2] = {0u, 0u};
         d.solve_failed.allocate(2);
         d.solve_failed.upload(zeros, 2);
-        d.root_flag.allocate(1);
-        d.bind_left.allocate(N);
-        d.bind_left.upload(m.bind_left.data(), N);
-        d.bind_right.allocate(N);
-        d.bind_right.upload(m.bind_right.data(), N);
-        d.bind_t.allocate(N);
-        d.bind_t.upload(m.bind_t.data(), N);
-        d.root_points.allocate(3 * size_t(d.P));
         d.root_elements.allocate(std::max<size_t>(1, roots.size()));
         if (!roots.empty()) d.root_elements.upload(roots.data(), roots.size());
     }
-    d.mapping.allocate(d.P);
-    d.mapping.upload(m.original_to_internal.data(), d.P);
 
-    // q = (position, zero rotation), v = 0 (reference cuda_backend.cu:360-371)
-    std::vector<double> q(6 * N, 0.0);
-    for (size_t n = 0; n < N; ++n) {
-        q[6 * n] = m.position[n].x;
-        q[6 * n + 1] = m.position[n].y;
-        q[6 * n + 2] = m.position[n].z;
-    }
+    // q = (position, zero rotation) from the model build, v = 0 (reference
+    // cuda_backend.cu:360-371)
     const std::vector<double> zeros(6 * N, 0.0);
-    for (Buffer<double> *b : {&d.q, &d.q_old, &d.vel, &d.q_base, &d.snap_q, &d.snap_v, &d.grad, &d.dir, &d.precond,
-                              &d.dp})
+    for (Buffer<double> *b : {&d.q_old, &d.vel, &d.q_base, &d.snap_q, &d.snap_v, &d.grad, &d.dir, &d.precond, &d.dp})
         b->allocate(6 * N);
-    d.q.upload(q.data(), q.size());
-    d.q_old.upload(q.data(), q.size());
-    d.snap_q.upload(q.data(), q.size());
+    HIP_TRY(hipMemcpyAsync(d.q_old.get(), d.q.get(), d.q.bytes(), hipMemcpyDeviceToDevice, d.stream));
+    HIP_TRY(hipMemcpyAsync(d.snap_q.get(), d.q.get(), d.q.bytes(), hipMemcpyDeviceToDevice, d.stream));
     d.vel.upload(zeros.data(), zeros.size());
     d.snap_v.upload(zeros.data(), zeros.size());
     d.dir.upload(zeros.data(), zeros.size());
@@ -615,11 +662,20 @@ KhsResult Solver::upload_model()
     d.Dblk.allocate(36 * N);
     d.Ublk.allocate(36 * N);
     d.Cp.allocate(36 * N);
-    // root targets start at the node positions and zero rotations
-    std::vector<double> pos = flatten(m.position), zero3(3 * N, 0.0);
+    // The root targets of the build's own points must have a frame
+    // (reference solver.cpp:463-467); they are computed into the pending
+    // targets as scratch. The targets then start at the node positions and
+    // zero rotations (cuda_backend.cu:341-407).
+    const std::vector<double> zero3(3 * N, 0.0);
     for (Buffer<double> *b : {&d.root_cur, &d.rot_cur, &d.root_pend, &d.rot_pend}) b->allocate(3 * N);
-    d.root_cur.upload(pos.data(), pos.size());
-    d.root_pend.upload(pos.data(), pos.size());
+    if (!device_root_targets(points_, d.root_pend.get(), d.rot_pend.get())) {
+        error_ = "Cannot build the initial root orientation.";
+        return KHS_ERROR_INVALID_HAIR;
+    }
+    hipLaunchKernelGGL(dev::k_gather_positions, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.q.get(),
+                       (const uint32_t *)nullptr, d.N, d.root_cur.get());
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(d.root_pend.get(), d.root_cur.get(), d.root_cur.bytes(), hipMemcpyDeviceToDevice, d.stream));
     d.rot_cur.upload(zero3.data(), zero3.size());
     d.rot_pend.upload(zero3.data(), zero3.size());
     d.out3.allocate(3 * std::max<size_t>(N, d.P));
@@ -819,10 +875,11 @@ KhsResult Solver::initial_feasibility()
 {
     // reference solver.cpp:634-657
     DeviceState &d = *dev_;
-    std::vector<double> pos = flatten(model_.position);
     Buffer<double> dpos;
-    dpos.allocate(pos.size());
-    dpos.upload(pos.data(), pos.size());
+    dpos.allocate(3 * size_t(d.N));
+    hipLaunchKernelGGL(dev::k_gather_positions, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.q.get(),
+                       (const uint32_t *)nullptr, d.N, dpos.get());
+    HIP_TRY(hipGetLastError());
     const RodView rv = d.rods();
     const ColliderView cv = d.collider();
     if (collider_.closed) {
@@ -873,9 +930,15 @@ KhsResult Solver::update_runtime_parameters(const KhsSolverDesc &desc, const Khs
     }
     desc_ = desc;
     material_ = material;
-    assign_masses(material_, model_);
-    dev_->mass.upload(model_.mass.data(), model_.node_count);
-    dev_->rmass.upload(model_.rotational_mass.data(), model_.node_count);
+    {
+        DeviceState &d = *dev_;
+        const MassDensity rho = mass_density(material_);
+        hipLaunchKernelGGL(dev::k_model_masses, dim3(grid_for(d.S, kBlock)), dim3(kBlock), 0, d.stream, d.rods(),
+                           (const double *)d.rest.get(), rho.translational, rho.rotational, d.mass.get(),
+                           d.rmass.get());
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(d.stream));
+    }
     trust_radius_ = std::max(0.25 * model_.maximum_rest_length,
                              8.0 * (material_.radius + material_.collider_offset + material_.barrier_distance));
     error_.clear();
@@ -1745,7 +1808,7 @@ KhsResult Solver::copy_internal_positions(KhsVec3 *positions, uint32_t capacity)
 KhsResult Solver::copy_mapping(uint32_t *indices, uint32_t capacity) const
 {
     if (!built_ || !indices || capacity < points_.size()) return KHS_ERROR_INVALID_ARGUMENT;
-    std::copy(model_.original_to_internal.begin(), model_.original_to_internal.end(), indices);
+    dev_->mapping.download(indices, points_.size());
     return KHS_OK;
 }
 
@@ -2022,7 +2085,8 @@ KhsResult Solver::contact_list_stats(KhsxContactListStats *out)
     std::vector<uint32_t> count(d.N);
     uint32_t listed = 0;
     HIP_TRY(hipMemcpyAsync(count.data(), d.cache_count.get(), d.cache_count.bytes(), hipMemcpyDeviceToHost, d.stream));
-    const std::vector<uint32_t> &flags = model_.flags;
+    std::vector<uint32_t> flags(d.N);
+    HIP_TRY(hipMemcpyAsync(flags.data(), d.flags.get(), d.flags.bytes(), hipMemcpyDeviceToHost, d.stream));
     HIP_TRY(hipMemcpyAsync(&listed, d.cache_list_count.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, d.stream));
     HIP_TRY(hipStreamSynchronize(d.stream));
     r.listed = listed;

@@ -88,6 +88,7 @@ private:
     KhsResult initial_feasibility();
     void release_device();
     void restore_state(uint32_t frame);
+    KhsResult build_model_on_device();
     void keep_in_ring(uint32_t frame);
     void rebuild_bvh(bool swept);
     void refit_bvh(bool swept);

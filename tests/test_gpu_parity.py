@@ -356,3 +356,78 @@ def test_device_root_targets_match_oracle(oracle_lib, gpu_lib, degrees):
     err = np.max(np.abs(qg[fixed] - qo[fixed]))
     print(f"fixed-node target difference {err:.3e}")
     assert err <= 1e-13
+
+
+def _curly_strands(rng, strands):
+    """Curly strands of 3..40 points with repeated points (zero-length
+    segments) scattered in, and some short strands for the extension."""
+    points, offsets = [], [0]
+    for s in range(strands):
+        count = int(rng.integers(3, 41))
+        root = rng.uniform(-0.05, 0.05, 3)
+        step = rng.uniform(0.002, 0.006)
+        angle = np.cumsum(rng.uniform(0.0, 0.4, count))
+        p = root + np.stack([0.004 * np.cos(angle), -step * np.arange(count), 0.004 * np.sin(angle)], axis=1)
+        if count > 4 and s % 3 == 0:
+            k = int(rng.integers(1, count - 1))
+            p = np.insert(p, k, p[k], axis=0)  # a zero-length segment
+        points.append(p)
+        offsets.append(offsets[-1] + len(p))
+    return np.concatenate(points), offsets
+
+
+def test_device_model_build_matches_oracle(oracle_lib, gpu_lib):
+    """The rod model built on the device (discretization, merges, virtual
+    extension, frames, rest strains, masses, mapping) against the oracle's
+    host build, and the masses again after a material change."""
+    print("[gpu] device model build against the oracle", flush=True)
+    rng = np.random.default_rng(7)
+    points, offsets = _curly_strands(rng, 300)
+    solvers = []
+    for lib in (oracle_lib, gpu_lib):
+        desc = kn.default_desc(lib)
+        desc.maximum_element_length = 0.003
+        desc.minimum_dynamic_length = 0.08
+        desc.fixed_root_nodes = 2
+        s = kn.HairSolver(lib, desc)
+        mat = kn.default_material(lib)
+        s.check(s.set_hair(points, offsets, mat))
+        s.check(s.build())
+        solvers.append((s, desc, mat))
+    (o, _, _), (g, _, _) = solvers
+    bo, bg = o.build_stats(), g.build_stats()
+    for f in ("internal_node_count", "element_count", "fixed_node_count", "degree_of_freedom_count",
+              "merged_zero_length_segment_count", "virtual_extension_strand_count", "virtual_extension_node_count"):
+        assert getattr(bo, f) == getattr(bg, f), f
+    assert bo.merged_zero_length_segment_count > 0 and bo.virtual_extension_strand_count > 0
+    assert abs(bo.virtual_extension_rest_length - bg.virtual_extension_rest_length) <= 1e-15 * bo.virtual_extension_rest_length
+    assert np.array_equal(o.mapping(), g.mapping())
+    assert np.max(np.abs(o.internal_positions() - g.internal_positions())) <= 1e-16
+    fo, mo, ro, flo = o.node_model()
+    fg, mg, rg, flg = g.node_model()
+    assert np.array_equal(flo, flg)
+    assert np.max(np.abs(fo - fg)) <= 1e-15
+    assert np.max(np.abs(mo - mg) / np.maximum(mo, 1e-300)) <= 1e-15
+    assert np.max(np.abs(ro - rg) / np.maximum(ro, 1e-300)) <= 1e-15
+    no, so, co, resto = o.element_model()
+    ng, sg, cg, restg = g.element_model()
+    assert np.array_equal(no, ng) and np.array_equal(so, sg) and np.array_equal(co, cg)
+    assert np.max(np.abs(resto - restg)) <= 1e-12
+    for s, desc, mat in solvers:
+        mat.density *= 1.5
+        mat.radius *= 0.75
+        s.check(s.update_runtime(desc, mat))
+    _, mo, ro, _ = o.node_model()
+    _, mg, rg, _ = g.node_model()
+    assert np.max(np.abs(mo - mg) / np.maximum(mo, 1e-300)) <= 1e-15
+    assert np.max(np.abs(ro - rg) / np.maximum(ro, 1e-300)) <= 1e-15
+    # a strand with every point at one place is refused, naming the strand
+    bad = np.concatenate([points[:offsets[2]], np.repeat(points[:1], 3, axis=0)])
+    for lib in (oracle_lib, gpu_lib):
+        s = kn.HairSolver(lib, kn.default_desc(lib))
+        s.check(s.set_hair(bad, [0, offsets[1], offsets[2], offsets[2] + 3], kn.default_material(lib)))
+        assert s.build() == kn.KHS_ERROR_INVALID_HAIR
+        assert "Strand 2" in s.last_error()
+        s.close()
+    for s, _, _ in solvers:
+        s.close()
