@@ -267,11 +267,12 @@ struct DeviceState {
         reduce_async(blocks, kEvalSlot);
         return read_back().eval;
     }
-    // Folds the partials into reduced[slot] without waiting.
-    void reduce_async(uint32_t blocks, int slot)
+    // Folds the partials into reduced[slot] without waiting (and applies
+    // the convergence test when one is given).
+    void reduce_async(uint32_t blocks, int slot, dev::ConvergenceTest test = {0.0, 0.0, 0.0, nullptr})
     {
         hipLaunchKernelGGL(dev::k_reduce, dim3(1), dim3(kBlock), 0, stream, partials.get(), blocks,
-                           reduced.get() + slot, mirror_slot(slot));
+                           reduced.get() + slot, mirror_slot(slot), test);
         HIP_TRY(hipGetLastError());
     }
     // One wait for everything the host reads after a batch of launches.
@@ -304,7 +305,7 @@ struct DeviceState {
     }
     // slots of reduced (3 holds the direction choice)
     static constexpr int kEvalSlot = 0, kSteepSlot = 1, kCcdSlot = 2, kGradSlot = 4, kDirSlot = 5;
-    Buffer<unsigned> stop;  // set by k_check_converged: work enqueued ahead skips itself
+    Buffer<unsigned> stop;  // set by the gradient's convergence test: work enqueued ahead skips itself
     bool skipping = false;  // inside the Newton loop, where that skip applies
     const unsigned *skip() const { return skipping ? stop.get() : nullptr; }
     HostView *host_view = nullptr;  // pinned, coherent
@@ -312,6 +313,7 @@ struct DeviceState {
     double *cache_host = nullptr;      // pinned staging of one cache frame (3P doubles)
 
     uint32_t tiled_blocks() const { return grid_for(size_t(N) + 1, kNodesPerBlock); }
+    uint32_t solve_blocks() const { return grid_for(S, dev::kSolveGroups); }
     uint32_t node_blocks() const { return grid_for(N, kBlock); }
 };
 
@@ -679,7 +681,7 @@ KhsResult Solver::upload_model()
     d.rot_cur.upload(zero3.data(), zero3.size());
     d.rot_pend.upload(zero3.data(), zero3.size());
     d.out3.allocate(3 * std::max<size_t>(N, d.P));
-    const uint32_t max_blocks = std::max(d.tiled_blocks(), d.node_blocks()) + 1;
+    const uint32_t max_blocks = std::max({d.tiled_blocks(), d.node_blocks(), d.solve_blocks()}) + 1;
     d.partials.allocate(max_blocks + kPairBlocks + 16);
     d.reduced.allocate(6);  // reductions, with the direction choice in slot 3 (at most one Partial)
     {
@@ -1075,7 +1077,8 @@ KhsResult Solver::finalize_animation()
 // Enqueues one evaluation of the incremental potential (the line-search
 // variant at alpha d + q_base, alpha read from the device) and its
 // reduction into reduced[kGradSlot] (with the gradient) or reduced[kEvalSlot].
-void Solver::evaluate_async(bool gradient, double dt, bool line_search)
+void Solver::evaluate_async(bool gradient, double dt, bool line_search, unsigned *stop, double abs_tol,
+                            double rel_tol, double initial)
 {
     DeviceState &d = *dev_;
     StepParams p{};
@@ -1129,7 +1132,8 @@ void Solver::evaluate_async(bool gradient, double dt, bool line_search)
     HIP_TRY(hipGetLastError());
     if (gradient) d.toc(t0, d.totals.assembly_ms, d.totals.assembly_launches);
     else d.toc(t0, d.totals.energy_ms, d.totals.energy_launches);
-    d.reduce_async(blocks + (list_pass ? kPairBlocks : 0), gradient ? DeviceState::kGradSlot : DeviceState::kEvalSlot);
+    d.reduce_async(blocks + (list_pass ? kPairBlocks : 0), gradient ? DeviceState::kGradSlot : DeviceState::kEvalSlot,
+                   dev::ConvergenceTest{abs_tol, rel_tol, initial, stop});
 }
 
 Solver::Reduced Solver::unpack(const Partial &r2)
@@ -1205,14 +1209,13 @@ void Solver::make_direction_async()
     const RodView rv = d.rods();
     unsigned *failed = d.solve_failed.get() + d.failed_slot;
     hipEvent_t t0 = d.tic();
-    hipLaunchKernelGGL(dev::k_block_solve, dim3(grid_for(d.S, dev::kSolveGroups)), dim3(kWave), kSolveLdsReserve,
-                       d.stream, rv, d.solve_order.get(), d.Dblk.get(), d.Ublk.get(), d.grad.get(), d.Cp.get(),
-                       d.dp.get(), d.dir.get(), failed, d.solve_failed.get() + (d.failed_slot ^ 1u), d.skip());
+    hipLaunchKernelGGL(dev::k_block_solve, dim3(d.solve_blocks()), dim3(kWave), kSolveLdsReserve, d.stream, rv,
+                       d.solve_order.get(), d.Dblk.get(), d.Ublk.get(), d.grad.get(), d.Cp.get(), d.dp.get(),
+                       d.dir.get(), failed, d.solve_failed.get() + (d.failed_slot ^ 1u), d.partials.get(), d.skip());
     d.failed_slot ^= 1u;
     d.toc(t0, d.totals.solve_ms, d.totals.solve_launches);
-    hipLaunchKernelGGL(dev::k_direction_stats, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, d.grad.get(),
-                       d.dir.get(), (const unsigned *)failed, (const DirectionChoice *)nullptr, d.partials.get());
-    d.reduce_async(d.node_blocks(), DeviceState::kDirSlot);
+    // the solve leaves the direction's statistics in one partial per block
+    d.reduce_async(d.solve_blocks(), DeviceState::kDirSlot);
     // the choice, and the line search's base state q_base = q
     const size_t dofs = 6 * size_t(d.N);
     hipLaunchKernelGGL(dev::k_apply_direction_choice, dim3(grid_for(dofs, kBlock)), dim3(kBlock), 0, d.stream,
@@ -1247,10 +1250,7 @@ KhsResult Solver::optimize_substep(double dt)
         DeviceState &d;
         ~SkipOff() { d.skipping = false; }
     } skip_off{d};
-    evaluate_async(true, dt, false);
-    hipLaunchKernelGGL(dev::k_check_converged, dim3(1), dim3(1), 0, d.stream,
-                       (const Partial *)(d.reduced.get() + DeviceState::kGradSlot), abs_tol, rel_tol, -1.0,
-                       d.stop.get());
+    evaluate_async(true, dt, false, d.stop.get(), abs_tol, rel_tol, -1.0);
     Reduced current = unpack(d.read_back().grad);
     if (!current.feasible) return fail(KHS_ERROR_NOT_CONVERGED, "State is outside the barrier feasible region.");
     const double initial = std::sqrt(current.grad_norm2);
@@ -1377,10 +1377,7 @@ KhsResult Solver::optimize_substep(double dt)
         // The gradient at the accepted iterate; unless the budget ends here,
         // the next iteration's direction, bound and first trial follow before
         // the one wait (they skip themselves if this gradient converged).
-        evaluate_async(true, dt, false);
-        hipLaunchKernelGGL(dev::k_check_converged, dim3(1), dim3(1), 0, d.stream,
-                           (const Partial *)(d.reduced.get() + DeviceState::kGradSlot), abs_tol, rel_tol, initial,
-                           d.stop.get());
+        evaluate_async(true, dt, false, d.stop.get(), abs_tol, rel_tol, initial);
         if (it + 1 < desc_.newton_iterations) {
             make_direction_async();
             bound_and_trial();
@@ -1946,9 +1943,9 @@ KhsResult Solver::test_evaluate(double dt, double *gradient, double *direction, 
     if (direction) {
         const RodView rv = d.rods();
         hipLaunchKernelGGL(dev::k_fill_u32, dim3(1), dim3(64), 0, d.stream, d.flag.get(), size_t(1), 0u);
-        hipLaunchKernelGGL(dev::k_block_solve, dim3(grid_for(d.S, dev::kSolveGroups)), dim3(kWave),
-                           kSolveLdsReserve, d.stream, rv, d.solve_order.get(), d.Dblk.get(), d.Ublk.get(),
-                           d.grad.get(), d.Cp.get(), d.dp.get(), d.dir.get(), d.flag.get(), (unsigned *)nullptr,
+        hipLaunchKernelGGL(dev::k_block_solve, dim3(d.solve_blocks()), dim3(kWave), kSolveLdsReserve, d.stream, rv,
+                           d.solve_order.get(), d.Dblk.get(), d.Ublk.get(), d.grad.get(), d.Cp.get(), d.dp.get(),
+                           d.dir.get(), d.flag.get(), (unsigned *)nullptr, d.partials.get(),
                            (const unsigned *)nullptr);
         HIP_TRY(hipGetLastError());
         unsigned failed = 0;

@@ -111,9 +111,27 @@ __device__ inline void block_reduce(Partial p, Partial *out, Partial *mirror = n
     }
 }
 
-// Folds per-block partials in index order (one block of kBlock threads).
+// The Newton convergence test on the device, the same comparisons as the
+// host's (cuda_backend.cu:662-672): stop = |g| <= abs_tol or |g| / initial
+// <= rel_tol, with initial < 0 meaning this gradient is the substep's first.
+// Work enqueued ahead for the next Newton iteration skips itself on stop.
+struct ConvergenceTest {
+    double abs_tol, rel_tol, initial;
+    unsigned *stop;  // null: no test
+};
+
+__device__ inline void converged_test(const Partial &grad, const ConvergenceTest &test)
+{
+    const double residual = sqrt(grad.sum[kGradNorm2]);
+    const double first = test.initial < 0.0 ? residual : test.initial;
+    const double relative = residual / fmax(first, 1e-30);
+    *test.stop = (residual <= test.abs_tol || relative <= test.rel_tol) ? 1u : 0u;
+}
+
+// Folds per-block partials in index order (one block of kBlock threads),
+// then applies the convergence test to the result when one is given.
 __global__ __launch_bounds__(kBlock) void k_reduce(const Partial *partials, uint32_t count, Partial *out,
-                                                   Partial *mirror)
+                                                   Partial *mirror, ConvergenceTest test)
 {
     Partial acc = empty_partial();
     for (uint32_t i = threadIdx.x; i < count; i += kBlock) {
@@ -128,6 +146,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const Partial *partials, uint
         acc.failed |= o.failed;
     }
     block_reduce(acc, out, mirror);
+    if (test.stop && threadIdx.x == 0) converged_test(*out, test);
 }
 
 
@@ -1089,7 +1108,12 @@ __device__ __forceinline__ void buffer_store(__amdgpu_buffer_rsrc_t r, uint32_t 
 // solve. A first pass without row exchanges stands whenever partial pivoting
 // would exchange nothing in any strand of the wavefront (the usual case for
 // these diagonally dominant blocks); otherwise the pivoting pass reruns.
-// The back substitution runs row-wise on lanes 0..5.
+// The back substitution runs row-wise on lanes 0..5; it also gathers the
+// direction's statistics for the choice that follows (cuda_backend.cu:
+// 623-630): g.d, each row lane summing its own products over the strand's
+// nodes, and the largest free-node translation |d_xyz| (its square kept,
+// one square root at the end), one Partial per block (a skipped launch
+// writes a failed one, so nothing is chosen from it).
 //
 // Memory: a ring of kSolveAhead (forward) or kBackAhead (back substitution)
 // nodes' inputs; a slot is refilled with the node that many steps later as
@@ -1115,11 +1139,16 @@ struct ForwardSlot {
 __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t *order, const double *Dblk,
                                                        const double *Ublk, const double *grad, double *Cp,
                                                        double *dp, double *dir, unsigned *failed,
-                                                       unsigned *next_failed, const unsigned *stop)
+                                                       unsigned *next_failed, Partial *partials, const unsigned *stop)
 {
     // the next solve's flag starts at zero (the two alternate)
     if (blockIdx.x == 0 && threadIdx.x == 0 && next_failed) *next_failed = 0;
-    if (stop && *stop) return;
+    if (stop && *stop) {
+        Partial skipped = empty_partial();
+        skipped.failed = 1;
+        block_reduce(skipped, partials + blockIdx.x);
+        return;
+    }
     __shared__ SolveShared sh;
     const int g = threadIdx.x / kSolveLanes, j = threadIdx.x % kSolveLanes;
     const bool acol = j < 6, icol = j >= 6 && j < 12, vlane = j == 12;
@@ -1312,7 +1341,7 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
     // Back substitution x_n = d'_n - C'_n x_{n+1}, lane r < 6 owning row r.
     const int r = acol ? j : 0;
     struct BackSlot {
-        double c[6], d;
+        double c[6], d, g;
     };
     auto fetch_back = [&](BackSlot &b, uint32_t t) __attribute__((always_inline)) {
         const uint32_t n = end - 1 - t;
@@ -1321,11 +1350,12 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
 #pragma unroll
         for (int k = 0; k < 6; ++k) b.c[k] = buffer_load(rC, oc + 8 * k);
         b.d = buffer_load(rP, live ? 48 * n + 8 * r : kNoAccess);
+        b.g = buffer_load(rG, live ? 48 * n + 8 * r : kNoAccess);
     };
     BackSlot back[kBackAhead];
 #pragma unroll
     for (int a = 0; a < kBackAhead; ++a) fetch_back(back[a], a);
-    double next = 0.0;
+    double next = 0.0, slope = 0.0, reach2 = 0.0;
     auto backward = [&](BackSlot &b, uint32_t t) __attribute__((always_inline)) {
         const uint32_t n = end - 1 - t;
         double crow[6];
@@ -1335,10 +1365,26 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
         if (t > 0) static_for<6>([&](auto K) { value -= crow[K] * row_bcast<K>(next); });
         buffer_store(rX, (t < len && ok && acol) ? 48 * n + 8 * j : kNoAccess, value);
         next = value;
+        // this row's part of g.d; |d_xyz|^2 of node n on lane 0
+        const double sq = value * value;
+        const double t2 = (sq + row_bcast<1>(sq)) + row_bcast<2>(sq);
+        if (t < len) {
+            slope += b.g * value;
+            reach2 = fmax(reach2, t2);
+        }
         fetch_back(b, t + kBackAhead);
     };
     for (uint32_t t0 = 0; t0 < steps; t0 += kBackAhead)
         static_for<kBackAhead>([&](auto A) { backward(back[A], t0 + A); });
+    double row_slope = 0.0;
+    static_for<6>([&](auto K) { row_slope += row_bcast<K>(acol ? slope : 0.0); });
+    Partial acc = empty_partial();
+    if (j == 0 && gi < m.strand_count) {
+        acc.sum[kSlope] = row_slope;
+        acc.max_translation = sqrt(reach2);
+        acc.failed = ok ? 0u : 1u;
+    }
+    block_reduce(acc, partials + blockIdx.x);
 }
 
 // Slope g.d and the largest translation of a free node (host then limits
@@ -1440,20 +1486,6 @@ __global__ void k_choose_alpha(const Partial *ccd, int has_collider, double *alp
 {
     if (threadIdx.x || blockIdx.x) return;
     *alpha = step_from_ccd(ccd, has_collider);
-}
-
-// The Newton convergence test on the device, the same comparisons as the
-// host's (cuda_backend.cu:662-672): stop = |g| <= abs_tol or |g| / initial
-// <= rel_tol, with initial < 0 meaning this gradient is the substep's first.
-// Work enqueued ahead for the next Newton iteration skips itself on stop.
-__global__ void k_check_converged(const Partial *grad, double abs_tol, double rel_tol, double initial,
-                                  unsigned *stop)
-{
-    if (threadIdx.x || blockIdx.x) return;
-    const double residual = sqrt(grad->sum[kGradNorm2]);
-    const double first = initial < 0.0 ? residual : initial;
-    const double relative = residual / fmax(first, 1e-30);
-    *stop = (residual <= abs_tol || relative <= rel_tol) ? 1u : 0u;
 }
 
 __global__ void k_set_double(double *x, double v)
