@@ -985,63 +985,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GRAD ? 2
 }
 
 // ---------------------------------------------------------- linear solve
-// Gauss-Jordan inverse with partial pivoting after the diagonal shift
-// max(1e-12 max|a_ii|, 1e-24) (cuda_backend.cu:855-864). Every index is a
-// compile-time constant, so both matrices stay in registers; the pivot row
-// exchange is a select.
-__device__ inline bool inverse6(double *a, double *inv)
-{
-#pragma unroll
-    for (int i = 0; i < 36; ++i) inv[i] = (i % 7 == 0) ? 1.0 : 0.0;
-    double scale = 0.0;
-#pragma unroll
-    for (int i = 0; i < 6; ++i) scale = fmax(scale, fabs(a[7 * i]));
-    const double shift = fmax(1.0e-12 * scale, 1.0e-24);
-#pragma unroll
-    for (int i = 0; i < 6; ++i) a[7 * i] += shift;
-#pragma unroll
-    for (int col = 0; col < 6; ++col) {
-        int pivot = col;
-        double best = fabs(a[7 * col]);
-#pragma unroll
-        for (int row = col + 1; row < 6; ++row) {
-            const double v = fabs(a[6 * row + col]);
-            if (v > best) { best = v; pivot = row; }
-        }
-        if (best < 1.0e-30 || !isfinite(best)) return false;
-#pragma unroll
-        for (int row = col + 1; row < 6; ++row) {
-            const bool sw = pivot == row;
-#pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                const double x = a[6 * col + k], y = a[6 * row + k];
-                a[6 * col + k] = sw ? y : x;
-                a[6 * row + k] = sw ? x : y;
-                const double u = inv[6 * col + k], w = inv[6 * row + k];
-                inv[6 * col + k] = sw ? w : u;
-                inv[6 * row + k] = sw ? u : w;
-            }
-        }
-        const double invp = 1.0 / a[7 * col];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            a[6 * col + k] *= invp;
-            inv[6 * col + k] *= invp;
-        }
-#pragma unroll
-        for (int row = 0; row < 6; ++row) {
-            if (row == col) continue;
-            const double f = a[6 * row + col];
-#pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                a[6 * row + k] -= f * a[6 * col + k];
-                inv[6 * row + k] -= f * inv[6 * col + k];
-            }
-        }
-    }
-    return true;
-}
-
 // Value of `v` in lane L of this lane's 16-lane row (DPP row_newbcast).
 // (one 64-bit DPP move: every lane of the row has a valid source lane)
 template <int L>
@@ -1101,11 +1044,12 @@ __device__ __forceinline__ void buffer_store(__amdgpu_buffer_rsrc_t r, uint32_t 
 // the previous node's U (for the Schur complement) and the finished inverse
 // (for C' = inv U and d' = inv rhs) are shared through LDS. Each entry is
 // computed with the sequential algorithm's operations in its order:
-// products sum k = 0..5 from zero, and the Gauss-Jordan inverse (diagonal
+// products sum k = 0..5 from zero by fused multiply-adds, as the
+// reference's CUDA build contracts them, and the Gauss-Jordan inverse (diagonal
 // shift max(1e-12 max|a_ii|, 1e-24), partial pivoting, pivot below 1e-30
-// fails; cuda_backend.cu:855-864) applies each column step's row operations
-// to all columns at once, so the result is bit-identical to a one-thread
-// solve. A first pass without row exchanges stands whenever partial pivoting
+// fails; cuda_backend.cu:855-864; each row update one fused multiply-add)
+// applies each column step's row operations to all columns at once, so the
+// result is bit-identical to a one-thread solve. A first pass without row exchanges stands whenever partial pivoting
 // would exchange nothing in any strand of the wavefront (the usual case for
 // these diagonally dominant blocks); otherwise the pivoting pass reruns.
 // The back substitution runs row-wise on lanes 0..5; it also gathers the
@@ -1217,7 +1161,7 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
             for (int r = 0; r < 6; ++r) {
                 double sum = 0.0;
 #pragma unroll
-                for (int k = 0; k < 6; ++k) sum += up[6 * r + k] * p[k];
+                for (int k = 0; k < 6; ++k) sum = fma(up[6 * r + k], p[k], sum);
                 col[r] -= sum;
             }
         }
@@ -1264,7 +1208,7 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
             col[C] *= invp;
 #pragma unroll
             for (int r = 0; r < 6; ++r)
-                if (r != C) col[r] -= cv[r] * col[C];
+                if (r != C) col[r] = fma(-cv[r], col[C], col[r]);
         });
         if (__builtin_amdgcn_ballot_w64(exchange)) {
             singular = false;
@@ -1299,7 +1243,7 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
                 col[C] *= invp;
 #pragma unroll
                 for (int r = 0; r < 6; ++r)
-                    if (r != C) col[r] -= cv[r] * col[C];
+                    if (r != C) col[r] = fma(-cv[r], col[C], col[r]);
             });
         }
         if (singular && t < len) ok = false;
@@ -1320,7 +1264,7 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
 #pragma unroll
         for (int k = 0; k < 6; ++k)
 #pragma unroll
-            for (int r = 0; r < 6; ++r) res[r] += inv[6 * k + r] * w[k];
+            for (int r = 0; r < 6; ++r) res[r] = fma(inv[6 * k + r], w[k], res[r]);
 #pragma unroll
         for (int r = 0; r < 6; ++r) p[r] = acol ? (t + 1 < len ? res[r] : 0.0) : (vlane ? res[r] : 0.0);
         const bool keep = t < len && ok;
@@ -1362,7 +1306,7 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
 #pragma unroll
         for (int k = 0; k < 6; ++k) crow[k] = b.c[k];
         double value = b.d;
-        if (t > 0) static_for<6>([&](auto K) { value -= crow[K] * row_bcast<K>(next); });
+        if (t > 0) static_for<6>([&](auto K) { value = fma(-crow[K], row_bcast<K>(next), value); });
         buffer_store(rX, (t < len && ok && acol) ? 48 * n + 8 * j : kNoAccess, value);
         next = value;
         // this row's part of g.d; |d_xyz|^2 of node n on lane 0

@@ -132,7 +132,9 @@ void rod_strains(const Dual<N> *q, const M3 &fi, const M3 &fj, const Element &e,
 
 // Gauss-Jordan inverse of a 6x6 block with partial pivoting after adding
 // max(1e-12 * max|a_ii|, 1e-24) to the diagonal; fails on a pivot below
-// 1e-30 or a non-finite pivot (cuda_backend.cu:855-864).
+// 1e-30 or a non-finite pivot (cuda_backend.cu:855-864). Each row update
+// a - f b is one fused multiply-add, as the reference's CUDA build contracts
+// it (nvcc's default); so are the sums of products in the block solve.
 bool inverse6(const double *src, double *inv)
 {
     double a[36];
@@ -158,8 +160,8 @@ bool inverse6(const double *src, double *inv)
             if (row != col) {
                 const double f = a[6 * row + col];
                 for (int k = 0; k < 6; ++k) {
-                    a[6 * row + k] -= f * a[6 * col + k];
-                    inv[6 * row + k] -= f * inv[6 * col + k];
+                    a[6 * row + k] = std::fma(-f, a[6 * col + k], a[6 * row + k]);
+                    inv[6 * row + k] = std::fma(-f, inv[6 * col + k], inv[6 * row + k]);
                 }
             }
     }
@@ -1411,11 +1413,11 @@ private:
                     for (int r = 0; r < 6; ++r) {
                         for (int c = 0; c < 6; ++c) {
                             double sum = 0;
-                            for (int k = 0; k < 6; ++k) sum += lower[6 * k + r] * pc[6 * k + c];
+                            for (int k = 0; k < 6; ++k) sum = std::fma(lower[6 * k + r], pc[6 * k + c], sum);
                             b[6 * r + c] -= sum;
                         }
                         double sum = 0;
-                        for (int k = 0; k < 6; ++k) sum += lower[6 * k + r] * pd[k];
+                        for (int k = 0; k < 6; ++k) sum = std::fma(lower[6 * k + r], pd[k], sum);
                         rhs[r] -= sum;
                     }
                 }
@@ -1426,7 +1428,7 @@ private:
                     for (int r = 0; r < 6; ++r)
                         for (int c = 0; c < 6; ++c) {
                             double sum = 0;
-                            for (int k = 0; k < 6; ++k) sum += inv[6 * r + k] * up[6 * k + c];
+                            for (int k = 0; k < 6; ++k) sum = std::fma(inv[6 * r + k], up[6 * k + c], sum);
                             cc[6 * r + c] = sum;
                         }
                 } else {
@@ -1434,7 +1436,7 @@ private:
                 }
                 for (int r = 0; r < 6; ++r) {
                     double sum = 0;
-                    for (int k = 0; k < 6; ++k) sum += inv[6 * r + k] * rhs[k];
+                    for (int k = 0; k < 6; ++k) sum = std::fma(inv[6 * r + k], rhs[k], sum);
                     cd[r] = sum;
                 }
             }
@@ -1442,7 +1444,7 @@ private:
                 for (int r = 0; r < 6; ++r) {
                     double value = dp_[6 * n + r];
                     if (n + 1 < end)
-                        for (int k = 0; k < 6; ++k) value -= C_[36 * n + 6 * r + k] * d_[6 * (n + 1) + k];
+                        for (int k = 0; k < 6; ++k) value = std::fma(-C_[36 * n + 6 * r + k], d_[6 * (n + 1) + k], value);
                     d_[6 * n + r] = value;
                 }
         });
