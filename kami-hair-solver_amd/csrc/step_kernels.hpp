@@ -226,6 +226,19 @@ __device__ __forceinline__ bool box_covered(const double *box, V3 lo, V3 hi, dou
     return ok;
 }
 
+// Where the skin holds more triangles than a list can (fine or sliver
+// triangles, as where a mesh's rings meet at a pole), a list is gathered
+// again for a box grown by only kThinSkin of the skin, so the element keeps
+// a list (refreshed more often) instead of every query walking the
+// hierarchy. thin_box shrinks a box grown by pad + skin to that.
+constexpr double kThinSkin = 1.0 / 16.0;
+__device__ __forceinline__ void thin_box(double skin, V3 &lo, V3 &hi)
+{
+    const double s = skin * (1.0 - kThinSkin);
+    lo = mk(lo.x + s, lo.y + s, lo.z + s);
+    hi = mk(hi.x - s, hi.y - s, hi.z - s);
+}
+
 // Before a query pass: every contact element whose query box (grown by
 // pad, the largest pad of the passes) is not inside its cached box walks
 // the hierarchy for that box grown by the skin and stores the overlapping
@@ -247,11 +260,19 @@ __global__ void k_contact_refresh(RodView m, ColliderView col, ContactCache cc, 
     V3 blo = mk(lo.x - grow, lo.y - grow, lo.z - grow), bhi = mk(hi.x + grow, hi.y + grow, hi.z + grow);
     uint32_t *tris = cc.tris + size_t(L) * kCacheCapacity;
     uint32_t n = 0;
-    bvh_visit(col.bvh, internal_count(col), col.leaf_box, blo, bhi, 0.0, [&](uint32_t t) {
-        if (n < kCacheCapacity) tris[n] = t;
-        ++n;
-        return n <= kCacheCapacity;  // an overflowed list is not used
-    });
+    auto gather = [&] {
+        n = 0;
+        bvh_visit(col.bvh, internal_count(col), col.leaf_box, blo, bhi, 0.0, [&](uint32_t t) {
+            if (n < kCacheCapacity) tris[n] = t;
+            ++n;
+            return n <= kCacheCapacity;  // an overflowed list is not used
+        });
+    };
+    gather();
+    if (n > kCacheCapacity) {
+        thin_box(cc.skin, blo, bhi);
+        gather();
+    }
     if (n == 0) {
         // Nothing near: keep a box eight skins wider if it is empty too, so
         // hair moving through free space rarely gathers again.
@@ -423,8 +444,12 @@ __global__ __launch_bounds__(kBlock) void k_contact_check(RodView m, ColliderVie
         const V3 lo = mk(__shfl(nlo.x, src, kWave), __shfl(nlo.y, src, kWave), __shfl(nlo.z, src, kWave));
         const V3 hi = mk(__shfl(nhi.x, src, kWave), __shfl(nhi.y, src, kWave), __shfl(nhi.z, src, kWave));
         uint32_t *tris = cc.tris + size_t(E) * kCacheCapacity;
-        const uint32_t n = wave_gather(col, lo, hi, tris, kCacheCapacity, queue, false);
         V3 blo = lo, bhi = hi;
+        uint32_t n = wave_gather(col, blo, bhi, tris, kCacheCapacity, queue, false);
+        if (n > kCacheCapacity) {
+            thin_box(cc.skin, blo, bhi);
+            n = wave_gather(col, blo, bhi, tris, kCacheCapacity, queue, false);
+        }
         if (n == 0) {
             const double far = 7.0 * cc.skin;
             const V3 flo = mk(lo.x - far, lo.y - far, lo.z - far), fhi = mk(hi.x + far, hi.y + far, hi.z + far);
@@ -464,9 +489,17 @@ __global__ __launch_bounds__(kBlock) void k_contact_gather(const uint32_t *flags
     for (uint32_t j = wave; j < pending; j += waves) {
         const uint32_t L = cc.pending[j];
         double *box = cc.box + 6 * size_t(L);
-        const V3 lo = mk(box[0], box[1], box[2]), hi = mk(box[3], box[4], box[5]);
+        V3 lo = mk(box[0], box[1], box[2]), hi = mk(box[3], box[4], box[5]);
         uint32_t *tris = cc.tris + size_t(L) * kCacheCapacity;
-        const uint32_t n = wave_gather(col, lo, hi, tris, kCacheCapacity, queue, false);
+        uint32_t n = wave_gather(col, lo, hi, tris, kCacheCapacity, queue, false);
+        if (n > kCacheCapacity) {
+            thin_box(cc.skin, lo, hi);
+            n = wave_gather(col, lo, hi, tris, kCacheCapacity, queue, false);
+            if (lane == 0) {
+                store3(box, lo);
+                store3(box + 3, hi);
+            }
+        }
         if (n == 0) {
             const double far = 7.0 * cc.skin;
             const V3 flo = mk(lo.x - far, lo.y - far, lo.z - far), fhi = mk(hi.x + far, hi.y + far, hi.z + far);
@@ -939,8 +972,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GRAD ? 2
             acc.sum[kDirNorm2] += dd;
         }
         const bool write_u = owner && elem && !fixed_i && !fixed_j;
-        // The blocks are staged per wavefront in LDS and written out as the
-        // tile's contiguous 36-entry records, whole lines at a time: first D
+        // Each element term L0 w_s J_s[a] J_s[b] is added by a fused
+        // multiply-add, as in the oracle. The blocks are staged per
+        // wavefront in LDS and written out as the tile's contiguous 36-entry
+        // records, whole lines at a time: first D
         // (inertia, both elements' j- and i-parts, both contact terms), then
         // U (this element's coupling and contact term).
         __shared__ double stage[kTilesPerBlock][kTile * kStageStride];
@@ -955,8 +990,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GRAD ? 2
                 double hj = 0.0, hi = 0.0;
 #pragma unroll
                 for (int s = 0; s < 6; ++s) {
-                    if (!structural_zero(s, 6 + a) && !structural_zero(s, 6 + b)) hj = hj + (Lw[s] * jac(s, 6 + a)) * jac(s, 6 + b);
-                    if (!structural_zero(s, a) && !structural_zero(s, b)) hi = hi + (Lw[s] * jac(s, a)) * jac(s, b);
+                    if (!structural_zero(s, 6 + a) && !structural_zero(s, 6 + b))
+                        hj = fma(Lw[s] * jac(s, 6 + a), jac(s, 6 + b), hj);
+                    if (!structural_zero(s, a) && !structural_zero(s, b)) hi = fma(Lw[s] * jac(s, a), jac(s, b), hi);
                 }
                 const double left = from_lane_below(hj);
                 double v = (a == b ? dnode[a] : 0.0) + left;
@@ -974,7 +1010,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GRAD ? 2
                 double hu = 0.0;
 #pragma unroll
                 for (int s = 0; s < 6; ++s)
-                    if (!structural_zero(s, a) && !structural_zero(s, 6 + b)) hu = hu + (Lw[s] * jac(s, a)) * jac(s, 6 + b);
+                    if (!structural_zero(s, a) && !structural_zero(s, 6 + b)) hu = fma(Lw[s] * jac(s, a), jac(s, 6 + b), hu);
                 if (a < 3 && b < 3) hu = hu + (curvature * dg[a]) * dg[3 + b];
                 if (owner) st[slot * kStageStride + 6 * a + b] = write_u ? hu : 0.0;
             }

@@ -1273,8 +1273,9 @@ private:
     }
 
     // Elastic energy of an element (Dual<12> strains, Gauss-Newton blocks
-    // L0 sum_s w_s J_s^T J_s) and its log-barrier contact against the
-    // closest collider triangle (cuda_backend.cu:899-939).
+    // L0 sum_s w_s J_s^T J_s, each term added by a fused multiply-add as
+    // the reference's CUDA build contracts it) and its log-barrier contact
+    // against the closest collider triangle (cuda_backend.cu:899-939).
     void element_term(uint32_t k, bool grad, std::vector<uint32_t> &cand, Stats &st)
     {
         const Element &e = elements_[k];
@@ -1302,7 +1303,7 @@ private:
             for (int a = 0; a < 12; ++a)
                 for (int b = 0; b < 12; ++b) {
                     double h = 0.0;
-                    for (int s = 0; s < 6; ++s) h += e.rest_length * w[s] * strain[s].d[a] * strain[s].d[b];
+                    for (int s = 0; s < 6; ++s) h = std::fma(e.rest_length * w[s] * strain[s].d[a], strain[s].d[b], h);
                     if (a < 6 && b < 6) { if (!ni.fixed) D_[36 * e.i + 6 * a + b] += h; }
                     else if (a >= 6 && b >= 6) { if (!nj.fixed) D_[36 * e.j + 6 * (a - 6) + (b - 6)] += h; }
                     else if (a < 6 && b >= 6 && !ni.fixed && !nj.fixed) U_[36 * e.i + 6 * a + (b - 6)] += h;
