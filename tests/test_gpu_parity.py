@@ -5,6 +5,8 @@ energy, gradient and Gauss-Newton direction within 1e-10 relative
 (norm-wise); broad-phase candidate pairs bit-exact; positions after one frame
 within 1e-7 m; minimum gap above the feasibility threshold.
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -431,3 +433,52 @@ def test_device_model_build_matches_oracle(oracle_lib, gpu_lib):
         s.close()
     for s, _, _ in solvers:
         s.close()
+
+
+def test_dense_pole_contact_matches_oracle(oracle_lib, gpu_lib):
+    """Hair lying across the pole of a UV sphere, where 160 sliver triangles
+    meet: candidate lists of more than 128 triangles, gathered again with
+    the thin skin where the skin holds more than a list can. Two frames
+    compared node by node with the oracle's exhaustive search."""
+    print("[gpu] dense pole contact frames", flush=True)
+    R = 0.05
+    verts, tris = grooms.uv_ellipsoid(24, 160, (R, R, R), (0.0, 0.0, 0.0))
+    strands, offsets = [], [0]
+    for k in range(8):
+        phi = math.pi * k / 8
+        u = np.array([math.cos(phi), math.sin(phi), 0.0])
+        s = np.linspace(-0.012, 0.012, 9)[:, None] * u + np.array([0.0, 0.0, R + 0.0004])
+        strands.append(s)
+        offsets.append(offsets[-1] + len(s))
+    pts = np.concatenate(strands)
+    out = []
+    for lib in (oracle_lib, gpu_lib):
+        desc = kn.default_desc(lib)
+        desc.substeps = 4
+        desc.fixed_root_nodes = 1
+        desc.maximum_element_length = 0.01
+        desc.newton_iterations = 40
+        mat = kn.default_material(lib)
+        mat.radius = 2.0e-4
+        mat.young_modulus = 1.0e6
+        mat.contact_stiffness = 100.0
+        mat.barrier_distance = 2.0e-3
+        mat.mass_damping = 2.0
+        s = kn.HairSolver(lib, desc)
+        s.check(s.set_hair(pts, offsets, mat))
+        s.check(s.set_collider(verts, tris))
+        s.check(s.build())
+        s.check(s.step(1.0 / 60.0))
+        lists = s.contact_list_stats() if lib is gpu_lib else None
+        first = s.step_stats()
+        s.check(s.step(1.0 / 60.0))
+        out.append((s.internal_positions(), first, lists))
+        s.close()
+    err = float(np.max(np.abs(out[0][0] - out[1][0])))
+    lists = out[1][2]
+    print(f"max position error {err:.3e} m; active oracle {out[0][1].active_contact_count} "
+          f"gpu {out[1][1].active_contact_count}; longest list {lists.longest}, overflowed {lists.overflowed}")
+    assert err <= 1e-7
+    assert out[1][1].active_contact_count > 0
+    assert out[1][1].minimum_gap > 1e-7
+    assert lists.longest > 128
