@@ -140,7 +140,7 @@ typedef struct KhsxContactListStats {
     uint64_t overflowed;              /* lists over capacity (queries walk the tree) */
     uint64_t entries;                 /* triangles over all lists within capacity */
     uint64_t longest;                 /* longest list within capacity */
-    uint64_t length_histogram[9];     /* lengths 1, 2-3, 4-7, ..., 128-255, 256 */
+    uint64_t length_histogram[9];     /* lengths 1, 2-3, 4-7, ..., 128-255, 256 and up */
     uint64_t root_nonempty;           /* of nonempty: elements whose left node is fixed */
     uint64_t root_entries;            /* of entries: theirs */
 } KhsxContactListStats;

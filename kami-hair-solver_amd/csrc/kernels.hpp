@@ -56,7 +56,7 @@ struct ColliderView {
 // again, gathering for their query box grown by a skin. A covered query thus
 // finds in the list every triangle the hierarchy would report. Lists longer
 // than kCacheCapacity are marked overflowed and their queries walk the tree.
-constexpr uint32_t kCacheCapacity = 256;
+constexpr uint32_t kCacheCapacity = 512;
 constexpr uint32_t kCacheOverflow = 0xffffffffu;
 constexpr uint32_t kNoTriangle = 0xffffffffu;
 constexpr uint32_t kPairBlocks = 1024;  // grid of the per-element narrow-phase passes
