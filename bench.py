@@ -36,7 +36,8 @@ HBM_PEAK_GBPS = 8000.0
 # (144 B), node constants R_ref, m, m_rot (88 B) and element constants L0,
 # rest shear, rest curvature (56 B) and write 48 B (gradient, or the trial
 # state). The block solve's only algorithmic traffic is the direction it
-# writes (48 B): the Gauss-Newton blocks are intermediates.
+# writes (48 B): the Gauss-Newton blocks are intermediates, and the gradient
+# it reads for the slope g.d is the assembly's output, not counted twice.
 KERNEL_CLASSES = {
     "assembly": ("k_evaluate<gradient> with its contact list pass (gradient + Gauss-Newton blocks)", 336),
     "energy": ("k_evaluate<line search> with its contact list pass (trial energy)", 336),
