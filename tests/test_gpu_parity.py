@@ -6,6 +6,7 @@ energy, gradient and Gauss-Newton direction within 1e-10 relative
 within 1e-7 m; minimum gap above the feasibility threshold.
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -208,11 +209,12 @@ def _grid_plane(n, h):
     return verts, np.array(tris, np.uint32)
 
 
-@pytest.mark.parametrize("quad", [5.0e-3, 1.0e-3])
+@pytest.mark.parametrize("quad", [5.0e-3, 1.0e-3, 4.0e-4])
 def test_draped_strand_on_grid_matches_oracle(oracle_lib, gpu_lib, quad):
-    """A strand falls onto a tessellated plane. With 1 mm quads an element's
-    padded box overlaps more triangles than a per-element candidate list
-    holds, so both the list path and the hierarchy path are exercised."""
+    """A strand falls onto a tessellated plane. With 5 mm quads the lists are
+    short; with 1 mm quads an element's skin holds several hundred triangles
+    (long lists, thin-skin regathers); with 0.4 mm quads even the thin box
+    holds more than a list can, so those elements take the hierarchy path."""
     print(f"[gpu] draped strand on grid, quad {quad}", flush=True)
     verts, tris = _grid_plane(int(round(0.08 / quad)), quad)
     pts = np.array([[-0.03 + 0.01 * k, 0.0013 * k, 0.006] for k in range(7)])
@@ -239,20 +241,24 @@ def test_draped_strand_on_grid_matches_oracle(oracle_lib, gpu_lib, quad):
         s.check(s.build())
         s.check(s.set_convergence_mode(1))
         cand, active = [], []
-        for _ in range(4):
+        for _ in range(4 if quad > 5.0e-4 else 2):  # the hierarchy path is slow: two frames
             s.check(s.step(1.0 / 30.0))
             st = s.step_stats()
             cand.append(st.contact_candidate_count)
             active.append(st.active_contact_count)
-        out.append((s.internal_positions(), cand, active, s.step_stats()))
+        lists = s.contact_list_stats() if lib is gpu_lib else None
+        out.append((s.internal_positions(), cand, active, s.step_stats(), lists))
     err = float(np.max(np.abs(out[0][0] - out[1][0])))
+    lists = out[1][4]
     print(f"max position error {err:.3e} m; candidates oracle {out[0][1]} gpu {out[1][1]}; "
-          f"active oracle {out[0][2]} gpu {out[1][2]}")
+          f"active oracle {out[0][2]} gpu {out[1][2]}; longest list {lists.longest}, overflowed {lists.overflowed}")
     assert err <= 1e-7
     assert out[1][1] == out[0][1]
     assert out[1][2] == out[0][2]
     assert sum(out[1][2]) > 0
     assert out[1][3].minimum_gap > 1e-7
+    if quad < 5.0e-4:
+        assert lists.overflowed > 0
 
 
 def test_trust_radius_frames_match_oracle(oracle_lib, gpu_lib):
@@ -482,3 +488,21 @@ def test_dense_pole_contact_matches_oracle(oracle_lib, gpu_lib):
     assert out[1][1].active_contact_count > 0
     assert out[1][1].minimum_gap > 1e-7
     assert lists.longest > 128
+
+
+def test_sphere_scene_matches_golden(gpu_lib):
+    """The GPU against the oracle's committed results on the 64-strand sphere
+    scene (tests/golden/config1_frame.npz), without running the oracle:
+    positions after one frame within 1e-7 m with the same Newton count, and
+    objective and gradient at the perturbed state within 1e-10 relative."""
+    import golden.make_golden as make_golden
+
+    print("[gpu] sphere scene against the committed oracle results", flush=True)
+    want = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "config1_frame.npz"))
+    got = make_golden.compute(gpu_lib)
+    err = float(np.max(np.abs(got["positions"] - want["positions"])))
+    print(f"max position error {err:.3e} m; newton {got['newton'][0]} (golden {want['newton'][0]})")
+    assert err <= 1e-7
+    assert got["newton"][0] == want["newton"][0]
+    assert abs(got["objective"][0] - want["objective"][0]) <= 1e-10 * abs(want["objective"][0])
+    assert _rel(got["gradient"], want["gradient"]) <= 1e-10
