@@ -67,9 +67,11 @@ def open_cache_for_calculation(temporary: Path, start: int, end: int, point_coun
 def write_cache_frame(file, solver: kn.HairSolver) -> None:
     """Appends the solver's current original-point positions as one frame
     record (addon.py:580-584). Python's buffer is flushed first so the
-    library's bytes follow everything already written."""
+    library's bytes follow everything already written, and the file object
+    is moved to the new end afterwards (it caches its position)."""
     file.flush()
     solver.check(solver.write_cache_frame(file.fileno()))
+    file.seek(0, os.SEEK_END)
 
 
 def read_cache_frame(path: Path, frame: int) -> np.ndarray | None:
