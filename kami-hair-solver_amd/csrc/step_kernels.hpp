@@ -535,6 +535,81 @@ __device__ __forceinline__ void group_closest(double &d, uint32_t &t)
     }
 }
 
+// Closest triangle to a segment among those whose box overlaps the segment's
+// box grown by pad, by a wavefront-cooperative walk of the hierarchy: the
+// same overlap tests as bvh_visit (so the same candidate set), breadth-first
+// through a ring of qcap entries, each lane computing the distances of the
+// leaves it finds. Every lane gets the closest (distance, triangle) and the
+// candidate count; false if the ring would overflow (nothing is then known).
+// All lanes call it with the same segment; loop control is wave-uniform.
+__device__ inline bool wave_closest(const ColliderView &col, V3 p0, V3 p1, double pad, volatile int *queue,
+                                    uint32_t qcap, double &best_d, uint32_t &best_t, uint32_t &count)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const unsigned long long below = __lanemask_lt();
+    const int ic = internal_count(col);
+    const V3 lo = vmin(p0, p1), hi = vmax(p0, p1);
+    double bd = 1.0e300;
+    uint32_t bt = kNoTriangle, n = 0, head = 0, tail = 1;
+    if (lane == 0) queue[0] = 0;
+    bool ok = true;
+    while (head < tail && head <= uint32_t(ic) + 1u) {
+        const uint32_t take = (tail - head) < uint32_t(kWave) ? (tail - head) : uint32_t(kWave);
+        const int node = uint32_t(lane) < take ? queue[(head + lane) % qcap] : -1;
+        head += take;
+        bool push[2] = {false, false};
+        int child[2] = {-1, -1};
+        if (node >= 0) {
+            const BvhNode &nd = col.bvh[node];
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const V3 bl = mk(nd.box[6 * c], nd.box[6 * c + 1], nd.box[6 * c + 2]);
+                const V3 bu = mk(nd.box[6 * c + 3], nd.box[6 * c + 4], nd.box[6 * c + 5]);
+                const int ch = nd.child[c];
+                child[c] = ch;
+                if (!overlap(lo, hi, bl, bu, pad) || ch >= ic) continue;
+                if (ch >= 0) {
+                    push[c] = true;
+                    continue;
+                }
+                const uint32_t t = uint32_t(-1 - ch);
+                if (col.leaf_box) {
+                    const double *b = col.leaf_box + 6 * size_t(t);
+                    if (!overlap(lo, hi, mk(b[0], b[1], b[2]), mk(b[3], b[4], b[5]), pad)) continue;
+                }
+                ++n;
+                const uint32_t *t3 = col.triangles + 3 * size_t(t);
+                const double d =
+                    closest_segment_triangle(p0, p1, vertex(col, t3[0]), vertex(col, t3[1]), vertex(col, t3[2])).distance;
+                if (d < bd || (d == bd && t < bt)) {
+                    bd = d;
+                    bt = t;
+                }
+            }
+        }
+        const unsigned long long m0 = __ballot(push[0]), m1 = __ballot(push[1]);
+        if (tail - head + __popcll(m0) + __popcll(m1) > qcap) {
+            ok = false;
+            break;
+        }
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const unsigned long long pm = c == 0 ? m0 : m1;
+            if (push[c]) queue[(tail + __popcll(pm & below)) % qcap] = child[c];
+            tail += __popcll(pm);
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    group_closest<kWave>(bd, bt);
+    uint32_t total = n;
+#pragma unroll
+    for (int o = kWave / 2; o >= 1; o >>= 1) total += uint32_t(__shfl_xor(int(total), o, kWave));
+    best_d = bd;
+    best_t = bt;
+    count = total;
+    return ok;
+}
+
 // Narrow phase, a group of kClosestLanes lanes per listed element: the
 // group box-tests its list kClosestLanes entries at a time, packs the
 // triangles whose leaf box overlaps the padded segment box into LDS, then
@@ -542,9 +617,13 @@ __device__ __forceinline__ void group_closest(double &d, uint32_t &t)
 // the lower index) goes to best_tri. The listed queue holds elements with a
 // free left node, the ones the energy queries, and their candidates are
 // counted; `extra` (the friction pass) appends the root elements (fixed left
-// node, free right node), whose lists only friction reads. Overflowed
-// elements are left to the consumer's tree walk.
+// node, free right node), whose lists only friction reads. An element whose
+// list overflowed is deferred: after its other elements the wavefront walks
+// the hierarchy for it cooperatively (wave_closest, its groups' packing rows
+// as the ring). One the wavefront cannot take (more than kOverflowSlots, or
+// a ring overflow) gets kUnresolved and is left to the consumer's walk.
 constexpr int kClosestLanes = 16;
+constexpr uint32_t kOverflowSlots = 16;
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, ColliderView col, ContactCache cc,
                                                          double pad, const double *q, const double *q_base,
@@ -556,6 +635,11 @@ __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, 
     if (stop && *stop) return;
     constexpr int G = kClosestLanes;
     __shared__ uint32_t packed[kBlock / G][kCacheCapacity];
+    __shared__ uint32_t deferred[kBlock / kWave][kOverflowSlots];
+    __shared__ uint32_t deferred_count[kBlock / kWave];
+    const int wave = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) deferred_count[wave] = 0;
+    __builtin_amdgcn_wave_barrier();
     const double alpha = MODE == kAtTrial ? *alpha_at : 0.0;
     const int lane = threadIdx.x & (G - 1);
     const int slot = threadIdx.x / G;
@@ -569,7 +653,15 @@ __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, 
     for (uint32_t j = group; j < listed + extra_count; j += groups) {
         const uint32_t L = j < listed ? cc.list[j] : extra[j - listed];
         const uint32_t c = cc.count[L];
-        if (c == 0 || c == kCacheOverflow) continue;
+        if (c == 0) continue;
+        if (c == kCacheOverflow) {
+            if (lane == 0) {
+                const uint32_t at = atomicAdd(&deferred_count[wave], 1u);
+                if (at < kOverflowSlots) deferred[wave][at] = L;
+                else best_tri[L] = kUnresolved;
+            }
+            continue;
+        }
         const bool counted = !(flags[L] & kNodeFixed);
         V3 p0, p1;
         element_points<MODE>(L, q, q_base, dir, alpha, p0, p1);
@@ -608,6 +700,22 @@ __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, 
             best_tri[L] = bt;
             if (counted) candidates += n;
         }
+    }
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t walks = min(deferred_count[wave], kOverflowSlots);
+    volatile int *ring = reinterpret_cast<volatile int *>(&packed[wave * (kWave / G)][0]);
+    for (uint32_t h = 0; h < walks; ++h) {
+        const uint32_t L = deferred[wave][h];
+        V3 p0, p1;
+        element_points<MODE>(L, q, q_base, dir, alpha, p0, p1);
+        double bd;
+        uint32_t bt, n;
+        const bool ok = wave_closest(col, p0, p1, pad, ring, (kWave / G) * kCacheCapacity, bd, bt, n);
+        if ((threadIdx.x & (kWave - 1)) == 0) {
+            best_tri[L] = ok ? bt : kUnresolved;
+            if (ok && !(flags[L] & kNodeFixed)) candidates += n;
+        }
+        __builtin_amdgcn_wave_barrier();
     }
     if (partials) {
         Partial acc = empty_partial();
@@ -740,7 +848,7 @@ __global__ __launch_bounds__(kBlock) void k_list_ccd(const uint32_t *flags, Coll
 // Whether element L's query in this pass was answered by the list pass.
 __device__ __forceinline__ bool listed_query(const ContactCache &cc, const uint32_t *best_tri, uint32_t L)
 {
-    return best_tri && cc.count[L] != kCacheOverflow;
+    return best_tri && (cc.count[L] != kCacheOverflow || best_tri[L] != kUnresolved);
 }
 
 // ------------------------------------------------------------ evaluation
