@@ -535,24 +535,21 @@ __device__ __forceinline__ void group_closest(double &d, uint32_t &t)
     }
 }
 
-// Closest triangle to a segment among those whose box overlaps the segment's
-// box grown by pad, by a wavefront-cooperative walk of the hierarchy: the
-// same overlap tests as bvh_visit (so the same candidate set), breadth-first
-// through a ring of qcap entries, each lane computing the distances of the
-// leaves it finds. Every lane gets the closest (distance, triangle) and the
-// candidate count; false if the ring would overflow (nothing is then known).
-// All lanes call it with the same segment; loop control is wave-uniform.
-__device__ inline bool wave_closest(const ColliderView &col, V3 p0, V3 p1, double pad, volatile int *queue,
-                                    uint32_t qcap, double &best_d, uint32_t &best_t, uint32_t &count)
+// Wavefront-cooperative walk of the hierarchy for the box [lo, hi] grown by
+// pad, with the same overlap tests as bvh_visit (so the same triangles):
+// breadth-first through a ring of qcap LDS entries, f(t) called by the lane
+// that reaches leaf t. False if the ring would overflow (the walk is then
+// incomplete). All lanes call it with the same box; loop control is
+// wave-uniform.
+template <class F>
+__device__ inline bool wave_walk(const ColliderView &col, V3 lo, V3 hi, double pad, volatile int *queue, uint32_t qcap,
+                                 F &&f)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const unsigned long long below = __lanemask_lt();
     const int ic = internal_count(col);
-    const V3 lo = vmin(p0, p1), hi = vmax(p0, p1);
-    double bd = 1.0e300;
-    uint32_t bt = kNoTriangle, n = 0, head = 0, tail = 1;
+    uint32_t head = 0, tail = 1;
     if (lane == 0) queue[0] = 0;
-    bool ok = true;
     while (head < tail && head <= uint32_t(ic) + 1u) {
         const uint32_t take = (tail - head) < uint32_t(kWave) ? (tail - head) : uint32_t(kWave);
         const int node = uint32_t(lane) < take ? queue[(head + lane) % qcap] : -1;
@@ -577,21 +574,11 @@ __device__ inline bool wave_closest(const ColliderView &col, V3 p0, V3 p1, doubl
                     const double *b = col.leaf_box + 6 * size_t(t);
                     if (!overlap(lo, hi, mk(b[0], b[1], b[2]), mk(b[3], b[4], b[5]), pad)) continue;
                 }
-                ++n;
-                const uint32_t *t3 = col.triangles + 3 * size_t(t);
-                const double d =
-                    closest_segment_triangle(p0, p1, vertex(col, t3[0]), vertex(col, t3[1]), vertex(col, t3[2])).distance;
-                if (d < bd || (d == bd && t < bt)) {
-                    bd = d;
-                    bt = t;
-                }
+                f(t);
             }
         }
         const unsigned long long m0 = __ballot(push[0]), m1 = __ballot(push[1]);
-        if (tail - head + __popcll(m0) + __popcll(m1) > qcap) {
-            ok = false;
-            break;
-        }
+        if (tail - head + __popcll(m0) + __popcll(m1) > qcap) return false;
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
             const unsigned long long pm = c == 0 ? m0 : m1;
@@ -600,6 +587,28 @@ __device__ inline bool wave_closest(const ColliderView &col, V3 p0, V3 p1, doubl
         }
         __builtin_amdgcn_wave_barrier();
     }
+    return true;
+}
+
+// Closest triangle to a segment among those whose box overlaps the
+// segment's box grown by pad (the set and tie rule of closest_triangle), by
+// a cooperative walk: every lane gets the closest (distance, triangle) and
+// the candidate count; false if the walk was incomplete.
+__device__ inline bool wave_closest(const ColliderView &col, V3 p0, V3 p1, double pad, volatile int *queue,
+                                    uint32_t qcap, double &best_d, uint32_t &best_t, uint32_t &count)
+{
+    double bd = 1.0e300;
+    uint32_t bt = kNoTriangle, n = 0;
+    const bool ok = wave_walk(col, vmin(p0, p1), vmax(p0, p1), pad, queue, qcap, [&](uint32_t t) {
+        ++n;
+        const uint32_t *t3 = col.triangles + 3 * size_t(t);
+        const double d =
+            closest_segment_triangle(p0, p1, vertex(col, t3[0]), vertex(col, t3[1]), vertex(col, t3[2])).distance;
+        if (d < bd || (d == bd && t < bt)) {
+            bd = d;
+            bt = t;
+        }
+    });
     group_closest<kWave>(bd, bt);
     uint32_t total = n;
 #pragma unroll
@@ -1683,8 +1692,13 @@ __global__ __launch_bounds__(kBlock) void k_ccd_slope(RodView m, ColliderView co
                                                       StepParams p, const double *q, const double *grad,
                                                       const double *dir, Partial *partials)
 {
+    __shared__ int queues[kTilesPerBlock][kGatherQueue];
+    volatile int *queue = queues[threadIdx.x / kWave];
     const uint32_t L = blockIdx.x * blockDim.x + threadIdx.x;
     Partial acc = empty_partial();
+    bool need = false;
+    V3 p0 = mk(0, 0, 0), p1 = p0, d0 = p0, d1 = p0;
+    double speed = 0.0;
     if (L < m.node_count) {
         double gd = 0.0;
 #pragma unroll
@@ -1694,18 +1708,59 @@ __global__ __launch_bounds__(kBlock) void k_ccd_slope(RodView m, ColliderView co
         // elements with a list were bounded by the list pass
         if (p.has_collider && (fl & kNodeHasElement) && (fl & kNodeContact) && !(fl & kNodeFixed) &&
             !(list_pass && cc.count[L] != kCacheOverflow)) {
-            const V3 p0 = load3(q + 6 * size_t(L)), p1 = load3(q + 6 * size_t(L + 1));
-            const V3 d0 = load3(dir + 6 * size_t(L)), d1 = load3(dir + 6 * size_t(L + 1));
-            const V3 lo = vmin(vmin(p0, p1), vmin(add(p0, d0), add(p1, d1)));
-            const V3 hi = vmax(vmax(p0, p1), vmax(add(p0, d0), add(p1, d1)));
-            const double speed = fmax(norm(d0), norm(d1));
-            double limit = 1.0;
-            if (speed >= 1e-30)
+            p0 = load3(q + 6 * size_t(L));
+            p1 = load3(q + 6 * size_t(L + 1));
+            d0 = load3(dir + 6 * size_t(L));
+            d1 = load3(dir + 6 * size_t(L + 1));
+            speed = fmax(norm(d0), norm(d1));
+            need = speed >= 1e-30;
+        }
+    }
+    // After a list pass the few elements left (lists that overflowed) each
+    // walk the swept box with their whole wavefront, every lane advancing
+    // against the leaves it reaches (each loop stops once it cannot lower
+    // the lane's own minimum; the minimum is unchanged). An incomplete walk
+    // (ring overflow) falls back to the element's own lane. Without lists
+    // every contact element walks, one lane each.
+    const int lane = threadIdx.x & (kWave - 1);
+    if (!list_pass && need) {
+        const V3 lo = vmin(vmin(p0, p1), vmin(add(p0, d0), add(p1, d1)));
+        const V3 hi = vmax(vmax(p0, p1), vmax(add(p0, d0), add(p1, d1)));
+        double limit = 1.0;
+        bvh_query(col.bvh, internal_count(col), col.leaf_box, lo, hi, p.sweep_target, [&](uint32_t t) {
+            const uint32_t *t3 = col.triangles + 3 * size_t(t);
+            limit = fmin(limit, advancement_limit(p0, p1, d0, d1, speed, vertex(col, t3[0]), vertex(col, t3[1]),
+                                                  vertex(col, t3[2]), p.sweep_target));
+        });
+        acc.alpha = limit;
+        need = false;
+    }
+    for (unsigned long long mask = __ballot(need); mask; mask &= mask - 1) {
+        const int src = __ffsll((long long)mask) - 1;
+        const V3 a0 = mk(__shfl(p0.x, src, kWave), __shfl(p0.y, src, kWave), __shfl(p0.z, src, kWave));
+        const V3 a1 = mk(__shfl(p1.x, src, kWave), __shfl(p1.y, src, kWave), __shfl(p1.z, src, kWave));
+        const V3 e0 = mk(__shfl(d0.x, src, kWave), __shfl(d0.y, src, kWave), __shfl(d0.z, src, kWave));
+        const V3 e1 = mk(__shfl(d1.x, src, kWave), __shfl(d1.y, src, kWave), __shfl(d1.z, src, kWave));
+        const double sp = __shfl(speed, src, kWave);
+        const V3 lo = vmin(vmin(a0, a1), vmin(add(a0, e0), add(a1, e1)));
+        const V3 hi = vmax(vmax(a0, a1), vmax(add(a0, e0), add(a1, e1)));
+        double limit = 1.0;
+        const bool ok = wave_walk(col, lo, hi, p.sweep_target, queue, kGatherQueue, [&](uint32_t t) {
+            const uint32_t *t3 = col.triangles + 3 * size_t(t);
+            limit = fmin(limit, advancement_limit(a0, a1, e0, e1, sp, vertex(col, t3[0]), vertex(col, t3[1]),
+                                                  vertex(col, t3[2]), p.sweep_target, limit / 0.9));
+        });
+#pragma unroll
+        for (int o = kWave / 2; o >= 1; o >>= 1) limit = fmin(limit, __shfl_xor(limit, o, kWave));
+        if (lane == src) {
+            if (!ok) {
+                limit = 1.0;
                 bvh_query(col.bvh, internal_count(col), col.leaf_box, lo, hi, p.sweep_target, [&](uint32_t t) {
                     const uint32_t *t3 = col.triangles + 3 * size_t(t);
-                    limit = fmin(limit, advancement_limit(p0, p1, d0, d1, speed, vertex(col, t3[0]), vertex(col, t3[1]),
+                    limit = fmin(limit, advancement_limit(a0, a1, e0, e1, sp, vertex(col, t3[0]), vertex(col, t3[1]),
                                                           vertex(col, t3[2]), p.sweep_target));
                 });
+            }
             acc.alpha = limit;
         }
     }
