@@ -1193,9 +1193,10 @@ __device__ __forceinline__ void buffer_store(__amdgpu_buffer_rsrc_t r, uint32_t 
 // and it runs the longest one's node count). In the forward sweep lane
 // j < 6 holds column j of the block being inverted and of U, lanes 6..11
 // the columns of its inverse, lane 12 the right-hand side. The pivot
-// column of each Gauss-Jordan step comes by row broadcasts from its lane;
-// the previous node's U (for the Schur complement) and the finished inverse
-// (for C' = inv U and d' = inv rhs) are shared through LDS. Each entry is
+// column of each Gauss-Jordan step, the previous node's U (for the Schur
+// complement, kept by lanes 0..5) and the finished inverse (for C' = inv U
+// and d' = inv rhs) all come by row broadcasts from the lanes that hold
+// them, with no LDS round trip. Each entry is
 // computed with the sequential algorithm's operations in its order:
 // products sum k = 0..5 from zero by fused multiply-adds, as the
 // reference's CUDA build contracts them, and the Gauss-Jordan inverse (diagonal
@@ -1222,11 +1223,6 @@ constexpr int kSolveAhead = 3;
 constexpr int kBackAhead = 8;
 constexpr uint64_t kMaxSolveNodes = kMaxBufferBytes / 288;  // 36 doubles per node block
 
-struct SolveShared {
-    double upper[kSolveGroups][36];    // previous node's U, column c at [6c]
-    double inverse[kSolveGroups][36];  // inverse of the current pivot block, column c at [6c]
-};
-
 struct ForwardSlot {
     double d[6];  // D column
     double g[6];  // gradient (rhs lane)
@@ -1246,7 +1242,6 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
         block_reduce(skipped, partials + blockIdx.x);
         return;
     }
-    __shared__ SolveShared sh;
     const int g = threadIdx.x / kSolveLanes, j = threadIdx.x % kSolveLanes;
     const bool acol = j < 6, icol = j >= 6 && j < 12, vlane = j == 12;
     const uint32_t gi = blockIdx.x * kSolveGroups + g;
@@ -1293,8 +1288,9 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
     ForwardSlot ring[kSolveAhead];
 #pragma unroll
     for (int a = 0; a < kSolveAhead; ++a) fetch(ring[a], a);
-    for (int e = j; e < 36; e += kSolveLanes) sh.upper[g][e] = 0.0;
-    __syncthreads();
+    double wprev[6];  // the previous node's U column (lanes 0..5); zero before the first
+#pragma unroll
+    for (int k = 0; k < 6; ++k) wprev[k] = 0.0;
 
     auto forward = [&](ForwardSlot &slot, uint32_t t) __attribute__((always_inline)) {
         const uint32_t n = begin + t;
@@ -1304,25 +1300,18 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
             col[r] = acol ? slot.d[r] : (vlane ? -slot.g[r] : ((icol && r == j - 6) ? 1.0 : 0.0));
             w[r] = slot.u[r];
         }
-        {
-            // col[r] -= sum_k U'[k][r] p[k]: D - U'^T C' on the block
-            // columns, rhs - U'^T d' on the rhs lane. At the first node U'
-            // and p are zero and this subtracts +0, which leaves every
-            // value (signed zeros, infinities and NaNs included) unchanged.
-            const double *up = sh.upper[g];
+        // col[r] -= sum_k U'[k][r] p[k]: D - U'^T C' on the block columns,
+        // rhs - U'^T d' on the rhs lane, U'[k][r] from lane r. At the first
+        // node U' and p are zero and this subtracts +0, which leaves every
+        // value (signed zeros, infinities and NaNs included) unchanged.
+        static_for<6>([&](auto R) {
+            double sum = 0.0;
 #pragma unroll
-            for (int r = 0; r < 6; ++r) {
-                double sum = 0.0;
+            for (int k = 0; k < 6; ++k) sum = fma(row_bcast<R>(wprev[k]), p[k], sum);
+            col[R] -= sum;
+        });
 #pragma unroll
-                for (int k = 0; k < 6; ++k) sum = fma(up[6 * r + k], p[k], sum);
-                col[r] -= sum;
-            }
-        }
-        __syncthreads();
-        if (acol) {
-#pragma unroll
-            for (int k = 0; k < 6; ++k) sh.upper[g][6 * j + k] = w[k];
-        }
+        for (int k = 0; k < 6; ++k) wprev[k] = w[k];
         // The rhs lane keeps its column (the right-hand side) in rhs; the
         // Gauss-Jordan steps below also transform its col, which nothing
         // reads (lane 12 is never a pivot column).
@@ -1400,24 +1389,18 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
             });
         }
         if (singular && t < len) ok = false;
-        if (icol) {
-#pragma unroll
-            for (int r = 0; r < 6; ++r) sh.inverse[g][6 * (j - 6) + r] = col[r];
-        }
-        __syncthreads();
         // res[r] = sum_k inv[r][k] w[k]: C' = inv U on the block columns,
-        // d' = inv rhs on the rhs lane
+        // d' = inv rhs on the rhs lane; inverse column k is on lane 6 + k
 #pragma unroll
         for (int r = 0; r < 6; ++r)
             if (vlane) w[r] = rhs[r];
-        const double *inv = sh.inverse[g];
         double res[6];
 #pragma unroll
         for (int r = 0; r < 6; ++r) res[r] = 0.0;
+        static_for<6>([&](auto K) {
 #pragma unroll
-        for (int k = 0; k < 6; ++k)
-#pragma unroll
-            for (int r = 0; r < 6; ++r) res[r] = fma(inv[6 * k + r], w[k], res[r]);
+            for (int r = 0; r < 6; ++r) res[r] = fma(row_bcast<6 + K>(col[r]), w[K], res[r]);
+        });
 #pragma unroll
         for (int r = 0; r < 6; ++r) p[r] = acol ? (t + 1 < len ? res[r] : 0.0) : (vlane ? res[r] : 0.0);
         const bool keep = t < len && ok;
