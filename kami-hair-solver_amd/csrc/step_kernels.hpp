@@ -631,7 +631,7 @@ __device__ inline bool wave_closest(const ColliderView &col, V3 p0, V3 p1, doubl
 // the hierarchy for it cooperatively (wave_closest, its groups' packing rows
 // as the ring). One the wavefront cannot take (more than kOverflowSlots, or
 // a ring overflow) gets kUnresolved and is left to the consumer's walk.
-constexpr int kClosestLanes = 16;
+constexpr int kClosestLanes = 32;
 constexpr uint32_t kOverflowSlots = 16;
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, ColliderView col, ContactCache cc,
@@ -823,7 +823,7 @@ __global__ __launch_bounds__(kBlock) void k_list_ccd(const uint32_t *flags, Coll
                 const double *b = cc.tri_box + 6 * size_t(t);
                 hit = overlap(lo, hi, mk(b[0], b[1], b[2]), mk(b[3], b[4], b[5]), p.sweep_target);
             }
-            const unsigned long long mask = (__ballot(hit) >> shift) & ((1ull << G) - 1);
+            const unsigned long long mask = (__ballot(hit) >> shift) & ((G == 64) ? ~0ull : ((1ull << G) - 1));
             uint32_t base = 0;
             if (lane == 0 && mask) base = atomicAdd(&pair_count, uint32_t(__popcll(mask)));
             base = __shfl(base, shift, kWave);
