@@ -150,6 +150,8 @@ struct DeviceState {
     Buffer<double> out3;
     Buffer<uint32_t> pairs;
     Buffer<double> alpha;  // line-search step read by the trial kernels
+    Buffer<unsigned long long> list_words;  // per-block candidate counts of the narrow-phase list pass
+    Buffer<double> list_alpha;              // per-block smallest step of the CCD list pass
     DirectionChoice *choice() const { return reinterpret_cast<DirectionChoice *>(reduced.get() + 3); }
 
     // Event timing of kernel classes (enabled on request) and of substeps.
@@ -269,10 +271,11 @@ struct DeviceState {
     }
     // Folds the partials into reduced[slot] without waiting (and applies
     // the convergence test when one is given).
-    void reduce_async(uint32_t blocks, int slot, dev::ConvergenceTest test = {0.0, 0.0, 0.0, nullptr})
+    void reduce_async(uint32_t blocks, int slot, dev::ConvergenceTest test = {0.0, 0.0, 0.0, nullptr},
+                      dev::ListPartials lists = {nullptr, nullptr, 0})
     {
         hipLaunchKernelGGL(dev::k_reduce, dim3(1), dim3(kBlock), 0, stream, partials.get(), blocks,
-                           reduced.get() + slot, mirror_slot(slot), test);
+                           reduced.get() + slot, mirror_slot(slot), test, lists);
         HIP_TRY(hipGetLastError());
     }
     // One wait for everything the host reads after a batch of launches.
@@ -682,7 +685,9 @@ KhsResult Solver::upload_model()
     d.rot_pend.upload(zero3.data(), zero3.size());
     d.out3.allocate(3 * std::max<size_t>(N, d.P));
     const uint32_t max_blocks = std::max({d.tiled_blocks(), d.node_blocks(), d.solve_blocks()}) + 1;
-    d.partials.allocate(max_blocks + kPairBlocks + 16);
+    d.partials.allocate(max_blocks + 16);
+    d.list_words.allocate(kPairBlocks);
+    d.list_alpha.allocate(kPairBlocks);
     d.reduced.allocate(6);  // reductions, with the direction choice in slot 3 (at most one Partial)
     {
         const unsigned zero = 0;
@@ -854,7 +859,7 @@ void Solver::refresh_contact_cache(int mode, bool full, bool swept_leaves, bool 
 // with the device's alpha), one wavefront per element; partials (may be
 // null) receive the candidate count. Elements with a fixed left node are
 // refreshed and included only for the friction pass (`with_roots`).
-void Solver::closest_pass(bool line_search, Partial *partials, bool with_roots)
+void Solver::closest_pass(bool line_search, unsigned long long *block_candidates, bool with_roots)
 {
     DeviceState &d = *dev_;
     const double contact_pad = material_.radius + material_.collider_offset + material_.barrier_distance;
@@ -863,13 +868,15 @@ void Solver::closest_pass(bool line_search, Partial *partials, bool with_roots)
     if (line_search)
         hipLaunchKernelGGL(dev::k_list_closest<dev::kAtTrial>, dim3(kPairBlocks), dim3(kBlock), 0, d.stream,
                            d.flags.get(), d.collider(), cc, contact_pad, d.q.get(), d.q_base.get(), d.dir.get(),
-                           d.alpha.get(), d.best_tri.get(), partials, with_roots ? d.root_elements.get() : nullptr,
-                           with_roots ? d.root_element_count : 0u, d.skip());
+                           d.alpha.get(), d.best_tri.get(), block_candidates,
+                           with_roots ? d.root_elements.get() : nullptr, with_roots ? d.root_element_count : 0u,
+                           d.skip());
     else
         hipLaunchKernelGGL(dev::k_list_closest<dev::kAtState>, dim3(kPairBlocks), dim3(kBlock), 0, d.stream,
                            d.flags.get(), d.collider(), cc, contact_pad, d.q.get(), d.q_base.get(), d.dir.get(),
-                           d.alpha.get(), d.best_tri.get(), partials, with_roots ? d.root_elements.get() : nullptr,
-                           with_roots ? d.root_element_count : 0u, (const unsigned *)nullptr);
+                           d.alpha.get(), d.best_tri.get(), block_candidates,
+                           with_roots ? d.root_elements.get() : nullptr, with_roots ? d.root_element_count : 0u,
+                           (const unsigned *)nullptr);
     HIP_TRY(hipGetLastError());
 }
 
@@ -1115,7 +1122,7 @@ void Solver::evaluate_async(bool gradient, double dt, bool line_search, unsigned
     // With current candidate lists the list pass finds the closest
     // triangles first; its partials follow the evaluation's.
     const bool list_pass = d.cache_valid;
-    if (list_pass) closest_pass(line_search, d.partials.get() + blocks);
+    if (list_pass) closest_pass(line_search, d.list_words.get());
     const uint32_t *best = list_pass ? d.best_tri.get() : nullptr;
     if (gradient)
         hipLaunchKernelGGL((dev::k_evaluate<true, false>), dim3(blocks), dim3(kBlock), 0, d.stream, rv, cv, cc, best, p,
@@ -1132,8 +1139,9 @@ void Solver::evaluate_async(bool gradient, double dt, bool line_search, unsigned
     HIP_TRY(hipGetLastError());
     if (gradient) d.toc(t0, d.totals.assembly_ms, d.totals.assembly_launches);
     else d.toc(t0, d.totals.energy_ms, d.totals.energy_launches);
-    d.reduce_async(blocks + (list_pass ? kPairBlocks : 0), gradient ? DeviceState::kGradSlot : DeviceState::kEvalSlot,
-                   dev::ConvergenceTest{abs_tol, rel_tol, initial, stop});
+    d.reduce_async(blocks, gradient ? DeviceState::kGradSlot : DeviceState::kEvalSlot,
+                   dev::ConvergenceTest{abs_tol, rel_tol, initial, stop},
+                   dev::ListPartials{list_pass ? d.list_words.get() : nullptr, nullptr, list_pass ? kPairBlocks : 0u});
 }
 
 Solver::Reduced Solver::unpack(const Partial &r2)
@@ -1267,15 +1275,16 @@ KhsResult Solver::optimize_substep(double dt)
         if (list_pass) {
             refresh_contact_cache(dev::kSwept, false);
             hipLaunchKernelGGL(dev::k_list_ccd, dim3(kPairBlocks), dim3(kBlock), 0, d.stream, d.flags.get(),
-                               d.collider(), d.cache(), cp, d.q.get(), d.dir.get(),
-                               d.partials.get() + d.node_blocks(), d.ccd_bound.get(), int(d.ccd_slot), d.skip());
+                               d.collider(), d.cache(), cp, d.q.get(), d.dir.get(), d.list_alpha.get(),
+                               d.ccd_bound.get(), int(d.ccd_slot), d.skip());
             d.ccd_slot ^= 1u;
         }
         hipLaunchKernelGGL(dev::k_ccd_slope, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, d.collider(),
                            d.cache(), list_pass ? 1 : 0, cp, d.q.get(), d.grad.get(), d.dir.get(), d.partials.get());
         HIP_TRY(hipGetLastError());
         d.toc(t0, d.totals.collision_ms, d.totals.collision_launches);
-        d.reduce_async(d.node_blocks() + (list_pass ? kPairBlocks : 0), DeviceState::kCcdSlot);
+        d.reduce_async(d.node_blocks(), DeviceState::kCcdSlot, dev::ConvergenceTest{0.0, 0.0, 0.0, nullptr},
+                       dev::ListPartials{nullptr, list_pass ? d.list_alpha.get() : nullptr, list_pass ? kPairBlocks : 0u});
         set_trial_step(dev::kAlphaFromCcd, 0.0);
         // The first trial runs at the device's step before the host has
         // seen the bound; one read-back returns direction choice, bound and

@@ -94,7 +94,7 @@ private:
     void refit_bvh(bool swept);
     void refit_boxes();
     void refresh_contact_cache(int mode, bool full, bool swept_leaves = false, bool with_roots = false);
-    void closest_pass(bool line_search, Partial *partials, bool with_roots = false);
+    void closest_pass(bool line_search, unsigned long long *block_candidates, bool with_roots = false);
     // With `stop`, the gradient's reduction also sets *stop (device) when
     // |g| <= abs_tol or |g| / initial <= rel_tol (initial < 0: this one).
     void evaluate_async(bool gradient, double dt, bool line_search, unsigned *stop = nullptr, double abs_tol = 0.0,

@@ -128,12 +128,26 @@ __device__ inline void converged_test(const Partial &grad, const ConvergenceTest
     *test.stop = (residual <= test.abs_tol || relative <= test.rel_tol) ? 1u : 0u;
 }
 
+// Per-block results of the list passes, one word per block: the narrow
+// phase's candidate count, the CCD pass's smallest step. Their sum and
+// minimum do not depend on the order they are folded in.
+struct ListPartials {
+    const unsigned long long *candidates;  // or null
+    const double *alpha;                   // or null
+    uint32_t count;
+};
+
 // Folds per-block partials in index order (one block of kBlock threads),
-// then applies the convergence test to the result when one is given.
+// and the list passes' per-block words, then applies the convergence test to
+// the result when one is given.
 __global__ __launch_bounds__(kBlock) void k_reduce(const Partial *partials, uint32_t count, Partial *out,
-                                                   Partial *mirror, ConvergenceTest test)
+                                                   Partial *mirror, ConvergenceTest test, ListPartials lists)
 {
     Partial acc = empty_partial();
+    for (uint32_t i = threadIdx.x; i < lists.count; i += kBlock) {
+        if (lists.candidates) acc.candidates += lists.candidates[i];
+        if (lists.alpha) acc.alpha = fmin(acc.alpha, lists.alpha[i]);
+    }
     for (uint32_t i = threadIdx.x; i < count; i += kBlock) {
         const Partial &o = partials[i];
         for (int k = 0; k < 8; ++k) acc.sum[k] += o.sum[k];
@@ -637,7 +651,7 @@ template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, ColliderView col, ContactCache cc,
                                                          double pad, const double *q, const double *q_base,
                                                          const double *dir, const double *alpha_at,
-                                                         uint32_t *best_tri, Partial *partials,
+                                                         uint32_t *best_tri, unsigned long long *block_candidates,
                                                          const uint32_t *extra, uint32_t extra_count,
                                                          const unsigned *stop)
 {
@@ -726,10 +740,16 @@ __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, 
         }
         __builtin_amdgcn_wave_barrier();
     }
-    if (partials) {
-        Partial acc = empty_partial();
-        acc.candidates = candidates;
-        block_reduce(acc, partials + blockIdx.x);
+    if (block_candidates) {
+        __shared__ unsigned long long wave_candidates[kBlock / kWave];
+        candidates = wave_sum_u64(candidates);
+        if ((threadIdx.x & (kWave - 1)) == 0) wave_candidates[wave] = candidates;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long total = 0;
+            for (int w = 0; w < kBlock / kWave; ++w) total += wave_candidates[w];
+            block_candidates[blockIdx.x] = total;
+        }
     }
 }
 
@@ -771,7 +791,7 @@ __device__ inline double advancement_limit(V3 p0, V3 p1, V3 d0, V3 d1, double sp
 constexpr int kCcdBatch = kBlock / kClosestLanes;
 __global__ __launch_bounds__(kBlock) void k_list_ccd(const uint32_t *flags, ColliderView col, ContactCache cc,
                                                      StepParams p, const double *q, const double *dir,
-                                                     Partial *partials, unsigned long long *bound, int slot,
+                                                     double *block_alpha, unsigned long long *bound, int slot,
                                                      const unsigned *stop)
 {
     constexpr int G = kClosestLanes;
@@ -849,9 +869,16 @@ __global__ __launch_bounds__(kBlock) void k_list_ccd(const uint32_t *flags, Coll
         }
         __syncthreads();
     }
-    Partial acc = empty_partial();
-    acc.alpha = limit;
-    block_reduce(acc, partials + blockIdx.x);
+    __shared__ double wave_alpha[kBlock / kWave];
+#pragma unroll
+    for (int o = kWave / 2; o >= 1; o >>= 1) limit = fmin(limit, __shfl_xor(limit, o, kWave));
+    if ((threadIdx.x & (kWave - 1)) == 0) wave_alpha[threadIdx.x / kWave] = limit;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = wave_alpha[0];
+        for (int w = 1; w < kBlock / kWave; ++w) a = fmin(a, wave_alpha[w]);
+        block_alpha[blockIdx.x] = a;
+    }
 }
 
 // Whether element L's query in this pass was answered by the list pass.
