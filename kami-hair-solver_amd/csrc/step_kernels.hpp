@@ -777,8 +777,8 @@ __device__ inline double advancement_limit(V3 p0, V3 p1, V3 d0, V3 d1, double sp
 }
 
 // Line-search step bound over the listed elements with a free left node.
-// A block takes the listed elements kBlock / kClosestLanes at a time: each
-// group of kClosestLanes lanes box-tests one element's list against its
+// A block takes the listed elements kBlock / kCcdLanes at a time: each
+// group of kCcdLanes lanes box-tests one element's list against its
 // swept segment box and appends the overlapping (element, triangle) pairs
 // to the block's LDS queue; then every lane of the block takes pairs from
 // the queue, so an element with many candidates spreads over the block.
@@ -788,13 +788,14 @@ __device__ inline double advancement_limit(V3 p0, V3 p1, V3 d0, V3 d1, double sp
 // with atomicMin and read without ordering; any value read is some
 // candidate's step or 1, so the minimum is unchanged). The kernel resets
 // the other slot to 1 for the next launch.
-constexpr int kCcdBatch = kBlock / kClosestLanes;
+constexpr int kCcdLanes = 32;
+constexpr int kCcdBatch = kBlock / kCcdLanes;
 __global__ __launch_bounds__(kBlock) void k_list_ccd(const uint32_t *flags, ColliderView col, ContactCache cc,
                                                      StepParams p, const double *q, const double *dir,
                                                      double *block_alpha, unsigned long long *bound, int slot,
                                                      const unsigned *stop)
 {
-    constexpr int G = kClosestLanes;
+    constexpr int G = kCcdLanes;
     __shared__ uint32_t pair_tri[kCcdBatch * kCacheCapacity];
     __shared__ uint8_t pair_elem[kCcdBatch * kCacheCapacity];
     __shared__ double elem[kCcdBatch][13];  // p0, p1, d0, d1, speed
