@@ -1367,12 +1367,8 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
 #pragma unroll
             for (int i = 0; i < 6; ++i) cv[i] = row_bcast<C>(col[i]);
             const double diag = fabs(cv[C]);
-            if constexpr (C < 5) {
-                double below = fabs(cv[C + 1]);
 #pragma unroll
-                for (int i = C + 2; i < 6; ++i) below = fmax(below, fabs(cv[i]));
-                exchange |= below > diag;
-            }
+            for (int i = C + 1; i < 6; ++i) exchange |= fabs(cv[i]) > diag;
             singular |= diag < 1.0e-30 || !isfinite(diag);
             const double invp = 1.0 / cv[C];
             col[C] *= invp;
@@ -1429,8 +1425,9 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
 #pragma unroll
             for (int r = 0; r < 6; ++r) res[r] = fma(row_bcast<6 + K>(col[r]), w[K], res[r]);
         });
+        const bool carry = (acol && t + 1 < len) || vlane;  // C' columns while a node follows, and d'
 #pragma unroll
-        for (int r = 0; r < 6; ++r) p[r] = acol ? (t + 1 < len ? res[r] : 0.0) : (vlane ? res[r] : 0.0);
+        for (int r = 0; r < 6; ++r) p[r] = carry ? res[r] : 0.0;
         const bool keep = t < len && ok;
         const uint32_t oc = (acol && keep) ? 288 * n + 8 * j : kNoAccess;
         const uint32_t op = (vlane && keep) ? 48 * n : kNoAccess;
