@@ -287,7 +287,37 @@ def rollback_ring(lib):
     s.close()
 
 
-EXTENSIONS = [rollback_ring]
+def every_node_fixed(lib):
+    """fixed_root_nodes at or above a strand's node count fixes the whole
+    strand (its count is min(fixed_root_nodes, nodes), reference
+    src/solver.cpp:227-318): that strand has no degrees of freedom and
+    follows its root targets, while a longer strand beside it moves freely."""
+    s, _ = _solver(lib, gravity=(0, 0, -9.81), fixed_root_nodes=6, maximum_element_length=0.05)
+    short = np.array([[0, 0, 0], [0, 0, -0.01], [0, 0, -0.02], [0, 0, -0.03]], float)
+    long = np.array([[0.01, 0, -0.01 * k] for k in range(12)], float)
+    points = np.concatenate([short, long])
+    s.check(s.set_hair(points, [0, 4, 16], kn.default_material(lib)))
+    s.check(s.build())
+    b = s.build_stats()
+    assert b.internal_node_count == 16 and b.fixed_node_count == 4 + 6
+    assert b.degree_of_freedom_count == 6 * 6
+    s.check(s.allocate_animation(3))
+    for k in range(3):
+        s.check(s.set_root_frame(k, points + [0.002 * k, 0.0, 0.001 * k]))
+        s.check(s.set_collider_frame(k, None))
+    s.check(s.finalize_animation())
+    start = s.original_positions()
+    for k in (1, 2):
+        s.check(s.step_frame(k, 1.0 / 24.0))
+    out = s.original_positions()
+    target = points + [0.004, 0.0, 0.002]
+    assert np.max(np.abs(out[:4] - target[:4])) <= 1e-15
+    assert np.max(np.abs(out[:6 + 4] - target[:6 + 4])) <= 1e-15
+    assert np.max(np.abs(out[10:] - start[10:])) > 1e-4  # the free part of the long strand fell
+    s.close()
+
+
+EXTENSIONS = [rollback_ring, every_node_fixed]
 
 ALL = [
     c_api_defaults,
