@@ -317,7 +317,50 @@ def every_node_fixed(lib):
     s.close()
 
 
-EXTENSIONS = [rollback_ring, every_node_fixed]
+def cancel_mid_frame(lib):
+    """khsRequestCancel from another thread while a frame runs: the step
+    returns KHS_ERROR_INVALID_STATE at its next Newton iteration, the state is
+    back at the frame's start, progress reports the cancel and records no
+    failure, and the next step runs normally (reference
+    src/cuda_backend.cu:556, 657, 693, 711-714)."""
+    import threading
+    import time
+
+    scene_points = []
+    for k in range(16):
+        x = 0.004 * (k % 4)
+        y = 0.004 * (k // 4)
+        scene_points.append(np.array([[x + 0.005 * j, y, 0.0] for j in range(24)], float))  # level, so it sags
+    points = np.concatenate(scene_points)
+    offsets = list(range(0, len(points) + 1, 24))
+    s, _ = _solver(lib, gravity=(0, 0, -9.81), substeps=32, maximum_substeps=32, fixed_root_nodes=1,
+                   newton_iterations=100, relative_tolerance=1e-9, absolute_tolerance=1e-30)
+    s.check(s.set_hair(points, offsets, kn.default_material(lib)))
+    s.check(s.build())
+    s.check(s.set_convergence_mode(1))  # thin hair would meet the default floors at once
+    before = s.internal_positions()
+    result = {}
+    worker = threading.Thread(target=lambda: result.setdefault("rc", s.step(1.0 / 24.0)))
+    worker.start()
+    deadline = time.time() + 60.0
+    while time.time() < deadline and "rc" not in result:
+        p = s.progress()
+        if p.substep >= 2:
+            s.check(s.request_cancel())
+            break
+    worker.join()
+    assert result["rc"] == kn.KHS_ERROR_INVALID_STATE, result["rc"]
+    assert "cancel" in s.last_error().lower()
+    assert s.progress().cancelled == 1
+    assert np.array_equal(s.internal_positions(), before)
+    assert s.failure().kind == 0
+    s.check(s.step(1.0 / 24.0))
+    assert s.progress().cancelled == 0
+    assert not np.array_equal(s.internal_positions(), before)
+    s.close()
+
+
+EXTENSIONS = [rollback_ring, every_node_fixed, cancel_mid_frame]
 
 ALL = [
     c_api_defaults,
