@@ -59,7 +59,6 @@ struct ColliderView {
 constexpr uint32_t kCacheCapacity = 512;
 constexpr uint32_t kCacheOverflow = 0xffffffffu;
 constexpr uint32_t kNoTriangle = 0xffffffffu;
-constexpr uint32_t kUnresolved = 0xfffffffeu;  // best_tri of an overflowed element the list pass left to the tree walk
 constexpr uint32_t kPairBlocks = 2048;  // grid of the per-element narrow-phase passes
 struct ContactCache {
     double skin;

@@ -1126,18 +1126,21 @@ void Solver::evaluate_async(bool gradient, double dt, bool line_search, unsigned
     const bool list_pass = d.cache_valid;
     if (list_pass) closest_pass(line_search, d.list_words.get());
     const uint32_t *best = list_pass ? d.best_tri.get() : nullptr;
-    if (gradient)
-        hipLaunchKernelGGL((dev::k_evaluate<true, false>), dim3(blocks), dim3(kBlock), 0, d.stream, rv, cv, cc, best, p,
-                           d.q.get(), d.q_old.get(), d.vel.get(), d.q_base.get(), d.dir.get(), d.alpha.get(),
-                           d.grad.get(), d.Dblk.get(), d.Ublk.get(), d.partials.get(), (const unsigned *)nullptr);
-    else if (line_search)
-        hipLaunchKernelGGL((dev::k_evaluate<false, true>), dim3(blocks), dim3(kBlock), 0, d.stream, rv, cv, cc, best, p,
-                           d.q.get(), d.q_old.get(), d.vel.get(), d.q_base.get(), d.dir.get(), d.alpha.get(),
-                           d.grad.get(), d.Dblk.get(), d.Ublk.get(), d.partials.get(), d.skip());
-    else
-        hipLaunchKernelGGL((dev::k_evaluate<false, false>), dim3(blocks), dim3(kBlock), 0, d.stream, rv, cv, cc, best, p,
-                           d.q.get(), d.q_old.get(), d.vel.get(), d.q_base.get(), d.dir.get(), d.alpha.get(),
-                           d.grad.get(), d.Dblk.get(), d.Ublk.get(), d.partials.get(), (const unsigned *)nullptr);
+    // the variants without the tree walk when the list pass answered every query
+    auto launch = [&](auto kernel, const unsigned *stop) {
+        hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kBlock), 0, d.stream, rv, cv, cc, best, p, d.q.get(),
+                           d.q_old.get(), d.vel.get(), d.q_base.get(), d.dir.get(), d.alpha.get(), d.grad.get(),
+                           d.Dblk.get(), d.Ublk.get(), d.partials.get(), stop);
+    };
+    if (gradient) {
+        if (list_pass) launch(dev::k_evaluate<true, false, true>, (const unsigned *)nullptr);
+        else launch(dev::k_evaluate<true, false, false>, (const unsigned *)nullptr);
+    } else if (line_search) {
+        if (list_pass) launch(dev::k_evaluate<false, true, true>, d.skip());
+        else launch(dev::k_evaluate<false, true, false>, d.skip());
+    } else {
+        launch(dev::k_evaluate<false, false, false>, (const unsigned *)nullptr);
+    }
     HIP_TRY(hipGetLastError());
     if (gradient) d.toc(t0, d.totals.assembly_ms, d.totals.assembly_launches);
     else d.toc(t0, d.totals.energy_ms, d.totals.energy_launches);

@@ -644,7 +644,8 @@ __device__ inline bool wave_closest(const ColliderView &col, V3 p0, V3 p1, doubl
 // list overflowed is deferred: after its other elements the wavefront walks
 // the hierarchy for it cooperatively (wave_closest, its groups' packing rows
 // as the ring). One the wavefront cannot take (more than kOverflowSlots, or
-// a ring overflow) gets kUnresolved and is left to the consumer's walk.
+// a ring overflow) is walked by one lane (closest_triangle). Every element
+// of the pass thus gets its best_tri, and the consumers walk nothing.
 constexpr int kClosestLanes = 32;
 constexpr uint32_t kOverflowSlots = 16;
 template <int MODE>
@@ -680,8 +681,18 @@ __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, 
         if (c == kCacheOverflow) {
             if (lane == 0) {
                 const uint32_t at = atomicAdd(&deferred_count[wave], 1u);
-                if (at < kOverflowSlots) deferred[wave][at] = L;
-                else best_tri[L] = kUnresolved;
+                if (at < kOverflowSlots) {
+                    deferred[wave][at] = L;
+                } else {
+                    V3 p0, p1;
+                    element_points<MODE>(L, q, q_base, dir, alpha, p0, p1);
+                    Closest pair;
+                    uint32_t tri;
+                    unsigned long long n = 0;
+                    closest_triangle(col, p0, p1, pad, pair, tri, n);
+                    best_tri[L] = tri;
+                    if (!(flags[L] & kNodeFixed)) candidates += n;
+                }
             }
             continue;
         }
@@ -735,8 +746,14 @@ __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, 
         uint32_t bt, n;
         const bool ok = wave_closest(col, p0, p1, pad, ring, (kWave / G) * kCacheCapacity, bd, bt, n);
         if ((threadIdx.x & (kWave - 1)) == 0) {
-            best_tri[L] = ok ? bt : kUnresolved;
-            if (ok && !(flags[L] & kNodeFixed)) candidates += n;
+            unsigned long long count = n;
+            if (!ok) {
+                Closest pair;
+                count = 0;
+                closest_triangle(col, p0, p1, pad, pair, bt, count);
+            }
+            best_tri[L] = bt;
+            if (!(flags[L] & kNodeFixed)) candidates += count;
         }
         __builtin_amdgcn_wave_barrier();
     }
@@ -882,10 +899,11 @@ __global__ __launch_bounds__(kBlock) void k_list_ccd(const uint32_t *flags, Coll
     }
 }
 
-// Whether element L's query in this pass was answered by the list pass.
+// Whether element L's query in this pass was answered by the list pass
+// (after one, every query is).
 __device__ __forceinline__ bool listed_query(const ContactCache &cc, const uint32_t *best_tri, uint32_t L)
 {
-    return best_tri && (cc.count[L] != kCacheOverflow || best_tri[L] != kUnresolved);
+    return best_tri != nullptr;
 }
 
 // ------------------------------------------------------------ evaluation
@@ -915,7 +933,9 @@ __device__ inline void store_tile_blocks(const double *st, double *out, long fir
 // state is q = alpha d + q_base, written back by the owners.
 // The assembly is held to 256 registers so two wavefronts share a SIMD
 // (a few values spill); at one wavefront per SIMD its loads are exposed.
-template <bool GRAD, bool LS>
+// With LISTS the list pass before it has chosen every element's closest
+// triangle, and the tree walk (and its stack) is not compiled in.
+template <bool GRAD, bool LS, bool LISTS>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GRAD ? 2 : 1))) void k_evaluate(
     RodView m, ColliderView col, ContactCache cc, const uint32_t *best_tri, StepParams p, double *q,
     const double *q_old, const double *vel, const double *q_base, const double *dir, const double *alpha_at,
@@ -979,7 +999,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GRAD ? 2
         Closest pair;
         uint32_t tri = 0;
         bool found;
-        if (listed_query(cc, best_tri, uint32_t(L))) {
+        if constexpr (LISTS) {
             // the list pass counted the candidates and chose the triangle
             tri = cc.count[L] ? best_tri[L] : kNoTriangle;
             found = tri != kNoTriangle;
