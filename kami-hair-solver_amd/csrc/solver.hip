@@ -274,7 +274,7 @@ struct DeviceState {
     void reduce_async(uint32_t blocks, int slot, dev::ConvergenceTest test = {0.0, 0.0, 0.0, nullptr},
                       dev::ListPartials lists = {nullptr, nullptr, 0})
     {
-        hipLaunchKernelGGL(dev::k_reduce, dim3(1), dim3(kBlock), 0, stream, partials.get(), blocks,
+        hipLaunchKernelGGL(dev::k_reduce, dim3(1), dim3(dev::kReduceBlock), 0, stream, partials.get(), blocks,
                            reduced.get() + slot, mirror_slot(slot), test, lists);
         HIP_TRY(hipGetLastError());
     }

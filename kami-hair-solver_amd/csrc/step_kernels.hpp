@@ -78,9 +78,10 @@ __device__ inline Partial empty_partial()
 // Block-wide fixed-order reduction of one Partial per thread; thread 0
 // writes the result (also to `mirror`, host memory the host reads after
 // its wait, when given).
+template <int BLOCK = kBlock>
 __device__ inline void block_reduce(Partial p, Partial *out, Partial *mirror = nullptr)
 {
-    __shared__ Partial waves[kBlock / kWave];
+    __shared__ Partial waves[BLOCK / kWave];
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
 #pragma unroll
     for (int k = 0; k < 8; ++k) p.sum[k] = wave_sum(p.sum[k]);
@@ -140,15 +141,16 @@ struct ListPartials {
 // Folds per-block partials in index order (one block of kBlock threads),
 // and the list passes' per-block words, then applies the convergence test to
 // the result when one is given.
-__global__ __launch_bounds__(kBlock) void k_reduce(const Partial *partials, uint32_t count, Partial *out,
-                                                   Partial *mirror, ConvergenceTest test, ListPartials lists)
+constexpr int kReduceBlock = 256;
+__global__ __launch_bounds__(kReduceBlock) void k_reduce(const Partial *partials, uint32_t count, Partial *out,
+                                                         Partial *mirror, ConvergenceTest test, ListPartials lists)
 {
     Partial acc = empty_partial();
-    for (uint32_t i = threadIdx.x; i < lists.count; i += kBlock) {
+    for (uint32_t i = threadIdx.x; i < lists.count; i += kReduceBlock) {
         if (lists.candidates) acc.candidates += lists.candidates[i];
         if (lists.alpha) acc.alpha = fmin(acc.alpha, lists.alpha[i]);
     }
-    for (uint32_t i = threadIdx.x; i < count; i += kBlock) {
+    for (uint32_t i = threadIdx.x; i < count; i += kReduceBlock) {
         const Partial &o = partials[i];
         for (int k = 0; k < 8; ++k) acc.sum[k] += o.sum[k];
         acc.min_gap = fmin(acc.min_gap, o.min_gap);
@@ -159,7 +161,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const Partial *partials, uint
         acc.infeasible |= o.infeasible;
         acc.failed |= o.failed;
     }
-    block_reduce(acc, out, mirror);
+    block_reduce<kReduceBlock>(acc, out, mirror);
     if (test.stop && threadIdx.x == 0) converged_test(*out, test);
 }
 
