@@ -290,9 +290,9 @@ __global__ void k_contact_refresh(RodView m, ColliderView col, ContactCache cc, 
         gather();
     }
     if (n == 0) {
-        // Nothing near: keep a box eight skins wider if it is empty too, so
+        // Nothing near: keep a box sixteen skins wider if it is empty too, so
         // hair moving through free space rarely gathers again.
-        const double far = pad + 8.0 * cc.skin;
+        const double far = pad + 16.0 * cc.skin;
         const V3 flo = mk(lo.x - far, lo.y - far, lo.z - far), fhi = mk(hi.x + far, hi.y + far, hi.z + far);
         bool empty = true;
         bvh_visit(col.bvh, internal_count(col), col.leaf_box, flo, fhi, 0.0, [&](uint32_t) {
@@ -415,7 +415,7 @@ __device__ __forceinline__ double resolve_step(const TrialStep &s, double *alpha
 // the rounding margin) left its cached box gets a new box (query box grown
 // by pad + skin), and its wavefront gathers the triangles overlapping each
 // such box in turn, breadth-first through an LDS queue; an empty result
-// keeps a box seven skins wider still when that is empty too. Elements with
+// keeps a box fifteen skins wider still when that is empty too. Elements with
 // a fixed left node take part only `with_roots` (the friction pass, the one
 // reader of their lists), and keep lists but stay off the listed queue.
 // The swept pass (boxes over a whole step, where many neighbouring elements
@@ -467,7 +467,7 @@ __global__ __launch_bounds__(kBlock) void k_contact_check(RodView m, ColliderVie
             n = wave_gather(col, blo, bhi, tris, kCacheCapacity, queue, false);
         }
         if (n == 0) {
-            const double far = 7.0 * cc.skin;
+            const double far = 15.0 * cc.skin;
             const V3 flo = mk(lo.x - far, lo.y - far, lo.z - far), fhi = mk(hi.x + far, hi.y + far, hi.z + far);
             uint32_t none;
             if (wave_gather(col, flo, fhi, &none, 0, queue, true) == 0) {
@@ -490,7 +490,7 @@ __global__ __launch_bounds__(kBlock) void k_contact_check(RodView m, ColliderVie
 
 // Second half of the swept pass: one wavefront per queued element gathers
 // the triangles overlapping its new box (stored by the check); an empty
-// result keeps a box seven skins wider still when that is empty too. It also
+// result keeps a box fifteen skins wider still when that is empty too. It also
 // clears the other pending counter, which the next swept pass appends to.
 __global__ __launch_bounds__(kBlock) void k_contact_gather(const uint32_t *flags, ColliderView col, ContactCache cc,
                                                            uint32_t *next_count)
@@ -517,7 +517,7 @@ __global__ __launch_bounds__(kBlock) void k_contact_gather(const uint32_t *flags
             }
         }
         if (n == 0) {
-            const double far = 7.0 * cc.skin;
+            const double far = 15.0 * cc.skin;
             const V3 flo = mk(lo.x - far, lo.y - far, lo.z - far), fhi = mk(hi.x + far, hi.y + far, hi.z + far);
             uint32_t none;
             if (wave_gather(col, flo, fhi, &none, 0, queue, true) == 0 && lane == 0) {
