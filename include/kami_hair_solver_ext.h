@@ -174,6 +174,50 @@ KHS_API KhsResult khsxWriteCacheFrame(KhsSolver *solver, int fd);
    library ignores it and returns KHS_OK). */
 KHS_API KhsResult khsxSetThreadCount(KhsSolver *solver, uint32_t threads);
 
+/* Stage hooks: each runs one stage of the substep on the current state, so
+   that a test can compare the stage between libraries and against closed
+   forms. None of them changes q.
+
+   khsxSetColliderState sets the collider's current and previous positions
+   (3 doubles per vertex; previous NULL means previous = current), as the
+   collider interpolation leaves them inside a substep
+   (reference src/cuda_backend.cu:770-774). */
+KHS_API KhsResult khsxSetColliderState(KhsSolver *solver, const double *current, const double *previous,
+                                       uint32_t vertex_count);
+
+/* The post-substep Coulomb friction impulse at the current q, v and
+   collider (current and previous), added to v (reference
+   src/cuda_backend.cu:941-964, 615-621). */
+KHS_API KhsResult khsxApplyFriction(KhsSolver *solver, double dt);
+
+/* Largest step alpha in [0, 1] along `direction` (6 doubles per node) that
+   conservative advancement of every contact-enabled free segment against
+   the current collider allows, before the line search's 0.9 factor
+   (reference src/cuda_backend.cu:989-1000). */
+KHS_API KhsResult khsxStepBound(KhsSolver *solver, const double *direction, uint64_t count, double *alpha);
+
+/* The moving-collider sweep at substep start: every candidate triangle
+   advanced from its previous to its current position against the
+   stationary segments (reference src/cuda_backend.cu:966-987).
+   *feasible = 1 when no triangle reaches the target clearance; otherwise 0
+   and `diagnostics` (may be NULL) receives the reported pair as
+   khsGetFailureDiagnostics would. */
+KHS_API KhsResult khsxSweepCheck(KhsSolver *solver, uint32_t *feasible, KhsFailureDiagnostics *diagnostics);
+
+/* Newton direction rule. 0 (default): the block direction when it solves
+   and descends, else Jacobi-preconditioned steepest descent
+   (reference src/cuda_backend.cu:623-648). 1: steepest descent always, the
+   fallback forced; khsxEvaluate then returns -P g, P the Jacobi
+   preconditioner (reference src/cuda_backend.cu:815-846), with zero rows
+   on fixed nodes and no trust-radius scaling. */
+KHS_API KhsResult khsxSetDirectionMode(KhsSolver *solver, uint32_t mode);
+
+/* Directions the Newton loop used in the last stepped frame (its final
+   adaptive attempt): block-tridiagonal directions and steepest-descent
+   directions (forced or fallback). */
+KHS_API KhsResult khsxGetDirectionCounts(const KhsSolver *solver, uint64_t *block_directions,
+                                         uint64_t *steepest_directions);
+
 #ifdef __cplusplus
 }
 #endif

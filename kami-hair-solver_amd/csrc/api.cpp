@@ -339,4 +339,34 @@ KhsResult khsxSetThreadCount(KhsSolver *s, uint32_t threads)
     return s ? KHS_OK : KHS_ERROR_INVALID_ARGUMENT;
 }
 
+KhsResult khsxSetColliderState(KhsSolver *s, const double *current, const double *previous, uint32_t count)
+{
+    return guarded(s, [&] { return s->impl.set_collider_state(current, previous, count); });
+}
+
+KhsResult khsxApplyFriction(KhsSolver *s, double dt)
+{
+    return guarded(s, [&] { return s->impl.friction_stage(dt); });
+}
+
+KhsResult khsxStepBound(KhsSolver *s, const double *direction, uint64_t count, double *alpha)
+{
+    return guarded(s, [&] { return s->impl.step_bound(direction, count, alpha); });
+}
+
+KhsResult khsxSweepCheck(KhsSolver *s, uint32_t *feasible, KhsFailureDiagnostics *diagnostics)
+{
+    return guarded(s, [&] { return s->impl.sweep_check(feasible, diagnostics); });
+}
+
+KhsResult khsxSetDirectionMode(KhsSolver *s, uint32_t mode)
+{
+    return guarded(s, [&] { return s->impl.set_direction_mode(mode); });
+}
+
+KhsResult khsxGetDirectionCounts(const KhsSolver *s, uint64_t *block, uint64_t *steepest)
+{
+    return guarded_const(s, [&] { return s->impl.direction_counts(block, steepest); });
+}
+
 }  // extern "C"

@@ -1086,10 +1086,10 @@ KhsResult Solver::finalize_animation()
 // Enqueues one evaluation of the incremental potential (the line-search
 // variant at alpha d + q_base, alpha read from the device) and its
 // reduction into reduced[kGradSlot] (with the gradient) or reduced[kEvalSlot].
-void Solver::evaluate_async(bool gradient, double dt, bool line_search, unsigned *stop, double abs_tol,
-                            double rel_tol, double initial)
+// The scalars of a substep of length dt (material weights as in the
+// reference's element kernel, cuda_backend.cu:905-910).
+StepParams Solver::step_params(double dt) const
 {
-    DeviceState &d = *dev_;
     StepParams p{};
     p.dt = dt;
     p.inv_dt = 1.0 / dt;
@@ -1115,7 +1115,15 @@ void Solver::evaluate_async(bool gradient, double dt, bool line_search, unsigned
     p.weights[5] = 2.0 * G * inertia;
     p.contact_pad = material_.radius + material_.collider_offset + material_.barrier_distance;
     p.sweep_target = material_.radius + material_.collider_offset + desc_.minimum_gap;
-    p.has_collider = d.T > 0;
+    p.has_collider = dev_ && dev_->T > 0;
+    return p;
+}
+
+void Solver::evaluate_async(bool gradient, double dt, bool line_search, unsigned *stop, double abs_tol,
+                            double rel_tol, double initial)
+{
+    DeviceState &d = *dev_;
+    const StepParams p = step_params(dt);
     const unsigned blocks = d.tiled_blocks();
     const RodView rv = d.rods();
     const ColliderView cv = d.collider();
@@ -1237,21 +1245,40 @@ void Solver::make_direction_async()
     HIP_TRY(hipGetLastError());
 }
 
+void Solver::direction_async()
+{
+    if (direction_mode_ == 0) {
+        make_direction_async();
+        return;
+    }
+    DeviceState &d = *dev_;
+    HIP_TRY(hipMemcpyAsync(d.q_base.get(), d.q.get(), d.q.bytes(), hipMemcpyDeviceToDevice, d.stream));
+    hipLaunchKernelGGL(dev::k_force_steepest, dim3(1), dim3(1), 0, d.stream, d.choice(), d.mirror_choice());
+    HIP_TRY(hipGetLastError());
+    steepest_direction_async();
+}
+
+// Jacobi diagonal of the substep (reference cuda_backend.cu:815-841).
+void Solver::launch_preconditioner(double dt)
+{
+    DeviceState &d = *dev_;
+    const double r = material_.radius;
+    const double area = kPi * r * r, inertia = kPi * r * r * r * r / 4.0;
+    const double G = material_.young_modulus / (2.0 * (1.0 + material_.poisson_ratio));
+    const double tk = std::fmax(material_.young_modulus * area, material_.shear_correction * G * area);
+    const double ra = material_.shear_correction * G * area;
+    const double rb = std::fmax(material_.young_modulus * inertia, 2.0 * G * inertia);
+    hipLaunchKernelGGL(dev::k_preconditioner, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.rods(), 1.0 / dt,
+                       material_.mass_damping, tk, ra, rb, d.precond.get());
+    HIP_TRY(hipGetLastError());
+}
+
 KhsResult Solver::optimize_substep(double dt)
 {
     // reference cuda_backend.cu:650-688
     DeviceState &d = *dev_;
     const RodView rv = d.rods();
-    {
-        const double r = material_.radius;
-        const double area = kPi * r * r, inertia = kPi * r * r * r * r / 4.0;
-        const double G = material_.young_modulus / (2.0 * (1.0 + material_.poisson_ratio));
-        const double tk = std::fmax(material_.young_modulus * area, material_.shear_correction * G * area);
-        const double ra = material_.shear_correction * G * area;
-        const double rb = std::fmax(material_.young_modulus * inertia, 2.0 * G * inertia);
-        hipLaunchKernelGGL(dev::k_preconditioner, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, 1.0 / dt,
-                           material_.mass_damping, tk, ra, rb, d.precond.get());
-    }
+    launch_preconditioner(dt);
     const double dofs = double(6 * size_t(d.N));
     const double abs_tol = convergence_mode_ ? desc_.absolute_tolerance
                                              : std::fmax(desc_.absolute_tolerance, 1.0e-5 * std::sqrt(dofs));
@@ -1331,10 +1358,11 @@ KhsResult Solver::optimize_substep(double dt)
             have_view = true;
             head_ready = false;
         } else {
-            make_direction_async();  // also q_base = q
+            direction_async();  // also q_base = q
         }
         ++step_stats_.linear_solves;
-        bool accepted = false, block_direction = true;
+        // with the forced rule the first direction already is steepest descent
+        bool accepted = false, block_direction = direction_mode_ == 0, counted = false;
         double alpha = 1.0;
         for (int attempt = 0; attempt < 2;) {
             if (!have_view) {
@@ -1349,6 +1377,10 @@ KhsResult Solver::optimize_substep(double dt)
                 steepest_direction_async();
                 block_direction = false;
                 continue;
+            }
+            if (!counted) {
+                ++(block_direction ? block_directions_ : steepest_directions_);
+                counted = true;
             }
             const Partial &cr = v.ccd;
             const double slope = cr.sum[kSlope];
@@ -1381,6 +1413,7 @@ KhsResult Solver::optimize_substep(double dt)
             HIP_TRY(hipMemcpyAsync(d.q.get(), d.q_base.get(), d.q.bytes(), hipMemcpyDeviceToDevice, d.stream));
             hipLaunchKernelGGL(dev::k_force_steepest, dim3(1), dim3(1), 0, d.stream, d.choice(), d.mirror_choice());
             steepest_direction_async();
+            ++steepest_directions_;
             block_direction = false;
             ++attempt;
         }
@@ -1393,7 +1426,7 @@ KhsResult Solver::optimize_substep(double dt)
         // the one wait (they skip themselves if this gradient converged).
         evaluate_async(true, dt, false, d.stop.get(), abs_tol, rel_tol, initial);
         if (it + 1 < desc_.newton_iterations) {
-            make_direction_async();
+            direction_async();
             bound_and_trial();
             head_ready = true;
         }
@@ -1420,6 +1453,39 @@ KhsResult Solver::optimize_substep(double dt)
     return KHS_OK;
 }
 
+void Solver::enqueue_sweep(const StepParams &sp)
+{
+    DeviceState &d = *dev_;
+    rebuild_bvh(true);
+    // Every list is gathered against the swept triangle boxes: the sweep
+    // below queries those, and since they contain the current boxes the
+    // lists also serve the substep's other queries, which filter with the
+    // current boxes.
+    refresh_contact_cache(dev::kAtState, true, true);
+    const unsigned long long none = ~0ull;
+    hipLaunchKernelGGL(dev::k_fill_u64, dim3(1), dim3(64), 0, d.stream, d.key64.get(), size_t(1), none);
+    hipLaunchKernelGGL(dev::k_list_sweep, dim3(kPairBlocks), dim3(kBlock), 0, d.stream, d.flags.get(),
+                       d.strand_of.get(), d.collider(), d.cache(), d.tri_box.get(), sp, d.q.get(), d.key64.get());
+    hipLaunchKernelGGL(dev::k_sweep, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.rods(), d.swept_collider(),
+                       sp, d.q.get(), (const uint32_t *)d.cache_count.get(), d.key64.get());
+    hipLaunchKernelGGL(dev::k_sweep_record, dim3(1), dim3(1), 0, d.stream, d.rods(), d.collider(), sp, d.q.get(),
+                       d.key64.get(), d.sweep.get());
+    HIP_TRY(hipGetLastError());
+}
+
+// Friction impulse of the substep's end state into vel (reference
+// cuda_backend.cu:615-621, 941-964).
+void Solver::enqueue_friction(const StepParams &sp)
+{
+    DeviceState &d = *dev_;
+    if (d.cache_valid) closest_pass(false, nullptr, true);
+    hipLaunchKernelGGL(dev::k_friction, dim3(d.tiled_blocks()), dim3(kBlock), 0, d.stream, d.rods(), d.collider(),
+                       d.cache(), d.cache_valid ? d.best_tri.get() : nullptr, sp, d.q.get(), d.vel.get(), d.dv.get());
+    hipLaunchKernelGGL(dev::k_apply_dv, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.N, d.dv.get(),
+                       d.vel.get());
+    HIP_TRY(hipGetLastError());
+}
+
 KhsResult Solver::run_frame(double frame_dt, const double *root_a, const double *rot_a, const double *root_b,
                             const double *rot_b, const double *collider_a, const double *collider_b, uint32_t frame,
                             uint32_t frame_count)
@@ -1442,16 +1508,6 @@ KhsResult Solver::run_frame(double frame_dt, const double *root_a, const double 
     const uint32_t max_sub = d.T ? desc_.maximum_substeps : desc_.substeps;
     uint32_t attempts = 0;
     const size_t col_bytes = d.col_cur.bytes();
-    StepParams sp{};
-    sp.radius_offset = material_.radius + material_.collider_offset;
-    sp.contact_pad = material_.radius + material_.collider_offset + material_.barrier_distance;
-    sp.sweep_target = material_.radius + material_.collider_offset + desc_.minimum_gap;
-    sp.barrier = material_.barrier_distance;
-    sp.stiffness = material_.contact_stiffness;
-    sp.friction = material_.friction;
-    sp.smoothing = material_.friction_smoothing;
-    sp.minimum_gap = desc_.minimum_gap;
-    sp.has_collider = d.T > 0;
     for (;;) {
         ++attempts;
         HIP_TRY(hipMemcpyAsync(d.q.get(), d.snap_q.get(), dof_bytes, hipMemcpyDeviceToDevice, d.stream));
@@ -1466,13 +1522,14 @@ KhsResult Solver::run_frame(double frame_dt, const double *root_a, const double 
         step_stats_.substeps = substeps;
         step_stats_.minimum_gap = kInf;
         step_stats_.ccd_step_limit = 1.0;
+        block_directions_ = steepest_directions_ = 0;
         if (d.T) {
             HIP_TRY(hipMemcpyAsync(d.col_cur.get(), collider_a, col_bytes, hipMemcpyDeviceToDevice, d.stream));
             if (collider_a != d.col_start.get())
                 HIP_TRY(hipMemcpyAsync(d.col_start.get(), collider_a, col_bytes, hipMemcpyDeviceToDevice, d.stream));
         }
         const double dt = frame_dt / substeps;
-        sp.dt = dt;
+        const StepParams sp = step_params(dt);
         bool crossed = false;
         KhsResult result = KHS_OK;
         while (d.substep_events.size() < size_t(substeps) + 1) {
@@ -1492,24 +1549,8 @@ KhsResult Solver::run_frame(double frame_dt, const double *root_a, const double 
                 hipLaunchKernelGGL(dev::k_collider_lerp, dim3(grid_for(3 * size_t(d.V), kBlock)), dim3(kBlock), 0,
                                    d.stream, d.V, d.col_cur.get(), d.col_old.get(), collider_a, collider_b, f);
                 hipEvent_t t0 = d.tic();
-                rebuild_bvh(true);
-                // Every list is gathered against the swept triangle boxes:
-                // the sweep below queries those, and since they contain the
-                // current boxes the lists also serve the substep's other
-                // queries, which filter with the current boxes.
-                refresh_contact_cache(dev::kAtState, true, true);
                 phase_.store(KHS_PHASE_CCD);
-                const unsigned long long none = ~0ull;
-                hipLaunchKernelGGL(dev::k_fill_u64, dim3(1), dim3(64), 0, d.stream, d.key64.get(), size_t(1), none);
-                hipLaunchKernelGGL(dev::k_list_sweep, dim3(kPairBlocks), dim3(kBlock), 0, d.stream, d.flags.get(),
-                                   d.strand_of.get(), d.collider(), d.cache(), d.tri_box.get(), sp, d.q.get(),
-                                   d.key64.get());
-                hipLaunchKernelGGL(dev::k_sweep, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv,
-                                   d.swept_collider(), sp, d.q.get(), (const uint32_t *)d.cache_count.get(),
-                                   d.key64.get());
-                hipLaunchKernelGGL(dev::k_sweep_record, dim3(1), dim3(1), 0, d.stream, rv, d.collider(), sp,
-                                   d.q.get(), d.key64.get(), d.sweep.get());
-                HIP_TRY(hipGetLastError());
+                enqueue_sweep(sp);
                 d.toc(t0, d.totals.collision_ms, d.totals.collision_launches);
                 SweepRecord rec{};
                 HIP_TRY(hipMemcpyAsync(&rec, d.sweep.get(), sizeof(rec), hipMemcpyDeviceToHost, d.stream));
@@ -1545,14 +1586,7 @@ KhsResult Solver::run_frame(double frame_dt, const double *root_a, const double 
                 }
                 break;
             }
-            if (d.T && material_.friction > 0.0) {
-                if (d.cache_valid) closest_pass(false, nullptr, true);
-                hipLaunchKernelGGL(dev::k_friction, dim3(d.tiled_blocks()), dim3(kBlock), 0, d.stream, rv,
-                                   d.collider(), d.cache(), d.cache_valid ? d.best_tri.get() : nullptr, sp, d.q.get(), d.vel.get(), d.dv.get());
-                hipLaunchKernelGGL(dev::k_apply_dv, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.N, d.dv.get(),
-                                   d.vel.get());
-                HIP_TRY(hipGetLastError());
-            }
+            if (d.T && material_.friction > 0.0) enqueue_friction(sp);
             ++step_stats_.converged_substeps;
         }
         if (!crossed && result == KHS_OK) break;
@@ -1954,7 +1988,17 @@ KhsResult Solver::test_evaluate(double dt, double *gradient, double *direction, 
     e.kinetic = r.kinetic;
     e.minimum_gap = r.min_gap;
     if (gradient) d.grad.download(gradient, dof_count());
-    if (direction) {
+    if (direction && direction_mode_ == 1) {
+        // the forced fallback: -P g, no trust-radius scaling
+        const RodView rv = d.rods();
+        launch_preconditioner(dt);
+        hipLaunchKernelGGL(dev::k_force_steepest, dim3(1), dim3(1), 0, d.stream, d.choice(), d.mirror_choice());
+        hipLaunchKernelGGL(dev::k_steepest, dim3(grid_for(6 * size_t(d.N), kBlock)), dim3(kBlock), 0, d.stream, rv,
+                           d.choice(), d.grad.get(), d.precond.get(), d.dir.get());
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(d.stream));
+        d.dir.download(direction, dof_count());
+    } else if (direction) {
         const RodView rv = d.rods();
         hipLaunchKernelGGL(dev::k_fill_u32, dim3(1), dim3(64), 0, d.stream, d.flag.get(), size_t(1), 0u);
         hipLaunchKernelGGL(dev::k_block_solve, dim3(d.solve_blocks()), dim3(kWave), kSolveLdsReserve, d.stream, rv,
@@ -2119,6 +2163,112 @@ KhsResult Solver::contact_list_stats(KhsxContactListStats *out)
         ++r.length_histogram[bin];
     }
     *out = r;
+    return KHS_OK;
+}
+
+KhsResult Solver::set_collider_state(const double *current, const double *previous, uint32_t count)
+{
+    if (!built_ || !dev_->T || !current || count != dev_->V) return KHS_ERROR_INVALID_ARGUMENT;
+    DeviceState &d = *dev_;
+    HIP_TRY(hipStreamSynchronize(d.stream));
+    d.col_cur.upload(current, 3 * size_t(count));
+    d.col_old.upload(previous ? previous : current, 3 * size_t(count));
+    d.cache_valid = false;
+    return KHS_OK;
+}
+
+// The friction stage as a substep ends: the tree swept over the collider's
+// motion and every list gathered, as at substep start, then the list pass
+// and the impulse.
+KhsResult Solver::friction_stage(double dt)
+{
+    if (!built_ || !(dt > 0.0) || !std::isfinite(dt)) return KHS_ERROR_INVALID_ARGUMENT;
+    DeviceState &d = *dev_;
+    if (d.T) {
+        rebuild_bvh(true);
+        refresh_contact_cache(dev::kAtState, true, true);
+        if (material_.friction > 0.0) enqueue_friction(step_params(dt));
+    }
+    HIP_TRY(hipStreamSynchronize(d.stream));
+    return KHS_OK;
+}
+
+// The line search's step bound along a given direction, by the step's own
+// path: lists gathered at the state, the swept-box refresh, the list pass
+// and the pass over overflowed lists.
+KhsResult Solver::step_bound(const double *direction, uint64_t count, double *alpha)
+{
+    if (!built_ || !direction || !alpha || count != dof_count()) return KHS_ERROR_INVALID_ARGUMENT;
+    DeviceState &d = *dev_;
+    HIP_TRY(hipStreamSynchronize(d.stream));
+    d.dir.upload(direction, count);
+    if (!d.T) {
+        *alpha = 1.0;
+        return KHS_OK;
+    }
+    rebuild_bvh(true);
+    refresh_contact_cache(dev::kAtState, true, true);
+    const StepParams cp = step_params(1.0);
+    refresh_contact_cache(dev::kSwept, false);
+    hipLaunchKernelGGL(dev::k_list_ccd, dim3(kPairBlocks), dim3(kBlock), 0, d.stream, d.flags.get(), d.collider(),
+                       d.cache(), cp, d.q.get(), d.dir.get(), d.list_alpha.get(), d.ccd_bound.get(), int(d.ccd_slot),
+                       (const unsigned *)nullptr);
+    d.ccd_slot ^= 1u;
+    hipLaunchKernelGGL(dev::k_ccd_slope, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.rods(), d.collider(),
+                       d.cache(), 1, cp, d.q.get(), d.grad.get(), d.dir.get(), d.partials.get());
+    HIP_TRY(hipGetLastError());
+    d.reduce_async(d.node_blocks(), DeviceState::kCcdSlot, dev::ConvergenceTest{0.0, 0.0, 0.0, nullptr},
+                   dev::ListPartials{nullptr, d.list_alpha.get(), kPairBlocks});
+    *alpha = d.read_back().ccd.alpha;
+    return KHS_OK;
+}
+
+KhsResult Solver::sweep_check(uint32_t *feasible, KhsFailureDiagnostics *diagnostics)
+{
+    if (!built_ || !feasible) return KHS_ERROR_INVALID_ARGUMENT;
+    if (diagnostics && diagnostics->struct_size < sizeof(KhsFailureDiagnostics)) return KHS_ERROR_INVALID_ARGUMENT;
+    DeviceState &d = *dev_;
+    SweepRecord rec{};
+    if (d.T) {
+        enqueue_sweep(step_params(1.0));
+        HIP_TRY(hipMemcpyAsync(&rec, d.sweep.get(), sizeof(rec), hipMemcpyDeviceToHost, d.stream));
+    }
+    d.wait();
+    *feasible = rec.claimed ? 0u : 1u;
+    if (diagnostics) {
+        KhsFailureDiagnostics f{};
+        f.struct_size = sizeof(f);
+        f.strand_index = f.element_index = f.collider_triangle_index = 0xffffffffu;
+        if (rec.claimed) {
+            f.kind = KHS_FAILURE_MOVING_COLLIDER_SWEEP;
+            f.strand_index = rec.strand;
+            f.element_index = rec.element;
+            f.collider_triangle_index = rec.triangle;
+            f.distance = rec.distance;
+            f.required_distance = rec.required;
+            f.clearance = rec.clearance;
+            f.collider_substep_displacement = rec.displacement;
+            f.hair_start = {rec.hair_start[0], rec.hair_start[1], rec.hair_start[2]};
+            f.hair_end = {rec.hair_end[0], rec.hair_end[1], rec.hair_end[2]};
+            f.collider_point = {rec.collider_point[0], rec.collider_point[1], rec.collider_point[2]};
+        }
+        *diagnostics = f;
+    }
+    return KHS_OK;
+}
+
+KhsResult Solver::set_direction_mode(uint32_t mode)
+{
+    if (mode > 1) return KHS_ERROR_INVALID_ARGUMENT;
+    direction_mode_ = mode;
+    return KHS_OK;
+}
+
+KhsResult Solver::direction_counts(uint64_t *block, uint64_t *steepest) const
+{
+    if (!block || !steepest) return KHS_ERROR_INVALID_ARGUMENT;
+    *block = block_directions_;
+    *steepest = steepest_directions_;
     return KHS_OK;
 }
 

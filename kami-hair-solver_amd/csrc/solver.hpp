@@ -16,6 +16,7 @@ namespace khs {
 
 struct DeviceState;  // device buffers, defined in solver.hip
 struct Partial;      // per-block reduction record of the kernels
+struct StepParams;   // scalars of one substep passed to the kernels
 
 class Solver {
 public:
@@ -73,6 +74,12 @@ public:
     KhsResult set_rollback_ring(uint32_t frames);
     KhsResult rollback(uint32_t frame);
     KhsResult write_cache_frame(int fd);
+    KhsResult set_collider_state(const double *current, const double *previous, uint32_t count);
+    KhsResult friction_stage(double dt);
+    KhsResult step_bound(const double *direction, uint64_t count, double *alpha);
+    KhsResult sweep_check(uint32_t *feasible, KhsFailureDiagnostics *diagnostics);
+    KhsResult set_direction_mode(uint32_t mode);
+    KhsResult direction_counts(uint64_t *block, uint64_t *steepest) const;
 
     const std::string &last_error() const { return error_; }
     void set_error(std::string message) { error_ = std::move(message); }
@@ -102,6 +109,15 @@ private:
     static Reduced unpack(const Partial &r);
     Reduced evaluate(bool gradient, double dt, bool line_search);
     void make_direction_async();
+    // The direction of the next Newton iteration by the direction rule:
+    // the block solve (mode 0) or forced steepest descent (mode 1).
+    void direction_async();
+    void launch_preconditioner(double dt);
+    StepParams step_params(double dt) const;
+    // The moving-collider sweep of a substep: swept tree, full list refresh
+    // and the sweep kernels (reads nothing back).
+    void enqueue_sweep(const StepParams &sp);
+    void enqueue_friction(const StepParams &sp);
     void set_trial_step(int source, double value);
     void steepest_direction_async();
     KhsResult optimize_substep(double dt);
@@ -125,6 +141,8 @@ private:
     std::unique_ptr<DeviceState> dev_;
     uint32_t convergence_mode_ = 0;
     uint32_t precision_mode_ = 0;
+    uint32_t direction_mode_ = 0;
+    uint64_t block_directions_ = 0, steepest_directions_ = 0;
     double trust_radius_ = 0.0;
     uint32_t animation_frames_ = 0, current_frame_ = 0;
     std::vector<uint32_t> ring_frame_;  // animation frame held by each rollback slot (~0u: empty)

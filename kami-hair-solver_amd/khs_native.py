@@ -308,6 +308,12 @@ EXT_FUNCTIONS = {
     "khsxSetRollbackRing": (C.c_int, [_P, C.c_uint32]),
     "khsxRollback": (C.c_int, [_P, C.c_uint32]),
     "khsxWriteCacheFrame": (C.c_int, [_P, C.c_int]),
+    "khsxSetColliderState": (C.c_int, [_P, _P, _P, C.c_uint32]),
+    "khsxApplyFriction": (C.c_int, [_P, C.c_double]),
+    "khsxStepBound": (C.c_int, [_P, _P, C.c_uint64, _D(C.c_double)]),
+    "khsxSweepCheck": (C.c_int, [_P, _D(C.c_uint32), _D(KhsFailureDiagnostics)]),
+    "khsxSetDirectionMode": (C.c_int, [_P, C.c_uint32]),
+    "khsxGetDirectionCounts": (C.c_int, [_P, _D(C.c_uint64), _D(C.c_uint64)]),
 }
 
 
@@ -571,6 +577,36 @@ class HairSolver:
 
     def contact_list_stats(self) -> KhsxContactListStats:
         return self._get(self.lib.khsxGetContactListStats, KhsxContactListStats)
+
+    def set_collider_state(self, current, previous=None) -> int:
+        cur = np.ascontiguousarray(current, dtype=np.float64)
+        prev = None if previous is None else np.ascontiguousarray(previous, dtype=np.float64)
+        return self.lib.khsxSetColliderState(self.handle, _ptr(cur), None if prev is None else _ptr(prev),
+                                             cur.size // 3)
+
+    def apply_friction(self, dt: float) -> int:
+        return self.lib.khsxApplyFriction(self.handle, dt)
+
+    def step_bound(self, direction) -> float:
+        d = np.ascontiguousarray(direction, dtype=np.float64)
+        alpha = C.c_double(0.0)
+        self.check(self.lib.khsxStepBound(self.handle, _ptr(d), d.size, C.byref(alpha)))
+        return alpha.value
+
+    def sweep_check(self):
+        feasible = C.c_uint32(0)
+        diag = KhsFailureDiagnostics()
+        diag.struct_size = C.sizeof(KhsFailureDiagnostics)
+        self.check(self.lib.khsxSweepCheck(self.handle, C.byref(feasible), C.byref(diag)))
+        return bool(feasible.value), diag
+
+    def set_direction_mode(self, mode: int) -> int:
+        return self.lib.khsxSetDirectionMode(self.handle, mode)
+
+    def direction_counts(self):
+        block, steepest = C.c_uint64(0), C.c_uint64(0)
+        self.check(self.lib.khsxGetDirectionCounts(self.handle, C.byref(block), C.byref(steepest)))
+        return block.value, steepest.value
 
     def substep_times(self) -> np.ndarray:
         count = C.c_uint32(0)

@@ -190,6 +190,7 @@ public:
     std::string error;
     uint32_t threads = 1;
     uint32_t convergence_mode = 0;
+    uint32_t direction_mode = 0;
 
     // ---------------------------------------------------------------- input
     KhsResult set_hair(const KhsVec3 *points, uint32_t point_count, const uint32_t *offsets,
@@ -795,13 +796,96 @@ public:
         r.kinetic = s.kinetic;
         r.minimum_gap = s.min_gap;
         if (gradient) std::copy(g_.begin(), g_.end(), gradient);
-        if (direction) {
+        if (direction && direction_mode == 1) {
+            // the forced fallback: -P g with the Jacobi preconditioner, no
+            // trust-radius scaling
+            preconditioner(dt);
+            for (size_t i = 0; i < d_.size(); ++i) d_[i] = (-g_[i]) * P_[i];
+            for (size_t n = 0; n < nodes_.size(); ++n)
+                if (nodes_[n].fixed) for (int d = 0; d < 6; ++d) d_[6 * n + d] = 0.0;
+            std::copy(d_.begin(), d_.end(), direction);
+        } else if (direction) {
             std::fill(d_.begin(), d_.end(), 0.0);
             const bool failed = block_solve();
             r.solve_failed = failed ? 1u : 0u;
             std::copy(d_.begin(), d_.end(), direction);
         }
         *out = r;
+        return KHS_OK;
+    }
+
+    KhsResult set_collider_state(const double *current, const double *previous, uint32_t count)
+    {
+        if (!built_ || !has_collider_ || !current || count != collider_cur_.size()) return KHS_ERROR_INVALID_ARGUMENT;
+        for (uint32_t i = 0; i < count; ++i) {
+            collider_cur_[i] = {current[3 * i], current[3 * i + 1], current[3 * i + 2]};
+            const double *p = previous ? previous : current;
+            collider_old_[i] = {p[3 * i], p[3 * i + 1], p[3 * i + 2]};
+        }
+        return KHS_OK;
+    }
+
+    // The friction stage alone, with the tree on the current boxes as the
+    // step leaves it after the sweep (cuda_backend.cu:786-792).
+    KhsResult friction_stage(double dt)
+    {
+        if (!built_ || !(dt > 0.0)) return KHS_ERROR_INVALID_ARGUMENT;
+        if (has_collider_) bvh_.refit(collider_cur_, nullptr);
+        apply_friction(dt);
+        return KHS_OK;
+    }
+
+    KhsResult step_bound(const double *direction, uint64_t count, double *alpha)
+    {
+        if (!built_ || !direction || !alpha || count != q_.size()) return KHS_ERROR_INVALID_ARGUMENT;
+        std::copy(direction, direction + count, d_.begin());
+        double a = 1.0;
+        if (has_collider_) {
+            bvh_.refit(collider_cur_, nullptr);
+            a = ccd_limit();
+        }
+        *alpha = a;
+        return KHS_OK;
+    }
+
+    KhsResult sweep_check(uint32_t *feasible, KhsFailureDiagnostics *diag)
+    {
+        if (!built_ || !feasible) return KHS_ERROR_INVALID_ARGUMENT;
+        if (diag && diag->struct_size < sizeof(KhsFailureDiagnostics)) return KHS_ERROR_INVALID_ARGUMENT;
+        SweepFailure sw;
+        bool ok = true;
+        if (has_collider_) {
+            bvh_.refit(collider_cur_, &collider_old_);
+            ok = sweep_feasible(sw);
+        }
+        *feasible = ok ? 1u : 0u;
+        if (diag) {
+            KhsFailureDiagnostics f{};
+            f.struct_size = sizeof(f);
+            f.strand_index = f.element_index = f.collider_triangle_index = 0xffffffffu;
+            if (!ok) {
+                f.kind = KHS_FAILURE_MOVING_COLLIDER_SWEEP;
+                f.strand_index = sw.strand;
+                f.element_index = sw.element;
+                f.collider_triangle_index = sw.triangle;
+                f.distance = sw.distance;
+                f.required_distance = sw.required;
+                f.clearance = sw.clearance;
+                f.collider_substep_displacement = sw.displacement;
+                f.hair_start = to_c(sw.hair_start);
+                f.hair_end = to_c(sw.hair_end);
+                f.collider_point = to_c(sw.collider_point);
+            }
+            *diag = f;
+        }
+        return KHS_OK;
+    }
+
+    KhsResult direction_counts(uint64_t *block, uint64_t *steepest) const
+    {
+        if (!block || !steepest) return KHS_ERROR_INVALID_ARGUMENT;
+        *block = block_directions_;
+        *steepest = steepest_directions_;
         return KHS_OK;
     }
 
@@ -1524,6 +1608,10 @@ private:
     bool make_direction()
     {
         // cuda_backend.cu:640-648
+        if (direction_mode == 1) {
+            steepest_direction();
+            return false;
+        }
         std::fill(d_.begin(), d_.end(), 0.0);
         const bool failed = block_solve();
         bool used = false;
@@ -1783,6 +1871,7 @@ private:
             if (it == desc_.newton_iterations) break;
             prog_phase_.store(KHS_PHASE_LINEAR_SOLVE);
             bool block_direction = make_direction();
+            ++(block_direction ? block_directions_ : steepest_directions_);
             ++step_stats_.linear_solves;
             q_base_ = q_;
             bool accepted = false;
@@ -1815,6 +1904,7 @@ private:
                 if (!block_direction) break;
                 q_ = q_base_;
                 steepest_direction();
+                ++steepest_directions_;
                 block_direction = false;
             }
             if (!accepted) {
@@ -1873,6 +1963,7 @@ private:
             step_stats_.substeps = substeps;
             step_stats_.minimum_gap = kInf;
             step_stats_.ccd_step_limit = 1.0;
+            block_directions_ = steepest_directions_ = 0;
             if (has_collider_) {
                 collider_cur_ = ca;
                 collider_frame_start_ = ca;
@@ -1959,6 +2050,7 @@ private:
         return KHS_OK;
     }
 
+    uint64_t block_directions_ = 0, steepest_directions_ = 0;
     KhsSolverDesc desc_{};
     KhsHairMaterial material_{};
     bool has_hair_ = false, has_collider_ = false, built_ = false, closed_ = false;
@@ -2342,6 +2434,38 @@ KhsResult khsxSetThreadCount(KhsSolver *s, uint32_t threads)
     if (!s) return KHS_ERROR_INVALID_ARGUMENT;
     s->impl.threads = threads == 0 ? 1 : threads;
     return KHS_OK;
+}
+
+KhsResult khsxSetColliderState(KhsSolver *s, const double *current, const double *previous, uint32_t count)
+{
+    return guarded(s, [&] { return s->impl.set_collider_state(current, previous, count); });
+}
+
+KhsResult khsxApplyFriction(KhsSolver *s, double dt)
+{
+    return guarded(s, [&] { return s->impl.friction_stage(dt); });
+}
+
+KhsResult khsxStepBound(KhsSolver *s, const double *direction, uint64_t count, double *alpha)
+{
+    return guarded(s, [&] { return s->impl.step_bound(direction, count, alpha); });
+}
+
+KhsResult khsxSweepCheck(KhsSolver *s, uint32_t *feasible, KhsFailureDiagnostics *diagnostics)
+{
+    return guarded(s, [&] { return s->impl.sweep_check(feasible, diagnostics); });
+}
+
+KhsResult khsxSetDirectionMode(KhsSolver *s, uint32_t mode)
+{
+    if (!s || mode > 1) return KHS_ERROR_INVALID_ARGUMENT;
+    s->impl.direction_mode = mode;
+    return KHS_OK;
+}
+
+KhsResult khsxGetDirectionCounts(const KhsSolver *s, uint64_t *block, uint64_t *steepest)
+{
+    return s ? s->impl.direction_counts(block, steepest) : KHS_ERROR_INVALID_ARGUMENT;
 }
 
 }  // extern "C"

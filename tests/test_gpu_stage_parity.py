@@ -1,0 +1,389 @@
+"""Stage-by-stage and moving-collider parity of the HIP step with the oracle.
+
+Each stage hook (friction, step bound, sweep, steepest descent) runs on both
+libraries from the same state; the oracle's side of each is pinned by
+test_oracle_stage_pins.py. Bars: 1e-10 relative for energies, gradients,
+blocks, velocity changes and directions; step bounds within 1e-12; sweep
+verdicts and reported pairs equal; positions after each frame within 1e-7 m
+with equal Newton, line-search, candidate and active counts.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import grooms
+import khs_native as kn
+
+pytestmark = pytest.mark.gpu
+
+LIFTED = dict(relative_tolerance=1e-6, absolute_tolerance=1e-30, increment_tolerance=1e-14)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _pulled_in(solver, seed=7, lo=5e-5, hi=2e-4):
+    """Free nodes of the sphere scene pulled onto shells lo..hi above the
+    collider's vertex sphere: by default inside the 0.2 mm barrier, so that
+    many barriers are active (about 150); 0.2-1.6 mm for the continuous
+    collision stages, which need room to move."""
+    q, v = solver.get_state()
+    x = q.reshape(-1, 6)
+    _, _, _, flags = solver.node_model()
+    rng = np.random.default_rng(seed)
+    for k in np.nonzero((flags & 1) == 0)[0]:
+        x[k, :3] *= (0.098 + rng.uniform(lo, hi)) / np.linalg.norm(x[k, :3])
+    return q, rng
+
+
+def _sphere_pair(oracle_lib, gpu_lib, **desc):
+    scene = grooms.small_sphere_scene()
+    out = []
+    for lib in (oracle_lib, gpu_lib):
+        s = grooms.build_solver(lib, scene, **desc)
+        out.append(s)
+    return scene, out[0], out[1]
+
+
+def test_active_barrier_evaluation_matches_oracle(oracle_lib, gpu_lib):
+    print("[gpu] evaluation with active barriers, sphere scene pulled in", flush=True)
+    scene, o, g = _sphere_pair(oracle_lib, gpu_lib)
+    q, rng = _pulled_in(o)
+    v = rng.uniform(-1e-3, 1e-3, q.size)
+    dt = scene.frame_dt / 4
+    res = []
+    for s in (o, g):
+        s.check(s.set_state(q, v, q - 1e-4 * v))
+        ev, grad, d = s.evaluate(dt)
+        D, U = s.blocks()
+        res.append((ev, grad, d, D, U))
+    (eo, go, do, Do, Uo), (eg, gg, dg, Dg, Ug) = res
+    print(f"active oracle {eo.active_contacts} gpu {eg.active_contacts}; candidates {eo.candidate_pairs} "
+          f"{eg.candidate_pairs}; contact energy {eo.contact:.6e} {eg.contact:.6e}")
+    assert eo.active_contacts > 100 and eg.active_contacts == eo.active_contacts
+    assert eg.candidate_pairs == eo.candidate_pairs
+    assert eg.feasible == eo.feasible == 1
+    for f in ("objective", "elastic", "contact", "kinetic"):
+        assert abs(getattr(eg, f) - getattr(eo, f)) <= 1e-10 * abs(getattr(eo, f)), f
+    assert eg.minimum_gap == eo.minimum_gap
+    assert _rel(gg, go) <= 1e-10
+    assert max(_rel(Dg, Do), _rel(Ug, Uo)) <= 1e-10
+    # the direction by its backward error in the oracle's system (the
+    # per-strand systems are too ill-conditioned for a forward bar)
+    from test_gpu_parity import _newton_residual
+    res_o = _newton_residual(Do, Uo, go, do, o)
+    res_g = _newton_residual(Do, Uo, go, dg, o)
+    print(f"direction backward error oracle {res_o:.3e} gpu {res_g:.3e}; forward {_rel(dg, do):.3e}")
+    assert res_g <= max(10 * res_o, 1e-10)
+
+
+def test_friction_stage_matches_oracle(oracle_lib, gpu_lib):
+    """Friction against a collider that moved (body velocity nonzero) at a
+    state with many elements inside the barrier."""
+    print("[gpu] friction stage, moving sphere collider", flush=True)
+    scene, o, g = _sphere_pair(oracle_lib, gpu_lib)
+    q, rng = _pulled_in(o, seed=8)
+    v = rng.uniform(-0.05, 0.05, q.size)
+    phi = 0.002
+    c, s_ = math.cos(phi), math.sin(phi)
+    prev = scene.vertices @ np.array([[c, -s_, 0], [s_, c, 0], [0, 0, 1]]).T
+    dv = []
+    for s in (o, g):
+        s.check(s.set_state(q, v, q))
+        s.check(s.set_collider_state(scene.vertices, prev))
+        s.check(s.apply_friction(scene.frame_dt / 4))
+        dv.append(s.get_state()[1] - v)
+    moved_o = np.count_nonzero(np.abs(dv[0]) > 0)
+    moved_g = np.count_nonzero(np.abs(dv[1]) > 0)
+    print(f"dofs changed oracle {moved_o} gpu {moved_g}; rel {_rel(dv[1], dv[0]):.3e}")
+    assert moved_o > 50 and moved_g == moved_o
+    assert _rel(dv[1], dv[0]) <= 1e-10
+
+
+def test_friction_closed_form_cases_match_oracle(oracle_lib, gpu_lib):
+    """The single-node slip, stick and moving-plane cases the oracle pins."""
+    print("[gpu] friction closed-form cases", flush=True)
+    import test_oracle_stage_pins as pins
+
+    cases = [dict(gap=4e-4, v2=np.array([0.3, 0.4, -0.05]), friction=0.002),
+             dict(gap=4e-4, v2=np.array([0.03, -0.04, 0.0]), friction=0.3),
+             dict(gap=4e-4, v2=np.zeros(3), friction=0.3, plane_velocity=np.array([0.02, 0.01, 0.0]))]
+    for case in cases:
+        got = [pins._friction_dv(lib, **case)[0] for lib in (oracle_lib, gpu_lib)]
+        assert np.abs(got[0]).max() > 0
+        assert _rel(got[1], got[0]) <= 1e-12
+
+
+def test_step_bound_matches_oracle(oracle_lib, gpu_lib):
+    print("[gpu] step bound, sphere scene pulled in", flush=True)
+    scene, o, g = _sphere_pair(oracle_lib, gpu_lib)
+    q, rng = _pulled_in(o, seed=9, lo=2e-4, hi=1.6e-3)
+    _, _, _, flags = o.node_model()
+    free = np.repeat((flags & 1) == 0, 6)
+    for trial in range(3):
+        d = rng.normal(size=q.size) * free
+        d.reshape(-1, 6)[:, :3] *= 2e-3  # up to a few mm of travel
+        d.reshape(-1, 6)[:, 3:] *= 0.05
+        alphas = []
+        for s in (o, g):
+            s.check(s.set_state(q, np.zeros_like(q), q))
+            alphas.append(s.step_bound(d))
+        print(f"trial {trial}: alpha oracle {alphas[0]!r} gpu {alphas[1]!r}")
+        assert alphas[0] < 1.0
+        assert abs(alphas[1] - alphas[0]) <= 1e-12
+
+
+def test_step_bound_plane_matches_oracle(oracle_lib, gpu_lib):
+    import test_oracle_stage_pins as pins
+
+    print("[gpu] step bound, segment driven at a plane", flush=True)
+    out = []
+    for lib in (oracle_lib, gpu_lib):
+        s, _, _ = pins._bound_solver(lib, 3.0e-3)
+        q, _ = s.get_state()
+        d = np.zeros_like(q)
+        d[8] = d[14] = -0.01
+        d[6] = d[12] = 0.004
+        out.append((s.step_bound(d), s.step_bound(2 * d), s.step_bound(-d)))
+    assert out[1] == out[0]
+
+
+def test_sweep_matches_oracle(oracle_lib, gpu_lib):
+    """The collider swelling outward at the pulled-in state: small swells
+    pass, larger ones fail on the same (element, triangle)."""
+    print("[gpu] collider sweep, sphere scene pulled in", flush=True)
+    scene, o, g = _sphere_pair(oracle_lib, gpu_lib)
+    q, _ = _pulled_in(o, seed=10, lo=2e-4, hi=1.6e-3)
+    verdicts = []
+    for swell in (1.0005, 1.002, 1.01, 1.03):
+        res = []
+        for s in (o, g):
+            s.check(s.set_state(q, np.zeros_like(q), q))
+            s.check(s.set_collider_state(scene.vertices * swell, scene.vertices))
+            res.append(s.sweep_check())
+        (ok_o, fo), (ok_g, fg) = res
+        print(f"swell {swell}: oracle {ok_o} ({fo.element_index}, {fo.collider_triangle_index}) "
+              f"gpu {ok_g} ({fg.element_index}, {fg.collider_triangle_index})")
+        assert ok_g == ok_o
+        if not ok_o:
+            assert (fg.strand_index, fg.element_index, fg.collider_triangle_index) == \
+                   (fo.strand_index, fo.element_index, fo.collider_triangle_index)
+            assert abs(fg.distance - fo.distance) <= 1e-15
+            assert abs(fg.collider_substep_displacement - fo.collider_substep_displacement) <= 1e-18
+        verdicts.append(ok_o)
+    assert True in verdicts and False in verdicts
+
+
+def test_steepest_direction_matches_oracle(oracle_lib, gpu_lib):
+    print("[gpu] forced steepest-descent direction", flush=True)
+    scene, o, g = _sphere_pair(oracle_lib, gpu_lib)
+    q, rng = _pulled_in(o, seed=11)
+    v = rng.uniform(-1e-3, 1e-3, q.size)
+    out = []
+    for s in (o, g):
+        s.check(s.set_state(q, v, q))
+        s.check(s.set_direction_mode(1))
+        out.append(s.evaluate(scene.frame_dt / 4))
+    (_, go, do), (_, gg, dg) = out
+    assert np.dot(go, do) < 0
+    assert _rel(dg, do) <= 1e-10
+
+
+def _frames_match(results, label):
+    (xo, so, co), (xg, sg, cg) = results
+    err = float(np.max(np.abs(xo - xg)))
+    print(f"{label}: max position error {err:.3e} m; newton {so[0]} {sg[0]}; line search {so[1]} {sg[1]}; "
+          f"directions oracle {co} gpu {cg}")
+    assert err <= 1e-7
+    assert sg == so
+    assert cg == co
+
+
+def _trust_strand(lib, min_step):
+    pts = np.array([[0.01 * k, 0.0, 0.0] for k in range(8)])
+    desc = kn.default_desc(lib)
+    desc.substeps = 1
+    desc.maximum_substeps = 1
+    desc.fixed_root_nodes = 1
+    desc.maximum_element_length = 0.02
+    desc.newton_iterations = 12
+    desc.minimum_line_search_step = min_step
+    mat = kn.default_material(lib)
+    mat.young_modulus = 1.0e7
+    mat.radius = 1.0e-4
+    s = kn.HairSolver(lib, desc)
+    s.check(s.set_hair(pts, [0, len(pts)], mat))
+    s.check(s.build())
+    s.check(s.set_convergence_mode(1))
+    return s
+
+
+def test_natural_fallback_frames_match_oracle(oracle_lib, gpu_lib):
+    """Trust-limited frames with a minimum line-search step of 0.6: a block
+    direction whose Armijo test needs a shorter step is rejected and the
+    steepest-descent retry takes over (reference cuda_backend.cu:668-676)."""
+    print("[gpu] natural steepest-descent fallback", flush=True)
+    results = []
+    for lib in (oracle_lib, gpu_lib):
+        s = _trust_strand(lib, 0.6)
+        stats, counts = [], []
+        for _ in range(2):
+            s.check(s.step(0.25))
+            st = s.step_stats()
+            stats.append((st.newton_iterations, st.line_search_evaluations, st.linear_solves))
+            counts.append(s.direction_counts())
+        results.append((s.internal_positions(), tuple(zip(*stats)), counts))
+    _frames_match(results, "fallback")
+    assert all(c[1] > 0 for c in results[1][2])  # the GPU took the fallback in each frame
+
+
+def test_forced_steepest_frames_match_oracle(oracle_lib, gpu_lib):
+    print("[gpu] forced steepest descent, two frames", flush=True)
+    results = []
+    for lib in (oracle_lib, gpu_lib):
+        s = _trust_strand(lib, 1e-9)
+        s.check(s.set_direction_mode(1))
+        stats, counts = [], []
+        for _ in range(2):
+            s.check(s.step(0.25))
+            st = s.step_stats()
+            stats.append((st.newton_iterations, st.line_search_evaluations, st.linear_solves))
+            counts.append(s.direction_counts())
+        results.append((s.internal_positions(), tuple(zip(*stats)), counts))
+    _frames_match(results, "forced steepest")
+    assert all(c[0] == 0 and c[1] > 0 for c in results[1][2])
+
+
+def _grid_plane(n, h):
+    xs = (np.arange(n + 1) - n / 2) * h
+    gx, gy = np.meshgrid(xs, xs, indexing="ij")
+    verts = np.stack([gx.ravel(), gy.ravel(), np.zeros(gx.size)], axis=1)
+    tris = []
+    for i in range(n):
+        for j in range(n):
+            a, b, c, d = i * (n + 1) + j, (i + 1) * (n + 1) + j, (i + 1) * (n + 1) + j + 1, i * (n + 1) + j + 1
+            tris += [[a, b, c], [a, c, d]]
+    return verts, np.array(tris, np.uint32)
+
+
+def test_turntable_contact_frames_match_oracle(oracle_lib, gpu_lib):
+    """Three strands draped on a tessellated plane that turns under them at
+    60 deg/s and rises 0.3 mm per frame: the swept tree with current-box
+    leaves, the sweep's success path on a moving mesh, and friction with the
+    body velocity, over four animation frames."""
+    print("[gpu] turntable contact frames", flush=True)
+    verts, tris = _grid_plane(20, 5e-3)
+    strands, offsets = [], [0]
+    for k in range(3):
+        y = -0.012 + 0.012 * k
+        strands.append(np.array([[-0.035 + 0.01 * i, y + 0.0013 * i, 0.006] for i in range(7)]))
+        offsets.append(offsets[-1] + 7)
+    pts = np.concatenate(strands)
+    spin, rise, frames = math.radians(60.0) / 30.0, 3.0e-4, 4
+
+    def collider(f):
+        c, s_ = math.cos(spin * f), math.sin(spin * f)
+        return verts @ np.array([[c, -s_, 0], [s_, c, 0], [0, 0, 1]]).T + [0.0, 0.0, rise * f]
+
+    results = []
+    for lib in (oracle_lib, gpu_lib):
+        desc = kn.default_desc(lib)
+        desc.substeps = 4
+        desc.fixed_root_nodes = 1
+        desc.maximum_element_length = 0.01
+        desc.newton_iterations = 60
+        for k, v in LIFTED.items():
+            setattr(desc, k, v)
+        mat = kn.default_material(lib)
+        mat.radius = 1.0e-4
+        mat.young_modulus = 1.0e7
+        mat.contact_stiffness = 100.0
+        mat.barrier_distance = 2.0e-3
+        mat.mass_damping = 2.0
+        mat.friction = 0.3
+        s = kn.HairSolver(lib, desc)
+        s.check(s.set_hair(pts, offsets, mat))
+        s.check(s.set_collider(verts, tris))
+        s.check(s.build())
+        s.check(s.set_convergence_mode(1))
+        s.check(s.allocate_animation(frames + 1))
+        for f in range(frames + 1):
+            s.check(s.set_root_frame(f, pts))
+            s.check(s.set_collider_frame(f, collider(f)))
+        s.check(s.finalize_animation())
+        stats, counts = [], []
+        for f in range(1, frames + 1):
+            s.check(s.step_frame(f, 1.0 / 30.0))
+            st = s.step_stats()
+            stats.append((st.newton_iterations, st.line_search_evaluations, st.contact_candidate_count,
+                          st.active_contact_count, st.substeps))
+            counts.append(s.direction_counts())
+            assert st.minimum_gap > desc.minimum_gap
+        results.append((s.internal_positions(), tuple(zip(*stats)), counts))
+        s.close()
+    _frames_match(results, "turntable")
+    assert sum(results[1][1][3]) > 0  # barriers were active
+
+
+def _groom_case(lib, scene, state, start, frames):
+    """`frames` frames of the groom from animation frame `start`, restored
+    from the checkpoint `state` (written after frame start - 1)."""
+    s = grooms.build_solver(lib, scene, **LIFTED)
+    s.check(s.set_convergence_mode(1))
+    s.set_threads(16)
+    total = start + frames
+    s.check(s.allocate_animation(total))
+    for f in range(total):
+        s.check(s.set_root_frame(f, scene.frame_points(f)))
+        s.check(s.set_collider_frame(f, scene.frame_vertices(f)))
+    s.check(s.finalize_animation())
+    if state is not None:
+        s.check(s.restore_checkpoint(state))
+    stats = []
+    for f in range(start, total):
+        s.check(s.step_frame(f, scene.frame_dt))
+        st = s.step_stats()
+        stats.append((st.newton_iterations, st.line_search_evaluations, st.contact_candidate_count,
+                      st.active_contact_count, st.substeps, st.minimum_gap))
+    x = s.internal_positions()
+    s.close()
+    return x, stats
+
+
+GROOM_STRANDS, GROOM_WARM = 600, 12
+
+
+@pytest.fixture(scope="module")
+def groom_subset():
+    return grooms.head_scene(strands=GROOM_STRANDS)
+
+
+def test_groom_subset_frames_match_oracle(oracle_lib, gpu_lib, groom_subset):
+    """The first 600 strands of the 6,757-strand groom on the full rotating
+    225,184-vertex body: the GPU steps the first frames alone, then both
+    libraries continue two frames from its checkpoint, to the 1e-6 residual
+    with the floors lifted."""
+    scene = groom_subset
+    print(f"[gpu] groom subset: {scene.strand_count} strands, frames 1-{GROOM_WARM} on the GPU", flush=True)
+    s = grooms.build_solver(gpu_lib, scene, **LIFTED)
+    s.check(s.set_convergence_mode(1))
+    s.check(s.allocate_animation(GROOM_WARM + 1))
+    for f in range(GROOM_WARM + 1):
+        s.check(s.set_root_frame(f, scene.frame_points(f)))
+        s.check(s.set_collider_frame(f, scene.frame_vertices(f)))
+    s.check(s.finalize_animation())
+    for f in range(1, GROOM_WARM + 1):
+        s.check(s.step_frame(f, scene.frame_dt))
+    state = s.save_checkpoint()
+    s.close()
+    print(f"[gpu] groom subset: frames {GROOM_WARM + 1}-{GROOM_WARM + 2} on both", flush=True)
+    xo, so = _groom_case(oracle_lib, scene, state, GROOM_WARM + 1, 2)
+    xg, sg = _groom_case(gpu_lib, scene, state, GROOM_WARM + 1, 2)
+    err = float(np.max(np.abs(xo - xg)))
+    print(f"max position error {err:.3e} m; oracle {so}; gpu {sg}")
+    assert err <= 1e-7
+    for a, b in zip(so, sg):
+        assert a[:5] == b[:5]
+        assert b[5] > 1e-7
