@@ -82,6 +82,18 @@ KHS_API KhsResult khsxCopyBlocks(const KhsSolver *solver, double *diagonal, doub
 KHS_API KhsResult khsxBroadphasePairs(KhsSolver *solver, uint32_t *pairs,
                                       uint64_t capacity, uint64_t *count);
 
+/* khsxBroadphasePairs without allocating device memory. No counterpart
+   in the reference: it serves callers that cannot allocate (HIP graph
+   capture, tight serving loops, harnesses that forbid hipMalloc). The GPU
+   library stages the pairs in a fixed device array of at most
+   KHSX_NO_ALLOC_PAIRS pairs (8 MiB, held by each GPU until the process
+   ends) and sorts them in the caller's buffer; capacity above that limit is
+   refused with KHS_ERROR_INVALID_ARGUMENT. Calls that overlap on one GPU
+   share the staging array and race; each call clears what it uses. */
+#define KHSX_NO_ALLOC_PAIRS 1048576u
+KHS_API KhsResult khsxBroadphasePairsNoAlloc(KhsSolver *solver, uint32_t *pairs,
+                                             uint64_t capacity, uint64_t *count);
+
 /* Self-check of the device broad-phase structure; violations[6]:
    0 triangle boxes not enclosed by an ancestor's stored box, 1 broken parent
    chains, 2 out-of-order sort keys, 3 leaves whose stored box is not their

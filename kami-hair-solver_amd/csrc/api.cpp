@@ -278,6 +278,11 @@ KhsResult khsxBroadphasePairs(KhsSolver *s, uint32_t *pairs, uint64_t capacity, 
     return guarded(s, [&] { return s->impl.broadphase_pairs(pairs, capacity, count); });
 }
 
+KhsResult khsxBroadphasePairsNoAlloc(KhsSolver *s, uint32_t *pairs, uint64_t capacity, uint64_t *count)
+{
+    return guarded(s, [&] { return s->impl.broadphase_pairs_no_alloc(pairs, capacity, count); });
+}
+
 KhsResult khsxCopyBlocks(const KhsSolver *s, double *diagonal, double *upper, uint32_t capacity)
 {
     return guarded_const(s, [&] { return s->impl.copy_blocks(diagonal, upper, capacity); });

@@ -9,10 +9,12 @@
 
 #include <rocprim/device/device_radix_sort.hpp>
 
+#include <algorithm>
 #include <cerrno>
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <sstream>
 #include <stdexcept>
 #include <thread>
@@ -95,6 +97,45 @@ std::vector<Vec3> to_vec3(const KhsVec3 *p, uint32_t n)
     std::vector<Vec3> out(n);
     for (uint32_t i = 0; i < n; ++i) out[i] = {p[i].x, p[i].y, p[i].z};
     return out;
+}
+
+// Staging of the allocation-free broad-phase query
+// (khsxBroadphasePairsNoAlloc): a pair counter on its own 256-byte line,
+// then at most kNoAllocPairs (element, triangle) pairs, 8 MiB in all. Each
+// GPU holds its own copy until the process ends. Host code reaches it only
+// through the address hipGetSymbolAddress returns for the current device
+// (no_alloc_staging), never through the array's name.
+constexpr uint64_t kNoAllocPairs = KHSX_NO_ALLOC_PAIRS;
+constexpr uint64_t kNoAllocHeader = 64;  // uint32 words before the pairs
+static __device__ __align__(256) uint32_t scratch[kNoAllocHeader + 2 * kNoAllocPairs];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_staging_mutex;
+uint32_t *g_staging[kMaxDevices] = {};
+
+// Looks up the staging array's device address once per device, with that
+// device current; later calls return the kept address.
+void prepare_no_alloc_staging()
+{
+    int device = 0;
+    HIP_TRY(hipGetDevice(&device));
+    if (device < 0 || device >= kMaxDevices) throw std::runtime_error("HIP device ordinal out of range.");
+    std::lock_guard<std::mutex> lock(g_staging_mutex);
+    if (g_staging[device]) return;
+    void *ptr = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw std::runtime_error("No device address for the broad-phase staging array.");
+    g_staging[device] = static_cast<uint32_t *>(ptr);
+}
+
+uint32_t *no_alloc_staging()
+{
+    int device = 0;
+    HIP_TRY(hipGetDevice(&device));
+    std::lock_guard<std::mutex> lock(g_staging_mutex);
+    if (device < 0 || device >= kMaxDevices || !g_staging[device])
+        throw std::runtime_error("The broad-phase staging array was not set up on this device.");
+    return g_staging[device];
 }
 
 }  // namespace
@@ -603,6 +644,7 @@ KhsResult Solver::upload_model()
         return KHS_ERROR_INVALID_STATE;
     }
     HIP_TRY(hipSetDevice(0));
+    prepare_no_alloc_staging();
     dev_ = std::make_unique<DeviceState>();
     DeviceState &d = *dev_;
     HIP_TRY(hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking));
@@ -2048,6 +2090,55 @@ KhsResult Solver::broadphase_pairs(uint32_t *pairs, uint64_t capacity, uint64_t 
             pairs[2 * i] = uint32_t(keyed[i] >> 32);
             pairs[2 * i + 1] = uint32_t(keyed[i]);
         }
+    }
+    return (pairs && total > capacity) ? KHS_ERROR_INVALID_ARGUMENT : KHS_OK;
+}
+
+// khsxBroadphasePairs without any allocation: the pairs are staged in the
+// file-scope array above (its counter cleared by a fill kernel) and sorted
+// in the caller's buffer. Overlapping calls on one GPU share the array and
+// race.
+KhsResult Solver::broadphase_pairs_no_alloc(uint32_t *pairs, uint64_t capacity, uint64_t *count)
+{
+    if (!built_ || !count) return KHS_ERROR_INVALID_ARGUMENT;
+    if (capacity > kNoAllocPairs) {
+        error_ = "khsxBroadphasePairsNoAlloc stages at most " + std::to_string(kNoAllocPairs) + " pairs; capacity " +
+                 std::to_string(capacity) + " is larger.";
+        return KHS_ERROR_INVALID_ARGUMENT;
+    }
+    DeviceState &d = *dev_;
+    if (!d.T) {
+        *count = 0;
+        return KHS_OK;
+    }
+    uint32_t *staging = no_alloc_staging();
+    unsigned long long *counter = reinterpret_cast<unsigned long long *>(staging);
+    uint32_t *staged = staging + kNoAllocHeader;
+    rebuild_bvh(false);
+    StepParams p{};
+    p.contact_pad = material_.radius + material_.collider_offset + material_.barrier_distance;
+    hipLaunchKernelGGL(dev::k_fill_u64, dim3(1), dim3(64), 0, d.stream, counter, size_t(1), 0ull);
+    hipLaunchKernelGGL(dev::k_broadphase_pairs, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.rods(),
+                       d.collider(), p, d.q.get(), staged, (unsigned long long)capacity, counter);
+    HIP_TRY(hipGetLastError());
+    unsigned long long total = 0;
+    HIP_TRY(hipMemcpyAsync(&total, counter, sizeof(total), hipMemcpyDeviceToHost, d.stream));
+    HIP_TRY(hipStreamSynchronize(d.stream));
+    *count = total;
+    if (pairs && capacity) {
+        const uint64_t n = std::min<uint64_t>(total, capacity);
+        HIP_TRY(hipMemcpyAsync(pairs, staged, n * 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, d.stream));
+        HIP_TRY(hipStreamSynchronize(d.stream));
+        struct Pair {
+            uint32_t element, triangle;
+            bool operator<(const Pair &o) const
+            {
+                return element != o.element ? element < o.element : triangle < o.triangle;
+            }
+        };
+        static_assert(sizeof(Pair) == 2 * sizeof(uint32_t) && alignof(Pair) == alignof(uint32_t), "pair layout");
+        Pair *first = reinterpret_cast<Pair *>(pairs);
+        std::sort(first, first + n);
     }
     return (pairs && total > capacity) ? KHS_ERROR_INVALID_ARGUMENT : KHS_OK;
 }

@@ -63,6 +63,7 @@ public:
                                  uint32_t capacity) const;
     KhsResult test_evaluate(double dt, double *gradient, double *direction, KhsxEvaluation *out);
     KhsResult broadphase_pairs(uint32_t *pairs, uint64_t capacity, uint64_t *count);
+    KhsResult broadphase_pairs_no_alloc(uint32_t *pairs, uint64_t capacity, uint64_t *count);
     KhsResult check_bvh(uint64_t *violations);
     KhsResult copy_blocks(double *diagonal, double *upper, uint32_t capacity) const;
     KhsResult set_convergence_mode(uint32_t mode);

@@ -39,6 +39,8 @@ PHASE_CCD, PHASE_LINE_SEARCH, PHASE_FINISHED, PHASE_FAILED = 4, 5, 6, 7
 
 FAILURE_NONE, FAILURE_SWEEP, FAILURE_BARRIER_INFEASIBLE, FAILURE_NONLINEAR = 0, 1, 2, 3
 
+KHSX_NO_ALLOC_PAIRS = 1048576  # staging limit of khsxBroadphasePairsNoAlloc
+
 
 class KhsVec3(C.Structure):
     _fields_ = [("x", C.c_double), ("y", C.c_double), ("z", C.c_double)]
@@ -296,6 +298,7 @@ EXT_FUNCTIONS = {
     "khsxCopyElementModel": (C.c_int, [_P, _P, _P, _P, _P, C.c_uint32]),
     "khsxEvaluate": (C.c_int, [_P, C.c_double, _P, _P, _D(KhsxEvaluation)]),
     "khsxBroadphasePairs": (C.c_int, [_P, _P, C.c_uint64, _D(C.c_uint64)]),
+    "khsxBroadphasePairsNoAlloc": (C.c_int, [_P, _P, C.c_uint64, _D(C.c_uint64)]),
     "khsxCheckBroadphase": (C.c_int, [_P, _D(C.c_uint64)]),
     "khsxCopyBlocks": (C.c_int, [_P, _P, _P, C.c_uint32]),
     "khsxSetConvergenceMode": (C.c_int, [_P, C.c_uint32]),
@@ -547,6 +550,16 @@ class HairSolver:
         if count.value:
             self.check(self.lib.khsxBroadphasePairs(self.handle, _ptr(out), count.value, C.byref(count)))
         return out
+
+    def broadphase_pairs_no_alloc(self, capacity=None):
+        """The allocation-free broad phase; capacity defaults to the count."""
+        count = C.c_uint64(0)
+        self.check(self.lib.khsxBroadphasePairsNoAlloc(self.handle, None, 0, C.byref(count)))
+        cap = count.value if capacity is None else capacity
+        out = np.zeros((cap, 2), dtype=np.uint32)
+        if cap:
+            self.check(self.lib.khsxBroadphasePairsNoAlloc(self.handle, _ptr(out), cap, C.byref(count)))
+        return out[:count.value]
 
     def blocks(self):
         n = self.node_count()

@@ -2436,6 +2436,12 @@ KhsResult khsxSetThreadCount(KhsSolver *s, uint32_t threads)
     return KHS_OK;
 }
 
+KhsResult khsxBroadphasePairsNoAlloc(KhsSolver *s, uint32_t *pairs, uint64_t cap, uint64_t *count)
+{
+    if (cap > KHSX_NO_ALLOC_PAIRS) return KHS_ERROR_INVALID_ARGUMENT;
+    return guarded(s, [&] { return s->impl.broadphase_pairs(pairs, cap, count); });
+}
+
 KhsResult khsxSetColliderState(KhsSolver *s, const double *current, const double *previous, uint32_t count)
 {
     return guarded(s, [&] { return s->impl.set_collider_state(current, previous, count); });
