@@ -69,6 +69,35 @@ def setup_animation(lib, scene, frames, **desc):
     return s
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def cpu_config1(threads):
+    """The 64-strand sphere scene, one frame at 4 substeps to a 1e-6
+    residual, on the oracle with `threads` threads."""
+    lib = kn.load_library(os.path.join(REPO, "oracle", "_build", "libkhs_oracle.so"))
+    scene = grooms.small_sphere_scene()
+    s = grooms.build_solver(lib, scene, **LIFTED)
+    s.check(s.set_convergence_mode(1))
+    s.check(s.set_threads(threads))
+    t0 = time.perf_counter()
+    s.check(s.step(scene.frame_dt))
+    elapsed = time.perf_counter() - t0
+    nodes, substeps = s.node_count(), s.step_stats().substeps
+    s.close()
+    return {"value": nodes * substeps / elapsed, "unit": "node-substeps/s", "cores": threads,
+            "sample": f"64 strands ({nodes} nodes) on the 1,280-triangle icosphere, 1 frame ({substeps} substeps), "
+                      f"{elapsed:.2f} s"}
+
+
 def cpu_baseline(scene, strands, frames, threads):
     oracle_path = os.path.join(REPO, "oracle", "_build", "libkhs_oracle.so")
     lib = kn.load_library(oracle_path)
@@ -89,7 +118,7 @@ def cpu_baseline(scene, strands, frames, threads):
         "kind": "port",
         "sample": f"{sub.strand_count} strands ({nodes} nodes) of the same groom and the full 225,184-vertex body, "
                   f"frames 1-{frames} ({substeps} substeps) to a 1e-6 residual, {elapsed:.1f} s on {threads} "
-                  f"threads ({platform.processor() or platform.machine()}, {os.cpu_count()} logical CPUs)",
+                  f"threads ({cpu_model()}, {os.cpu_count()} logical CPUs)",
     }
 
 
@@ -144,6 +173,44 @@ def run_batched(lib, scene, jobs, warmup, frames):
     }
 
 
+def run_mixed(lib, scene, warmup, frames, fp64_positions):
+    """The same workload with precision mode 1 (FP32 blocks and block solve,
+    FP64 residual, energies, barrier, step bound and convergence test) from
+    the same start: node-substeps/s over `frames` frames after `warmup`, the
+    Newton work it took, the relative residual its substeps reached, and the
+    position error against the FP64 step after the same frames."""
+    s = setup_animation(lib, scene, warmup + frames + 1, **LIFTED)
+    s.check(s.set_precision_mode(1))
+    for f in range(1, warmup + 1):
+        s.check(s.step_frame(f, scene.frame_dt))
+    substeps = newton = ls_evals = 0
+    residuals = []
+    t0 = time.perf_counter()
+    for f in range(warmup + 1, warmup + frames + 1):
+        s.check(s.step_frame(f, scene.frame_dt))
+        st = s.step_stats()
+        substeps += st.substeps
+        newton += st.newton_iterations
+        ls_evals += st.line_search_evaluations
+        residuals.append(st.relative_residual_norm)
+    elapsed = time.perf_counter() - t0
+    nodes = s.node_count()
+    err = float(np.max(np.abs(s.internal_positions() - fp64_positions))) if fp64_positions is not None else None
+    s.close()
+    return {
+        "value": nodes * substeps / elapsed,
+        "unit": "node-substeps/s",
+        "frames": frames,
+        "form": "FP32 Gauss-Newton blocks and block solve; FP64 assembly, residual, barrier, CCD, line search "
+                "and convergence test",
+        "newton_iterations_per_substep": newton / max(substeps, 1),
+        "line_search_evaluations_per_substep": ls_evals / max(substeps, 1),
+        "relative_residual_last_substep_median": float(np.median(residuals)),
+        "reaches_1e-6_residual": bool(max(residuals) <= 1e-6),
+        "max_position_error_vs_fp64_m": err,
+    }
+
+
 def copy_calibration(mib):
     """One device-to-device copy of `mib` MiB (reads and writes mib MiB),
     the known kernel against which the PMC traffic counters are scaled. It
@@ -173,10 +240,13 @@ def main():
     ap.add_argument("--cpu-strands", type=int, default=6757, help="strands of the CPU sample (all by default)")
     ap.add_argument("--cpu-frames", type=int, default=2, help="frames of the CPU sample")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-strands-one-thread", type=int, default=400,
+                    help="strands of the one-thread CPU sample (frame 1)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one device copy of this size first (for scaling PMC traffic counters)")
     ap.add_argument("--batch-jobs", type=int, default=4, help="concurrent jobs of the batched form (0: skip)")
+    ap.add_argument("--mixed-frames", type=int, default=8, help="timed frames of the mixed-precision form (0: skip)")
     ap.add_argument("--batch-frames", type=int, default=None,
                     help="timed frames per job of the batched form (default: --steps)")
     args = ap.parse_args()
@@ -206,9 +276,16 @@ def main():
     s.check(s.set_kernel_timing(True))
     substep_ms, total_substeps, newton, ls_evals, pairs = [], 0, 0, 0, 0
     min_gap, penetrations = float("inf"), 0
+    mixed_last = args.warmup + args.mixed_frames
+    phase_ms = {"assembly": 0.0, "collision": 0.0, "optimization": 0.0}
+    fp64_at_mixed_last = None
     t0 = time.perf_counter()
     for f in range(args.warmup + 1, args.warmup + args.steps + 1):
         s.check(s.step_frame(f, scene.frame_dt))
+        if f == mixed_last:
+            # the positions the mixed form is compared with (outside the
+            # step; the copy costs a few milliseconds of the timed region)
+            fp64_at_mixed_last = s.internal_positions()
         st = s.step_stats()
         total_substeps += st.substeps
         newton += st.newton_iterations
@@ -217,6 +294,9 @@ def main():
         min_gap = min(min_gap, st.minimum_gap)
         if st.minimum_gap <= 1e-7:
             penetrations += 1
+        gs = s.gpu_stats()
+        for k in phase_ms:
+            phase_ms[k] += getattr(gs, f"last_{k}_milliseconds")
         substep_ms.extend(s.substep_times().tolist())
     elapsed = time.perf_counter() - t0
     timing = s.kernel_timing()
@@ -265,6 +345,8 @@ def main():
         "line_search_evaluations_per_substep": ls_evals / max(total_substeps, 1),
         "minimum_gap_m": min_gap,
         "penetrating_frames": penetrations,
+        # khsGetGpuStats phase times summed over the timed frames
+        "phase_ms": phase_ms,
         "kernel_ms": {
             "assembly_avg": timing.assembly_ms / max(timing.assembly_launches, 1),
             "assembly_launches": timing.assembly_launches,
@@ -301,24 +383,29 @@ def main():
             "frac": step_bytes / elapsed / 1e9 / HBM_PEAK_GBPS,
         },
         "reference_derived_node_substeps_per_s": 3.16e5,
-        # The metric compares FP64 with a mixed-precision form; this build has
-        # none (tests/test_mixed_precision.py measures why).
-        "mixed_precision": {
-            "value": None,
-            "reason": "not built: per-strand systems have condition numbers near 3e12; an FP32 block solve "
-                      "leaves relative residuals near 1e-3 (FP64: 3e-10), and non-packed FP32 vector issue "
-                      "is no faster than FP64 on MI355X",
-        },
     }
-    if args.batch_jobs > 0:
+    if args.mixed_frames > 0:
         s.close()
+        t = time.perf_counter()
+        result["mixed_precision"] = run_mixed(lib, scene, args.warmup, args.mixed_frames, fp64_at_mixed_last)
+        log(f"[bench] mixed form: {args.mixed_frames} frames in {time.perf_counter() - t:.1f} s, "
+            f"{result['mixed_precision']['value']:.4g} node-substeps/s")
+    if args.batch_jobs > 0:
+        if args.mixed_frames <= 0:
+            s.close()
         t = time.perf_counter()
         result["batched"] = run_batched(lib, scene, args.batch_jobs, args.warmup, args.batch_frames or args.steps)
         log(f"[bench] batched form: {args.batch_jobs} jobs in {time.perf_counter() - t:.1f} s, "
             f"{result['batched']['value']:.4g} node-substeps/s")
     if not args.no_cpu_baseline:
         t = time.perf_counter()
-        result["cpu_baseline"] = cpu_baseline(scene, args.cpu_strands, args.cpu_frames, args.cpu_threads)
+        base = cpu_baseline(scene, args.cpu_strands, args.cpu_frames, args.cpu_threads)
+        base["cpu_model"] = cpu_model()
+        # one thread on a bounded sample of the same groom (the first
+        # strands, the full body, frame 1), and the 64-strand scene in full
+        base["one_thread"] = cpu_baseline(scene, args.cpu_strands_one_thread, 1, 1)
+        base["sphere_scene"] = {"one_thread": cpu_config1(1), "threads": cpu_config1(args.cpu_threads)}
+        result["cpu_baseline"] = base
         log(f"[bench] cpu baseline in {time.perf_counter() - t:.1f} s")
     print(json.dumps(result), flush=True)
 

@@ -217,13 +217,13 @@ typedef struct KhsFailureDiagnostics {
     KhsVec3 collider_point;
 } KhsFailureDiagnostics;
 
-/* Library identity and defaults (reference api.cpp:222-274). */
+/* Library identity and defaults (reference src/api.cpp:64-116). */
 KHS_API uint32_t khsGetAbiVersion(void);
 KHS_API KhsResult khsGetGpuInfo(KhsGpuInfo *info);
 KHS_API void khsDefaultSolverDesc(KhsSolverDesc *desc);
 KHS_API void khsDefaultHairMaterial(KhsHairMaterial *material);
 
-/* Returns NULL when the desc is invalid (reference api.cpp:195-205, 276-281). */
+/* Returns NULL when the desc is invalid (reference src/api.cpp:37-51, 118-125). */
 KHS_API KhsSolver *khsCreate(const KhsSolverDesc *desc);
 KHS_API void khsDestroy(KhsSolver *solver);
 

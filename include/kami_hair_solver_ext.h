@@ -106,12 +106,16 @@ KHS_API KhsResult khsxCheckBroadphase(KhsSolver *solver, uint64_t *violations);
    so the desc tolerances apply as given (used to run to a 1e-6 residual). */
 KHS_API KhsResult khsxSetConvergenceMode(KhsSolver *solver, uint32_t mode);
 
-/* Arithmetic mode of the step. 0 (default): FP64 throughout, the only mode
-   either library accepts. 1 is reserved for a mixed-precision step (FP32
-   assembly and block solve with FP64 refinement) and is rejected with
-   KHS_ERROR_INVALID_ARGUMENT: the per-strand systems have condition numbers
-   near 3e12, an FP32 block solve leaves relative residuals near 1e-3, and
-   non-packed FP32 vector instructions issue no faster than FP64 ones. */
+/* Arithmetic mode of the step. 0 (default): FP64 throughout. 1: mixed
+   precision: the Gauss-Newton blocks and the gradient are rounded to FP32
+   as the block solve loads them, and every Schur complement, inverse and
+   Thomas factor is formed and stored in FP32; the direction is widened to
+   FP64. Assembly, energies, barrier, step bound, line search and the
+   convergence test stay FP64, so each Newton iteration corrects the FP32
+   direction from an FP64 residual. The per-strand systems have condition
+   numbers near 3e12: the FP32 direction leaves relative linear residuals
+   near 1e-3 and the Newton loop converges far more slowly (measured in
+   tests/test_mixed_precision.py). */
 KHS_API KhsResult khsxSetPrecisionMode(KhsSolver *solver, uint32_t mode);
 
 /* Device-side timing of the step, from HIP events recorded on the

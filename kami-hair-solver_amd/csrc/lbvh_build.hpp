@@ -1,15 +1,12 @@
 // Device construction of the collider LBVH: triangle boxes, Morton codes,
 // Karras hierarchy over the sorted codes, bottom-up boxes, and a structural
-// self-check. Compiles for the host too (thread loops emulated by the CPU
-// test of the construction).
+// self-check. Compiles for the host too, when the includer first provides
+// host stand-ins for the HIP thread indices and atomics (the CPU test of the
+// construction runs each kernel thread by thread).
 #pragma once
 
 #include "kernels.hpp"
 #include "lbvh.hpp"
-
-#if !defined(__HIPCC__)
-#include "host_thread_shim.hpp"
-#endif
 
 namespace khs {
 namespace dev {

@@ -1272,9 +1272,10 @@ void Solver::make_direction_async()
     const RodView rv = d.rods();
     unsigned *failed = d.solve_failed.get() + d.failed_slot;
     hipEvent_t t0 = d.tic();
-    hipLaunchKernelGGL(dev::k_block_solve, dim3(d.solve_blocks()), dim3(kWave), kSolveLdsReserve, d.stream, rv,
-                       d.solve_order.get(), d.Dblk.get(), d.Ublk.get(), d.grad.get(), d.Cp.get(), d.dp.get(),
-                       d.dir.get(), failed, d.solve_failed.get() + (d.failed_slot ^ 1u), d.partials.get(), d.skip());
+    hipLaunchKernelGGL(precision_mode_ ? dev::k_block_solve<float> : dev::k_block_solve<double>,
+                       dim3(d.solve_blocks()), dim3(kWave), kSolveLdsReserve, d.stream, rv, d.solve_order.get(),
+                       d.Dblk.get(), d.Ublk.get(), d.grad.get(), d.Cp.get(), d.dp.get(), d.dir.get(), failed,
+                       d.solve_failed.get() + (d.failed_slot ^ 1u), d.partials.get(), d.skip());
     d.failed_slot ^= 1u;
     d.toc(t0, d.totals.solve_ms, d.totals.solve_launches);
     // the solve leaves the direction's statistics in one partial per block
@@ -1543,6 +1544,7 @@ KhsResult Solver::run_frame(double frame_dt, const double *root_a, const double 
     frame_count_.store(frame_count);
     iteration_.store(0);
     HIP_TRY(hipEventRecord(d.frame_begin, d.stream));
+    const KhsxKernelTiming before = d.totals;  // phase times of this frame are the differences
     const size_t dof_bytes = d.q.bytes();
     HIP_TRY(hipMemcpyAsync(d.snap_q.get(), d.q.get(), dof_bytes, hipMemcpyDeviceToDevice, d.stream));
     HIP_TRY(hipMemcpyAsync(d.snap_v.get(), d.vel.get(), dof_bytes, hipMemcpyDeviceToDevice, d.stream));
@@ -1657,6 +1659,19 @@ KhsResult Solver::run_frame(double frame_dt, const double *root_a, const double 
     float ms = 0.0f;
     HIP_TRY(hipEventElapsedTime(&ms, d.frame_begin, d.frame_end));
     gpu_stats_.last_frame_milliseconds = ms;
+    // Phase times from the per-launch HIP events (with kernel timing on):
+    // assembly = gradient and line-search evaluations with their list
+    // passes, collision = tree rebuilds, sweeps and step bounds,
+    // optimization = the block solves.
+    if (d.timing) {
+        gpu_stats_.last_assembly_milliseconds =
+            (d.totals.assembly_ms - before.assembly_ms) + (d.totals.energy_ms - before.energy_ms);
+        gpu_stats_.last_collision_milliseconds = d.totals.collision_ms - before.collision_ms;
+        gpu_stats_.last_optimization_milliseconds = d.totals.solve_ms - before.solve_ms;
+    } else {
+        gpu_stats_.last_assembly_milliseconds = gpu_stats_.last_collision_milliseconds =
+            gpu_stats_.last_optimization_milliseconds = 0.0;
+    }
     d.substep_ms.assign(substeps, 0.0);
     for (uint32_t k = 0; k < substeps; ++k) {
         float sms = 0.0f;
@@ -2043,10 +2058,10 @@ KhsResult Solver::test_evaluate(double dt, double *gradient, double *direction, 
     } else if (direction) {
         const RodView rv = d.rods();
         hipLaunchKernelGGL(dev::k_fill_u32, dim3(1), dim3(64), 0, d.stream, d.flag.get(), size_t(1), 0u);
-        hipLaunchKernelGGL(dev::k_block_solve, dim3(d.solve_blocks()), dim3(kWave), kSolveLdsReserve, d.stream, rv,
-                           d.solve_order.get(), d.Dblk.get(), d.Ublk.get(), d.grad.get(), d.Cp.get(), d.dp.get(),
-                           d.dir.get(), d.flag.get(), (unsigned *)nullptr, d.partials.get(),
-                           (const unsigned *)nullptr);
+        hipLaunchKernelGGL(precision_mode_ ? dev::k_block_solve<float> : dev::k_block_solve<double>,
+                           dim3(d.solve_blocks()), dim3(kWave), kSolveLdsReserve, d.stream, rv, d.solve_order.get(),
+                           d.Dblk.get(), d.Ublk.get(), d.grad.get(), d.Cp.get(), d.dp.get(), d.dir.get(),
+                           d.flag.get(), (unsigned *)nullptr, d.partials.get(), (const unsigned *)nullptr);
         HIP_TRY(hipGetLastError());
         unsigned failed = 0;
         HIP_TRY(hipMemcpyAsync(&failed, d.flag.get(), sizeof(unsigned), hipMemcpyDeviceToHost, d.stream));
@@ -2182,9 +2197,8 @@ KhsResult Solver::set_convergence_mode(uint32_t mode)
 
 KhsResult Solver::set_precision_mode(uint32_t mode)
 {
-    if (mode > 0) {
-        error_ = "Only the FP64 step (mode 0) is available: an FP32 block solve of these strand systems "
-                 "(condition numbers near 3e12) leaves relative residuals near 1e-3.";
+    if (mode > 1) {
+        error_ = "Precision mode must be 0 (FP64) or 1 (FP32 block solve).";
         return KHS_ERROR_INVALID_ARGUMENT;
     }
     precision_mode_ = mode;

@@ -1195,6 +1195,11 @@ __device__ __forceinline__ double row_bcast(double v)
 {
     return __builtin_amdgcn_mov_dpp(v, 0x150 + L, 0xf, 0xf, false);
 }
+template <int L>
+__device__ __forceinline__ float row_bcast(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x150 + L, 0xf, 0xf, false));
+}
 
 // f(std::integral_constant<int, 0>), ..., f(std::integral_constant<int, N - 1>).
 template <class F, int... Is>
@@ -1234,6 +1239,30 @@ __device__ __forceinline__ void buffer_store(__amdgpu_buffer_rsrc_t r, uint32_t 
     v[1] = unsigned(__double2hiint(x));
     __builtin_amdgcn_raw_buffer_store_b64(v, r, int(offset), 0, 0);
 }
+__device__ __forceinline__ float buffer_load_f32(__amdgpu_buffer_rsrc_t r, uint32_t offset)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, int(offset), 0, 0));
+}
+__device__ __forceinline__ void buffer_store_f32(__amdgpu_buffer_rsrc_t r, uint32_t offset, float x)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, int(offset), 0, 0);
+}
+// Loads and stores of the solve's own factors (C', d') at their stored width.
+__device__ __forceinline__ void factor_store(__amdgpu_buffer_rsrc_t r, uint32_t offset, double x)
+{
+    buffer_store(r, offset, x);
+}
+__device__ __forceinline__ void factor_store(__amdgpu_buffer_rsrc_t r, uint32_t offset, float x)
+{
+    buffer_store_f32(r, offset, x);
+}
+template <class T>
+__device__ __forceinline__ T factor_load(__amdgpu_buffer_rsrc_t r, uint32_t offset)
+{
+    if constexpr (sizeof(T) == 8) return buffer_load(r, offset);
+    else return buffer_load_f32(r, offset);
+}
+
 
 // Block Thomas per strand on the free nodes; fixed root nodes are
 // eliminated and get a zero direction (cuda_backend.cu:866-881).
@@ -1267,23 +1296,32 @@ __device__ __forceinline__ void buffer_store(__amdgpu_buffer_rsrc_t r, uint32_t 
 // nodes' inputs; a slot is refilled with the node that many steps later as
 // soon as its step is done. Loads and stores are raw buffer operations
 // predicated by offset, so no step waits for the stores of the one before.
+//
+// T is the arithmetic type: double for the FP64 step; float for the mixed
+// step, where the FP64 blocks and gradient are rounded to FP32 as they are
+// loaded, every Schur complement, inverse and factor is formed and stored in
+// FP32 (C' and d' at half the bytes), and the direction is widened to FP64
+// as it is stored.
 constexpr int kSolveLanes = 16;
 constexpr int kSolveGroups = kWave / kSolveLanes;
 constexpr int kSolveAhead = 3;
 constexpr int kBackAhead = 8;
 constexpr uint64_t kMaxSolveNodes = kMaxBufferBytes / 288;  // 36 doubles per node block
 
+template <class T>
 struct ForwardSlot {
-    double d[6];  // D column
-    double g[6];  // gradient (rhs lane)
-    double u[6];  // U column
+    T d[6];  // D column
+    T g[6];  // gradient (rhs lane)
+    T u[6];  // U column
 };
 
+template <class T>
 __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t *order, const double *Dblk,
                                                        const double *Ublk, const double *grad, double *Cp,
                                                        double *dp, double *dir, unsigned *failed,
                                                        unsigned *next_failed, Partial *partials, const unsigned *stop)
 {
+    constexpr uint32_t E = sizeof(T), EB = 36 * E, EV = 6 * E;  // factor entry, block and vector bytes
     // the next solve's flag starts at zero (the two alternate)
     if (blockIdx.x == 0 && threadIdx.x == 0 && next_failed) *next_failed = 0;
     if (stop && *stop) {
@@ -1318,36 +1356,36 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
 
     // node t's inputs: D column j (acol), gradient (rhs lane), U column j
     // (acol, when a node follows)
-    auto fetch = [&](ForwardSlot &f, uint32_t t) __attribute__((always_inline)) {
+    auto fetch = [&](ForwardSlot<T> &f, uint32_t t) __attribute__((always_inline)) {
         const uint32_t n = begin + t;
         const uint32_t od = (acol && t < len) ? 288 * n + 8 * j : kNoAccess;
         const uint32_t og = (vlane && t < len) ? 48 * n : kNoAccess;
         const uint32_t ou = (acol && t + 1 < len) ? 288 * n + 8 * j : kNoAccess;
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
-            f.d[r] = buffer_load(rD, od + 48 * r);
-            f.g[r] = buffer_load(rG, og + 8 * r);
-            f.u[r] = buffer_load(rU, ou + 48 * r);
+            f.d[r] = T(buffer_load(rD, od + 48 * r));
+            f.g[r] = T(buffer_load(rG, og + 8 * r));
+            f.u[r] = T(buffer_load(rU, ou + 48 * r));
         }
     };
 
     bool ok = true;
-    double p[6];  // previous C' column (d' on the rhs lane)
+    T p[6];  // previous C' column (d' on the rhs lane)
 #pragma unroll
-    for (int r = 0; r < 6; ++r) p[r] = 0.0;
-    ForwardSlot ring[kSolveAhead];
+    for (int r = 0; r < 6; ++r) p[r] = T(0);
+    ForwardSlot<T> ring[kSolveAhead];
 #pragma unroll
     for (int a = 0; a < kSolveAhead; ++a) fetch(ring[a], a);
-    double wprev[6];  // the previous node's U column (lanes 0..5); zero before the first
+    T wprev[6];  // the previous node's U column (lanes 0..5); zero before the first
 #pragma unroll
-    for (int k = 0; k < 6; ++k) wprev[k] = 0.0;
+    for (int k = 0; k < 6; ++k) wprev[k] = T(0);
 
-    auto forward = [&](ForwardSlot &slot, uint32_t t) __attribute__((always_inline)) {
+    auto forward = [&](ForwardSlot<T> &slot, uint32_t t) __attribute__((always_inline)) {
         const uint32_t n = begin + t;
-        double col[6], w[6];
+        T col[6], w[6];
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
-            col[r] = acol ? slot.d[r] : (vlane ? -slot.g[r] : ((icol && r == j - 6) ? 1.0 : 0.0));
+            col[r] = acol ? slot.d[r] : (vlane ? -slot.g[r] : ((icol && r == j - 6) ? T(1) : T(0)));
             w[r] = slot.u[r];
         }
         // col[r] -= sum_k U'[k][r] p[k]: D - U'^T C' on the block columns,
@@ -1355,7 +1393,7 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
         // node U' and p are zero and this subtracts +0, which leaves every
         // value (signed zeros, infinities and NaNs included) unchanged.
         static_for<6>([&](auto R) {
-            double sum = 0.0;
+            T sum = T(0);
 #pragma unroll
             for (int k = 0; k < 6; ++k) sum = fma(row_bcast<R>(wprev[k]), p[k], sum);
             col[R] -= sum;
@@ -1365,13 +1403,13 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
         // The rhs lane keeps its column (the right-hand side) in rhs; the
         // Gauss-Jordan steps below also transform its col, which nothing
         // reads (lane 12 is never a pivot column).
-        double rhs[6];
+        T rhs[6];
 #pragma unroll
         for (int r = 0; r < 6; ++r) rhs[r] = col[r];
         // Gauss-Jordan inverse on the columns of [a | inv].
-        double scale = 0.0;
+        T scale = T(0);
         static_for<6>([&](auto I) { scale = fmax(scale, fabs(row_bcast<I>(col[I]))); });
-        const double shift = fmax(1.0e-12 * scale, 1.0e-24);
+        const T shift = fmax(T(1.0e-12) * scale, T(1.0e-24));
 #pragma unroll
         for (int r = 0; r < 6; ++r)
             if (acol && r == j) col[r] += shift;
@@ -1380,19 +1418,19 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
         // partial pivoting would exchange nothing and this pass performs
         // exactly its operations; otherwise (some strand of the wavefront)
         // the pivoting pass reruns from the saved columns.
-        double start[6];
+        T start[6];
 #pragma unroll
         for (int r = 0; r < 6; ++r) start[r] = col[r];
         bool exchange = false, singular = false;
         static_for<6>([&](auto C) {
-            double cv[6];
+            T cv[6];
 #pragma unroll
             for (int i = 0; i < 6; ++i) cv[i] = row_bcast<C>(col[i]);
-            const double diag = fabs(cv[C]);
+            const T diag = fabs(cv[C]);
 #pragma unroll
             for (int i = C + 1; i < 6; ++i) exchange |= fabs(cv[i]) > diag;
-            singular |= diag < 1.0e-30 || !isfinite(diag);
-            const double invp = 1.0 / cv[C];
+            singular |= diag < T(1.0e-30) || !isfinite(diag);
+            const T invp = T(1) / cv[C];
             col[C] *= invp;
 #pragma unroll
             for (int r = 0; r < 6; ++r)
@@ -1403,31 +1441,31 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
 #pragma unroll
             for (int r = 0; r < 6; ++r) col[r] = start[r];
             static_for<6>([&](auto C) {
-                double cv[6];
+                T cv[6];
 #pragma unroll
                 for (int i = 0; i < 6; ++i) cv[i] = row_bcast<C>(col[i]);
                 int pivot = C;
-                double best = fabs(cv[C]);
+                T best = fabs(cv[C]);
 #pragma unroll
                 for (int i = C + 1; i < 6; ++i) {
-                    const double v = fabs(cv[i]);
+                    const T v = fabs(cv[i]);
                     if (v > best) { best = v; pivot = i; }
                 }
-                if (best < 1.0e-30 || !isfinite(best)) {
+                if (best < T(1.0e-30) || !isfinite(best)) {
                     singular = true;
                     pivot = C;
                 }
 #pragma unroll
                 for (int i = C + 1; i < 6; ++i) {
                     const bool sw = pivot == i;
-                    const double x = col[C], y = col[i];
+                    const T x = col[C], y = col[i];
                     col[C] = sw ? y : x;
                     col[i] = sw ? x : y;
-                    const double cx = cv[C], cy = cv[i];
+                    const T cx = cv[C], cy = cv[i];
                     cv[C] = sw ? cy : cx;
                     cv[i] = sw ? cx : cy;
                 }
-                const double invp = 1.0 / cv[C];
+                const T invp = T(1) / cv[C];
                 col[C] *= invp;
 #pragma unroll
                 for (int r = 0; r < 6; ++r)
@@ -1440,23 +1478,23 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
 #pragma unroll
         for (int r = 0; r < 6; ++r)
             if (vlane) w[r] = rhs[r];
-        double res[6];
+        T res[6];
 #pragma unroll
-        for (int r = 0; r < 6; ++r) res[r] = 0.0;
+        for (int r = 0; r < 6; ++r) res[r] = T(0);
         static_for<6>([&](auto K) {
 #pragma unroll
             for (int r = 0; r < 6; ++r) res[r] = fma(row_bcast<6 + K>(col[r]), w[K], res[r]);
         });
         const bool carry = (acol && t + 1 < len) || vlane;  // C' columns while a node follows, and d'
 #pragma unroll
-        for (int r = 0; r < 6; ++r) p[r] = carry ? res[r] : 0.0;
+        for (int r = 0; r < 6; ++r) p[r] = carry ? res[r] : T(0);
         const bool keep = t < len && ok;
-        const uint32_t oc = (acol && keep) ? 288 * n + 8 * j : kNoAccess;
-        const uint32_t op = (vlane && keep) ? 48 * n : kNoAccess;
+        const uint32_t oc = (acol && keep) ? EB * n + E * j : kNoAccess;
+        const uint32_t op = (vlane && keep) ? EV * n : kNoAccess;
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
-            buffer_store(rC, oc + 48 * r, p[r]);
-            buffer_store(rP, op + 8 * r, p[r]);
+            factor_store(rC, oc + EV * r, p[r]);
+            factor_store(rP, op + E * r, p[r]);
         }
         // refilled once its values are dead, so it keeps its registers
         fetch(slot, t + kSolveAhead);
@@ -1468,30 +1506,33 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
     // Back substitution x_n = d'_n - C'_n x_{n+1}, lane r < 6 owning row r.
     const int r = acol ? j : 0;
     struct BackSlot {
-        double c[6], d, g;
+        T c[6], d;
+        double g;
     };
     auto fetch_back = [&](BackSlot &b, uint32_t t) __attribute__((always_inline)) {
         const uint32_t n = end - 1 - t;
         const bool live = acol && t < len;
-        const uint32_t oc = live ? 288 * n + 48 * r : kNoAccess;
+        const uint32_t oc = live ? EB * n + EV * r : kNoAccess;
 #pragma unroll
-        for (int k = 0; k < 6; ++k) b.c[k] = buffer_load(rC, oc + 8 * k);
-        b.d = buffer_load(rP, live ? 48 * n + 8 * r : kNoAccess);
+        for (int k = 0; k < 6; ++k) b.c[k] = factor_load<T>(rC, oc + E * k);
+        b.d = factor_load<T>(rP, live ? EV * n + E * r : kNoAccess);
         b.g = buffer_load(rG, live ? 48 * n + 8 * r : kNoAccess);
     };
     BackSlot back[kBackAhead];
 #pragma unroll
     for (int a = 0; a < kBackAhead; ++a) fetch_back(back[a], a);
-    double next = 0.0, slope = 0.0, reach2 = 0.0;
+    T next = T(0);
+    double slope = 0.0, reach2 = 0.0;
     auto backward = [&](BackSlot &b, uint32_t t) __attribute__((always_inline)) {
         const uint32_t n = end - 1 - t;
-        double crow[6];
+        T crow[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) crow[k] = b.c[k];
-        double value = b.d;
-        if (t > 0) static_for<6>([&](auto K) { value = fma(-crow[K], row_bcast<K>(next), value); });
+        T value_t = b.d;
+        if (t > 0) static_for<6>([&](auto K) { value_t = fma(-crow[K], row_bcast<K>(next), value_t); });
+        next = value_t;
+        const double value = double(value_t);
         buffer_store(rX, (t < len && ok && acol) ? 48 * n + 8 * j : kNoAccess, value);
-        next = value;
         // this row's part of g.d; |d_xyz|^2 of node n on lane 0
         const double sq = value * value;
         const double t2 = (sq + row_bcast<1>(sq)) + row_bcast<2>(sq);

@@ -135,6 +135,41 @@ void rod_strains(const Dual<N> *q, const M3 &fi, const M3 &fj, const Element &e,
 // 1e-30 or a non-finite pivot (cuda_backend.cu:855-864). Each row update
 // a - f b is one fused multiply-add, as the reference's CUDA build contracts
 // it (nvcc's default); so are the sums of products in the block solve.
+// The same Gauss-Jordan inverse in single precision, for the
+// mixed-precision step (the blocks and factors rounded to FP32).
+bool inverse6f(const float *src, float *inv)
+{
+    float a[36];
+    for (int i = 0; i < 36; ++i) { a[i] = src[i]; inv[i] = 0.0f; }
+    for (int i = 0; i < 6; ++i) inv[6 * i + i] = 1.0f;
+    float scale = 0.0f;
+    for (int i = 0; i < 6; ++i) scale = std::fmax(scale, std::fabs(a[6 * i + i]));
+    for (int i = 0; i < 6; ++i) a[6 * i + i] += std::fmax(1.0e-12f * scale, 1.0e-24f);
+    for (int col = 0; col < 6; ++col) {
+        int pivot = col;
+        float best = std::fabs(a[6 * col + col]);
+        for (int row = col + 1; row < 6; ++row)
+            if (std::fabs(a[6 * row + col]) > best) { best = std::fabs(a[6 * row + col]); pivot = row; }
+        if (best < 1.0e-30f || !std::isfinite(best)) return false;
+        if (pivot != col)
+            for (int k = 0; k < 6; ++k) {
+                std::swap(a[6 * col + k], a[6 * pivot + k]);
+                std::swap(inv[6 * col + k], inv[6 * pivot + k]);
+            }
+        const float invp = 1.0f / a[6 * col + col];
+        for (int k = 0; k < 6; ++k) { a[6 * col + k] *= invp; inv[6 * col + k] *= invp; }
+        for (int row = 0; row < 6; ++row)
+            if (row != col) {
+                const float f = a[6 * row + col];
+                for (int k = 0; k < 6; ++k) {
+                    a[6 * row + k] = std::fma(-f, a[6 * col + k], a[6 * row + k]);
+                    inv[6 * row + k] = std::fma(-f, inv[6 * col + k], inv[6 * row + k]);
+                }
+            }
+    }
+    return true;
+}
+
 bool inverse6(const double *src, double *inv)
 {
     double a[36];
@@ -191,6 +226,7 @@ public:
     uint32_t threads = 1;
     uint32_t convergence_mode = 0;
     uint32_t direction_mode = 0;
+    uint32_t precision_mode = 0;
 
     // ---------------------------------------------------------------- input
     KhsResult set_hair(const KhsVec3 *points, uint32_t point_count, const uint32_t *offsets,
@@ -1482,8 +1518,72 @@ private:
 
     // Per-strand block Thomas on the free nodes; the root Dirichlet nodes
     // are eliminated (cuda_backend.cu:866-881). Returns true on failure.
+    // Mixed precision: the same block Thomas with the Gauss-Newton blocks,
+    // the gradient and every factor rounded to FP32 (products summed by
+    // single-precision fused multiply-adds); the direction is widened to
+    // FP64. The Newton loop around it keeps its FP64 gradient, energy,
+    // barrier, step bound and convergence test, so each iteration corrects
+    // the FP32 direction's error from an FP64 residual.
+    bool block_solve_fp32()
+    {
+        std::vector<uint8_t> failed(strands_.size(), 0);
+        parallel_for(threads, uint32_t(strands_.size()), [&](uint32_t s) {
+            const Strand &st = strands_[s];
+            const uint32_t begin = st.first + st.fixed_count, end = st.first + st.count;
+            if (begin >= end) return;
+            std::vector<float> C(36 * size_t(end - begin)), dp(6 * size_t(end - begin));
+            for (uint32_t n = begin; n < end; ++n) {
+                const size_t k = n - begin;
+                float b[36], inv[36], rhs[6];
+                for (int i = 0; i < 36; ++i) b[i] = float(D_[36 * n + i]);
+                for (int i = 0; i < 6; ++i) rhs[i] = -float(g_[6 * n + i]);
+                if (n > begin) {
+                    const float *pc = &C[36 * (k - 1)], *pd = &dp[6 * (k - 1)];
+                    for (int r = 0; r < 6; ++r) {
+                        for (int c = 0; c < 6; ++c) {
+                            float sum = 0;
+                            for (int j = 0; j < 6; ++j) sum = std::fma(float(U_[36 * (n - 1) + 6 * j + r]), pc[6 * j + c], sum);
+                            b[6 * r + c] -= sum;
+                        }
+                        float sum = 0;
+                        for (int j = 0; j < 6; ++j) sum = std::fma(float(U_[36 * (n - 1) + 6 * j + r]), pd[j], sum);
+                        rhs[r] -= sum;
+                    }
+                }
+                if (!inverse6f(b, inv)) { failed[s] = 1; return; }
+                float *cc = &C[36 * k], *cd = &dp[6 * k];
+                for (int r = 0; r < 6; ++r)
+                    for (int c = 0; c < 6; ++c) {
+                        float sum = 0;
+                        if (n + 1 < end)
+                            for (int j = 0; j < 6; ++j) sum = std::fma(inv[6 * r + j], float(U_[36 * n + 6 * j + c]), sum);
+                        cc[6 * r + c] = sum;
+                    }
+                for (int r = 0; r < 6; ++r) {
+                    float sum = 0;
+                    for (int j = 0; j < 6; ++j) sum = std::fma(inv[6 * r + j], rhs[j], sum);
+                    cd[r] = sum;
+                }
+            }
+            std::vector<float> x(6 * size_t(end - begin));
+            for (uint32_t n = end; n-- > begin;) {
+                const size_t k = n - begin;
+                for (int r = 0; r < 6; ++r) {
+                    float value = dp[6 * k + r];
+                    if (n + 1 < end)
+                        for (int j = 0; j < 6; ++j) value = std::fma(-C[36 * k + 6 * r + j], x[6 * (k + 1) + j], value);
+                    x[6 * k + r] = value;
+                    d_[6 * n + r] = double(value);
+                }
+            }
+        });
+        for (uint8_t f : failed) if (f) return true;
+        return false;
+    }
+
     bool block_solve()
     {
+        if (precision_mode == 1) return block_solve_fp32();
         std::vector<uint8_t> failed(strands_.size(), 0);
         parallel_for(threads, uint32_t(strands_.size()), [&](uint32_t s) {
             const Strand &st = strands_[s];
@@ -2372,7 +2472,8 @@ KhsResult khsxSetConvergenceMode(KhsSolver *s, uint32_t mode)
 
 KhsResult khsxSetPrecisionMode(KhsSolver *s, uint32_t mode)
 {
-    if (!s || mode != 0) return KHS_ERROR_INVALID_ARGUMENT;
+    if (!s || mode > 1) return KHS_ERROR_INVALID_ARGUMENT;
+    s->impl.precision_mode = mode;
     return KHS_OK;
 }
 

@@ -4,6 +4,8 @@
 // self-check and a comparison of box queries with brute force.
 // Input: V T, then V vertices (x y z), T triangles (i j k), then Q query
 // boxes (lo hi pad). Prints JSON.
+#include "../../kami-hair-solver_amd/csrc/kernels.hpp"
+#include "host_thread_shim.hpp"
 #include "../../kami-hair-solver_amd/csrc/lbvh_build.hpp"
 
 #include <cstdio>

@@ -506,3 +506,29 @@ def test_sphere_scene_matches_golden(gpu_lib):
     assert got["newton"][0] == want["newton"][0]
     assert abs(got["objective"][0] - want["objective"][0]) <= 1e-10 * abs(want["objective"][0])
     assert _rel(got["gradient"], want["gradient"]) <= 1e-10
+
+
+def test_gpu_stats_phase_times(gpu_lib):
+    """With kernel timing on, khsGetGpuStats reports the frame's assembly,
+    collision and optimization (block solve) milliseconds from the
+    per-launch HIP events; they are the frame's share of the kernel-class
+    totals and fit inside the frame."""
+    print("[gpu] phase times in the GPU stats", flush=True)
+    scene = grooms.small_sphere_scene()
+    s = grooms.build_solver(gpu_lib, scene, **LIFTED)
+    s.check(s.set_convergence_mode(1))
+    s.check(s.set_kernel_timing(True))
+    s.check(s.step(scene.frame_dt))
+    st, t = s.gpu_stats(), s.kernel_timing()
+    print(f"frame {st.last_frame_milliseconds:.3f} ms: assembly {st.last_assembly_milliseconds:.3f}, collision "
+          f"{st.last_collision_milliseconds:.3f}, optimization {st.last_optimization_milliseconds:.3f}")
+    assert st.last_assembly_milliseconds > 0 and st.last_collision_milliseconds > 0
+    assert st.last_optimization_milliseconds > 0
+    assert abs(st.last_assembly_milliseconds - (t.assembly_ms + t.energy_ms)) <= 1e-9
+    assert abs(st.last_optimization_milliseconds - t.solve_ms) <= 1e-9
+    assert (st.last_assembly_milliseconds + st.last_collision_milliseconds + st.last_optimization_milliseconds
+            <= st.last_frame_milliseconds * 1.001)
+    s.check(s.set_kernel_timing(False))
+    s.check(s.step(scene.frame_dt))
+    assert s.gpu_stats().last_optimization_milliseconds == 0.0
+    s.close()

@@ -436,3 +436,38 @@ def test_groom_contact_state_stages_match_oracle(oracle_lib, gpu_lib, groom_subs
     assert np.count_nonzero(dv[0]) > 20
     assert np.count_nonzero(dv[1]) == np.count_nonzero(dv[0])
     assert _rel(dv[1], dv[0]) <= 1e-10
+
+
+def test_fp32_block_solve_matches_oracle(oracle_lib, gpu_lib):
+    """Precision mode 1: the FP32 block solve from the same FP64 blocks and
+    gradient gives the oracle's FP32 direction (the same single-precision
+    operations in the same order), and a frame in that mode stalls at the
+    Newton cap on the GPU as on the oracle."""
+    print("[gpu] FP32 block solve against the oracle's", flush=True)
+    scene, o, g = _sphere_pair(oracle_lib, gpu_lib)
+    q, rng = _pulled_in(o, seed=12, lo=2e-4, hi=1.6e-3)
+    v = rng.uniform(-1e-3, 1e-3, q.size)
+    out = []
+    for s in (o, g):
+        s.check(s.set_state(q, v, q - 1e-4 * v))
+        s.check(s.set_precision_mode(1))
+        out.append(s.evaluate(scene.frame_dt / 4))
+    (eo, go, do), (eg, gg, dg) = out
+    same = float(np.mean(dg == do))
+    print(f"FP32 direction: relative difference {_rel(dg, do):.3e}, bit-equal entries {same:.4f}")
+    assert eo.solve_failed == eg.solve_failed == 0
+    assert _rel(gg, go) <= 1e-10
+    assert _rel(dg, do) <= 1e-5
+    stats = []
+    for lib in (oracle_lib, gpu_lib):
+        s = grooms.build_solver(lib, scene, **LIFTED)
+        s.check(s.set_convergence_mode(1))
+        s.check(s.set_precision_mode(1))
+        s.set_threads(16)
+        s.check(s.step(scene.frame_dt))
+        st = s.step_stats()
+        stats.append((st.newton_iterations, st.relative_residual_norm, st.phase))
+        s.close()
+    print(f"FP32 frame: oracle {stats[0]} gpu {stats[1]}")
+    assert stats[1][2] == kn.PHASE_FINISHED
+    assert stats[1][0] == stats[0][0] == 4 * 24
