@@ -247,6 +247,10 @@ def main():
                     help="run one device copy of this size first (for scaling PMC traffic counters)")
     ap.add_argument("--batch-jobs", type=int, default=4, help="concurrent jobs of the batched form (0: skip)")
     ap.add_argument("--mixed-frames", type=int, default=8, help="timed frames of the mixed-precision form (0: skip)")
+    ap.add_argument("--serve-window", type=float, default=20.0, help="arrival window of the served form, s (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=1.0, help="mean job arrivals per second of the served form")
+    ap.add_argument("--serve-max-strands", type=int, default=6000, help="largest job of the served form, strands")
+    ap.add_argument("--serve-slots", type=int, default=4, help="concurrent jobs of the served form")
     ap.add_argument("--batch-frames", type=int, default=None,
                     help="timed frames per job of the batched form (default: --steps)")
     args = ap.parse_args()
@@ -397,6 +401,22 @@ def main():
         result["batched"] = run_batched(lib, scene, args.batch_jobs, args.warmup, args.batch_frames or args.steps)
         log(f"[bench] batched form: {args.batch_jobs} jobs in {time.perf_counter() - t:.1f} s, "
             f"{result['batched']['value']:.4g} node-substeps/s")
+    if args.serve_window > 0:
+        import serve
+        t = time.perf_counter()
+        jobs = serve.poisson_jobs(args.serve_rate, args.serve_window, seed=5, strands=(500, args.serve_max_strands),
+                                  frames=(10, 30))
+        log(f"[bench] serving: {len(jobs)} jobs generated in {time.perf_counter() - t:.1f} s")
+        t = time.perf_counter()
+        served = serve.BakeServer(lib, slots=args.serve_slots, hbm_budget=200e9).run(jobs)
+        served["arrivals"] = (f"Poisson, {args.serve_rate}/s over {args.serve_window} s; strands log-uniform "
+                              f"500..{args.serve_max_strands}, nodes lognormal median 40 (8..80), frames 10..30, "
+                              f"each on its own rotating ellipsoid (icosphere subdivision 5)")
+        served["form"] = ("continuous admission by slots and a 200 GB HBM budget; each job resident with a "
+                          "10-frame rollback ring on its own stream; plain FP64")
+        result["served"] = served
+        log(f"[bench] served {served['completed']} jobs in {time.perf_counter() - t:.1f} s, "
+            f"{served['value']:.4g} node-substeps/s")
     if not args.no_cpu_baseline:
         t = time.perf_counter()
         base = cpu_baseline(scene, args.cpu_strands, args.cpu_frames, args.cpu_threads)
