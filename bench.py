@@ -211,6 +211,58 @@ def run_mixed(lib, scene, warmup, frames, fp64_positions):
     }
 
 
+def run_uneven(lib, warmup, frames):
+    """The uneven-length groom (50,000 strands of 4..256 nodes, about 6.5M
+    nodes, per-strand E and radius) on the rotating head + torso body, plain
+    FP64 to the 1e-6 residual: node-substeps/s over `frames` frames after
+    `warmup`, with substep latency, Newton work and the block solve's
+    average launch time."""
+    t = time.perf_counter()
+    scene, young, radius = grooms.uneven_scene()
+    s = grooms.build_solver(lib, scene, **LIFTED)
+    s.check(s.set_convergence_mode(1))
+    s.check(s.set_strand_materials(young, radius))
+    total = warmup + frames + 1
+    s.check(s.allocate_animation(total))
+    for f in range(total):
+        s.check(s.set_root_frame(f, scene.frame_points(f)))
+        s.check(s.set_collider_frame(f, scene.frame_vertices(f)))
+    s.check(s.finalize_animation())
+    nodes = s.node_count()
+    log(f"[bench] uneven groom: {scene.strand_count} strands, {nodes} nodes, ready in {time.perf_counter() - t:.1f} s")
+    for f in range(1, warmup + 1):
+        s.check(s.step_frame(f, scene.frame_dt))
+    s.check(s.set_kernel_timing(True))
+    substeps = newton = 0
+    lat, gaps = [], []
+    t0 = time.perf_counter()
+    for f in range(warmup + 1, warmup + frames + 1):
+        s.check(s.step_frame(f, scene.frame_dt))
+        st = s.step_stats()
+        substeps += st.substeps
+        newton += st.newton_iterations
+        gaps.append(st.minimum_gap)
+        lat.extend(s.substep_times().tolist())
+    elapsed = time.perf_counter() - t0
+    timing = s.kernel_timing()
+    resident = int(s.gpu_stats().resident_bytes)
+    s.close()
+    return {
+        "value": nodes * substeps / elapsed,
+        "unit": "node-substeps/s",
+        "workload": f"{scene.strand_count} strands of 4..256 nodes ({nodes} nodes), per-strand E U(1,5) GPa and "
+                    f"radius U(30,80) um, 225,184-vertex body rotating 30 deg/s, 8 substeps/frame, relative "
+                    f"Newton residual 1e-6, plain FP64",
+        "frames": frames,
+        "substep_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "newton_iterations_per_substep": newton / max(substeps, 1),
+        "minimum_gap_m": float(min(gaps)),
+        "solve_avg_ms": timing.solve_ms / max(timing.solve_launches, 1),
+        "assembly_avg_ms": timing.assembly_ms / max(timing.assembly_launches, 1),
+        "resident_bytes": resident,
+    }
+
+
 def copy_calibration(mib):
     """One device-to-device copy of `mib` MiB (reads and writes mib MiB),
     the known kernel against which the PMC traffic counters are scaled. It
@@ -247,6 +299,8 @@ def main():
                     help="run one device copy of this size first (for scaling PMC traffic counters)")
     ap.add_argument("--batch-jobs", type=int, default=4, help="concurrent jobs of the batched form (0: skip)")
     ap.add_argument("--mixed-frames", type=int, default=8, help="timed frames of the mixed-precision form (0: skip)")
+    ap.add_argument("--uneven-frames", type=int, default=3,
+                    help="timed frames of the 50,000-strand uneven-length groom (0: skip)")
     ap.add_argument("--serve-window", type=float, default=20.0, help="arrival window of the served form, s (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=1.0, help="mean job arrivals per second of the served form")
     ap.add_argument("--serve-max-strands", type=int, default=6000, help="largest job of the served form, strands")
@@ -401,6 +455,15 @@ def main():
         result["batched"] = run_batched(lib, scene, args.batch_jobs, args.warmup, args.batch_frames or args.steps)
         log(f"[bench] batched form: {args.batch_jobs} jobs in {time.perf_counter() - t:.1f} s, "
             f"{result['batched']['value']:.4g} node-substeps/s")
+    if args.uneven_frames > 0:
+        t = time.perf_counter()
+        try:
+            result["uneven"] = run_uneven(lib, 1, args.uneven_frames)
+            log(f"[bench] uneven groom in {time.perf_counter() - t:.1f} s, "
+                f"{result['uneven']['value']:.4g} node-substeps/s")
+        except kn.SolverError as e:  # reported in the line; the headline stands
+            result["uneven"] = {"value": None, "error": str(e)}
+            log(f"[bench] uneven groom failed: {e}")
     if args.serve_window > 0:
         import serve
         t = time.perf_counter()

@@ -190,6 +190,20 @@ KHS_API KhsResult khsxWriteCacheFrame(KhsSolver *solver, int fd);
    library ignores it and returns KHS_OK). */
 KHS_API KhsResult khsxSetThreadCount(KhsSolver *solver, uint32_t threads);
 
+/* Per-strand Young's modulus (Pa) and hair radius (m), one value each per
+   hair strand (strand_count = the strands of khsSetHairCurves). They replace
+   the material's young_modulus and radius for each strand everywhere the
+   step uses them: masses, stretch/shear/bend/twist stiffness, the Jacobi
+   preconditioner, and the contact radius (barrier gap, query padding,
+   step bound and sweep targets); the trust radius uses the largest radius.
+   Every other material field stays solver-wide. Both pointers NULL returns
+   to the material's values. On a built solver it applies at once (masses
+   are recomputed); new hair curves clear it. The reference's material is
+   global per solver (include/kami_hair_solver.h:80-93, 214-219); this is
+   the additive input of the uneven-material groom. */
+KHS_API KhsResult khsxSetStrandMaterials(KhsSolver *solver, const double *young_modulus, const double *radius,
+                                         uint32_t strand_count);
+
 /* Stage hooks: each runs one stage of the substep on the current state, so
    that a test can compare the stage between libraries and against closed
    forms. None of them changes q.

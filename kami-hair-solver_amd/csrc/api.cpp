@@ -344,6 +344,11 @@ KhsResult khsxSetThreadCount(KhsSolver *s, uint32_t threads)
     return s ? KHS_OK : KHS_ERROR_INVALID_ARGUMENT;
 }
 
+KhsResult khsxSetStrandMaterials(KhsSolver *s, const double *young_modulus, const double *radius, uint32_t count)
+{
+    return guarded(s, [&] { return s->impl.set_strand_materials(young_modulus, radius, count); });
+}
+
 KhsResult khsxSetColliderState(KhsSolver *s, const double *current, const double *previous, uint32_t count)
 {
     return guarded(s, [&] { return s->impl.set_collider_state(current, previous, count); });

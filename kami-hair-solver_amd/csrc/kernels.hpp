@@ -92,6 +92,11 @@ struct StepParams {
     double contact_pad;    // radius + offset + barrier
     double sweep_target;   // radius + offset + minimum gap
     int has_collider;
+    // Per-strand materials: 2 doubles per node, the Young's modulus and
+    // radius of the node's strand, or null when the material's values
+    // (above) hold for every strand.
+    const double *material;
+    double collider_offset, poisson, shear_correction;
 };
 
 struct SweepRecord {

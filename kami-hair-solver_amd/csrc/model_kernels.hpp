@@ -229,10 +229,15 @@ __global__ void k_model_nodes(uint32_t strands, const double *points, const uint
 // Lumped masses again after a material change, lane per strand, from the
 // rest lengths already on the device.
 __global__ void k_model_masses(RodView m, const double *rest, double mass_per_length, double rotational_per_length,
-                               double *mass, double *rmass)
+                               const double *strand_density, double *mass, double *rmass)
 {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= m.strand_count) return;
+    // per-strand materials: the strand's own mass and rotational mass per length
+    if (strand_density) {
+        mass_per_length = strand_density[2 * size_t(s)];
+        rotational_per_length = strand_density[2 * size_t(s) + 1];
+    }
     const uint32_t first = m.strand_first[s], count = m.strand_count_of[s];
     double carried_mass = 0.0, carried_rot = 0.0;
     for (uint32_t k = 0; k + 1 < count; ++k) {

@@ -193,6 +193,9 @@ struct DeviceState {
     Buffer<double> alpha;  // line-search step read by the trial kernels
     Buffer<unsigned long long> list_words;  // per-block candidate counts of the narrow-phase list pass
     Buffer<double> list_alpha;              // per-block smallest step of the CCD list pass
+    // per-strand materials: (E, r) per node of its strand, and per strand
+    // the mass and rotational mass per length; empty without them
+    Buffer<double> elem_material, strand_density;
     DirectionChoice *choice() const { return reinterpret_cast<DirectionChoice *>(reduced.get() + 3); }
 
     // Event timing of kernel classes (enabled on request) and of substeps.
@@ -442,6 +445,8 @@ KhsResult Solver::set_hair(const KhsVec3 *points, uint32_t point_count, const ui
     release_device();
     points_ = std::move(pts);
     offsets_.assign(offsets, offsets + strand_count + 1);
+    strand_E_.clear();
+    strand_r_.clear();
     material_ = m;
     has_hair_ = true;
     error_.clear();
@@ -492,6 +497,7 @@ KhsResult Solver::build()
         release_device();
         return r;
     }
+    if (!strand_r_.empty()) apply_strand_materials();
     double initial_gap = kInf;
     if (has_collider_) {
         r = initial_feasibility();
@@ -786,7 +792,7 @@ KhsResult Solver::upload_model()
         rebuild_bvh(false);
     }
     trust_radius_ = std::max(0.25 * model_.maximum_rest_length,
-                             8.0 * (material_.radius + material_.collider_offset + material_.barrier_distance));
+                             8.0 * (max_radius() + material_.collider_offset + material_.barrier_distance));
     HIP_TRY(hipStreamSynchronize(d.stream));
     return KHS_OK;
 }
@@ -852,8 +858,9 @@ constexpr unsigned kGatherBlocks = 512;
 void Solver::refresh_contact_cache(int mode, bool full, bool swept_leaves, bool with_roots)
 {
     DeviceState &d = *dev_;
-    const double contact_pad = material_.radius + material_.collider_offset + material_.barrier_distance;
-    const double sweep_target = material_.radius + material_.collider_offset + desc_.minimum_gap;
+    // the largest pad of any element's queries (per-strand radii included)
+    const double contact_pad = max_radius() + material_.collider_offset + material_.barrier_distance;
+    const double sweep_target = max_radius() + material_.collider_offset + desc_.minimum_gap;
     const double pad = std::fmax(contact_pad, sweep_target);
     // Half the pad: measured against 0.35-2 pads on the groom, a thinner
     // skin shortens every list pass by more than the extra regathers cost.
@@ -906,18 +913,18 @@ void Solver::refresh_contact_cache(int mode, bool full, bool swept_leaves, bool 
 void Solver::closest_pass(bool line_search, unsigned long long *block_candidates, bool with_roots)
 {
     DeviceState &d = *dev_;
-    const double contact_pad = material_.radius + material_.collider_offset + material_.barrier_distance;
+    const StepParams sp = step_params(1.0);
     refresh_contact_cache(line_search ? dev::kAtTrial : dev::kAtState, false, false, with_roots);
     const ContactCache cc = d.cache();
     if (line_search)
         hipLaunchKernelGGL(dev::k_list_closest<dev::kAtTrial>, dim3(kPairBlocks), dim3(kBlock), 0, d.stream,
-                           d.flags.get(), d.collider(), cc, contact_pad, d.q.get(), d.q_base.get(), d.dir.get(),
+                           d.flags.get(), d.collider(), cc, sp, d.q.get(), d.q_base.get(), d.dir.get(),
                            d.alpha.get(), d.best_tri.get(), block_candidates,
                            with_roots ? d.root_elements.get() : nullptr, with_roots ? d.root_element_count : 0u,
                            d.skip());
     else
         hipLaunchKernelGGL(dev::k_list_closest<dev::kAtState>, dim3(kPairBlocks), dim3(kBlock), 0, d.stream,
-                           d.flags.get(), d.collider(), cc, contact_pad, d.q.get(), d.q_base.get(), d.dir.get(),
+                           d.flags.get(), d.collider(), cc, sp, d.q.get(), d.q_base.get(), d.dir.get(),
                            d.alpha.get(), d.best_tri.get(), block_candidates,
                            with_roots ? d.root_elements.get() : nullptr, with_roots ? d.root_element_count : 0u,
                            (const unsigned *)nullptr);
@@ -951,9 +958,7 @@ KhsResult Solver::initial_feasibility()
             return KHS_ERROR_INITIAL_INTERSECTION;
         }
     }
-    StepParams p{};
-    p.radius_offset = material_.radius + material_.collider_offset;
-    p.contact_pad = material_.radius + material_.collider_offset + material_.barrier_distance;
+    const StepParams p = step_params(1.0);
     hipLaunchKernelGGL(dev::k_initial_gap, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, cv, p, dpos.get(),
                        d.partials.get());
     HIP_TRY(hipGetLastError());
@@ -983,17 +988,7 @@ KhsResult Solver::update_runtime_parameters(const KhsSolverDesc &desc, const Khs
     }
     desc_ = desc;
     material_ = material;
-    {
-        DeviceState &d = *dev_;
-        const MassDensity rho = mass_density(material_);
-        hipLaunchKernelGGL(dev::k_model_masses, dim3(grid_for(d.S, kBlock)), dim3(kBlock), 0, d.stream, d.rods(),
-                           (const double *)d.rest.get(), rho.translational, rho.rotational, d.mass.get(),
-                           d.rmass.get());
-        HIP_TRY(hipGetLastError());
-        HIP_TRY(hipStreamSynchronize(d.stream));
-    }
-    trust_radius_ = std::max(0.25 * model_.maximum_rest_length,
-                             8.0 * (material_.radius + material_.collider_offset + material_.barrier_distance));
+    apply_strand_materials();  // masses and trust radius of the new material
     error_.clear();
     return KHS_OK;
 }
@@ -1158,6 +1153,10 @@ StepParams Solver::step_params(double dt) const
     p.contact_pad = material_.radius + material_.collider_offset + material_.barrier_distance;
     p.sweep_target = material_.radius + material_.collider_offset + desc_.minimum_gap;
     p.has_collider = dev_ && dev_->T > 0;
+    p.material = (dev_ && dev_->elem_material.size()) ? dev_->elem_material.get() : nullptr;
+    p.collider_offset = material_.collider_offset;
+    p.poisson = material_.poisson_ratio;
+    p.shear_correction = material_.shear_correction;
     return p;
 }
 
@@ -1312,7 +1311,9 @@ void Solver::launch_preconditioner(double dt)
     const double ra = material_.shear_correction * G * area;
     const double rb = std::fmax(material_.young_modulus * inertia, 2.0 * G * inertia);
     hipLaunchKernelGGL(dev::k_preconditioner, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.rods(), 1.0 / dt,
-                       material_.mass_damping, tk, ra, rb, d.precond.get());
+                       material_.mass_damping, tk, ra, rb,
+                       d.elem_material.size() ? (const double *)d.elem_material.get() : nullptr,
+                       material_.poisson_ratio, material_.shear_correction, d.precond.get());
     HIP_TRY(hipGetLastError());
 }
 
@@ -1338,10 +1339,7 @@ KhsResult Solver::optimize_substep(double dt)
     if (!current.feasible) return fail(KHS_ERROR_NOT_CONVERGED, "State is outside the barrier feasible region.");
     const double initial = std::sqrt(current.grad_norm2);
     step_stats_.initial_residual_norm = std::fmax(step_stats_.initial_residual_norm, initial);
-    StepParams cp{};
-    cp.radius_offset = material_.radius + material_.collider_offset;
-    cp.sweep_target = material_.radius + material_.collider_offset + desc_.minimum_gap;
-    cp.has_collider = d.T > 0;
+    const StepParams cp = step_params(dt);
     // The step bound along the current direction and the first trial at it.
     auto bound_and_trial = [&]() {
         phase_.store(KHS_PHASE_CCD);
@@ -2084,8 +2082,7 @@ KhsResult Solver::broadphase_pairs(uint32_t *pairs, uint64_t capacity, uint64_t 
     rebuild_bvh(false);
     Buffer<uint32_t> out;
     out.allocate(std::max<uint64_t>(2 * capacity, 2));
-    StepParams p{};
-    p.contact_pad = material_.radius + material_.collider_offset + material_.barrier_distance;
+    const StepParams p = step_params(1.0);
     hipLaunchKernelGGL(dev::k_fill_u64, dim3(1), dim3(64), 0, d.stream, d.key64.get(), size_t(1), 0ull);
     hipLaunchKernelGGL(dev::k_broadphase_pairs, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.rods(),
                        d.collider(), p, d.q.get(), out.get(), (unsigned long long)capacity, d.key64.get());
@@ -2130,8 +2127,7 @@ KhsResult Solver::broadphase_pairs_no_alloc(uint32_t *pairs, uint64_t capacity, 
     unsigned long long *counter = reinterpret_cast<unsigned long long *>(staging);
     uint32_t *staged = staging + kNoAllocHeader;
     rebuild_bvh(false);
-    StepParams p{};
-    p.contact_pad = material_.radius + material_.collider_offset + material_.barrier_distance;
+    const StepParams p = step_params(1.0);
     hipLaunchKernelGGL(dev::k_fill_u64, dim3(1), dim3(64), 0, d.stream, counter, size_t(1), 0ull);
     hipLaunchKernelGGL(dev::k_broadphase_pairs, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.rods(),
                        d.collider(), p, d.q.get(), staged, (unsigned long long)capacity, counter);
@@ -2268,6 +2264,79 @@ KhsResult Solver::contact_list_stats(KhsxContactListStats *out)
         ++r.length_histogram[bin];
     }
     *out = r;
+    return KHS_OK;
+}
+
+double Solver::max_radius() const
+{
+    double r = material_.radius;
+    if (!strand_r_.empty()) {
+        r = strand_r_[0];
+        for (double x : strand_r_) r = std::fmax(r, x);
+    }
+    return r;
+}
+
+// Uploads the per-strand materials of a built solver: (E, r) of each node's
+// strand for the step's kernels, the strands' mass densities, and the
+// masses and trust radius they give.
+void Solver::apply_strand_materials()
+{
+    DeviceState &d = *dev_;
+    if (strand_r_.empty()) {
+        d.elem_material.reset();
+        d.strand_density.reset();
+    } else {
+        std::vector<uint32_t> strand_of(d.N);
+        d.strand_of.download(strand_of.data(), d.N);
+        std::vector<double> per_node(2 * size_t(d.N)), density(2 * size_t(d.S));
+        for (uint32_t n = 0; n < d.N; ++n) {
+            per_node[2 * size_t(n)] = strand_E_[strand_of[n]];
+            per_node[2 * size_t(n) + 1] = strand_r_[strand_of[n]];
+        }
+        for (uint32_t k = 0; k < d.S; ++k) {
+            KhsHairMaterial m = material_;
+            m.radius = strand_r_[k];
+            const MassDensity rho = mass_density(m);
+            density[2 * size_t(k)] = rho.translational;
+            density[2 * size_t(k) + 1] = rho.rotational;
+        }
+        d.elem_material.allocate(per_node.size());
+        d.elem_material.upload(per_node.data(), per_node.size());
+        d.strand_density.allocate(density.size());
+        d.strand_density.upload(density.data(), density.size());
+    }
+    const MassDensity rho = mass_density(material_);
+    hipLaunchKernelGGL(dev::k_model_masses, dim3(grid_for(d.S, kBlock)), dim3(kBlock), 0, d.stream, d.rods(),
+                       (const double *)d.rest.get(), rho.translational, rho.rotational,
+                       d.strand_density.size() ? (const double *)d.strand_density.get() : nullptr, d.mass.get(),
+                       d.rmass.get());
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(d.stream));
+    d.cache_valid = false;
+    trust_radius_ = std::max(0.25 * model_.maximum_rest_length,
+                             8.0 * (max_radius() + material_.collider_offset + material_.barrier_distance));
+}
+
+KhsResult Solver::set_strand_materials(const double *E, const double *r, uint32_t count)
+{
+    if (!E && !r) {
+        strand_E_.clear();
+        strand_r_.clear();
+    } else {
+        if (!E || !r || !has_hair_ || count + 1 != offsets_.size()) {
+            error_ = "Per-strand materials need both arrays, one value per hair strand.";
+            return KHS_ERROR_INVALID_ARGUMENT;
+        }
+        for (uint32_t k = 0; k < count; ++k)
+            if (!(E[k] > 0.0 && std::isfinite(E[k]) && r[k] > 0.0 && std::isfinite(r[k]))) {
+                error_ = "Strand " + std::to_string(k) + " has a material value out of range.";
+                return KHS_ERROR_INVALID_ARGUMENT;
+            }
+        strand_E_.assign(E, E + count);
+        strand_r_.assign(r, r + count);
+    }
+    if (built_) apply_strand_materials();
     return KHS_OK;
 }
 

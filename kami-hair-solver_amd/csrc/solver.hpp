@@ -76,6 +76,7 @@ public:
     KhsResult rollback(uint32_t frame);
     KhsResult write_cache_frame(int fd);
     KhsResult set_collider_state(const double *current, const double *previous, uint32_t count);
+    KhsResult set_strand_materials(const double *young_modulus, const double *radius, uint32_t count);
     KhsResult friction_stage(double dt);
     KhsResult step_bound(const double *direction, uint64_t count, double *alpha);
     KhsResult sweep_check(uint32_t *feasible, KhsFailureDiagnostics *diagnostics);
@@ -114,6 +115,8 @@ private:
     // the block solve (mode 0) or forced steepest descent (mode 1).
     void direction_async();
     void launch_preconditioner(double dt);
+    double max_radius() const;
+    void apply_strand_materials();
     StepParams step_params(double dt) const;
     // The moving-collider sweep of a substep: swept tree, full list refresh
     // and the sweep kernels (reads nothing back).
@@ -144,6 +147,7 @@ private:
     uint32_t precision_mode_ = 0;
     uint32_t direction_mode_ = 0;
     uint64_t block_directions_ = 0, steepest_directions_ = 0;
+    std::vector<double> strand_E_, strand_r_;  // per-strand materials (empty: the material's)
     double trust_radius_ = 0.0;
     uint32_t animation_frames_ = 0, current_frame_ = 0;
     std::vector<uint32_t> ring_frame_;  // animation frame held by each rollback slot (~0u: empty)

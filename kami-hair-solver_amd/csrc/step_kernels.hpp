@@ -166,6 +166,41 @@ __global__ __launch_bounds__(kReduceBlock) void k_reduce(const Partial *partials
 }
 
 
+// The contact radius terms and stiffness weights of the element whose left
+// node is L: the material's (precomputed on the host) or, with per-strand
+// materials, its strand's, formed in the same operation order.
+__device__ __forceinline__ double radius_offset_of(const StepParams &p, size_t L)
+{
+    return p.material ? p.material[2 * L + 1] + p.collider_offset : p.radius_offset;
+}
+__device__ __forceinline__ double contact_pad_of(const StepParams &p, size_t L)
+{
+    return p.material ? (p.material[2 * L + 1] + p.collider_offset) + p.barrier : p.contact_pad;
+}
+__device__ __forceinline__ double sweep_target_of(const StepParams &p, size_t L)
+{
+    return p.material ? (p.material[2 * L + 1] + p.collider_offset) + p.minimum_gap : p.sweep_target;
+}
+constexpr double kPi = 3.1415926535897932384626433832795;
+// kGA, kGA, EA, EI, EI, 2GI (cuda_backend.cu:905-910)
+__device__ __forceinline__ void weights_of(const StepParams &p, size_t L, double w[6])
+{
+    if (!p.material) {
+#pragma unroll
+        for (int s = 0; s < 6; ++s) w[s] = p.weights[s];
+        return;
+    }
+    const double E = p.material[2 * L], r = p.material[2 * L + 1];
+    const double area = kPi * r * r, inertia = kPi * r * r * r * r / 4.0;
+    const double G = E / (2.0 * (1.0 + p.poisson));
+    w[0] = p.shear_correction * G * area;
+    w[1] = p.shear_correction * G * area;
+    w[2] = E * area;
+    w[3] = E * inertia;
+    w[4] = E * inertia;
+    w[5] = 2.0 * G * inertia;
+}
+
 __device__ __forceinline__ int internal_count(const ColliderView &c)
 {
     return c.triangle_count > 1 ? int(c.triangle_count) - 1 : 1;
@@ -652,7 +687,7 @@ constexpr int kClosestLanes = 32;
 constexpr uint32_t kOverflowSlots = 16;
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, ColliderView col, ContactCache cc,
-                                                         double pad, const double *q, const double *q_base,
+                                                         StepParams sp, const double *q, const double *q_base,
                                                          const double *dir, const double *alpha_at,
                                                          uint32_t *best_tri, unsigned long long *block_candidates,
                                                          const uint32_t *extra, uint32_t extra_count,
@@ -691,7 +726,7 @@ __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, 
                     Closest pair;
                     uint32_t tri;
                     unsigned long long n = 0;
-                    closest_triangle(col, p0, p1, pad, pair, tri, n);
+                    closest_triangle(col, p0, p1, contact_pad_of(sp, L), pair, tri, n);
                     best_tri[L] = tri;
                     if (!(flags[L] & kNodeFixed)) candidates += n;
                 }
@@ -699,6 +734,7 @@ __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, 
             continue;
         }
         const bool counted = !(flags[L] & kNodeFixed);
+        const double pad = contact_pad_of(sp, L);
         V3 p0, p1;
         element_points<MODE>(L, q, q_base, dir, alpha, p0, p1);
         const V3 lo = vmin(p0, p1), hi = vmax(p0, p1);
@@ -742,6 +778,7 @@ __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, 
     volatile int *ring = reinterpret_cast<volatile int *>(&packed[wave * (kWave / G)][0]);
     for (uint32_t h = 0; h < walks; ++h) {
         const uint32_t L = deferred[wave][h];
+        const double pad = contact_pad_of(sp, L);
         V3 p0, p1;
         element_points<MODE>(L, q, q_base, dir, alpha, p0, p1);
         double bd;
@@ -817,7 +854,7 @@ __global__ __launch_bounds__(kBlock) void k_list_ccd(const uint32_t *flags, Coll
     constexpr int G = kCcdLanes;
     __shared__ uint32_t pair_tri[kCcdBatch * kCacheCapacity];
     __shared__ uint8_t pair_elem[kCcdBatch * kCacheCapacity];
-    __shared__ double elem[kCcdBatch][13];  // p0, p1, d0, d1, speed
+    __shared__ double elem[kCcdBatch][14];  // p0, p1, d0, d1, speed, target clearance
     __shared__ uint32_t pair_count;
     const int lane = threadIdx.x & (G - 1), my = threadIdx.x / G;
     const int shift = (threadIdx.x & (kWave - 1)) & ~(G - 1);
@@ -838,6 +875,7 @@ __global__ __launch_bounds__(kBlock) void k_list_ccd(const uint32_t *flags, Coll
         }
         V3 p0 = mk(0, 0, 0), p1 = p0, d0 = p0, d1 = p0;
         double speed = 0.0;
+        const double target = sweep_target_of(p, L);
         if (c > 0) {
             p0 = load3(q + 6 * size_t(L));
             p1 = load3(q + 6 * size_t(L + 1));
@@ -847,9 +885,9 @@ __global__ __launch_bounds__(kBlock) void k_list_ccd(const uint32_t *flags, Coll
             if (speed < 1e-30) c = 0;
         }
         if (c > 0 && lane == 0) {
-            const double v[13] = {p0.x, p0.y, p0.z, p1.x, p1.y, p1.z, d0.x, d0.y, d0.z, d1.x, d1.y, d1.z, speed};
+            const double v[14] = {p0.x, p0.y, p0.z, p1.x, p1.y, p1.z, d0.x, d0.y, d0.z, d1.x, d1.y, d1.z, speed, target};
 #pragma unroll
-            for (int k = 0; k < 13; ++k) elem[my][k] = v[k];
+            for (int k = 0; k < 14; ++k) elem[my][k] = v[k];
         }
         const V3 lo = vmin(vmin(p0, p1), vmin(add(p0, d0), add(p1, d1)));
         const V3 hi = vmax(vmax(p0, p1), vmax(add(p0, d0), add(p1, d1)));
@@ -861,7 +899,7 @@ __global__ __launch_bounds__(kBlock) void k_list_ccd(const uint32_t *flags, Coll
             if (k < c) {
                 t = tris[k];
                 const double *b = cc.tri_box + 6 * size_t(t);
-                hit = overlap(lo, hi, mk(b[0], b[1], b[2]), mk(b[3], b[4], b[5]), p.sweep_target);
+                hit = overlap(lo, hi, mk(b[0], b[1], b[2]), mk(b[3], b[4], b[5]), target);
             }
             const unsigned long long mask = (__ballot(hit) >> shift) & ((G == 64) ? ~0ull : ((1ull << G) - 1));
             uint32_t base = 0;
@@ -883,7 +921,7 @@ __global__ __launch_bounds__(kBlock) void k_list_ccd(const uint32_t *flags, Coll
                                                 bound + slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
             const double r = advancement_limit(mk(e[0], e[1], e[2]), mk(e[3], e[4], e[5]), mk(e[6], e[7], e[8]),
                                                mk(e[9], e[10], e[11]), e[12], vertex(col, t3[0]), vertex(col, t3[1]),
-                                               vertex(col, t3[2]), p.sweep_target, seen / 0.9);
+                                               vertex(col, t3[2]), e[13], seen / 0.9);
             if (r < seen) atomicMin(bound + slot, (unsigned long long)__double_as_longlong(r));
             limit = fmin(limit, r);
         }
@@ -1011,13 +1049,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GRAD ? 2
             }
         } else {
             unsigned long long count = 0;
-            found = closest_triangle(col, p0, p1, p.contact_pad, pair, tri, count);
+            found = closest_triangle(col, p0, p1, contact_pad_of(p, L), pair, tri, count);
             // Lane 0 recomputes the previous tile's element for its coupling
             // terms only; statistics are counted by the element's owner lane.
             if (owner) acc.candidates += count;
         }
         if (found) {
-            const double gap = pair.distance - p.radius_offset;
+            const double radius_offset = radius_offset_of(p, L);
+            const double gap = pair.distance - radius_offset;
             if (owner) acc.min_gap = fmin(acc.min_gap, gap);
             if (gap <= p.minimum_gap) {
                 if (owner) acc.infeasible = 1;
@@ -1030,7 +1069,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GRAD ? 2
                 const V3 hp = add(mul(p0, w0), mul(p1, w1));
                 const V3 diff = sub(hp, bp);
                 const double dist = sqrt(dot(diff, diff) + 1e-30);
-                const double dgap = dist - p.radius_offset;
+                const double dgap = dist - radius_offset;
                 if (dgap < p.barrier) {
                     const double aa = p.barrier - dgap, lg = log(dgap * (1.0 / p.barrier));
                     contact_energy = -p.stiffness * aa * aa * lg;
@@ -1060,12 +1099,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GRAD ? 2
     // Cosserat elasticity (cuda_backend.cu:905-919).
     ElementJacobian J;
     double elastic = 0.0;
+    double weight[6];
+    weights_of(p, elem ? L : 0, weight);
     if (elem) {
         element_strains<GRAD>(qi, qj, m.frame + 9 * L, m.frame + 9 * (L + 1), m.rest + 8 * L, J);
         const double L0 = m.rest[8 * L];
 #pragma unroll
         for (int s = 0; s < 6; ++s) {
-            const double c = (0.5 * L0) * p.weights[s];
+            const double c = (0.5 * L0) * weight[s];
             elastic = elastic + (c * J.strain[s]) * J.strain[s];
         }
     }
@@ -1081,8 +1122,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GRAD ? 2
         double Lw[6], cw[6];
 #pragma unroll
         for (int s = 0; s < 6; ++s) {
-            Lw[s] = L0 * p.weights[s];
-            cw[s] = (0.5 * L0) * p.weights[s];
+            Lw[s] = L0 * weight[s];
+            cw[s] = (0.5 * L0) * weight[s];
         }
         // Jacobian entry of strain s by coordinate a of (xi, ui, xj, uj).
         auto jac = [&](int s, int a) -> double {
@@ -1676,23 +1717,43 @@ __global__ void k_fill_u64(unsigned long long *x, size_t count, unsigned long lo
 // Jacobi diagonal: inertia plus per-element stretch and bend/twist
 // stiffness bounds, node-gathered in element order (cuda_backend.cu:815-841).
 __global__ void k_preconditioner(RodView m, double inv_dt, double damping, double translation_k, double rotation_a,
-                                 double rotation_b, double *precond)
+                                 double rotation_b, const double *material, double poisson, double shear_correction,
+                                 double *precond)
 {
     const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= m.node_count) return;
     const double inv2 = inv_dt * inv_dt;
     const double pt = m.mass[n] * (inv2 + damping * inv_dt);
     const double pr = m.rotational_mass[n] * (inv2 + damping * inv_dt);
+    // the material's bounds, or those of element L's strand
+    auto bounds = [&](size_t L, double &tk, double &ra, double &rb) {
+        if (!material) {
+            tk = translation_k;
+            ra = rotation_a;
+            rb = rotation_b;
+            return;
+        }
+        const double E = material[2 * L], rad = material[2 * L + 1];
+        const double area = kPi * rad * rad, inertia = kPi * rad * rad * rad * rad / 4.0;
+        const double G = E / (2.0 * (1.0 + poisson));
+        tk = fmax(E * area, shear_correction * G * area);
+        ra = shear_correction * G * area;
+        rb = fmax(E * inertia, 2.0 * G * inertia);
+    };
     double t = pt, r = pr;
     if (n > 0 && (m.flags[n - 1] & kNodeHasElement)) {
         const double L0 = m.rest[8 * size_t(n - 1)];
-        t += translation_k / L0;
-        r += rotation_a * L0 + rotation_b / L0;
+        double tk, ra, rb;
+        bounds(n - 1, tk, ra, rb);
+        t += tk / L0;
+        r += ra * L0 + rb / L0;
     }
     if (m.flags[n] & kNodeHasElement) {
         const double L0 = m.rest[8 * size_t(n)];
-        t += translation_k / L0;
-        r += rotation_a * L0 + rotation_b / L0;
+        double tk, ra, rb;
+        bounds(n, tk, ra, rb);
+        t += tk / L0;
+        r += ra * L0 + rb / L0;
     }
     const bool fixed = m.flags[n] & kNodeFixed;
     for (int d = 0; d < 3; ++d) {
@@ -1769,7 +1830,7 @@ __global__ __launch_bounds__(kBlock) void k_ccd_slope(RodView m, ColliderView co
     Partial acc = empty_partial();
     bool need = false;
     V3 p0 = mk(0, 0, 0), p1 = p0, d0 = p0, d1 = p0;
-    double speed = 0.0;
+    double speed = 0.0, target = p.sweep_target;
     if (L < m.node_count) {
         double gd = 0.0;
 #pragma unroll
@@ -1785,6 +1846,7 @@ __global__ __launch_bounds__(kBlock) void k_ccd_slope(RodView m, ColliderView co
             d1 = load3(dir + 6 * size_t(L + 1));
             speed = fmax(norm(d0), norm(d1));
             need = speed >= 1e-30;
+            target = sweep_target_of(p, L);
         }
     }
     // After a list pass the few elements left (lists that overflowed) each
@@ -1798,10 +1860,10 @@ __global__ __launch_bounds__(kBlock) void k_ccd_slope(RodView m, ColliderView co
         const V3 lo = vmin(vmin(p0, p1), vmin(add(p0, d0), add(p1, d1)));
         const V3 hi = vmax(vmax(p0, p1), vmax(add(p0, d0), add(p1, d1)));
         double limit = 1.0;
-        bvh_query(col.bvh, internal_count(col), col.leaf_box, lo, hi, p.sweep_target, [&](uint32_t t) {
+        bvh_query(col.bvh, internal_count(col), col.leaf_box, lo, hi, target, [&](uint32_t t) {
             const uint32_t *t3 = col.triangles + 3 * size_t(t);
             limit = fmin(limit, advancement_limit(p0, p1, d0, d1, speed, vertex(col, t3[0]), vertex(col, t3[1]),
-                                                  vertex(col, t3[2]), p.sweep_target));
+                                                  vertex(col, t3[2]), target));
         });
         acc.alpha = limit;
         need = false;
@@ -1812,24 +1874,24 @@ __global__ __launch_bounds__(kBlock) void k_ccd_slope(RodView m, ColliderView co
         const V3 a1 = mk(__shfl(p1.x, src, kWave), __shfl(p1.y, src, kWave), __shfl(p1.z, src, kWave));
         const V3 e0 = mk(__shfl(d0.x, src, kWave), __shfl(d0.y, src, kWave), __shfl(d0.z, src, kWave));
         const V3 e1 = mk(__shfl(d1.x, src, kWave), __shfl(d1.y, src, kWave), __shfl(d1.z, src, kWave));
-        const double sp = __shfl(speed, src, kWave);
+        const double sp = __shfl(speed, src, kWave), tg = __shfl(target, src, kWave);
         const V3 lo = vmin(vmin(a0, a1), vmin(add(a0, e0), add(a1, e1)));
         const V3 hi = vmax(vmax(a0, a1), vmax(add(a0, e0), add(a1, e1)));
         double limit = 1.0;
-        const bool ok = wave_walk(col, lo, hi, p.sweep_target, queue, kGatherQueue, [&](uint32_t t) {
+        const bool ok = wave_walk(col, lo, hi, tg, queue, kGatherQueue, [&](uint32_t t) {
             const uint32_t *t3 = col.triangles + 3 * size_t(t);
             limit = fmin(limit, advancement_limit(a0, a1, e0, e1, sp, vertex(col, t3[0]), vertex(col, t3[1]),
-                                                  vertex(col, t3[2]), p.sweep_target, limit / 0.9));
+                                                  vertex(col, t3[2]), tg, limit / 0.9));
         });
 #pragma unroll
         for (int o = kWave / 2; o >= 1; o >>= 1) limit = fmin(limit, __shfl_xor(limit, o, kWave));
         if (lane == src) {
             if (!ok) {
                 limit = 1.0;
-                bvh_query(col.bvh, internal_count(col), col.leaf_box, lo, hi, p.sweep_target, [&](uint32_t t) {
+                bvh_query(col.bvh, internal_count(col), col.leaf_box, lo, hi, tg, [&](uint32_t t) {
                     const uint32_t *t3 = col.triangles + 3 * size_t(t);
                     limit = fmin(limit, advancement_limit(a0, a1, e0, e1, sp, vertex(col, t3[0]), vertex(col, t3[1]),
-                                                          vertex(col, t3[2]), p.sweep_target));
+                                                          vertex(col, t3[2]), tg));
                 });
             }
             acc.alpha = limit;
@@ -1878,9 +1940,10 @@ __global__ void k_sweep(RodView m, ColliderView col, StepParams p, const double 
     if (only_overflow && only_overflow[L] != kCacheOverflow) return;
     const V3 p0 = load3(q + 6 * size_t(L)), p1 = load3(q + 6 * size_t(L + 1));
     const uint32_t element = L - m.strand_of[L];
+    const double target = sweep_target_of(p, L);
     unsigned long long worst = ~0ull;
-    bvh_query(col.bvh, internal_count(col), col.leaf_box, vmin(p0, p1), vmax(p0, p1), p.sweep_target, [&](uint32_t t) {
-        if (sweep_fails(col, p0, p1, t, p.sweep_target)) {
+    bvh_query(col.bvh, internal_count(col), col.leaf_box, vmin(p0, p1), vmax(p0, p1), target, [&](uint32_t t) {
+        if (sweep_fails(col, p0, p1, t, target)) {
             const unsigned long long key = (static_cast<unsigned long long>(element) << 32) | t;
             worst = key < worst ? key : worst;
         }
@@ -1908,13 +1971,14 @@ __global__ __launch_bounds__(kBlock) void k_list_sweep(const uint32_t *flags, co
         const V3 p0 = load3(q + 6 * size_t(L)), p1 = load3(q + 6 * size_t(L + 1));
         const V3 lo = vmin(p0, p1), hi = vmax(p0, p1);
         const uint32_t element = L - strand_of[L];
+        const double target = sweep_target_of(p, L);
         const uint32_t *tris = cc.tris + size_t(L) * kCacheCapacity;
         unsigned long long worst = ~0ull;
         for (uint32_t k = lane; k < c; k += kWave) {
             const uint32_t t = tris[k];
             const double *b = swept_box + 6 * size_t(t);
-            if (!overlap(lo, hi, mk(b[0], b[1], b[2]), mk(b[3], b[4], b[5]), p.sweep_target)) continue;
-            if (sweep_fails(col, p0, p1, t, p.sweep_target)) {
+            if (!overlap(lo, hi, mk(b[0], b[1], b[2]), mk(b[3], b[4], b[5]), target)) continue;
+            if (sweep_fails(col, p0, p1, t, target)) {
                 const unsigned long long key = (static_cast<unsigned long long>(element) << 32) | t;
                 worst = key < worst ? key : worst;
             }
@@ -1948,6 +2012,7 @@ __global__ void k_sweep_record(RodView m, ColliderView col, StepParams p, const 
         if (m.strand_first[mid] - mid <= element) lo = mid; else hi = mid;
     }
     const uint32_t L = element + lo;
+    const double target = sweep_target_of(p, L);
     const V3 p0 = load3(q + 6 * size_t(L)), p1 = load3(q + 6 * size_t(L + 1));
     const uint32_t *t3 = col.triangles + 3 * size_t(t);
     const V3 oa = vertex_prev(col, t3[0]), ob = vertex_prev(col, t3[1]), oc = vertex_prev(col, t3[2]);
@@ -1961,7 +2026,7 @@ __global__ void k_sweep_record(RodView m, ColliderView col, StepParams p, const 
         b = add(ob, mul(db, alpha));
         c = add(oc, mul(dc, alpha));
         pair = closest_segment_triangle(p0, p1, a, b, c);
-        const double clearance = pair.distance - p.sweep_target;
+        const double clearance = pair.distance - target;
         if (clearance <= 0) break;
         const double advance = 0.8 * clearance / speed;
         if (alpha + advance >= 1 || advance < 1e-12) break;
@@ -1972,8 +2037,8 @@ __global__ void k_sweep_record(RodView m, ColliderView col, StepParams p, const 
     r.element = element;
     r.triangle = t;
     r.distance = pair.distance;
-    r.required = p.sweep_target;
-    r.clearance = pair.distance - p.sweep_target;
+    r.required = target;
+    r.clearance = pair.distance - target;
     r.displacement = speed;
     store3(r.hair_start, p0);
     store3(r.hair_end, p1);
@@ -2013,10 +2078,10 @@ __global__ __launch_bounds__(kBlock) void k_friction(RodView m, ColliderView col
                 }
             } else {
                 unsigned long long count = 0;
-                found = closest_triangle(col, p0, p1, p.contact_pad, pair, tri, count);
+                found = closest_triangle(col, p0, p1, contact_pad_of(p, L), pair, tri, count);
             }
             if (found) {
-                const double gap = pair.distance - p.radius_offset;
+                const double gap = pair.distance - radius_offset_of(p, L);
                 if (gap > 0.0 && gap < p.barrier) {
                     const uint32_t *t3 = col.triangles + 3 * size_t(tri);
                     const double w0 = 1.0 - pair.t, w1 = pair.t;
@@ -2114,11 +2179,13 @@ __global__ __launch_bounds__(kBlock) void k_initial_gap(RodView m, ColliderView 
         const uint32_t fl = m.flags[L];
         if ((fl & kNodeHasElement) && (fl & kNodeContact) && !(fl & kNodeFixed)) {
             const V3 p0 = load3(pos + 3 * size_t(L)), p1 = load3(pos + 3 * size_t(L + 1));
-            bvh_query(col.bvh, internal_count(col), col.leaf_box, vmin(p0, p1), vmax(p0, p1), p.contact_pad, [&](uint32_t t) {
+            const double radius_offset = radius_offset_of(p, L);
+            bvh_query(col.bvh, internal_count(col), col.leaf_box, vmin(p0, p1), vmax(p0, p1), contact_pad_of(p, L),
+                      [&](uint32_t t) {
                 const uint32_t *t3 = col.triangles + 3 * size_t(t);
                 const Closest c = closest_segment_triangle(p0, p1, vertex(col, t3[0]), vertex(col, t3[1]),
                                                            vertex(col, t3[2]), true);
-                acc.min_gap = fmin(acc.min_gap, c.distance - p.radius_offset);
+                acc.min_gap = fmin(acc.min_gap, c.distance - radius_offset);
                 ++acc.candidates;
             });
         }
@@ -2136,7 +2203,8 @@ __global__ void k_broadphase_pairs(RodView m, ColliderView col, StepParams p, co
     if (!(fl & kNodeHasElement) || !(fl & kNodeContact) || (fl & kNodeFixed)) return;
     const V3 p0 = load3(q + 6 * size_t(L)), p1 = load3(q + 6 * size_t(L + 1));
     const uint32_t element = L - m.strand_of[L];
-    bvh_query(col.bvh, internal_count(col), col.leaf_box, vmin(p0, p1), vmax(p0, p1), p.contact_pad, [&](uint32_t t) {
+    bvh_query(col.bvh, internal_count(col), col.leaf_box, vmin(p0, p1), vmax(p0, p1), contact_pad_of(p, L),
+              [&](uint32_t t) {
         const unsigned long long slot = atomicAdd(count, 1ull);
         if (slot < capacity) {
             pairs[2 * slot] = element;

@@ -243,6 +243,61 @@ def _head_point(d: np.ndarray) -> np.ndarray:
     return _head_bulge(d)[..., None] * d * np.asarray(HEAD_AXES)
 
 
+def head_body():
+    """The head + torso body of the groom scenes: the deformed ellipsoid head
+    (icosphere subdivision 6, 40,962 vertices) and a UV-ellipsoid torso
+    (305 x 604 + 2 poles, 184,222 vertices); 225,184 vertices and 450,360
+    triangles."""
+    hv, ht = icosphere(6, 1.0)
+    head_v = _head_point(hv)
+    tv, tt = uv_ellipsoid(305, 604, TORSO_AXES, TORSO_CENTER)
+    vertices = np.concatenate([head_v, tv])
+    triangles = np.concatenate([ht, tt + head_v.shape[0]]).astype(np.uint32)
+    return vertices, triangles
+
+
+TORSO_AXES, TORSO_CENTER = (0.19, 0.12, 0.28), (0.0, 0.0, -0.43)
+
+
+def uneven_scene(strands: int = 50000, seed: int = 3, lengths=(4, 256), spin_deg_per_s: float = 30.0,
+                 fps: float = 24.0):
+    """The uneven-length groom: `strands` strands with node counts uniform
+    over `lengths` (inclusive; about 6.5M nodes at 50,000 strands) rooted
+    on the upper part of the head body, 8 mm segments leaving along the
+    surface normal and turning sideways by at most 1 rad over the strand (so
+    no strand turns down into the body); per-strand Young's modulus U(1, 5)
+    GPa and radius U(30, 80) um. Returns the scene and the two per-strand
+    arrays. Generated with numpy's seeded generator (vectorized per strand)."""
+    rng = np.random.default_rng(seed)
+    vertices, triangles = head_body()
+    d = rng.normal(size=(3 * strands, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d = d[d[:, 2] > 0.2][:strands]
+    surface = _head_point(d)
+    axes = np.asarray(HEAD_AXES)
+    normal = surface / axes ** 2
+    normal /= np.linalg.norm(normal, axis=1, keepdims=True)
+    side = np.cross(normal, [0.0, 0.0, 1.0])
+    side /= np.maximum(np.linalg.norm(side, axis=1, keepdims=True), 1e-12)
+    counts = rng.integers(lengths[0], lengths[1] + 1, strands)
+    turn = rng.uniform(0.0, 1.0, strands) / np.maximum(counts - 1, 1)
+    pts, offsets = [], [0]
+    for s in range(strands):
+        ang = turn[s] * np.arange(counts[s] - 1)
+        t = np.cos(ang)[:, None] * normal[s] + np.sin(ang)[:, None] * side[s]
+        # 3 mm off the head surface so the deformed surface's facets stay clear
+        p = surface[s] + 0.003 * normal[s] + 0.008 * np.concatenate([[np.zeros(3)], np.cumsum(t, axis=0)])
+        pts.append(p)
+        offsets.append(offsets[-1] + counts[s])
+    young = rng.uniform(1.0e9, 5.0e9, strands)
+    radius = rng.uniform(3.0e-5, 8.0e-5, strands)
+    scene = Scene(points=np.concatenate(pts), offsets=np.array(offsets, dtype=np.uint32), vertices=vertices,
+                  triangles=triangles, desc=dict(substeps=8, maximum_substeps=32),
+                  material=dict(friction=0.3, barrier_distance=1.0e-3), frame_dt=1.0 / fps,
+                  spin_per_frame=math.radians(spin_deg_per_s) / fps)
+    return scene, young, radius
+
+
 def head_scene(strands: int = 6757, seed: int = 2, spin_deg_per_s: float = 30.0, fps: float = 24.0) -> Scene:
     """Groom-scale scene: 6,757 strands with lognormal node counts
     (median 36, sigma 0.41, clipped 8..80) 8 mm apart on the upper part of a
@@ -250,12 +305,8 @@ def head_scene(strands: int = 6757, seed: int = 2, spin_deg_per_s: float = 30.0,
     vertices) + torso (UV ellipsoid 305 x 604 + 2 poles, 184,222 vertices),
     225,184 vertices and 450,360 triangles; mu = 0.3, barrier 1 mm."""
     rng = SplitMix64(seed)
-    hv, ht = icosphere(6, 1.0)
-    head_v = _head_point(hv)
-    torso_axes, torso_center = (0.19, 0.12, 0.28), (0.0, 0.0, -0.43)
-    tv, tt = uv_ellipsoid(305, 604, torso_axes, torso_center)
-    vertices = np.concatenate([head_v, tv])
-    triangles = np.concatenate([ht, tt + head_v.shape[0]]).astype(np.uint32)
+    vertices, triangles = head_body()
+    torso_axes, torso_center = TORSO_AXES, TORSO_CENTER
     seg = 0.008
     pts, offsets = [], [0]
     redraws = 0

@@ -311,6 +311,7 @@ EXT_FUNCTIONS = {
     "khsxSetRollbackRing": (C.c_int, [_P, C.c_uint32]),
     "khsxRollback": (C.c_int, [_P, C.c_uint32]),
     "khsxWriteCacheFrame": (C.c_int, [_P, C.c_int]),
+    "khsxSetStrandMaterials": (C.c_int, [_P, _P, _P, C.c_uint32]),
     "khsxSetColliderState": (C.c_int, [_P, _P, _P, C.c_uint32]),
     "khsxApplyFriction": (C.c_int, [_P, C.c_double]),
     "khsxStepBound": (C.c_int, [_P, _P, C.c_uint64, _D(C.c_double)]),
@@ -590,6 +591,13 @@ class HairSolver:
 
     def contact_list_stats(self) -> KhsxContactListStats:
         return self._get(self.lib.khsxGetContactListStats, KhsxContactListStats)
+
+    def set_strand_materials(self, young_modulus=None, radius=None) -> int:
+        if young_modulus is None and radius is None:
+            return self.lib.khsxSetStrandMaterials(self.handle, None, None, 0)
+        E = np.ascontiguousarray(young_modulus, dtype=np.float64)
+        r = np.ascontiguousarray(radius, dtype=np.float64)
+        return self.lib.khsxSetStrandMaterials(self.handle, _ptr(E), _ptr(r), E.size)
 
     def set_collider_state(self, current, previous=None) -> int:
         cur = np.ascontiguousarray(current, dtype=np.float64)

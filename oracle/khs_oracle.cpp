@@ -260,6 +260,8 @@ public:
         clear_built();
         input_points_ = pts;
         input_offsets_.assign(offsets, offsets + strand_count + 1);
+        strand_E_.clear();
+        strand_r_.clear();
         material_ = m;
         has_hair_ = true;
         error.clear();
@@ -393,7 +395,7 @@ public:
         material_ = m;
         init_masses();
         trust_radius_ = std::fmax(0.25 * max_rest_length_,
-                                  8.0 * (material_.radius + material_.collider_offset + material_.barrier_distance));
+                                  8.0 * (max_radius() + material_.collider_offset + material_.barrier_distance));
         error.clear();
         return KHS_OK;
     }
@@ -917,6 +919,47 @@ public:
         return KHS_OK;
     }
 
+    double radius_of(uint32_t strand) const { return strand_r_.empty() ? material_.radius : strand_r_[strand]; }
+    double young_of(uint32_t strand) const { return strand_E_.empty() ? material_.young_modulus : strand_E_[strand]; }
+    double max_radius() const
+    {
+        double r = material_.radius;
+        if (!strand_r_.empty()) {
+            r = strand_r_[0];
+            for (double x : strand_r_) r = std::fmax(r, x);
+        }
+        return r;
+    }
+
+    // Per-strand Young's modulus and radius; both NULL clears them. Applies
+    // at once to a built solver (masses and trust radius) and to later
+    // builds; new hair curves clear them.
+    KhsResult set_strand_materials(const double *E, const double *r, uint32_t count)
+    {
+        if (!E && !r) {
+            strand_E_.clear();
+            strand_r_.clear();
+        } else {
+            if (!E || !r || !has_hair_ || count != input_offsets_.size() - 1) {
+                error = "Per-strand materials need both arrays, one value per hair strand.";
+                return KHS_ERROR_INVALID_ARGUMENT;
+            }
+            for (uint32_t k = 0; k < count; ++k)
+                if (!(E[k] > 0.0 && std::isfinite(E[k]) && r[k] > 0.0 && std::isfinite(r[k]))) {
+                    error = "Strand " + std::to_string(k) + " has a material value out of range.";
+                    return KHS_ERROR_INVALID_ARGUMENT;
+                }
+            strand_E_.assign(E, E + count);
+            strand_r_.assign(r, r + count);
+        }
+        if (built_) {
+            init_masses();
+            trust_radius_ = std::fmax(0.25 * max_rest_length_,
+                                      8.0 * (max_radius() + material_.collider_offset + material_.barrier_distance));
+        }
+        return KHS_OK;
+    }
+
     KhsResult direction_counts(uint64_t *block, uint64_t *steepest) const
     {
         if (!block || !steepest) return KHS_ERROR_INVALID_ARGUMENT;
@@ -939,11 +982,11 @@ public:
         uint64_t total = 0;
         if (has_collider_) {
             bvh_.refit(collider_cur_, nullptr);
-            const double pad = material_.radius + material_.collider_offset + material_.barrier_distance;
             std::vector<uint32_t> cand;
             for (uint32_t k = 0; k < elements_.size(); ++k) {
                 const Element &e = elements_[k];
                 if (nodes_[e.i].fixed || !e.contact) continue;
+                const double pad = radius_of(e.strand) + material_.collider_offset + material_.barrier_distance;
                 const V3 p0 = pos(e.i), p1 = pos(e.j);
                 bvh_.query(vmin(p0, p1), vmax(p0, p1), pad, cand);
                 for (uint32_t t : cand) {
@@ -1108,11 +1151,12 @@ private:
     {
         // solver.cpp:358-373
         const double pi = 3.14159265358979323846;
-        const double area = pi * material_.radius * material_.radius;
-        const double inertia = pi * std::pow(material_.radius, 4.0) / 4.0;
-        const double polar = 2.0 * inertia;
         for (Node &n : nodes_) { n.mass = 0.0; n.rot_mass = 0.0; }
         for (const Element &e : elements_) {
+            const double r = radius_of(e.strand);
+            const double area = pi * r * r;
+            const double inertia = pi * std::pow(r, 4.0) / 4.0;
+            const double polar = 2.0 * inertia;
             const double mass = material_.density * area * e.rest_length;
             const double rot = material_.density * (2.0 * inertia + polar) / 3.0 * e.rest_length;
             nodes_[e.i].mass += 0.5 * mass;
@@ -1267,15 +1311,15 @@ private:
         double minimum = kInf;
         uint64_t count = 0;
         std::vector<uint32_t> cand;
-        const double pad = material_.radius + material_.collider_offset + material_.barrier_distance;
         for (const Element &e : elements_) {
             if (nodes_[e.i].fixed || !e.contact) continue;
+            const double pad = radius_of(e.strand) + material_.collider_offset + material_.barrier_distance;
             bvh_.query(vmin(nodes_[e.i].x, nodes_[e.j].x), vmax(nodes_[e.i].x, nodes_[e.j].x), pad, cand);
             for (uint32_t t : cand) {
                 const Tri &tr = tris_[t];
                 const Closest c = closest_segment_triangle(nodes_[e.i].x, nodes_[e.j].x, collider_cur_[tr.i0],
                                                            collider_cur_[tr.i1], collider_cur_[tr.i2], true);
-                minimum = std::min(minimum, c.distance - material_.radius - material_.collider_offset);
+                minimum = std::min(minimum, c.distance - radius_of(e.strand) - material_.collider_offset);
                 ++count;
             }
         }
@@ -1321,7 +1365,7 @@ private:
         collider_frame_start_ = collider_input_;
         collider_pending_ = collider_input_;
         trust_radius_ = std::fmax(0.25 * max_rest_length_,
-                                  8.0 * (material_.radius + material_.collider_offset + material_.barrier_distance));
+                                  8.0 * (max_radius() + material_.collider_offset + material_.barrier_distance));
         anim_ready_ = false;
         anim_frames_ = 0;
         anim_current_ = 0;
@@ -1407,12 +1451,11 @@ private:
         }
         Dual<12> strain[6];
         rod_strains(dq, ni.frame, nj.frame, e, strain);
-        const double r = material_.radius;
+        const double r = radius_of(e.strand), E = young_of(e.strand);
         const double area = kPi * r * r, inertia = kPi * r * r * r * r / 4.0;
-        const double G = material_.young_modulus / (2.0 * (1.0 + material_.poisson_ratio));
+        const double G = E / (2.0 * (1.0 + material_.poisson_ratio));
         const double w[6] = {material_.shear_correction * G * area, material_.shear_correction * G * area,
-                             material_.young_modulus * area, material_.young_modulus * inertia,
-                             material_.young_modulus * inertia, 2.0 * G * inertia};
+                             E * area, E * inertia, E * inertia, 2.0 * G * inertia};
         Dual<12> re(0.0);
         for (int s = 0; s < 6; ++s) re = re + Dual<12>(0.5 * e.rest_length * w[s]) * strain[s] * strain[s];
         st.objective += re.v;
@@ -1431,12 +1474,12 @@ private:
         }
         if (!has_collider_ || ni.fixed || !e.contact) return;
         const V3 p0 = pos(e.i), p1 = pos(e.j);
-        const double pad = material_.radius + material_.collider_offset + material_.barrier_distance;
+        const double pad = r + material_.collider_offset + material_.barrier_distance;
         Closest pair;
         uint32_t ti = 0;
         if (!closest_triangle(p0, p1, pad, cand, pair, ti, &st.candidates)) return;
         const Tri &t = tris_[ti];
-        const double gap = pair.distance - material_.radius - material_.collider_offset;
+        const double gap = pair.distance - r - material_.collider_offset;
         st.min_gap = std::fmin(st.min_gap, gap);
         if (gap <= desc_.minimum_gap) { st.feasible = false; return; }
         if (gap >= material_.barrier_distance) return;
@@ -1457,7 +1500,7 @@ private:
         const V3 bp = A * pair.bary.x + B * pair.bary.y + C * pair.bary.z;
         const TV3<D6> diff = tsub(p, TV3<D6>{D6(bp.x), D6(bp.y), D6(bp.z)});
         const D6 dist = dsqrt(tdot(diff, diff) + D6(1e-30));
-        const D6 dgap = dist - D6(material_.radius + material_.collider_offset);
+        const D6 dgap = dist - D6(r + material_.collider_offset);
         D6 ce(0.0);
         if (dgap.v >= material_.barrier_distance) {
             ce = cq[0] * D6(0.0);
@@ -1657,18 +1700,23 @@ private:
     {
         // cuda_backend.cu:815-841
         const double inv = 1.0 / dt, inv2 = inv * inv;
-        const double r = material_.radius;
-        const double area = kPi * r * r, inertia = kPi * r * r * r * r / 4.0;
-        const double shear = material_.young_modulus / (2.0 * (1.0 + material_.poisson_ratio));
-        const double tr_el = std::fmax(material_.young_modulus * area, material_.shear_correction * shear * area);
-        const double rot_a = material_.shear_correction * shear * area;
-        const double rot_b = std::fmax(material_.young_modulus * inertia, 2.0 * shear * inertia);
+        // per element: stretch and bend/twist bounds of its strand's material
+        auto bounds = [&](uint32_t strand, double &tr_el, double &rot_a, double &rot_b) {
+            const double r = radius_of(strand), E = young_of(strand);
+            const double area = kPi * r * r, inertia = kPi * r * r * r * r / 4.0;
+            const double shear = E / (2.0 * (1.0 + material_.poisson_ratio));
+            tr_el = std::fmax(E * area, material_.shear_correction * shear * area);
+            rot_a = material_.shear_correction * shear * area;
+            rot_b = std::fmax(E * inertia, 2.0 * shear * inertia);
+        };
         for (size_t n = 0; n < nodes_.size(); ++n) {
             const double p = nodes_[n].mass * (inv2 + material_.mass_damping * inv);
             const double rr = nodes_[n].rot_mass * (inv2 + material_.mass_damping * inv);
             for (int d = 0; d < 3; ++d) { P_[6 * n + d] = p; P_[6 * n + 3 + d] = rr; }
         }
         for (const Element &e : elements_) {
+            double tr_el, rot_a, rot_b;
+            bounds(e.strand, tr_el, rot_a, rot_b);
             const double t = tr_el / e.rest_length;
             const double rot = rot_a * e.rest_length + rot_b / e.rest_length;
             for (int d = 0; d < 3; ++d) {
@@ -1732,10 +1780,10 @@ private:
     // (cuda_backend.cu:989-1000).
     double ccd_limit()
     {
-        const double target = material_.radius + material_.collider_offset + desc_.minimum_gap;
         std::vector<double> part(strands_.size(), 1.0);
         parallel_for(threads, uint32_t(strands_.size()), [&](uint32_t s) {
             const Strand &st = strands_[s];
+            const double target = radius_of(s) + material_.collider_offset + desc_.minimum_gap;
             double limit = 1.0;
             std::vector<uint32_t> cand;
             const uint32_t e0 = st.first - s;
@@ -1781,10 +1829,10 @@ private:
     // element, then its lowest failing triangle, is reported.
     bool sweep_feasible(SweepFailure &fail_out)
     {
-        const double target = material_.radius + material_.collider_offset + desc_.minimum_gap;
         std::vector<SweepFailure> part(strands_.size());
         parallel_for(threads, uint32_t(strands_.size()), [&](uint32_t s) {
             const Strand &st = strands_[s];
+            const double target = radius_of(s) + material_.collider_offset + desc_.minimum_gap;
             std::vector<uint32_t> cand;
             const uint32_t e0 = st.first - s;
             for (uint32_t k = e0; k + 1 < e0 + st.count && !part[s].claimed; ++k) {
@@ -1844,9 +1892,9 @@ private:
     {
         if (!has_collider_ || !(material_.friction > 0.0)) return;
         std::fill(dv_.begin(), dv_.end(), 0.0);
-        const double pad = material_.radius + material_.collider_offset + material_.barrier_distance;
         parallel_for(threads, uint32_t(strands_.size()), [&](uint32_t s) {
             const Strand &st = strands_[s];
+            const double pad = radius_of(s) + material_.collider_offset + material_.barrier_distance;
             std::vector<uint32_t> cand;
             const uint32_t e0 = st.first - s;
             for (uint32_t k = e0; k + 1 < e0 + st.count; ++k) {
@@ -1858,7 +1906,7 @@ private:
                 Closest pair;
                 uint32_t ti = 0;
                 if (!closest_triangle(p0, p1, pad, cand, pair, ti, nullptr)) continue;
-                const double gap = pair.distance - material_.radius - material_.collider_offset;
+                const double gap = pair.distance - radius_of(s) - material_.collider_offset;
                 if (!(gap > 0.0 && gap < material_.barrier_distance)) continue;
                 const Tri &t = tris_[ti];
                 const double w0 = 1.0 - pair.t, w1 = pair.t;
@@ -2151,6 +2199,9 @@ private:
     }
 
     uint64_t block_directions_ = 0, steepest_directions_ = 0;
+    // per-strand Young's modulus and radius (khsxSetStrandMaterials); empty:
+    // the material's values for every strand
+    std::vector<double> strand_E_, strand_r_;
     KhsSolverDesc desc_{};
     KhsHairMaterial material_{};
     bool has_hair_ = false, has_collider_ = false, built_ = false, closed_ = false;
@@ -2561,6 +2612,11 @@ KhsResult khsxStepBound(KhsSolver *s, const double *direction, uint64_t count, d
 KhsResult khsxSweepCheck(KhsSolver *s, uint32_t *feasible, KhsFailureDiagnostics *diagnostics)
 {
     return guarded(s, [&] { return s->impl.sweep_check(feasible, diagnostics); });
+}
+
+KhsResult khsxSetStrandMaterials(KhsSolver *s, const double *young_modulus, const double *radius, uint32_t count)
+{
+    return guarded(s, [&] { return s->impl.set_strand_materials(young_modulus, radius, count); });
 }
 
 KhsResult khsxSetDirectionMode(KhsSolver *s, uint32_t mode)
