@@ -1181,14 +1181,21 @@ void Solver::evaluate_async(bool gradient, double dt, bool line_search, unsigned
                            d.q_old.get(), d.vel.get(), d.q_base.get(), d.dir.get(), d.alpha.get(), d.grad.get(),
                            d.Dblk.get(), d.Ublk.get(), d.partials.get(), stop);
     };
+    // variants with per-strand stiffness weights only when they are set
+    const bool mat = d.elem_material.size() != 0;
     if (gradient) {
-        if (list_pass) launch(dev::k_evaluate<true, false, true>, (const unsigned *)nullptr);
-        else launch(dev::k_evaluate<true, false, false>, (const unsigned *)nullptr);
+        if (list_pass) launch(mat ? dev::k_evaluate<true, false, true, true> : dev::k_evaluate<true, false, true, false>,
+                              (const unsigned *)nullptr);
+        else launch(mat ? dev::k_evaluate<true, false, false, true> : dev::k_evaluate<true, false, false, false>,
+                    (const unsigned *)nullptr);
     } else if (line_search) {
-        if (list_pass) launch(dev::k_evaluate<false, true, true>, d.skip());
-        else launch(dev::k_evaluate<false, true, false>, d.skip());
+        if (list_pass) launch(mat ? dev::k_evaluate<false, true, true, true> : dev::k_evaluate<false, true, true, false>,
+                              d.skip());
+        else launch(mat ? dev::k_evaluate<false, true, false, true> : dev::k_evaluate<false, true, false, false>,
+                    d.skip());
     } else {
-        launch(dev::k_evaluate<false, false, false>, (const unsigned *)nullptr);
+        launch(mat ? dev::k_evaluate<false, false, false, true> : dev::k_evaluate<false, false, false, false>,
+               (const unsigned *)nullptr);
     }
     HIP_TRY(hipGetLastError());
     if (gradient) d.toc(t0, d.totals.assembly_ms, d.totals.assembly_launches);

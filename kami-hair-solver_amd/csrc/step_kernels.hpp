@@ -182,10 +182,13 @@ __device__ __forceinline__ double sweep_target_of(const StepParams &p, size_t L)
     return p.material ? (p.material[2 * L + 1] + p.collider_offset) + p.minimum_gap : p.sweep_target;
 }
 constexpr double kPi = 3.1415926535897932384626433832795;
-// kGA, kGA, EA, EI, EI, 2GI (cuda_backend.cu:905-910)
+// kGA, kGA, EA, EI, EI, 2GI (cuda_backend.cu:905-910); MAT selects at
+// compile time whether per-strand materials can be present, so the
+// material-wide kernels keep their registers.
+template <bool MAT>
 __device__ __forceinline__ void weights_of(const StepParams &p, size_t L, double w[6])
 {
-    if (!p.material) {
+    if (!MAT || !p.material) {
 #pragma unroll
         for (int s = 0; s < 6; ++s) w[s] = p.weights[s];
         return;
@@ -975,7 +978,7 @@ __device__ inline void store_tile_blocks(const double *st, double *out, long fir
 // (a few values spill); at one wavefront per SIMD its loads are exposed.
 // With LISTS the list pass before it has chosen every element's closest
 // triangle, and the tree walk (and its stack) is not compiled in.
-template <bool GRAD, bool LS, bool LISTS>
+template <bool GRAD, bool LS, bool LISTS, bool MAT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GRAD ? 2 : 1))) void k_evaluate(
     RodView m, ColliderView col, ContactCache cc, const uint32_t *best_tri, StepParams p, double *q,
     const double *q_old, const double *vel, const double *q_base, const double *dir, const double *alpha_at,
@@ -1100,7 +1103,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GRAD ? 2
     ElementJacobian J;
     double elastic = 0.0;
     double weight[6];
-    weights_of(p, elem ? L : 0, weight);
+    weights_of<MAT>(p, elem ? L : 0, weight);
     if (elem) {
         element_strains<GRAD>(qi, qj, m.frame + 9 * L, m.frame + 9 * (L + 1), m.rest + 8 * L, J);
         const double L0 = m.rest[8 * L];
