@@ -211,6 +211,62 @@ def run_mixed(lib, scene, warmup, frames, fp64_positions):
     }
 
 
+SOLVE_KERNELS = {
+    0: "k_block_solve (block-tridiagonal Thomas per strand)",
+    1: "k_block_solve_twisted (block-tridiagonal, from both ends)",
+    2: "k_block_solve_fused (block-tridiagonal, fused elimination)",
+}
+SOLVE_ALGORITHMS = {
+    0: "block Thomas with the Gauss-Jordan inverse (the reference's)",
+    1: "twisted block factorization",
+    2: "fused elimination (factors from Gauss-Jordan on [S | U | rhs], no inverse)",
+}
+
+
+def run_solve_algorithm(lib, scene, warmup, frames, algorithm, reference_positions):
+    """The same FP64 workload with another block-solve algorithm (0 block
+    Thomas, 1 twisted factorization, 2 fused elimination) from the same
+    start: node-substeps/s,
+    the solve kernel's average launch, the Newton work, the relative residual
+    the substeps reached, and the largest position difference from the main
+    form after the same frames (in contact, where substeps stop at the
+    iteration cap, rounding differences grow to millimetres)."""
+    s = setup_animation(lib, scene, warmup + frames + 1, **LIFTED)
+    s.check(s.set_solve_algorithm(algorithm))
+    for f in range(1, warmup + 1):
+        s.check(s.step_frame(f, scene.frame_dt))
+    s.check(s.set_kernel_timing(True))
+    substeps = newton = 0
+    residuals, lat = [], []
+    t0 = time.perf_counter()
+    for f in range(warmup + 1, warmup + frames + 1):
+        s.check(s.step_frame(f, scene.frame_dt))
+        st = s.step_stats()
+        substeps += st.substeps
+        newton += st.newton_iterations
+        residuals.append(st.relative_residual_norm)
+        lat.extend(s.substep_times().tolist())
+    elapsed = time.perf_counter() - t0
+    timing = s.kernel_timing()
+    nodes = s.node_count()
+    diff = (float(np.max(np.abs(s.internal_positions() - reference_positions)))
+            if reference_positions is not None else None)
+    s.close()
+    return {
+        "solve_algorithm": SOLVE_ALGORITHMS[algorithm],
+        "value": nodes * substeps / elapsed,
+        "unit": "node-substeps/s",
+        "frames": frames,
+        "ms_per_frame": elapsed * 1e3 / frames,
+        "solve_avg_ms": timing.solve_ms / max(timing.solve_launches, 1),
+        "solve_launches": timing.solve_launches,
+        "substep_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "newton_iterations_per_substep": newton / max(substeps, 1),
+        "relative_residual_last_substep_median": float(np.median(residuals)),
+        "max_position_difference_vs_main_m": diff,
+    }
+
+
 def run_uneven(lib, warmup, frames):
     """The uneven-length groom (50,000 strands of 4..256 nodes, about 6.5M
     nodes, per-strand E and radius) on the rotating head + torso body, plain
@@ -299,6 +355,10 @@ def main():
                     help="run one device copy of this size first (for scaling PMC traffic counters)")
     ap.add_argument("--batch-jobs", type=int, default=4, help="concurrent jobs of the batched form (0: skip)")
     ap.add_argument("--mixed-frames", type=int, default=8, help="timed frames of the mixed-precision form (0: skip)")
+    ap.add_argument("--solve-algorithm", type=int, default=2, choices=(0, 1, 2),
+                    help="block solve of the main form: 0 block Thomas, 1 twisted factorization, 2 fused elimination")
+    ap.add_argument("--other-solve-frames", type=int, default=8,
+                    help="timed frames of the main workload with each other solve algorithm (0: skip)")
     ap.add_argument("--uneven-frames", type=int, default=3,
                     help="timed frames of the 50,000-strand uneven-length groom (0: skip)")
     ap.add_argument("--serve-window", type=float, default=20.0, help="arrival window of the served form, s (0: skip)")
@@ -324,6 +384,7 @@ def main():
     frames = args.warmup + args.steps + 1
     t = time.perf_counter()
     s = setup_animation(lib, scene, frames, **LIFTED)
+    s.check(s.set_solve_algorithm(args.solve_algorithm))
     nodes = s.node_count()
     log(f"[bench] built and uploaded {frames} frames in {time.perf_counter() - t:.1f} s; {nodes} nodes")
     for f in range(1, args.warmup + 1):
@@ -335,6 +396,8 @@ def main():
     substep_ms, total_substeps, newton, ls_evals, pairs = [], 0, 0, 0, 0
     min_gap, penetrations = float("inf"), 0
     mixed_last = args.warmup + args.mixed_frames
+    other_last = args.warmup + args.other_solve_frames
+    main_at_other_last = None
     phase_ms = {"assembly": 0.0, "collision": 0.0, "optimization": 0.0}
     last_residuals = []  # relative residual of each frame's last substep
     fp64_at_mixed_last = None
@@ -345,6 +408,8 @@ def main():
             # the positions the mixed form is compared with (outside the
             # step; the copy costs a few milliseconds of the timed region)
             fp64_at_mixed_last = s.internal_positions()
+        if f == other_last:
+            main_at_other_last = fp64_at_mixed_last if f == mixed_last else s.internal_positions()
         st = s.step_stats()
         total_substeps += st.substeps
         newton += st.newton_iterations
@@ -368,6 +433,8 @@ def main():
     dom_ms, dom_launches = totals[dominant]
     dom_avg_ms = dom_ms / max(dom_launches, 1)
     kernel_name, bytes_per_node = KERNEL_CLASSES[dominant]
+    if dominant == "solve":
+        kernel_name = SOLVE_KERNELS[args.solve_algorithm]
     achieved = bytes_per_node * nodes / (dom_avg_ms * 1e-3) / 1e9 if dom_launches else None
     traffic = None
     traffic_file = os.path.join(REPO, "profiles", f"{dominant}_hbm_traffic.json")
@@ -391,7 +458,7 @@ def main():
         "config": {
             "workload": "6,757-strand groom, 261,305 nodes, 225,184-vertex body rotating 30 deg/s, mu 0.3, "
                         "barrier 1 mm, 8 substeps/frame at 24 fps, relative Newton residual 1e-6",
-            "form": "plain FP64 step",
+            "form": "plain FP64 step, " + SOLVE_ALGORITHMS[args.solve_algorithm],
             "strands": scene.strand_count,
             "nodes": nodes,
             "collider_triangles": int(scene.triangles.shape[0]),
@@ -453,15 +520,24 @@ def main():
         },
         "reference_derived_node_substeps_per_s": 3.16e5,
     }
+    s.close()
+    if args.other_solve_frames > 0:
+        result["other_solve_algorithms"] = []
+        for algorithm in (0, 1, 2):
+            if algorithm == args.solve_algorithm:
+                continue
+            t = time.perf_counter()
+            other = run_solve_algorithm(lib, scene, args.warmup, args.other_solve_frames, algorithm,
+                                        main_at_other_last)
+            result["other_solve_algorithms"].append(other)
+            log(f"[bench] solve algorithm {algorithm}: {args.other_solve_frames} frames in "
+                f"{time.perf_counter() - t:.1f} s, solve {other['solve_avg_ms'] * 1e3:.1f} us")
     if args.mixed_frames > 0:
-        s.close()
         t = time.perf_counter()
         result["mixed_precision"] = run_mixed(lib, scene, args.warmup, args.mixed_frames, fp64_at_mixed_last)
         log(f"[bench] mixed form: {args.mixed_frames} frames in {time.perf_counter() - t:.1f} s, "
             f"{result['mixed_precision']['value']:.4g} node-substeps/s")
     if args.batch_jobs > 0:
-        if args.mixed_frames <= 0:
-            s.close()
         t = time.perf_counter()
         result["batched"] = run_batched(lib, scene, args.batch_jobs, args.warmup, args.batch_frames or args.steps)
         log(f"[bench] batched form: {args.batch_jobs} jobs in {time.perf_counter() - t:.1f} s, "

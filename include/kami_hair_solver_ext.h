@@ -242,6 +242,21 @@ KHS_API KhsResult khsxSweepCheck(KhsSolver *solver, uint32_t *feasible, KhsFailu
    on fixed nodes and no trust-radius scaling. */
 KHS_API KhsResult khsxSetDirectionMode(KhsSolver *solver, uint32_t mode);
 
+/* Block-tridiagonal solve algorithm. 0: block Thomas from the root to the
+   tip with the explicit Gauss-Jordan inverse and partial pivoting, the
+   reference's (src/cuda_backend.cu:855-881). 1: twisted factorization,
+   eliminating from the root and from the tip towards the middle node at
+   once. 2: fused: block Thomas whose factors C' = S^-1 U and d' come out of
+   one Gauss-Jordan elimination of [S | U | rhs] without row exchanges (the
+   blocks are symmetric positive definite), with the reference's diagonal
+   shift; no inverse is formed. All three give the same direction in exact
+   arithmetic, rounded differently. Both libraries implement each with the
+   same operations (the device's reciprocal in 2 is within an ulp of a
+   division), so parity holds in any. Algorithm 2 on a solver whose D, U
+   and gradient arrays reach 4 GiB together (about 6.8M nodes) runs 0.
+   Precision mode 1 always runs the FP32 form of 0. */
+KHS_API KhsResult khsxSetSolveAlgorithm(KhsSolver *solver, uint32_t algorithm);
+
 /* Directions the Newton loop used in the last stepped frame (its final
    adaptive attempt): block-tridiagonal directions and steepest-descent
    directions (forced or fallback). */

@@ -303,6 +303,11 @@ KhsResult khsxSetPrecisionMode(KhsSolver *s, uint32_t mode)
     return guarded(s, [&] { return s->impl.set_precision_mode(mode); });
 }
 
+KhsResult khsxSetSolveAlgorithm(KhsSolver *s, uint32_t algorithm)
+{
+    return guarded(s, [&] { return s->impl.set_solve_algorithm(algorithm); });
+}
+
 KhsResult khsxSetKernelTiming(KhsSolver *s, uint32_t enable)
 {
     return guarded(s, [&] { return s->impl.set_kernel_timing(enable); });

@@ -53,10 +53,19 @@ public:
         reset();
         count_ = count;
         if (count) HIP_TRY(hipMalloc(&ptr_, count * sizeof(T)));
+        owned_ = true;
+    }
+    // `count` elements of another buffer's allocation, which outlives this
+    void view(T *ptr, size_t count)
+    {
+        reset();
+        ptr_ = ptr;
+        count_ = count;
+        owned_ = false;
     }
     void reset()
     {
-        if (ptr_) (void)hipFree(ptr_);
+        if (ptr_ && owned_) (void)hipFree(ptr_);
         ptr_ = nullptr;
         count_ = 0;
     }
@@ -77,6 +86,7 @@ public:
 private:
     T *ptr_ = nullptr;
     size_t count_ = 0;
+    bool owned_ = true;
 };
 
 unsigned grid_for(size_t n, unsigned block) { return unsigned((n + block - 1) / block); }
@@ -160,6 +170,10 @@ struct DeviceState {
     dev::TrialStep trial_step{};     // how the next trial's first kernel sets alpha
     // state and solver vectors
     Buffer<double> q, q_old, vel, q_base, snap_q, snap_v, grad, dir, precond, Dblk, Ublk, Cp, dp, dv;
+    // D, U and the gradient in one allocation (D | U | g), which the fused
+    // block solve reads through one buffer resource; Dblk, Ublk and grad
+    // are views of it
+    Buffer<double> solve_in;
     // root targets (3 per node)
     Buffer<double> root_cur, rot_cur, root_pend, rot_pend, root_anim, rot_anim;
     Buffer<double> ring_q, ring_v;  // rollback ring: 6N doubles per slot each
@@ -361,6 +375,7 @@ struct DeviceState {
 
     uint32_t tiled_blocks() const { return grid_for(size_t(N) + 1, kNodesPerBlock); }
     uint32_t solve_blocks() const { return grid_for(S, dev::kSolveGroups); }
+    uint32_t twisted_blocks() const { return grid_for(S, dev::kTwistGroups); }
     uint32_t node_blocks() const { return grid_for(N, kBlock); }
 };
 
@@ -703,8 +718,12 @@ KhsResult Solver::upload_model()
     // q = (position, zero rotation) from the model build, v = 0 (reference
     // cuda_backend.cu:360-371)
     const std::vector<double> zeros(6 * N, 0.0);
-    for (Buffer<double> *b : {&d.q_old, &d.vel, &d.q_base, &d.snap_q, &d.snap_v, &d.grad, &d.dir, &d.precond, &d.dp})
+    for (Buffer<double> *b : {&d.q_old, &d.vel, &d.q_base, &d.snap_q, &d.snap_v, &d.dir, &d.precond, &d.dp})
         b->allocate(6 * N);
+    d.solve_in.allocate(78 * N);
+    d.Dblk.view(d.solve_in.get(), 36 * N);
+    d.Ublk.view(d.solve_in.get() + 36 * N, 36 * N);
+    d.grad.view(d.solve_in.get() + 72 * N, 6 * N);
     HIP_TRY(hipMemcpyAsync(d.q_old.get(), d.q.get(), d.q.bytes(), hipMemcpyDeviceToDevice, d.stream));
     HIP_TRY(hipMemcpyAsync(d.snap_q.get(), d.q.get(), d.q.bytes(), hipMemcpyDeviceToDevice, d.stream));
     d.vel.upload(zeros.data(), zeros.size());
@@ -712,8 +731,6 @@ KhsResult Solver::upload_model()
     d.dir.upload(zeros.data(), zeros.size());
     d.grad.upload(zeros.data(), zeros.size());
     d.dv.allocate(3 * N);
-    d.Dblk.allocate(36 * N);
-    d.Ublk.allocate(36 * N);
     d.Cp.allocate(36 * N);
     // The root targets of the build's own points must have a frame
     // (reference solver.cpp:463-467); they are computed into the pending
@@ -732,7 +749,7 @@ KhsResult Solver::upload_model()
     d.rot_cur.upload(zero3.data(), zero3.size());
     d.rot_pend.upload(zero3.data(), zero3.size());
     d.out3.allocate(3 * std::max<size_t>(N, d.P));
-    const uint32_t max_blocks = std::max({d.tiled_blocks(), d.node_blocks(), d.solve_blocks()}) + 1;
+    const uint32_t max_blocks = std::max({d.tiled_blocks(), d.node_blocks(), d.solve_blocks(), d.twisted_blocks()}) + 1;
     d.partials.allocate(max_blocks + 16);
     d.list_words.allocate(kPairBlocks);
     d.list_alpha.allocate(kPairBlocks);
@@ -1267,6 +1284,38 @@ void Solver::set_trial_step(int source, double value)
     HIP_TRY(hipGetLastError());
 }
 
+// One block-tridiagonal solve on the stream: block Thomas (FP64, or FP32
+// in precision mode 1) or, in FP64, the twisted factorization (solve
+// algorithm 1) or the fused elimination (2; block Thomas where D, U and g
+// together reach kMaxBufferBytes, about 6.8M nodes). Returns the grid
+// size, the count of Partials it leaves.
+uint32_t Solver::launch_block_solve(unsigned *failed, unsigned *next_failed, const unsigned *stop)
+{
+    DeviceState &d = *dev_;
+    const RodView rv = d.rods();
+    uint32_t blocks = d.solve_blocks();
+    if (precision_mode_ == 1) {
+        hipLaunchKernelGGL(dev::k_block_solve<float>, dim3(blocks), dim3(kWave), kSolveLdsReserve, d.stream, rv,
+                           d.solve_order.get(), d.Dblk.get(), d.Ublk.get(), d.grad.get(), d.Cp.get(), d.dp.get(),
+                           d.dir.get(), failed, next_failed, d.partials.get(), stop);
+    } else if (solve_algorithm_ == 1) {
+        blocks = d.twisted_blocks();
+        hipLaunchKernelGGL(dev::k_block_solve_twisted, dim3(blocks), dim3(kWave), kSolveLdsReserve, d.stream, rv,
+                           d.solve_order.get(), d.Dblk.get(), d.Ublk.get(), d.grad.get(), d.Cp.get(), d.dp.get(),
+                           d.dir.get(), failed, next_failed, d.partials.get(), stop);
+    } else if (solve_algorithm_ == 2 && d.solve_in.bytes() < dev::kMaxBufferBytes) {
+        hipLaunchKernelGGL(dev::k_block_solve_fused, dim3(blocks), dim3(kWave), kSolveLdsReserve, d.stream, rv,
+                           d.solve_order.get(), (const double *)d.solve_in.get(), d.Cp.get(), d.dp.get(), d.dir.get(),
+                           failed, next_failed, d.partials.get(), stop);
+    } else {
+        hipLaunchKernelGGL(dev::k_block_solve<double>, dim3(blocks), dim3(kWave), kSolveLdsReserve, d.stream, rv,
+                           d.solve_order.get(), d.Dblk.get(), d.Ublk.get(), d.grad.get(), d.Cp.get(), d.dp.get(),
+                           d.dir.get(), failed, next_failed, d.partials.get(), stop);
+    }
+    HIP_TRY(hipGetLastError());
+    return blocks;
+}
+
 // Block direction limited to the trust radius, with the choice whether it
 // is usable (solved and descending) made on the device (reference
 // cuda_backend.cu:623-648). The choice reaches the host with the next
@@ -1278,14 +1327,11 @@ void Solver::make_direction_async()
     const RodView rv = d.rods();
     unsigned *failed = d.solve_failed.get() + d.failed_slot;
     hipEvent_t t0 = d.tic();
-    hipLaunchKernelGGL(precision_mode_ ? dev::k_block_solve<float> : dev::k_block_solve<double>,
-                       dim3(d.solve_blocks()), dim3(kWave), kSolveLdsReserve, d.stream, rv, d.solve_order.get(),
-                       d.Dblk.get(), d.Ublk.get(), d.grad.get(), d.Cp.get(), d.dp.get(), d.dir.get(), failed,
-                       d.solve_failed.get() + (d.failed_slot ^ 1u), d.partials.get(), d.skip());
+    const uint32_t blocks = launch_block_solve(failed, d.solve_failed.get() + (d.failed_slot ^ 1u), d.skip());
     d.failed_slot ^= 1u;
     d.toc(t0, d.totals.solve_ms, d.totals.solve_launches);
     // the solve leaves the direction's statistics in one partial per block
-    d.reduce_async(d.solve_blocks(), DeviceState::kDirSlot);
+    d.reduce_async(blocks, DeviceState::kDirSlot);
     // the choice, and the line search's base state q_base = q
     const size_t dofs = 6 * size_t(d.N);
     hipLaunchKernelGGL(dev::k_apply_direction_choice, dim3(grid_for(dofs, kBlock)), dim3(kBlock), 0, d.stream,
@@ -2063,11 +2109,7 @@ KhsResult Solver::test_evaluate(double dt, double *gradient, double *direction, 
     } else if (direction) {
         const RodView rv = d.rods();
         hipLaunchKernelGGL(dev::k_fill_u32, dim3(1), dim3(64), 0, d.stream, d.flag.get(), size_t(1), 0u);
-        hipLaunchKernelGGL(precision_mode_ ? dev::k_block_solve<float> : dev::k_block_solve<double>,
-                           dim3(d.solve_blocks()), dim3(kWave), kSolveLdsReserve, d.stream, rv, d.solve_order.get(),
-                           d.Dblk.get(), d.Ublk.get(), d.grad.get(), d.Cp.get(), d.dp.get(), d.dir.get(),
-                           d.flag.get(), (unsigned *)nullptr, d.partials.get(), (const unsigned *)nullptr);
-        HIP_TRY(hipGetLastError());
+        launch_block_solve(d.flag.get(), nullptr, nullptr);
         unsigned failed = 0;
         HIP_TRY(hipMemcpyAsync(&failed, d.flag.get(), sizeof(unsigned), hipMemcpyDeviceToHost, d.stream));
         HIP_TRY(hipStreamSynchronize(d.stream));
@@ -2205,6 +2247,16 @@ KhsResult Solver::set_precision_mode(uint32_t mode)
         return KHS_ERROR_INVALID_ARGUMENT;
     }
     precision_mode_ = mode;
+    return KHS_OK;
+}
+
+KhsResult Solver::set_solve_algorithm(uint32_t algorithm)
+{
+    if (algorithm > 2) {
+        error_ = "Solve algorithm must be 0 (block Thomas), 1 (twisted) or 2 (fused).";
+        return KHS_ERROR_INVALID_ARGUMENT;
+    }
+    solve_algorithm_ = algorithm;
     return KHS_OK;
 }
 

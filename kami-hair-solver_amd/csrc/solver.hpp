@@ -68,6 +68,7 @@ public:
     KhsResult copy_blocks(double *diagonal, double *upper, uint32_t capacity) const;
     KhsResult set_convergence_mode(uint32_t mode);
     KhsResult set_precision_mode(uint32_t mode);
+    KhsResult set_solve_algorithm(uint32_t algorithm);
     KhsResult set_kernel_timing(uint32_t enable);
     KhsResult kernel_timing(KhsxKernelTiming *out) const;
     KhsResult substep_times(double *ms, uint32_t capacity, uint32_t *count) const;
@@ -111,6 +112,7 @@ private:
     static Reduced unpack(const Partial &r);
     Reduced evaluate(bool gradient, double dt, bool line_search);
     void make_direction_async();
+    uint32_t launch_block_solve(unsigned *failed, unsigned *next_failed, const unsigned *stop);
     // The direction of the next Newton iteration by the direction rule:
     // the block solve (mode 0) or forced steepest descent (mode 1).
     void direction_async();
@@ -145,6 +147,7 @@ private:
     std::unique_ptr<DeviceState> dev_;
     uint32_t convergence_mode_ = 0;
     uint32_t precision_mode_ = 0;
+    uint32_t solve_algorithm_ = 0;  // 0 block Thomas, 1 twisted (FP64 only)
     uint32_t direction_mode_ = 0;
     uint64_t block_directions_ = 0, steepest_directions_ = 0;
     std::vector<double> strand_E_, strand_r_;  // per-strand materials (empty: the material's)

@@ -1599,6 +1599,560 @@ __global__ __launch_bounds__(kWave) void k_block_solve(RodView m, const uint32_t
     block_reduce(acc, partials + blockIdx.x);
 }
 
+// The same solve from both ends (twisted block factorization; the oracle's
+// block_solve_twisted restates it operation for operation). Two strands
+// per wavefront, two 16-lane rows each: the first row runs block Thomas
+// from the first free node up to the middle node m (free count - 1) / 2
+// further on, the second from the tip down to m on the reversed chain,
+// whose coupling blocks are the transposes (its U column j is row j of the
+// original U of the node before). The first row then forms the middle
+// node's Schur complement from both sides (the second row's last factor
+// and coupling block come by lane shuffles), solves it, and both rows
+// back-substitute outward from m. Lanes, registers, the Gauss-Jordan
+// passes and the buffer accesses are those of k_block_solve; a row whose
+// side is shorter than the wavefront's longest keeps its last factor
+// (its later steps are masked). The serial chain is about half the
+// strand's free length.
+constexpr int kTwistGroups = kWave / (2 * kSolveLanes);
+
+__global__ __launch_bounds__(kWave) void k_block_solve_twisted(RodView m, const uint32_t *order, const double *Dblk,
+                                                               const double *Ublk, const double *grad, double *Cp,
+                                                               double *dp, double *dir, unsigned *failed,
+                                                               unsigned *next_failed, Partial *partials,
+                                                               const unsigned *stop)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0 && next_failed) *next_failed = 0;
+    if (stop && *stop) {
+        Partial skipped = empty_partial();
+        skipped.failed = 1;
+        block_reduce(skipped, partials + blockIdx.x);
+        return;
+    }
+    const int row = threadIdx.x / kSolveLanes, j = threadIdx.x % kSolveLanes;
+    const int side = row & 1;  // 0: from the root, 1: from the tip
+    const bool acol = j < 6, icol = j >= 6 && j < 12, vlane = j == 12;
+    const uint32_t gi = blockIdx.x * kTwistGroups + (row >> 1);
+    uint32_t begin = 0, end = 0;
+    if (gi < m.strand_count) {
+        const uint32_t s = order[gi];
+        const uint32_t first = m.strand_first[s];
+        begin = first + m.strand_fixed[s];
+        end = first + m.strand_count_of[s];
+        if (acol && side == 0)
+            for (uint32_t n = first; n < begin; ++n) dir[6 * size_t(n) + j] = 0.0;
+        if (begin > end) begin = end;
+    }
+    const uint32_t len = end - begin;
+    const bool has_free = len > 0;
+    const uint32_t na = has_free ? (len - 1) / 2 : 0, nb = has_free ? len - 1 - na : 0;
+    const uint32_t mid = begin + na;
+    const uint32_t lside = side ? nb : na;
+    uint32_t steps = lside;
+#pragma unroll
+    for (int o = kWave / 2; o >= 1; o >>= 1) steps = max(steps, uint32_t(__shfl_xor(int(steps), o, kWave)));
+    steps = __builtin_amdgcn_readfirstlane(steps);
+
+    const uint64_t block_bytes = 288 * uint64_t(m.node_count), vec_bytes = 48 * uint64_t(m.node_count);
+    const __amdgpu_buffer_rsrc_t rD = buffer_of(Dblk, block_bytes), rU = buffer_of(Ublk, block_bytes);
+    const __amdgpu_buffer_rsrc_t rG = buffer_of(grad, vec_bytes), rC = buffer_of(Cp, block_bytes);
+    const __amdgpu_buffer_rsrc_t rP = buffer_of(dp, vec_bytes), rX = buffer_of(dir, vec_bytes);
+    // the root side's U column j (stride 48 between rows) or, on the tip
+    // side, row j of the node before's U (stride 8 between columns)
+    const uint32_t ustride = side ? 8 : 48;
+    auto node_at = [&](uint32_t t) { return side ? end - 1 - t : begin + t; };
+    auto fetch = [&](ForwardSlot<double> &f, uint32_t t) __attribute__((always_inline)) {
+        const uint32_t n = node_at(t);
+        const bool live = t < lside;
+        const uint32_t od = (acol && live) ? 288 * n + 8 * j : kNoAccess;
+        const uint32_t og = (vlane && live) ? 48 * n : kNoAccess;
+        const uint32_t ou = (acol && live) ? (side ? 288 * (n - 1) + 48 * j : 288 * n + 8 * j) : kNoAccess;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            f.d[r] = buffer_load(rD, od + 48 * r);
+            f.g[r] = buffer_load(rG, og + 8 * r);
+            f.u[r] = buffer_load(rU, ou + ustride * r);
+        }
+    };
+
+    bool ok = true;
+    double p[6], wprev[6];  // last factor column (d' on the rhs lane) and coupling column
+#pragma unroll
+    for (int r = 0; r < 6; ++r) p[r] = wprev[r] = 0.0;
+
+    // col[R] -= sum_k W[k][R] P[k], W[k][R] from lane R
+    auto schur = [&](double col[6], const double wv[6], const double pv[6]) __attribute__((always_inline)) {
+        static_for<6>([&](auto R) {
+            double sum = 0.0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) sum = fma(row_bcast<R>(wv[k]), pv[k], sum);
+            col[R] -= sum;
+        });
+    };
+    // Gauss-Jordan inverse on the columns of [a | inv]; returns singular
+    auto invert = [&](double col[6], bool live) __attribute__((always_inline)) {
+        double scale = 0.0;
+        static_for<6>([&](auto I) { scale = fmax(scale, fabs(row_bcast<I>(col[I]))); });
+        const double shift = fmax(1.0e-12 * scale, 1.0e-24);
+#pragma unroll
+        for (int r = 0; r < 6; ++r)
+            if (acol && r == j) col[r] += shift;
+        double start[6];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) start[r] = col[r];
+        bool exchange = false, singular = false;
+        static_for<6>([&](auto C) {
+            double cv[6];
+#pragma unroll
+            for (int i = 0; i < 6; ++i) cv[i] = row_bcast<C>(col[i]);
+            const double diag = fabs(cv[C]);
+#pragma unroll
+            for (int i = C + 1; i < 6; ++i) exchange |= live && fabs(cv[i]) > diag;
+            singular |= diag < 1.0e-30 || !isfinite(diag);
+            const double invp = 1.0 / cv[C];
+            col[C] *= invp;
+#pragma unroll
+            for (int r = 0; r < 6; ++r)
+                if (r != C) col[r] = fma(-cv[r], col[C], col[r]);
+        });
+        if (__builtin_amdgcn_ballot_w64(exchange)) {
+            singular = false;
+#pragma unroll
+            for (int r = 0; r < 6; ++r) col[r] = start[r];
+            static_for<6>([&](auto C) {
+                double cv[6];
+#pragma unroll
+                for (int i = 0; i < 6; ++i) cv[i] = row_bcast<C>(col[i]);
+                int pivot = C;
+                double best = fabs(cv[C]);
+#pragma unroll
+                for (int i = C + 1; i < 6; ++i) {
+                    const double v = fabs(cv[i]);
+                    if (v > best) { best = v; pivot = i; }
+                }
+                if (best < 1.0e-30 || !isfinite(best)) {
+                    singular = true;
+                    pivot = C;
+                }
+#pragma unroll
+                for (int i = C + 1; i < 6; ++i) {
+                    const bool sw = pivot == i;
+                    const double x = col[C], y = col[i];
+                    col[C] = sw ? y : x;
+                    col[i] = sw ? x : y;
+                    const double cx = cv[C], cy = cv[i];
+                    cv[C] = sw ? cy : cx;
+                    cv[i] = sw ? cx : cy;
+                }
+                const double invp = 1.0 / cv[C];
+                col[C] *= invp;
+#pragma unroll
+                for (int r = 0; r < 6; ++r)
+                    if (r != C) col[r] = fma(-cv[r], col[C], col[r]);
+            });
+        }
+        return singular;
+    };
+    // res[r] = sum_K inv[r][K] w[K], inverse column K on lane 6 + K
+    auto apply_inverse = [&](const double col[6], const double w[6], double res[6]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int r = 0; r < 6; ++r) res[r] = 0.0;
+        static_for<6>([&](auto K) {
+#pragma unroll
+            for (int r = 0; r < 6; ++r) res[r] = fma(row_bcast<6 + K>(col[r]), w[K], res[r]);
+        });
+    };
+
+    ForwardSlot<double> ring[kSolveAhead];
+#pragma unroll
+    for (int a = 0; a < kSolveAhead; ++a) fetch(ring[a], a);
+    auto forward = [&](ForwardSlot<double> &slot, uint32_t t) __attribute__((always_inline)) {
+        const uint32_t n = node_at(t);
+        const bool live = t < lside;
+        double col[6], w[6];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            col[r] = acol ? slot.d[r] : (vlane ? -slot.g[r] : ((icol && r == j - 6) ? 1.0 : 0.0));
+            w[r] = slot.u[r];
+        }
+        schur(col, wprev, p);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) wprev[k] = live ? w[k] : wprev[k];
+        double rhs[6];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) rhs[r] = col[r];
+        const bool singular = invert(col, live);
+        if (singular && live) ok = false;
+#pragma unroll
+        for (int r = 0; r < 6; ++r)
+            if (vlane) w[r] = rhs[r];
+        double res[6];
+        apply_inverse(col, w, res);
+        // every side node has a follower (the middle node at last)
+#pragma unroll
+        for (int r = 0; r < 6; ++r) p[r] = live ? ((acol || vlane) ? res[r] : 0.0) : p[r];
+        const bool keep = live && ok;
+        const uint32_t oc = (acol && keep) ? 288 * n + 8 * j : kNoAccess;
+        const uint32_t op = (vlane && keep) ? 48 * n : kNoAccess;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            buffer_store(rC, oc + 48 * r, p[r]);
+            buffer_store(rP, op + 8 * r, p[r]);
+        }
+        fetch(slot, t + kSolveAhead);
+    };
+    for (uint32_t t0 = 0; t0 < steps; t0 += kSolveAhead)
+        static_for<kSolveAhead>([&](auto A) { forward(ring[A], t0 + A); });
+
+    // The middle node, on the root-side row: its Schur complement takes the
+    // root side's term, then the tip side's (U_m times the tip side's last
+    // factor: the tip row's coupling column j is row j of U_m).
+    const int base = threadIdx.x & ~(2 * kSolveLanes - 1);  // the strand's first lane
+    double pt[6], wt[6];
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+        pt[r] = __shfl(p[r], base + kSolveLanes + j, kWave);
+        wt[r] = __shfl(wprev[r], base + kSolveLanes + j, kWave);
+    }
+    const bool mlive = side == 0 && has_free;
+    double xm[6];
+    {
+        const uint32_t od = (acol && mlive) ? 288 * mid + 8 * j : kNoAccess;
+        const uint32_t og = (vlane && mlive) ? 48 * mid : kNoAccess;
+        double col[6];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            const double dv = buffer_load(rD, od + 48 * r), gv = buffer_load(rG, og + 8 * r);
+            col[r] = acol ? dv : (vlane ? -gv : ((icol && r == j - 6) ? 1.0 : 0.0));
+        }
+        schur(col, wprev, p);
+        schur(col, wt, pt);
+        double rhs[6];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) rhs[r] = col[r];
+        const bool singular = invert(col, mlive);
+        if (singular && mlive) ok = false;
+        apply_inverse(col, rhs, xm);  // x_m on the rhs lane
+    }
+    // both rows of the strand learn the middle solution and the strand's
+    // verdict
+    const bool strand_ok = __shfl(int(ok), base, kWave) && __shfl(int(ok), base + kSolveLanes, kWave);
+    const int r = acol ? j : 0;
+    double next = 0.0;
+    static_for<6>([&](auto Q) {
+        const double v = __shfl(xm[Q], base + 12, kWave);
+        if (r == Q) next = v;
+    });
+    double slope = 0.0, reach2 = 0.0;
+    if (mlive && acol) {
+        buffer_store(rX, strand_ok ? 48 * mid + 8 * j : kNoAccess, next);
+        slope += buffer_load(rG, 48 * mid + 8 * r) * next;
+    }
+    {
+        const double sq = next * next;
+        const double t2 = (sq + row_bcast<1>(sq)) + row_bcast<2>(sq);
+        if (mlive) reach2 = fmax(reach2, t2);
+    }
+
+    // Back substitution outward from m: x_n = d'_n - C'_n x_(n -+ 1), lane
+    // r < 6 owning row r.
+    struct BackSlot {
+        double c[6], d, g;
+    };
+    auto back_node = [&](uint32_t t) { return side ? mid + 1 + t : mid - 1 - t; };
+    auto fetch_back = [&](BackSlot &b, uint32_t t) __attribute__((always_inline)) {
+        const uint32_t n = back_node(t);
+        const bool live = acol && t < lside;
+        const uint32_t oc = live ? 288 * n + 48 * r : kNoAccess;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) b.c[k] = buffer_load(rC, oc + 8 * k);
+        b.d = buffer_load(rP, live ? 48 * n + 8 * r : kNoAccess);
+        b.g = buffer_load(rG, live ? 48 * n + 8 * r : kNoAccess);
+    };
+    BackSlot back[kBackAhead];
+#pragma unroll
+    for (int a = 0; a < kBackAhead; ++a) fetch_back(back[a], a);
+    auto backward = [&](BackSlot &b, uint32_t t) __attribute__((always_inline)) {
+        const uint32_t n = back_node(t);
+        double crow[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) crow[k] = b.c[k];
+        double value = b.d;
+        static_for<6>([&](auto K) { value = fma(-crow[K], row_bcast<K>(next), value); });
+        const bool live = t < lside;
+        buffer_store(rX, (live && strand_ok && acol) ? 48 * n + 8 * j : kNoAccess, value);
+        next = live ? value : next;
+        const double sq = value * value;
+        const double t2 = (sq + row_bcast<1>(sq)) + row_bcast<2>(sq);
+        if (live) {
+            slope += b.g * value;
+            reach2 = fmax(reach2, t2);
+        }
+        fetch_back(b, t + kBackAhead);
+    };
+    for (uint32_t t0 = 0; t0 < steps; t0 += kBackAhead)
+        static_for<kBackAhead>([&](auto A) { backward(back[A], t0 + A); });
+    if (!strand_ok && j == 0 && side == 0 && has_free) atomicOr(failed, 1u);
+    double row_slope = 0.0;
+    static_for<6>([&](auto K) { row_slope += row_bcast<K>(acol ? slope : 0.0); });
+    Partial acc = empty_partial();
+    if (j == 0 && gi < m.strand_count) {
+        acc.sum[kSlope] = row_slope;
+        acc.max_translation = sqrt(reach2);
+        acc.failed = strand_ok ? 0u : 1u;
+    }
+    block_reduce(acc, partials + blockIdx.x);
+}
+
+// Block Thomas with the factors taken straight out of one elimination
+// (solve algorithm 2; the oracle's block_solve_fused restates it). Per free
+// node n the 16-lane row holds the columns of [S | U_n | e]: S = D_n -
+// U_{n-1}^T C'_{n-1}, e = g_n - U_{n-1}^T e_{n-1} with e = -d' (the rhs
+// carried with the opposite sign, which rounds identically); Gauss-Jordan
+// without pivoting reduces S to I and leaves C'_n = S^-1 U_n and e_n in the
+// other columns, with no inverse formed and no product by it. The blocks
+// are symmetric positive definite (Gauss-Newton Hessian, mass, the
+// reference's diagonal shift), so their Schur complements need no row
+// exchanges; a pivot below 1e-30 or a non-finite result fails the solve as
+// a singular pivot fails the reference's (cuda_backend.cu:855-881).
+//
+// The S and U column halves alternate between lanes 0..5 and 6..11 from
+// node to node: the U lanes of node n, whose columns become C'_n, are the
+// S lanes of node n + 1, so the Schur complement reads its C' column in
+// place and U_n (kept in the lane's ring slot) by row broadcast. Lane 12
+// holds e; every lane reads its one column per node (6 loads of D, U or g
+// out of one buffer holding all three). Each Schur and row-operation term
+// is one v_fmac_f64_dpp: the product's first factor comes from the
+// broadcasting lane inside the fused multiply-add (the compiler forms
+// these only as a separate DPP move and FMA). Per node that is 36 DPP-FMAs
+// for the Schur complement and 6 x (1 reciprocal, 5 DPP-FMAs) for the
+// elimination, against the reference form's inverse, its products and
+// its moves: about a third of the instructions.
+constexpr int kFusedAhead = 4;  // ring slots; even, as the lane halves alternate
+
+// v_fmac_f64_dpp ACC, X, -Y: ACC += X(lane L of the row) * -Y
+#define KHS_FMAC_NEG(ACC, X, Y, L) "v_fmac_f64_dpp " ACC ", " X ", -" Y " row_newbcast:" L " row_mask:0xf bank_mask:0xf\n\t"
+// col[R] += U[k][R] * -p[k] for R = 0..5; U[k][R] is w[k] of lane B + R
+#define KHS_SCHUR_K(W, P, L0, L1, L2, L3, L4, L5)                                                                  \
+    KHS_FMAC_NEG("%0", W, P, L0) KHS_FMAC_NEG("%1", W, P, L1) KHS_FMAC_NEG("%2", W, P, L2)                         \
+        KHS_FMAC_NEG("%3", W, P, L3) KHS_FMAC_NEG("%4", W, P, L4) KHS_FMAC_NEG("%5", W, P, L5)
+// k outermost: consecutive DPP-FMAs never read what the one before wrote;
+// the leading s_nop covers a VALU write of an input just before the block
+#define KHS_SCHUR(L0, L1, L2, L3, L4, L5)                                                                          \
+    "s_nop 1\n\t" KHS_SCHUR_K("%6", "%12", L0, L1, L2, L3, L4, L5) KHS_SCHUR_K("%7", "%13", L0, L1, L2, L3, L4, L5) \
+        KHS_SCHUR_K("%8", "%14", L0, L1, L2, L3, L4, L5) KHS_SCHUR_K("%9", "%15", L0, L1, L2, L3, L4, L5)           \
+            KHS_SCHUR_K("%10", "%16", L0, L1, L2, L3, L4, L5) KHS_SCHUR_K("%11", "%17", L0, L1, L2, L3, L4, L5)
+// pivot row C scaled by %6 (1 / pivot), then row r -= S[r][C] * row C for
+// the other rows, S[r][C] from the pivot column's lane L
+#define KHS_GJ(C, L, A1, A2, A3, A4, A5)                                                                           \
+    "v_mul_f64 %" #C ", %" #C ", %6\n\ts_nop 1\n\t" KHS_FMAC_NEG("%" #A1, "%" #A1, "%" #C, #L)                    \
+        KHS_FMAC_NEG("%" #A2, "%" #A2, "%" #C, #L) KHS_FMAC_NEG("%" #A3, "%" #A3, "%" #C, #L)                      \
+            KHS_FMAC_NEG("%" #A4, "%" #A4, "%" #C, #L) KHS_FMAC_NEG("%" #A5, "%" #A5, "%" #C, #L)
+#define KHS_GJ_ASM(C, L, A1, A2, A3, A4, A5)                                                                       \
+    asm(KHS_GJ(C, L, A1, A2, A3, A4, A5)                                                                         \
+        : "+v"(col[0]), "+v"(col[1]), "+v"(col[2]), "+v"(col[3]), "+v"(col[4]), "+v"(col[5])                    \
+        : "v"(inv))
+
+template <int B>  // lanes B..B+5 hold the S columns and the previous node's U columns
+__device__ __forceinline__ void fused_schur(double (&col)[6], const double (&w)[6], const double (&p)[6])
+{
+#define KHS_SCHUR_OPERANDS                                                                                         \
+    : "+v"(col[0]), "+v"(col[1]), "+v"(col[2]), "+v"(col[3]), "+v"(col[4]), "+v"(col[5])                        \
+    : "v"(w[0]), "v"(w[1]), "v"(w[2]), "v"(w[3]), "v"(w[4]), "v"(w[5]), "v"(p[0]), "v"(p[1]), "v"(p[2]),         \
+      "v"(p[3]), "v"(p[4]), "v"(p[5])
+    if constexpr (B == 0) asm(KHS_SCHUR("0", "1", "2", "3", "4", "5") KHS_SCHUR_OPERANDS);
+    else asm(KHS_SCHUR("6", "7", "8", "9", "10", "11") KHS_SCHUR_OPERANDS);
+#undef KHS_SCHUR_OPERANDS
+}
+
+template <int B, int C>
+__device__ __forceinline__ void fused_gj_step(double (&col)[6], double inv)
+{
+    static_assert(B == 0 || B == 6, "S columns on lanes 0..5 or 6..11");
+    if constexpr (B == 0) {
+        if constexpr (C == 0) KHS_GJ_ASM(0, 0, 1, 2, 3, 4, 5);
+        if constexpr (C == 1) KHS_GJ_ASM(1, 1, 0, 2, 3, 4, 5);
+        if constexpr (C == 2) KHS_GJ_ASM(2, 2, 0, 1, 3, 4, 5);
+        if constexpr (C == 3) KHS_GJ_ASM(3, 3, 0, 1, 2, 4, 5);
+        if constexpr (C == 4) KHS_GJ_ASM(4, 4, 0, 1, 2, 3, 5);
+        if constexpr (C == 5) KHS_GJ_ASM(5, 5, 0, 1, 2, 3, 4);
+    } else {
+        if constexpr (C == 0) KHS_GJ_ASM(0, 6, 1, 2, 3, 4, 5);
+        if constexpr (C == 1) KHS_GJ_ASM(1, 7, 0, 2, 3, 4, 5);
+        if constexpr (C == 2) KHS_GJ_ASM(2, 8, 0, 1, 3, 4, 5);
+        if constexpr (C == 3) KHS_GJ_ASM(3, 9, 0, 1, 2, 4, 5);
+        if constexpr (C == 4) KHS_GJ_ASM(4, 10, 0, 1, 2, 3, 5);
+        if constexpr (C == 5) KHS_GJ_ASM(5, 11, 0, 1, 2, 3, 4);
+    }
+}
+#undef KHS_GJ_ASM
+#undef KHS_GJ
+#undef KHS_SCHUR
+#undef KHS_SCHUR_K
+#undef KHS_FMAC_NEG
+
+// 1 / x from the hardware reciprocal and two Newton steps (pivots are
+// normal numbers here: the shift keeps them above 1e-24 unless the solve
+// fails anyway)
+__device__ __forceinline__ double reciprocal(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return fma(r, fma(-x, r, 1.0), r);
+}
+
+// blocks: D (36 N doubles), U (36 N) and g (6 N) in one allocation, in that
+// order, below kMaxBufferBytes in all.
+__global__ __launch_bounds__(kWave) void k_block_solve_fused(RodView m, const uint32_t *order, const double *blocks,
+                                                             double *Cp, double *dp, double *dir, unsigned *failed,
+                                                             unsigned *next_failed, Partial *partials,
+                                                             const unsigned *stop)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0 && next_failed) *next_failed = 0;
+    if (stop && *stop) {
+        Partial skipped = empty_partial();
+        skipped.failed = 1;
+        block_reduce(skipped, partials + blockIdx.x);
+        return;
+    }
+    const int g = threadIdx.x / kSolveLanes, j = threadIdx.x % kSolveLanes;
+    const bool acol = j < 6, rlane = j == 12;
+    const int half = j < 6 ? 0 : (j < 12 ? 1 : 2), ci = j < 12 ? j % 6 : 0;
+    const uint32_t gi = blockIdx.x * kSolveGroups + g;
+    uint32_t begin = 0, end = 0;
+    if (gi < m.strand_count) {
+        const uint32_t s = order[gi];
+        const uint32_t first = m.strand_first[s];
+        begin = first + m.strand_fixed[s];
+        end = first + m.strand_count_of[s];
+        if (acol)
+            for (uint32_t n = first; n < begin; ++n) dir[6 * size_t(n) + j] = 0.0;
+        if (begin > end) begin = end;
+    }
+    const uint32_t len = end - begin;
+    uint32_t steps = len;
+#pragma unroll
+    for (int o = kWave / 2; o >= 1; o >>= 1) steps = max(steps, uint32_t(__shfl_xor(int(steps), o, kWave)));
+    steps = __builtin_amdgcn_readfirstlane(steps);
+
+    const uint32_t N = m.node_count;
+    const uint32_t u_part = 288 * N, g_part = 576 * N;
+    const __amdgpu_buffer_rsrc_t rB = buffer_of(blocks, 624 * uint64_t(N));
+    const uint64_t block_bytes = 288 * uint64_t(N), vec_bytes = 48 * uint64_t(N);
+    const __amdgpu_buffer_rsrc_t rC = buffer_of(Cp, block_bytes), rP = buffer_of(dp, vec_bytes);
+    const __amdgpu_buffer_rsrc_t rG = buffer_of(blocks + 72 * size_t(N), vec_bytes), rX = buffer_of(dir, vec_bytes);
+
+    // this lane's column of node t in parity P (S lanes: D; U lanes: U, while
+    // a node follows; lane 12: g); row r at base + r * stride
+    const uint32_t stride = rlane ? 8 : 48, per_node = rlane ? 48 : 288;
+    const uint32_t lane_at[2] = {rlane ? g_part : 8 * ci + (half == 1 ? u_part : 0),
+                                 rlane ? g_part : 8 * ci + (half == 0 ? u_part : 0)};
+    auto fetch = [&](double (&f)[6], uint32_t t, auto P) __attribute__((always_inline)) {
+        // bitwise and by selects, so that no branch splits the wavefront
+        const uint32_t u_lane = half == 1 - int(P), now = (half == int(P)) | rlane;
+        const uint32_t want = (now & uint32_t(t < len)) | (u_lane & uint32_t(t + 1 < len));
+        const uint32_t at = __umul24(begin + t, per_node) + lane_at[P];
+        const uint32_t base = want ? at : kNoAccess;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) f[r] = buffer_load(rB, base + stride * r);
+    };
+
+    bool ok = true;
+    double col[6];
+#pragma unroll
+    for (int r = 0; r < 6; ++r) col[r] = 0.0;
+    // slot t % 4 holds node t's column; the slot of node t - 1 keeps U_{t-1}
+    // (on the lanes that are node t's S lanes) until node t's Schur
+    // complement, then takes node t + 3
+    double ring[kFusedAhead][6];
+#pragma unroll
+    for (int r = 0; r < 6; ++r) ring[kFusedAhead - 1][r] = 0.0;
+    static_for<kFusedAhead - 1>([&](auto A) { fetch(ring[A], A, std::integral_constant<int, A & 1>{}); });
+
+    auto forward = [&](uint32_t t, auto A) __attribute__((always_inline)) {
+        constexpr int P = A & 1, B = P ? 6 : 0, prev = (A + kFusedAhead - 1) % kFusedAhead;
+        const uint32_t n = begin + t;
+        const bool live = t < len;
+        // C'_{n-1} (S lanes) and e_{n-1} (lane 12) stay; the rest is zero
+        const double keep_p = (half == P || rlane) ? 1.0 : 0.0;
+        double p[6];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            p[r] = col[r] * keep_p;
+            col[r] = ring[A][r];
+        }
+        fused_schur<B>(col, ring[prev], p);
+        fetch(ring[prev], t + kFusedAhead - 1, std::integral_constant<int, (A + kFusedAhead - 1) & 1>{});
+        double scale = 0.0;
+        static_for<6>([&](auto I) { scale = fmax(scale, fabs(row_bcast<B + I>(col[I]))); });
+        const double shift = fmax(1.0e-12 * scale, 1.0e-24);
+        double smallest = 0.0;
+        static_for<6>([&](auto C) {
+            const double pivot = row_bcast<B + C>(col[C]) + shift;
+            smallest = C == 0 ? pivot : fmin(smallest, pivot);
+            fused_gj_step<B, C>(col, reciprocal(pivot));
+        });
+        const double e0 = row_bcast<12>(col[0]);
+        const bool bad = !(smallest >= 1.0e-30) || !isfinite(e0);
+        if (live && bad) ok = false;
+        const bool keep = live && ok;
+        const uint32_t oc = (keep && half == 1 - P) ? 288 * n + 8 * ci : kNoAccess;
+        const uint32_t op = (keep && rlane) ? 48 * n : kNoAccess;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            buffer_store(rC, oc + 48 * r, col[r]);
+            buffer_store(rP, op + 8 * r, col[r]);
+        }
+    };
+    for (uint32_t t0 = 0; t0 < steps; t0 += kFusedAhead)
+        static_for<kFusedAhead>([&](auto A) { forward(t0 + A, A); });
+    if (!ok && j == 0) atomicOr(failed, 1u);
+
+    // Back substitution x_n = d'_n - C'_n x_{n+1} with d' = -e, lane r < 6
+    // owning row r (as in k_block_solve).
+    const int r = acol ? j : 0;
+    struct BackSlot {
+        double c[6], e, g;
+    };
+    auto fetch_back = [&](BackSlot &b, uint32_t t) __attribute__((always_inline)) {
+        const uint32_t n = end - 1 - t;
+        const bool bl = acol && t < len;
+        const uint32_t oc = bl ? 288 * n + 48 * r : kNoAccess;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) b.c[k] = buffer_load(rC, oc + 8 * k);
+        b.e = buffer_load(rP, bl ? 48 * n + 8 * r : kNoAccess);
+        b.g = buffer_load(rG, bl ? 48 * n + 8 * r : kNoAccess);
+    };
+    BackSlot back[kBackAhead];
+#pragma unroll
+    for (int a = 0; a < kBackAhead; ++a) fetch_back(back[a], a);
+    double next = 0.0, slope = 0.0, reach2 = 0.0;
+    auto backward = [&](BackSlot &b, uint32_t t) __attribute__((always_inline)) {
+        const uint32_t n = end - 1 - t;
+        double value = -b.e;
+        if (t > 0) static_for<6>([&](auto K) { value = fma(-b.c[K], row_bcast<K>(next), value); });
+        next = value;
+        buffer_store(rX, (t < len && ok && acol) ? 48 * n + 8 * j : kNoAccess, value);
+        const double sq = value * value;
+        const double t2 = (sq + row_bcast<1>(sq)) + row_bcast<2>(sq);
+        if (t < len) {
+            slope += b.g * value;
+            reach2 = fmax(reach2, t2);
+        }
+        fetch_back(b, t + kBackAhead);
+    };
+    for (uint32_t t0 = 0; t0 < steps; t0 += kBackAhead)
+        static_for<kBackAhead>([&](auto A) { backward(back[A], t0 + A); });
+    double row_slope = 0.0;
+    static_for<6>([&](auto K) { row_slope += row_bcast<K>(acol ? slope : 0.0); });
+    Partial acc = empty_partial();
+    if (j == 0 && gi < m.strand_count) {
+        acc.sum[kSlope] = row_slope;
+        acc.max_translation = sqrt(reach2);
+        acc.failed = ok ? 0u : 1u;
+    }
+    block_reduce(acc, partials + blockIdx.x);
+}
+
 // Slope g.d and the largest translation of a free node (host then limits
 // the direction by the trust radius, cuda_backend.cu:623-630).
 __global__ __launch_bounds__(kBlock) void k_direction_stats(RodView m, const double *grad, const double *dir,

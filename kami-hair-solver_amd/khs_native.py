@@ -317,6 +317,7 @@ EXT_FUNCTIONS = {
     "khsxStepBound": (C.c_int, [_P, _P, C.c_uint64, _D(C.c_double)]),
     "khsxSweepCheck": (C.c_int, [_P, _D(C.c_uint32), _D(KhsFailureDiagnostics)]),
     "khsxSetDirectionMode": (C.c_int, [_P, C.c_uint32]),
+    "khsxSetSolveAlgorithm": (C.c_int, [_P, C.c_uint32]),
     "khsxGetDirectionCounts": (C.c_int, [_P, _D(C.c_uint64), _D(C.c_uint64)]),
 }
 
@@ -620,6 +621,9 @@ class HairSolver:
         diag.struct_size = C.sizeof(KhsFailureDiagnostics)
         self.check(self.lib.khsxSweepCheck(self.handle, C.byref(feasible), C.byref(diag)))
         return bool(feasible.value), diag
+
+    def set_solve_algorithm(self, algorithm: int) -> int:
+        return self.lib.khsxSetSolveAlgorithm(self.handle, algorithm)
 
     def set_direction_mode(self, mode: int) -> int:
         return self.lib.khsxSetDirectionMode(self.handle, mode)

@@ -227,6 +227,7 @@ public:
     uint32_t convergence_mode = 0;
     uint32_t direction_mode = 0;
     uint32_t precision_mode = 0;
+    uint32_t solve_algorithm = 0;  // 0 block Thomas from the root, 1 twisted (from both ends), 2 fused
 
     // ---------------------------------------------------------------- input
     KhsResult set_hair(const KhsVec3 *points, uint32_t point_count, const uint32_t *offsets,
@@ -1624,9 +1625,199 @@ private:
         return false;
     }
 
+    // The same block-tridiagonal system solved from both ends ("twisted"
+    // block factorization): block Thomas eliminates the free nodes from the
+    // root up to the middle node m and, on the reversed chain (coupling
+    // blocks transposed), from the tip down to m; the middle node's Schur
+    // complement takes both sides' terms (the root side's first), its
+    // solution starts both back substitutions. Mathematically the same
+    // direction as block_solve's; the serial chain is half as long. Each
+    // entry follows the sequential operations of the GPU kernel
+    // (k_block_solve_twisted): sums of products by fused multiply-adds from
+    // zero, the same inverse6.
+    bool block_solve_twisted()
+    {
+        std::vector<uint8_t> failed(strands_.size(), 0);
+        parallel_for(threads, uint32_t(strands_.size()), [&](uint32_t s) {
+            const Strand &st = strands_[s];
+            const uint32_t begin = st.first + st.fixed_count, end = st.first + st.count;
+            if (begin >= end) return;
+            const uint32_t len = end - begin, na = (len - 1) / 2, m = begin + na;
+            double b[36], inv[36], rhs[6];
+            auto load = [&](uint32_t n) {
+                for (int i = 0; i < 36; ++i) b[i] = D_[36 * n + i];
+                for (int i = 0; i < 6; ++i) rhs[i] = -g_[6 * n + i];
+            };
+            // b -= L C and rhs -= L d with L(r, j) = lower(r, j)
+            auto schur = [&](auto lower, const double *pc, const double *pd) {
+                for (int r = 0; r < 6; ++r) {
+                    for (int c = 0; c < 6; ++c) {
+                        double sum = 0;
+                        for (int j = 0; j < 6; ++j) sum = std::fma(lower(r, j), pc[6 * j + c], sum);
+                        b[6 * r + c] -= sum;
+                    }
+                    double sum = 0;
+                    for (int j = 0; j < 6; ++j) sum = std::fma(lower(r, j), pd[j], sum);
+                    rhs[r] -= sum;
+                }
+            };
+            // C = inv W with W(j, c) = w(j, c); d = inv rhs
+            auto factor = [&](uint32_t n, auto w) {
+                double *cc = &C_[36 * n], *cd = &dp_[6 * n];
+                for (int r = 0; r < 6; ++r)
+                    for (int c = 0; c < 6; ++c) {
+                        double sum = 0;
+                        for (int j = 0; j < 6; ++j) sum = std::fma(inv[6 * r + j], w(j, c), sum);
+                        cc[6 * r + c] = sum;
+                    }
+                for (int r = 0; r < 6; ++r) {
+                    double sum = 0;
+                    for (int j = 0; j < 6; ++j) sum = std::fma(inv[6 * r + j], rhs[j], sum);
+                    cd[r] = sum;
+                }
+            };
+            // root side: nodes begin .. m - 1, coupling U_{n-1}^T C'_{n-1}
+            for (uint32_t n = begin; n < m; ++n) {
+                load(n);
+                if (n > begin) {
+                    const double *U = &U_[36 * (n - 1)];
+                    schur([&](int r, int j) { return U[6 * j + r]; }, &C_[36 * (n - 1)], &dp_[6 * (n - 1)]);
+                }
+                if (!inverse6(b, inv)) { failed[s] = 1; return; }
+                const double *Un = &U_[36 * n];
+                factor(n, [&](int j, int c) { return Un[6 * j + c]; });
+            }
+            // tip side: nodes end - 1 .. m + 1, coupling U_n C''_{n+1}; its
+            // factor is inv U_{n-1}^T
+            for (uint32_t n = end; n-- > m + 1;) {
+                load(n);
+                if (n + 1 < end) {
+                    const double *U = &U_[36 * n];
+                    schur([&](int r, int j) { return U[6 * r + j]; }, &C_[36 * (n + 1)], &dp_[6 * (n + 1)]);
+                }
+                if (!inverse6(b, inv)) { failed[s] = 1; return; }
+                const double *Up = &U_[36 * (n - 1)];
+                factor(n, [&](int j, int c) { return Up[6 * c + j]; });
+            }
+            // middle node m: both sides' Schur terms, the root side's first
+            load(m);
+            if (m > begin) {
+                const double *U = &U_[36 * (m - 1)];
+                schur([&](int r, int j) { return U[6 * j + r]; }, &C_[36 * (m - 1)], &dp_[6 * (m - 1)]);
+            }
+            if (m + 1 < end) {
+                const double *U = &U_[36 * m];
+                schur([&](int r, int j) { return U[6 * r + j]; }, &C_[36 * (m + 1)], &dp_[6 * (m + 1)]);
+            }
+            if (!inverse6(b, inv)) { failed[s] = 1; return; }
+            for (int r = 0; r < 6; ++r) {
+                double sum = 0;
+                for (int j = 0; j < 6; ++j) sum = std::fma(inv[6 * r + j], rhs[j], sum);
+                d_[6 * m + r] = sum;
+            }
+            // back substitution outward from m
+            for (uint32_t n = m; n-- > begin;)
+                for (int r = 0; r < 6; ++r) {
+                    double value = dp_[6 * n + r];
+                    for (int k = 0; k < 6; ++k) value = std::fma(-C_[36 * n + 6 * r + k], d_[6 * (n + 1) + k], value);
+                    d_[6 * n + r] = value;
+                }
+            for (uint32_t n = m + 1; n < end; ++n)
+                for (int r = 0; r < 6; ++r) {
+                    double value = dp_[6 * n + r];
+                    for (int k = 0; k < 6; ++k) value = std::fma(-C_[36 * n + 6 * r + k], d_[6 * (n - 1) + k], value);
+                    d_[6 * n + r] = value;
+                }
+        });
+        for (uint8_t f : failed) if (f) return true;
+        return false;
+    }
+
+    // Block Thomas whose factors come straight out of one elimination
+    // (k_block_solve_fused): per node, S = D_n - U_{n-1}^T C'_{n-1} and
+    // e = g_n - U_{n-1}^T e_{n-1} (e = -d'), then Gauss-Jordan without
+    // pivoting on the columns of [S | U_n | e] turns them into
+    // [I | C'_n | e_n]; no inverse is formed. The diagonal shift is the
+    // reference's, max(1e-12 max|S_ii|, 1e-24), added to each pivot as it is
+    // taken. The blocks are symmetric positive definite (Gauss-Newton
+    // Hessian, mass and shift), whose Schur complements need no pivoting; a
+    // pivot below 1e-30 (or negative) or a non-finite result fails the solve
+    // as a singular pivot fails the reference's. The device forms 1 / pivot
+    // from the hardware reciprocal and two Newton steps (within an ulp of
+    // the division here); everything else is the device's operations in its
+    // order: Schur terms accumulated into S by fused multiply-adds over
+    // k = 0..5, row operations S[r][c] = fma(S[r][C], -S[C][c], S[r][c]).
+    bool block_solve_fused()
+    {
+        std::vector<uint8_t> failed(strands_.size(), 0);
+        parallel_for(threads, uint32_t(strands_.size()), [&](uint32_t s) {
+            const Strand &st = strands_[s];
+            const uint32_t begin = st.first + st.fixed_count, end = st.first + st.count;
+            if (begin >= end) return;
+            double e_prev[6] = {0, 0, 0, 0, 0, 0};
+            for (uint32_t n = begin; n < end; ++n) {
+                double S[36], W[36], e[6];
+                for (int i = 0; i < 36; ++i) {
+                    S[i] = D_[36 * n + i];
+                    W[i] = n + 1 < end ? U_[36 * n + i] : 0.0;
+                }
+                for (int i = 0; i < 6; ++i) e[i] = g_[6 * n + i];
+                if (n > begin) {
+                    const double *U = &U_[36 * (n - 1)], *Cprev = &C_[36 * (n - 1)];
+                    for (int k = 0; k < 6; ++k)
+                        for (int R = 0; R < 6; ++R) {
+                            const double u = U[6 * k + R];
+                            for (int c = 0; c < 6; ++c) S[6 * R + c] = std::fma(u, -Cprev[6 * k + c], S[6 * R + c]);
+                            e[R] = std::fma(u, -e_prev[k], e[R]);
+                        }
+                }
+                double scale = 0.0;
+                for (int i = 0; i < 6; ++i) scale = std::fmax(scale, std::fabs(S[6 * i + i]));
+                const double shift = std::fmax(1.0e-12 * scale, 1.0e-24);
+                double smallest = 0.0;
+                for (int C = 0; C < 6; ++C) {
+                    const double pivot = S[6 * C + C] + shift;
+                    smallest = C == 0 ? pivot : std::fmin(smallest, pivot);
+                    const double inv = 1.0 / pivot;
+                    for (int c = 0; c < 6; ++c) {
+                        S[6 * C + c] *= inv;
+                        W[6 * C + c] *= inv;
+                    }
+                    e[C] *= inv;
+                    for (int r = 0; r < 6; ++r) {
+                        if (r == C) continue;
+                        const double f = S[6 * r + C];
+                        for (int c = 0; c < 6; ++c) {
+                            S[6 * r + c] = std::fma(f, -S[6 * C + c], S[6 * r + c]);
+                            W[6 * r + c] = std::fma(f, -W[6 * C + c], W[6 * r + c]);
+                        }
+                        e[r] = std::fma(f, -e[C], e[r]);
+                    }
+                }
+                if (!(smallest >= 1.0e-30) || !std::isfinite(e[0])) { failed[s] = 1; return; }
+                for (int i = 0; i < 36; ++i) C_[36 * n + i] = W[i];
+                for (int i = 0; i < 6; ++i) {
+                    dp_[6 * n + i] = -e[i];
+                    e_prev[i] = e[i];
+                }
+            }
+            for (uint32_t n = end; n-- > begin;)
+                for (int r = 0; r < 6; ++r) {
+                    double value = dp_[6 * n + r];
+                    if (n + 1 < end)
+                        for (int k = 0; k < 6; ++k) value = std::fma(-C_[36 * n + 6 * r + k], d_[6 * (n + 1) + k], value);
+                    d_[6 * n + r] = value;
+                }
+        });
+        for (uint8_t f : failed) if (f) return true;
+        return false;
+    }
+
     bool block_solve()
     {
         if (precision_mode == 1) return block_solve_fp32();
+        if (solve_algorithm == 1) return block_solve_twisted();
+        if (solve_algorithm == 2) return block_solve_fused();
         std::vector<uint8_t> failed(strands_.size(), 0);
         parallel_for(threads, uint32_t(strands_.size()), [&](uint32_t s) {
             const Strand &st = strands_[s];
@@ -2617,6 +2808,13 @@ KhsResult khsxSweepCheck(KhsSolver *s, uint32_t *feasible, KhsFailureDiagnostics
 KhsResult khsxSetStrandMaterials(KhsSolver *s, const double *young_modulus, const double *radius, uint32_t count)
 {
     return guarded(s, [&] { return s->impl.set_strand_materials(young_modulus, radius, count); });
+}
+
+KhsResult khsxSetSolveAlgorithm(KhsSolver *s, uint32_t algorithm)
+{
+    if (!s || algorithm > 2) return KHS_ERROR_INVALID_ARGUMENT;
+    s->impl.solve_algorithm = algorithm;
+    return KHS_OK;
 }
 
 KhsResult khsxSetDirectionMode(KhsSolver *s, uint32_t mode)
