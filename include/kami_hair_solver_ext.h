@@ -263,6 +263,13 @@ KHS_API KhsResult khsxSetSolveAlgorithm(KhsSolver *solver, uint32_t algorithm);
 KHS_API KhsResult khsxGetDirectionCounts(const KhsSolver *solver, uint64_t *block_directions,
                                          uint64_t *steepest_directions);
 
+/* Work enqueued on the solver's stream since it was built: kernel launches
+   and device-to-device copies (each a GPU dispatch), and host waits for the
+   stream (read-backs of reductions and synchronizations). Callers take
+   differences around the frames they measure. The CPU oracle reports zero
+   for both. */
+KHS_API KhsResult khsxGetLaunchCounts(const KhsSolver *solver, uint64_t *dispatches, uint64_t *host_waits);
+
 #ifdef __cplusplus
 }
 #endif

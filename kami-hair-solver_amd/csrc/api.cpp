@@ -384,4 +384,9 @@ KhsResult khsxGetDirectionCounts(const KhsSolver *s, uint64_t *block, uint64_t *
     return guarded_const(s, [&] { return s->impl.direction_counts(block, steepest); });
 }
 
+KhsResult khsxGetLaunchCounts(const KhsSolver *s, uint64_t *dispatches, uint64_t *host_waits)
+{
+    return guarded_const(s, [&] { return s->impl.launch_counts(dispatches, host_waits); });
+}
+
 }  // extern "C"

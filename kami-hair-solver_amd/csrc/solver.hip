@@ -23,6 +23,13 @@
 
 namespace khs {
 
+// A kernel launch on a solver's stream, counted (khsxGetLaunchCounts).
+#define KHS_LAUNCH(state, ...)                                                                                      \
+    do {                                                                                                          \
+        ++(state).launch_count;                                                                                   \
+        hipLaunchKernelGGL(__VA_ARGS__);                                                                          \
+    } while (0)
+
 #define HIP_TRY(call)                                                                                         \
     do {                                                                                                      \
         const hipError_t khs_err_ = (call);                                                                   \
@@ -151,6 +158,8 @@ uint32_t *no_alloc_staging()
 }  // namespace
 
 struct DeviceState {
+    uint64_t launch_count = 0;  // kernel launches and device-to-device copies on `stream`
+    uint64_t wait_count = 0;    // host waits for `stream`
     hipStream_t stream = nullptr;
     hipEvent_t frame_begin = nullptr, frame_end = nullptr;
     uint32_t N = 0, S = 0, V = 0, T = 0, P = 0;
@@ -332,7 +341,7 @@ struct DeviceState {
     void reduce_async(uint32_t blocks, int slot, dev::ConvergenceTest test = {0.0, 0.0, 0.0, nullptr},
                       dev::ListPartials lists = {nullptr, nullptr, 0})
     {
-        hipLaunchKernelGGL(dev::k_reduce, dim3(1), dim3(dev::kReduceBlock), 0, stream, partials.get(), blocks,
+        KHS_LAUNCH(*this, dev::k_reduce, dim3(1), dim3(dev::kReduceBlock), 0, stream, partials.get(), blocks,
                            reduced.get() + slot, mirror_slot(slot), test, lists);
         HIP_TRY(hipGetLastError());
     }
@@ -361,6 +370,7 @@ struct DeviceState {
     }
     void wait()
     {
+        ++wait_count;
         HIP_TRY(hipStreamSynchronize(stream));
         settle();
     }
@@ -577,7 +587,7 @@ KhsResult Solver::build_model_on_device()
     Buffer<dev::StrandShape> shape;
     shape.allocate(S);
     const double hmax = desc_.maximum_element_length, minimum = desc_.minimum_dynamic_length;
-    hipLaunchKernelGGL(dev::k_model_count, dim3(grid_for(S, kBlock)), dim3(kBlock), 0, d.stream, S,
+    KHS_LAUNCH(d, dev::k_model_count, dim3(grid_for(S, kBlock)), dim3(kBlock), 0, d.stream, S,
                        (const double *)d.root_points.get(), (const uint32_t *)offsets.get(), hmax, minimum,
                        shape.get());
     HIP_TRY(hipGetLastError());
@@ -645,7 +655,7 @@ KhsResult Solver::build_model_on_device()
     dev::ModelOut o{d.q.get(),        d.frame.get(),    d.rest.get(),       d.mass.get(),   d.rmass.get(),
                     d.flags.get(),    d.strand_of.get(), d.bind_left.get(), d.bind_right.get(), d.bind_t.get(),
                     d.mapping.get(),  strand_max.get()};
-    hipLaunchKernelGGL(dev::k_model_nodes, dim3(grid_for(S, kBlock)), dim3(kBlock), 0, d.stream, S,
+    KHS_LAUNCH(d, dev::k_model_nodes, dim3(grid_for(S, kBlock)), dim3(kBlock), 0, d.stream, S,
                        (const double *)d.root_points.get(), (const uint32_t *)offsets.get(), hmax, minimum,
                        desc_.fixed_root_nodes, (const uint32_t *)d.strand_first.get(), rho.translational,
                        rho.rotational, o);
@@ -724,7 +734,9 @@ KhsResult Solver::upload_model()
     d.Dblk.view(d.solve_in.get(), 36 * N);
     d.Ublk.view(d.solve_in.get() + 36 * N, 36 * N);
     d.grad.view(d.solve_in.get() + 72 * N, 6 * N);
+    ++d.launch_count;
     HIP_TRY(hipMemcpyAsync(d.q_old.get(), d.q.get(), d.q.bytes(), hipMemcpyDeviceToDevice, d.stream));
+    ++d.launch_count;
     HIP_TRY(hipMemcpyAsync(d.snap_q.get(), d.q.get(), d.q.bytes(), hipMemcpyDeviceToDevice, d.stream));
     d.vel.upload(zeros.data(), zeros.size());
     d.snap_v.upload(zeros.data(), zeros.size());
@@ -742,9 +754,10 @@ KhsResult Solver::upload_model()
         error_ = "Cannot build the initial root orientation.";
         return KHS_ERROR_INVALID_HAIR;
     }
-    hipLaunchKernelGGL(dev::k_gather_positions, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.q.get(),
+    KHS_LAUNCH(d, dev::k_gather_positions, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.q.get(),
                        (const uint32_t *)nullptr, d.N, d.root_cur.get());
     HIP_TRY(hipGetLastError());
+    ++d.launch_count;
     HIP_TRY(hipMemcpyAsync(d.root_pend.get(), d.root_cur.get(), d.root_cur.bytes(), hipMemcpyDeviceToDevice, d.stream));
     d.rot_cur.upload(zero3.data(), zero3.size());
     d.rot_pend.upload(zero3.data(), zero3.size());
@@ -822,15 +835,15 @@ void Solver::rebuild_bvh(bool swept)
     d.cache_valid = false;
     const ColliderView cv = d.collider();
     const unsigned tb = grid_for(d.T, kBlock);
-    hipLaunchKernelGGL(dev::k_triangle_boxes, dim3(tb), dim3(kBlock), 0, d.stream, cv, swept ? 1 : 0, d.tri_box.get(),
+    KHS_LAUNCH(d, dev::k_triangle_boxes, dim3(tb), dim3(kBlock), 0, d.stream, cv, swept ? 1 : 0, d.tri_box.get(),
                        d.tri_box_static.get(), d.bound_partials.get());
-    hipLaunchKernelGGL(dev::k_bounds_final, dim3(1), dim3(256), 0, d.stream, d.bound_partials.get(), tb, d.bounds.get());
-    hipLaunchKernelGGL(dev::k_morton, dim3(tb), dim3(kBlock), 0, d.stream, cv, d.bounds.get(), d.keys.get(),
+    KHS_LAUNCH(d, dev::k_bounds_final, dim3(1), dim3(256), 0, d.stream, d.bound_partials.get(), tb, d.bounds.get());
+    KHS_LAUNCH(d, dev::k_morton, dim3(tb), dim3(kBlock), 0, d.stream, cv, d.bounds.get(), d.keys.get(),
                        d.ids.get());
     size_t temp = d.sort_temp_bytes;
     HIP_TRY(rocprim::radix_sort_pairs(d.sort_temp.get(), temp, d.keys.get(), d.keys_sorted.get(), d.ids.get(),
                                       d.ids_sorted.get(), d.T, 0, 30, d.stream));
-    hipLaunchKernelGGL(dev::k_hierarchy, dim3(tb), dim3(kBlock), 0, d.stream, d.T, d.keys_sorted.get(),
+    KHS_LAUNCH(d, dev::k_hierarchy, dim3(tb), dim3(kBlock), 0, d.stream, d.T, d.keys_sorted.get(),
                        d.ids_sorted.get(), d.bvh.get(), d.parent_internal.get(), d.parent_leaf.get(), d.counters.get());
     refit_boxes();
 }
@@ -840,15 +853,15 @@ void Solver::refit_boxes()
 {
     DeviceState &d = *dev_;
     if (d.T == 1) {
-        hipLaunchKernelGGL(dev::k_refit, dim3(1), dim3(kBlock), 0, d.stream, d.T, d.ids_sorted.get(), d.tri_box.get(),
+        KHS_LAUNCH(d, dev::k_refit, dim3(1), dim3(kBlock), 0, d.stream, d.T, d.ids_sorted.get(), d.tri_box.get(),
                            d.parent_internal.get(), d.parent_leaf.get(), d.bvh.get(), d.counters.get());
     } else {
-        hipLaunchKernelGGL(dev::k_fill_u32, dim3(1), dim3(64), 0, d.stream, d.refit_ready_count.get(), size_t(1), 0u);
-        hipLaunchKernelGGL(dev::k_refit_local, dim3(grid_for(d.T, dev::kRefitChunk)), dim3(dev::kRefitChunk), 0,
+        KHS_LAUNCH(d, dev::k_fill_u32, dim3(1), dim3(64), 0, d.stream, d.refit_ready_count.get(), size_t(1), 0u);
+        KHS_LAUNCH(d, dev::k_refit_local, dim3(grid_for(d.T, dev::kRefitChunk)), dim3(dev::kRefitChunk), 0,
                            d.stream, d.T, d.ids_sorted.get(), d.tri_box.get(), d.parent_internal.get(),
                            d.parent_leaf.get(), d.bvh.get(), d.counters.get(), d.refit_ready.get(),
                            d.refit_ready_count.get());
-        hipLaunchKernelGGL(dev::k_refit_top, dim3(1), dim3(1024), 0, d.stream, d.parent_internal.get(), d.bvh.get(),
+        KHS_LAUNCH(d, dev::k_refit_top, dim3(1), dim3(1024), 0, d.stream, d.parent_internal.get(), d.bvh.get(),
                            d.counters.get(), d.refit_ready.get(), d.refit_ready_count.get());
     }
     HIP_TRY(hipGetLastError());
@@ -861,9 +874,9 @@ void Solver::refit_bvh(bool swept)
     if (!d.T) return;
     d.cache_valid = false;
     const unsigned tb = grid_for(d.T, kBlock);
-    hipLaunchKernelGGL(dev::k_triangle_boxes, dim3(tb), dim3(kBlock), 0, d.stream, d.collider(), swept ? 1 : 0,
+    KHS_LAUNCH(d, dev::k_triangle_boxes, dim3(tb), dim3(kBlock), 0, d.stream, d.collider(), swept ? 1 : 0,
                        d.tri_box.get(), d.tri_box_static.get(), d.bound_partials.get());
-    hipLaunchKernelGGL(dev::k_zero_counters, dim3(tb), dim3(kBlock), 0, d.stream, d.counters.get(),
+    KHS_LAUNCH(d, dev::k_zero_counters, dim3(tb), dim3(kBlock), 0, d.stream, d.counters.get(),
                        d.T > 1 ? d.T - 1 : 1u);
     refit_boxes();
 }
@@ -884,14 +897,14 @@ void Solver::refresh_contact_cache(int mode, bool full, bool swept_leaves, bool 
     d.cache_skin = 0.5 * pad;
     const ContactCache cc = d.cache();
     if (full) {
-        hipLaunchKernelGGL(dev::k_fill_u32, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.cache_listed.get(),
+        KHS_LAUNCH(d, dev::k_fill_u32, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.cache_listed.get(),
                            size_t(d.N), 0u);
-        hipLaunchKernelGGL(dev::k_fill_u32, dim3(1), dim3(64), 0, d.stream, d.cache_list_count.get(), size_t(1), 0u);
+        KHS_LAUNCH(d, dev::k_fill_u32, dim3(1), dim3(64), 0, d.stream, d.cache_list_count.get(), size_t(1), 0u);
     }
     const dim3 grid(d.node_blocks()), block(kBlock);
     if (full) {
         // every element gathers: one lane each
-        hipLaunchKernelGGL(dev::k_contact_refresh<dev::kAtState>, grid, block, 0, d.stream, d.rods(),
+        KHS_LAUNCH(d, dev::k_contact_refresh<dev::kAtState>, grid, block, 0, d.stream, d.rods(),
                            swept_leaves ? d.swept_collider() : d.collider(), cc, d.q.get(), d.q_base.get(),
                            d.dir.get(), d.alpha.get(), pad, 1);
     } else {
@@ -899,11 +912,11 @@ void Solver::refresh_contact_cache(int mode, bool full, bool swept_leaves, bool 
         // wavefront
         const ColliderView col = d.collider();
         if (mode == dev::kAtState)
-            hipLaunchKernelGGL(dev::k_contact_check<dev::kAtState>, grid, block, 0, d.stream, d.rods(), col, cc,
+            KHS_LAUNCH(d, dev::k_contact_check<dev::kAtState>, grid, block, 0, d.stream, d.rods(), col, cc,
                                d.q.get(), d.q_base.get(), d.dir.get(), d.alpha.get(), dev::TrialStep{}, pad,
                                with_roots ? 1 : 0);
         else if (mode == dev::kAtTrial) {
-            hipLaunchKernelGGL(dev::k_contact_check<dev::kAtTrial>, grid, block, 0, d.stream, d.rods(), col, cc,
+            KHS_LAUNCH(d, dev::k_contact_check<dev::kAtTrial>, grid, block, 0, d.stream, d.rods(), col, cc,
                                d.q.get(), d.q_base.get(), d.dir.get(), d.alpha.get(), d.trial_step, pad,
                                with_roots ? 1 : 0);
             d.trial_step.source = dev::kAlphaStored;
@@ -911,10 +924,10 @@ void Solver::refresh_contact_cache(int mode, bool full, bool swept_leaves, bool 
         else {
             // the swept pass queues its elements; the two pending counters
             // alternate, each cleared by the gather before its next use
-            hipLaunchKernelGGL(dev::k_contact_check<dev::kSwept>, grid, block, 0, d.stream, d.rods(), col, cc,
+            KHS_LAUNCH(d, dev::k_contact_check<dev::kSwept>, grid, block, 0, d.stream, d.rods(), col, cc,
                                d.q.get(), d.q_base.get(), d.dir.get(), d.alpha.get(), dev::TrialStep{}, pad,
                                with_roots ? 1 : 0);
-            hipLaunchKernelGGL(dev::k_contact_gather, dim3(kGatherBlocks), block, 0, d.stream, d.flags.get(), col, cc,
+            KHS_LAUNCH(d, dev::k_contact_gather, dim3(kGatherBlocks), block, 0, d.stream, d.flags.get(), col, cc,
                                d.cache_pending_count.get() + (d.pending_slot ^ 1u));
             d.pending_slot ^= 1u;
         }
@@ -934,13 +947,13 @@ void Solver::closest_pass(bool line_search, unsigned long long *block_candidates
     refresh_contact_cache(line_search ? dev::kAtTrial : dev::kAtState, false, false, with_roots);
     const ContactCache cc = d.cache();
     if (line_search)
-        hipLaunchKernelGGL(dev::k_list_closest<dev::kAtTrial>, dim3(kPairBlocks), dim3(kBlock), 0, d.stream,
+        KHS_LAUNCH(d, dev::k_list_closest<dev::kAtTrial>, dim3(kPairBlocks), dim3(kBlock), 0, d.stream,
                            d.flags.get(), d.collider(), cc, sp, d.q.get(), d.q_base.get(), d.dir.get(),
                            d.alpha.get(), d.best_tri.get(), block_candidates,
                            with_roots ? d.root_elements.get() : nullptr, with_roots ? d.root_element_count : 0u,
                            d.skip());
     else
-        hipLaunchKernelGGL(dev::k_list_closest<dev::kAtState>, dim3(kPairBlocks), dim3(kBlock), 0, d.stream,
+        KHS_LAUNCH(d, dev::k_list_closest<dev::kAtState>, dim3(kPairBlocks), dim3(kBlock), 0, d.stream,
                            d.flags.get(), d.collider(), cc, sp, d.q.get(), d.q_base.get(), d.dir.get(),
                            d.alpha.get(), d.best_tri.get(), block_candidates,
                            with_roots ? d.root_elements.get() : nullptr, with_roots ? d.root_element_count : 0u,
@@ -954,7 +967,7 @@ KhsResult Solver::initial_feasibility()
     DeviceState &d = *dev_;
     Buffer<double> dpos;
     dpos.allocate(3 * size_t(d.N));
-    hipLaunchKernelGGL(dev::k_gather_positions, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.q.get(),
+    KHS_LAUNCH(d, dev::k_gather_positions, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.q.get(),
                        (const uint32_t *)nullptr, d.N, dpos.get());
     HIP_TRY(hipGetLastError());
     const RodView rv = d.rods();
@@ -964,7 +977,7 @@ KhsResult Solver::initial_feasibility()
         for (const Vec3 &v : collider_.vertices) x_max = std::max(x_max, v.x);
         const unsigned init = 0xffffffffu;
         HIP_TRY(hipMemcpyAsync(d.flag.get(), &init, sizeof(unsigned), hipMemcpyHostToDevice, d.stream));
-        hipLaunchKernelGGL(dev::k_point_inside, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, cv, dpos.get(),
+        KHS_LAUNCH(d, dev::k_point_inside, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, cv, dpos.get(),
                            x_max, d.flag.get());
         HIP_TRY(hipGetLastError());
         unsigned inside = 0;
@@ -976,7 +989,7 @@ KhsResult Solver::initial_feasibility()
         }
     }
     const StepParams p = step_params(1.0);
-    hipLaunchKernelGGL(dev::k_initial_gap, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, cv, p, dpos.get(),
+    KHS_LAUNCH(d, dev::k_initial_gap, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, cv, p, dpos.get(),
                        d.partials.get());
     HIP_TRY(hipGetLastError());
     const Partial r = d.reduce(d.node_blocks());
@@ -1060,11 +1073,11 @@ bool Solver::device_root_targets(const std::vector<Vec3> &points, double *pos, d
     DeviceState &d = *dev_;
     const std::vector<double> flat = flatten(points);
     d.root_points.upload(flat.data(), flat.size());
-    hipLaunchKernelGGL(dev::k_fill_u32, dim3(1), dim3(64), 0, d.stream, d.root_flag.get(), size_t(1), 0u);
-    hipLaunchKernelGGL(dev::k_root_positions, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.N,
+    KHS_LAUNCH(d, dev::k_fill_u32, dim3(1), dim3(64), 0, d.stream, d.root_flag.get(), size_t(1), 0u);
+    KHS_LAUNCH(d, dev::k_root_positions, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.N,
                        (const uint32_t *)d.bind_left.get(), (const uint32_t *)d.bind_right.get(),
                        (const double *)d.bind_t.get(), (const double *)d.root_points.get(), pos);
-    hipLaunchKernelGGL(dev::k_root_rotations, dim3(grid_for(d.S, kBlock)), dim3(kBlock), 0, d.stream, d.rods(),
+    KHS_LAUNCH(d, dev::k_root_rotations, dim3(grid_for(d.S, kBlock)), dim3(kBlock), 0, d.stream, d.rods(),
                        (const double *)pos, rot, d.root_flag.get());
     HIP_TRY(hipGetLastError());
     unsigned bad = 0;
@@ -1194,7 +1207,7 @@ void Solver::evaluate_async(bool gradient, double dt, bool line_search, unsigned
     const uint32_t *best = list_pass ? d.best_tri.get() : nullptr;
     // the variants without the tree walk when the list pass answered every query
     auto launch = [&](auto kernel, const unsigned *stop) {
-        hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kBlock), 0, d.stream, rv, cv, cc, best, p, d.q.get(),
+        KHS_LAUNCH(d, kernel, dim3(blocks), dim3(kBlock), 0, d.stream, rv, cv, cc, best, p, d.q.get(),
                            d.q_old.get(), d.vel.get(), d.q_base.get(), d.dir.get(), d.alpha.get(), d.grad.get(),
                            d.Dblk.get(), d.Ublk.get(), d.partials.get(), stop);
     };
@@ -1252,16 +1265,16 @@ void Solver::steepest_direction_async()
     DeviceState &d = *dev_;
     const RodView rv = d.rods();
     const size_t dofs = 6 * size_t(d.N);
-    hipLaunchKernelGGL(dev::k_steepest, dim3(grid_for(dofs, kBlock)), dim3(kBlock), 0, d.stream, rv, d.choice(),
+    KHS_LAUNCH(d, dev::k_steepest, dim3(grid_for(dofs, kBlock)), dim3(kBlock), 0, d.stream, rv, d.choice(),
                        d.grad.get(), d.precond.get(), d.dir.get());
-    hipLaunchKernelGGL(dev::k_direction_stats, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, d.grad.get(),
+    KHS_LAUNCH(d, dev::k_direction_stats, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, d.grad.get(),
                        d.dir.get(), (const unsigned *)nullptr, (const DirectionChoice *)d.choice(),
                        d.partials.get());
     d.reduce_async(d.node_blocks(), DeviceState::kSteepSlot);
-    hipLaunchKernelGGL(dev::k_choose_steepest_scale, dim3(1), dim3(1), 0, d.stream,
+    KHS_LAUNCH(d, dev::k_choose_steepest_scale, dim3(1), dim3(1), 0, d.stream,
                        (const Partial *)(d.reduced.get() + DeviceState::kSteepSlot), trust_radius_, d.choice(),
                        d.mirror_choice());
-    hipLaunchKernelGGL(dev::k_scale_direction, dim3(grid_for(dofs, kBlock)), dim3(kBlock), 0, d.stream,
+    KHS_LAUNCH(d, dev::k_scale_direction, dim3(grid_for(dofs, kBlock)), dim3(kBlock), 0, d.stream,
                        (const DirectionChoice *)d.choice(), 0, d.dir.get(), dofs);
     HIP_TRY(hipGetLastError());
 }
@@ -1278,9 +1291,9 @@ void Solver::set_trial_step(int source, double value)
         return;
     }
     if (source == dev::kAlphaFromCcd)
-        hipLaunchKernelGGL(dev::k_choose_alpha, dim3(1), dim3(1), 0, d.stream, ccd, d.T ? 1 : 0, d.alpha.get());
+        KHS_LAUNCH(d, dev::k_choose_alpha, dim3(1), dim3(1), 0, d.stream, ccd, d.T ? 1 : 0, d.alpha.get());
     else
-        hipLaunchKernelGGL(dev::k_set_double, dim3(1), dim3(1), 0, d.stream, d.alpha.get(), value);
+        KHS_LAUNCH(d, dev::k_set_double, dim3(1), dim3(1), 0, d.stream, d.alpha.get(), value);
     HIP_TRY(hipGetLastError());
 }
 
@@ -1295,20 +1308,20 @@ uint32_t Solver::launch_block_solve(unsigned *failed, unsigned *next_failed, con
     const RodView rv = d.rods();
     uint32_t blocks = d.solve_blocks();
     if (precision_mode_ == 1) {
-        hipLaunchKernelGGL(dev::k_block_solve<float>, dim3(blocks), dim3(kWave), kSolveLdsReserve, d.stream, rv,
+        KHS_LAUNCH(d, dev::k_block_solve<float>, dim3(blocks), dim3(kWave), kSolveLdsReserve, d.stream, rv,
                            d.solve_order.get(), d.Dblk.get(), d.Ublk.get(), d.grad.get(), d.Cp.get(), d.dp.get(),
                            d.dir.get(), failed, next_failed, d.partials.get(), stop);
     } else if (solve_algorithm_ == 1) {
         blocks = d.twisted_blocks();
-        hipLaunchKernelGGL(dev::k_block_solve_twisted, dim3(blocks), dim3(kWave), kSolveLdsReserve, d.stream, rv,
+        KHS_LAUNCH(d, dev::k_block_solve_twisted, dim3(blocks), dim3(kWave), kSolveLdsReserve, d.stream, rv,
                            d.solve_order.get(), d.Dblk.get(), d.Ublk.get(), d.grad.get(), d.Cp.get(), d.dp.get(),
                            d.dir.get(), failed, next_failed, d.partials.get(), stop);
     } else if (solve_algorithm_ == 2 && d.solve_in.bytes() < dev::kMaxBufferBytes) {
-        hipLaunchKernelGGL(dev::k_block_solve_fused, dim3(blocks), dim3(kWave), kSolveLdsReserve, d.stream, rv,
+        KHS_LAUNCH(d, dev::k_block_solve_fused, dim3(blocks), dim3(kWave), kSolveLdsReserve, d.stream, rv,
                            d.solve_order.get(), (const double *)d.solve_in.get(), d.Cp.get(), d.dp.get(), d.dir.get(),
                            failed, next_failed, d.partials.get(), stop);
     } else {
-        hipLaunchKernelGGL(dev::k_block_solve<double>, dim3(blocks), dim3(kWave), kSolveLdsReserve, d.stream, rv,
+        KHS_LAUNCH(d, dev::k_block_solve<double>, dim3(blocks), dim3(kWave), kSolveLdsReserve, d.stream, rv,
                            d.solve_order.get(), d.Dblk.get(), d.Ublk.get(), d.grad.get(), d.Cp.get(), d.dp.get(),
                            d.dir.get(), failed, next_failed, d.partials.get(), stop);
     }
@@ -1334,7 +1347,7 @@ void Solver::make_direction_async()
     d.reduce_async(blocks, DeviceState::kDirSlot);
     // the choice, and the line search's base state q_base = q
     const size_t dofs = 6 * size_t(d.N);
-    hipLaunchKernelGGL(dev::k_apply_direction_choice, dim3(grid_for(dofs, kBlock)), dim3(kBlock), 0, d.stream,
+    KHS_LAUNCH(d, dev::k_apply_direction_choice, dim3(grid_for(dofs, kBlock)), dim3(kBlock), 0, d.stream,
                        (const Partial *)(d.reduced.get() + DeviceState::kDirSlot), trust_radius_, d.choice(),
                        d.mirror_choice(), d.dir.get(), dofs, (const double *)d.q.get(), d.q_base.get());
     HIP_TRY(hipGetLastError());
@@ -1347,8 +1360,9 @@ void Solver::direction_async()
         return;
     }
     DeviceState &d = *dev_;
+    ++d.launch_count;
     HIP_TRY(hipMemcpyAsync(d.q_base.get(), d.q.get(), d.q.bytes(), hipMemcpyDeviceToDevice, d.stream));
-    hipLaunchKernelGGL(dev::k_force_steepest, dim3(1), dim3(1), 0, d.stream, d.choice(), d.mirror_choice());
+    KHS_LAUNCH(d, dev::k_force_steepest, dim3(1), dim3(1), 0, d.stream, d.choice(), d.mirror_choice());
     HIP_TRY(hipGetLastError());
     steepest_direction_async();
 }
@@ -1363,7 +1377,7 @@ void Solver::launch_preconditioner(double dt)
     const double tk = std::fmax(material_.young_modulus * area, material_.shear_correction * G * area);
     const double ra = material_.shear_correction * G * area;
     const double rb = std::fmax(material_.young_modulus * inertia, 2.0 * G * inertia);
-    hipLaunchKernelGGL(dev::k_preconditioner, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.rods(), 1.0 / dt,
+    KHS_LAUNCH(d, dev::k_preconditioner, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.rods(), 1.0 / dt,
                        material_.mass_damping, tk, ra, rb,
                        d.elem_material.size() ? (const double *)d.elem_material.get() : nullptr,
                        material_.poisson_ratio, material_.shear_correction, d.precond.get());
@@ -1400,12 +1414,12 @@ KhsResult Solver::optimize_substep(double dt)
         const bool list_pass = d.cache_valid;
         if (list_pass) {
             refresh_contact_cache(dev::kSwept, false);
-            hipLaunchKernelGGL(dev::k_list_ccd, dim3(kPairBlocks), dim3(kBlock), 0, d.stream, d.flags.get(),
+            KHS_LAUNCH(d, dev::k_list_ccd, dim3(kPairBlocks), dim3(kBlock), 0, d.stream, d.flags.get(),
                                d.collider(), d.cache(), cp, d.q.get(), d.dir.get(), d.list_alpha.get(),
                                d.ccd_bound.get(), int(d.ccd_slot), d.skip());
             d.ccd_slot ^= 1u;
         }
-        hipLaunchKernelGGL(dev::k_ccd_slope, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, d.collider(),
+        KHS_LAUNCH(d, dev::k_ccd_slope, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, d.collider(),
                            d.cache(), list_pass ? 1 : 0, cp, d.q.get(), d.grad.get(), d.dir.get(), d.partials.get());
         HIP_TRY(hipGetLastError());
         d.toc(t0, d.totals.collision_ms, d.totals.collision_launches);
@@ -1438,7 +1452,7 @@ KhsResult Solver::optimize_substep(double dt)
         if (residual <= abs_tol || relative <= rel_tol) {
             // work enqueued ahead skipped itself (the device made the same
             // test), so q is still this iterate
-            hipLaunchKernelGGL(dev::k_update_velocity, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv,
+            KHS_LAUNCH(d, dev::k_update_velocity, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv,
                                d.q.get(), d.q_old.get(), d.vel.get(), 1.0 / dt);
             HIP_TRY(hipGetLastError());
             return KHS_OK;
@@ -1467,6 +1481,7 @@ KhsResult Solver::optimize_substep(double dt)
             if (attempt == 0 && block_direction && !v.choice.used) {
                 // The block direction failed or does not descend: the step
                 // starts from steepest descent instead, with no retry.
+                ++d.launch_count;
                 HIP_TRY(hipMemcpyAsync(d.q.get(), d.q_base.get(), d.q.bytes(), hipMemcpyDeviceToDevice, d.stream));
                 steepest_direction_async();
                 block_direction = false;
@@ -1504,14 +1519,16 @@ KhsResult Solver::optimize_substep(double dt)
             }
             if (accepted) break;
             if (!block_direction) break;
+            ++d.launch_count;
             HIP_TRY(hipMemcpyAsync(d.q.get(), d.q_base.get(), d.q.bytes(), hipMemcpyDeviceToDevice, d.stream));
-            hipLaunchKernelGGL(dev::k_force_steepest, dim3(1), dim3(1), 0, d.stream, d.choice(), d.mirror_choice());
+            KHS_LAUNCH(d, dev::k_force_steepest, dim3(1), dim3(1), 0, d.stream, d.choice(), d.mirror_choice());
             steepest_direction_async();
             ++steepest_directions_;
             block_direction = false;
             ++attempt;
         }
         if (!accepted) {
+            ++d.launch_count;
             HIP_TRY(hipMemcpyAsync(d.q.get(), d.q_base.get(), d.q.bytes(), hipMemcpyDeviceToDevice, d.stream));
             return fail(KHS_ERROR_NOT_CONVERGED, "Gauss-Newton line search failed.");
         }
@@ -1533,15 +1550,17 @@ KhsResult Solver::optimize_substep(double dt)
         phase_.store(KHS_PHASE_ASSEMBLING);
         if (step_stats_.increment_norm / std::sqrt(dofs) <= 2.0 * desc_.increment_tolerance) {
             // a trial enqueued ahead may have moved q: back to the iterate
-            if (head_ready)
+            if (head_ready) {
+                ++d.launch_count;
                 HIP_TRY(hipMemcpyAsync(d.q.get(), d.q_base.get(), d.q.bytes(), hipMemcpyDeviceToDevice, d.stream));
-            hipLaunchKernelGGL(dev::k_update_velocity, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv,
+            }
+            KHS_LAUNCH(d, dev::k_update_velocity, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv,
                                d.q.get(), d.q_old.get(), d.vel.get(), 1.0 / dt);
             HIP_TRY(hipGetLastError());
             return KHS_OK;
         }
     }
-    hipLaunchKernelGGL(dev::k_update_velocity, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, d.q.get(),
+    KHS_LAUNCH(d, dev::k_update_velocity, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, d.q.get(),
                        d.q_old.get(), d.vel.get(), 1.0 / dt);
     HIP_TRY(hipGetLastError());
     return KHS_OK;
@@ -1557,12 +1576,12 @@ void Solver::enqueue_sweep(const StepParams &sp)
     // current boxes.
     refresh_contact_cache(dev::kAtState, true, true);
     const unsigned long long none = ~0ull;
-    hipLaunchKernelGGL(dev::k_fill_u64, dim3(1), dim3(64), 0, d.stream, d.key64.get(), size_t(1), none);
-    hipLaunchKernelGGL(dev::k_list_sweep, dim3(kPairBlocks), dim3(kBlock), 0, d.stream, d.flags.get(),
+    KHS_LAUNCH(d, dev::k_fill_u64, dim3(1), dim3(64), 0, d.stream, d.key64.get(), size_t(1), none);
+    KHS_LAUNCH(d, dev::k_list_sweep, dim3(kPairBlocks), dim3(kBlock), 0, d.stream, d.flags.get(),
                        d.strand_of.get(), d.collider(), d.cache(), d.tri_box.get(), sp, d.q.get(), d.key64.get());
-    hipLaunchKernelGGL(dev::k_sweep, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.rods(), d.swept_collider(),
+    KHS_LAUNCH(d, dev::k_sweep, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.rods(), d.swept_collider(),
                        sp, d.q.get(), (const uint32_t *)d.cache_count.get(), d.key64.get());
-    hipLaunchKernelGGL(dev::k_sweep_record, dim3(1), dim3(1), 0, d.stream, d.rods(), d.collider(), sp, d.q.get(),
+    KHS_LAUNCH(d, dev::k_sweep_record, dim3(1), dim3(1), 0, d.stream, d.rods(), d.collider(), sp, d.q.get(),
                        d.key64.get(), d.sweep.get());
     HIP_TRY(hipGetLastError());
 }
@@ -1573,9 +1592,9 @@ void Solver::enqueue_friction(const StepParams &sp)
 {
     DeviceState &d = *dev_;
     if (d.cache_valid) closest_pass(false, nullptr, true);
-    hipLaunchKernelGGL(dev::k_friction, dim3(d.tiled_blocks()), dim3(kBlock), 0, d.stream, d.rods(), d.collider(),
+    KHS_LAUNCH(d, dev::k_friction, dim3(d.tiled_blocks()), dim3(kBlock), 0, d.stream, d.rods(), d.collider(),
                        d.cache(), d.cache_valid ? d.best_tri.get() : nullptr, sp, d.q.get(), d.vel.get(), d.dv.get());
-    hipLaunchKernelGGL(dev::k_apply_dv, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.N, d.dv.get(),
+    KHS_LAUNCH(d, dev::k_apply_dv, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.N, d.dv.get(),
                        d.vel.get());
     HIP_TRY(hipGetLastError());
 }
@@ -1597,7 +1616,9 @@ KhsResult Solver::run_frame(double frame_dt, const double *root_a, const double 
     HIP_TRY(hipEventRecord(d.frame_begin, d.stream));
     const KhsxKernelTiming before = d.totals;  // phase times of this frame are the differences
     const size_t dof_bytes = d.q.bytes();
+    ++d.launch_count;
     HIP_TRY(hipMemcpyAsync(d.snap_q.get(), d.q.get(), dof_bytes, hipMemcpyDeviceToDevice, d.stream));
+    ++d.launch_count;
     HIP_TRY(hipMemcpyAsync(d.snap_v.get(), d.vel.get(), dof_bytes, hipMemcpyDeviceToDevice, d.stream));
     uint32_t substeps = desc_.substeps;
     const uint32_t max_sub = d.T ? desc_.maximum_substeps : desc_.substeps;
@@ -1605,7 +1626,9 @@ KhsResult Solver::run_frame(double frame_dt, const double *root_a, const double 
     const size_t col_bytes = d.col_cur.bytes();
     for (;;) {
         ++attempts;
+        ++d.launch_count;
         HIP_TRY(hipMemcpyAsync(d.q.get(), d.snap_q.get(), dof_bytes, hipMemcpyDeviceToDevice, d.stream));
+        ++d.launch_count;
         HIP_TRY(hipMemcpyAsync(d.vel.get(), d.snap_v.get(), dof_bytes, hipMemcpyDeviceToDevice, d.stream));
         clear_failure();
         error_.clear();
@@ -1619,9 +1642,12 @@ KhsResult Solver::run_frame(double frame_dt, const double *root_a, const double 
         step_stats_.ccd_step_limit = 1.0;
         block_directions_ = steepest_directions_ = 0;
         if (d.T) {
+            ++d.launch_count;
             HIP_TRY(hipMemcpyAsync(d.col_cur.get(), collider_a, col_bytes, hipMemcpyDeviceToDevice, d.stream));
-            if (collider_a != d.col_start.get())
+            if (collider_a != d.col_start.get()) {
+                ++d.launch_count;
                 HIP_TRY(hipMemcpyAsync(d.col_start.get(), collider_a, col_bytes, hipMemcpyDeviceToDevice, d.stream));
+            }
         }
         const double dt = frame_dt / substeps;
         const StepParams sp = step_params(dt);
@@ -1637,11 +1663,11 @@ KhsResult Solver::run_frame(double frame_dt, const double *root_a, const double 
             iteration_.store(0);
             HIP_TRY(hipEventRecord(d.substep_events[sub - 1], d.stream));
             const double f = double(sub) / substeps;
-            hipLaunchKernelGGL(dev::k_begin_substep, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, d.q.get(),
+            KHS_LAUNCH(d, dev::k_begin_substep, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, d.q.get(),
                                d.q_old.get(), d.vel.get(), root_a, rot_a, root_b, rot_b, f, dt);
             HIP_TRY(hipGetLastError());
             if (d.T) {
-                hipLaunchKernelGGL(dev::k_collider_lerp, dim3(grid_for(3 * size_t(d.V), kBlock)), dim3(kBlock), 0,
+                KHS_LAUNCH(d, dev::k_collider_lerp, dim3(grid_for(3 * size_t(d.V), kBlock)), dim3(kBlock), 0,
                                    d.stream, d.V, d.col_cur.get(), d.col_old.get(), collider_a, collider_b, f);
                 hipEvent_t t0 = d.tic();
                 phase_.store(KHS_PHASE_CCD);
@@ -1685,7 +1711,9 @@ KhsResult Solver::run_frame(double frame_dt, const double *root_a, const double 
             ++step_stats_.converged_substeps;
         }
         if (!crossed && result == KHS_OK) break;
+        ++d.launch_count;
         HIP_TRY(hipMemcpyAsync(d.q.get(), d.snap_q.get(), dof_bytes, hipMemcpyDeviceToDevice, d.stream));
+        ++d.launch_count;
         HIP_TRY(hipMemcpyAsync(d.vel.get(), d.snap_v.get(), dof_bytes, hipMemcpyDeviceToDevice, d.stream));
         HIP_TRY(hipStreamSynchronize(d.stream));
         if (cancel_.load()) return result;
@@ -1697,7 +1725,9 @@ KhsResult Solver::run_frame(double frame_dt, const double *root_a, const double 
     }
     if (!animation_ready_) {
         const size_t n3 = 3 * size_t(d.N) * sizeof(double);
+        ++d.launch_count;
         HIP_TRY(hipMemcpyAsync(d.root_cur.get(), d.root_pend.get(), n3, hipMemcpyDeviceToDevice, d.stream));
+        ++d.launch_count;
         HIP_TRY(hipMemcpyAsync(d.rot_cur.get(), d.rot_pend.get(), n3, hipMemcpyDeviceToDevice, d.stream));
         if (d.T) HIP_TRY(hipMemcpyAsync(d.col_start.get(), d.col_pend.get(), col_bytes, hipMemcpyDeviceToDevice, d.stream));
     }
@@ -1780,8 +1810,10 @@ void Solver::keep_in_ring(uint32_t frame)
     if (ring_frame_.empty()) return;
     DeviceState &d = *dev_;
     const size_t dofs = 6 * size_t(d.N), slot = frame % ring_frame_.size();
+    ++d.launch_count;
     HIP_TRY(hipMemcpyAsync(d.ring_q.get() + slot * dofs, d.q.get(), dofs * sizeof(double), hipMemcpyDeviceToDevice,
                            d.stream));
+    ++d.launch_count;
     HIP_TRY(hipMemcpyAsync(d.ring_v.get() + slot * dofs, d.vel.get(), dofs * sizeof(double), hipMemcpyDeviceToDevice,
                            d.stream));
     HIP_TRY(hipStreamSynchronize(d.stream));
@@ -1815,10 +1847,14 @@ KhsResult Solver::rollback(uint32_t frame)
     const size_t dofs = 6 * size_t(d.N), slot = frame % ring_frame_.size();
     const size_t bytes = dofs * sizeof(double);
     const double *q = d.ring_q.get() + slot * dofs, *v = d.ring_v.get() + slot * dofs;
-    for (double *dst : {d.q.get(), d.q_old.get(), d.snap_q.get()})
+    for (double *dst : {d.q.get(), d.q_old.get(), d.snap_q.get()}) {
+        ++d.launch_count;
         HIP_TRY(hipMemcpyAsync(dst, q, bytes, hipMemcpyDeviceToDevice, d.stream));
-    for (double *dst : {d.vel.get(), d.snap_v.get()})
+    }
+    for (double *dst : {d.vel.get(), d.snap_v.get()}) {
+        ++d.launch_count;
         HIP_TRY(hipMemcpyAsync(dst, v, bytes, hipMemcpyDeviceToDevice, d.stream));
+    }
     HIP_TRY(hipStreamSynchronize(d.stream));
     restore_state(frame);
     return KHS_OK;
@@ -1901,7 +1937,7 @@ KhsResult Solver::copy_original_positions(KhsVec3 *positions, uint32_t capacity)
     if (!built_ || !positions || capacity < points_.size()) return KHS_ERROR_INVALID_ARGUMENT;
     try {
         DeviceState &d = *dev_;
-        hipLaunchKernelGGL(dev::k_gather_positions, dim3(grid_for(d.P, kBlock)), dim3(kBlock), 0, d.stream, d.q.get(),
+        KHS_LAUNCH(d, dev::k_gather_positions, dim3(grid_for(d.P, kBlock)), dim3(kBlock), 0, d.stream, d.q.get(),
                            d.mapping.get(), d.P, d.out3.get());
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(d.stream));
@@ -1923,7 +1959,7 @@ KhsResult Solver::write_cache_frame(int fd)
     DeviceState &d = *dev_;
     const size_t bytes = 3 * size_t(d.P) * sizeof(double);
     if (!d.cache_host) HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&d.cache_host), bytes, hipHostMallocDefault));
-    hipLaunchKernelGGL(dev::k_gather_positions, dim3(grid_for(d.P, kBlock)), dim3(kBlock), 0, d.stream, d.q.get(),
+    KHS_LAUNCH(d, dev::k_gather_positions, dim3(grid_for(d.P, kBlock)), dim3(kBlock), 0, d.stream, d.q.get(),
                        d.mapping.get(), d.P, d.out3.get());
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(d.cache_host, d.out3.get(), bytes, hipMemcpyDeviceToHost, d.stream));
@@ -1947,7 +1983,7 @@ KhsResult Solver::copy_internal_positions(KhsVec3 *positions, uint32_t capacity)
     if (!built_ || !positions || capacity < model_.node_count) return KHS_ERROR_INVALID_ARGUMENT;
     try {
         DeviceState &d = *dev_;
-        hipLaunchKernelGGL(dev::k_gather_positions, dim3(grid_for(d.N, kBlock)), dim3(kBlock), 0, d.stream, d.q.get(),
+        KHS_LAUNCH(d, dev::k_gather_positions, dim3(grid_for(d.N, kBlock)), dim3(kBlock), 0, d.stream, d.q.get(),
                            (const uint32_t *)nullptr, d.N, d.out3.get());
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(d.stream));
@@ -2100,15 +2136,15 @@ KhsResult Solver::test_evaluate(double dt, double *gradient, double *direction, 
         // the forced fallback: -P g, no trust-radius scaling
         const RodView rv = d.rods();
         launch_preconditioner(dt);
-        hipLaunchKernelGGL(dev::k_force_steepest, dim3(1), dim3(1), 0, d.stream, d.choice(), d.mirror_choice());
-        hipLaunchKernelGGL(dev::k_steepest, dim3(grid_for(6 * size_t(d.N), kBlock)), dim3(kBlock), 0, d.stream, rv,
+        KHS_LAUNCH(d, dev::k_force_steepest, dim3(1), dim3(1), 0, d.stream, d.choice(), d.mirror_choice());
+        KHS_LAUNCH(d, dev::k_steepest, dim3(grid_for(6 * size_t(d.N), kBlock)), dim3(kBlock), 0, d.stream, rv,
                            d.choice(), d.grad.get(), d.precond.get(), d.dir.get());
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(d.stream));
         d.dir.download(direction, dof_count());
     } else if (direction) {
         const RodView rv = d.rods();
-        hipLaunchKernelGGL(dev::k_fill_u32, dim3(1), dim3(64), 0, d.stream, d.flag.get(), size_t(1), 0u);
+        KHS_LAUNCH(d, dev::k_fill_u32, dim3(1), dim3(64), 0, d.stream, d.flag.get(), size_t(1), 0u);
         launch_block_solve(d.flag.get(), nullptr, nullptr);
         unsigned failed = 0;
         HIP_TRY(hipMemcpyAsync(&failed, d.flag.get(), sizeof(unsigned), hipMemcpyDeviceToHost, d.stream));
@@ -2132,8 +2168,8 @@ KhsResult Solver::broadphase_pairs(uint32_t *pairs, uint64_t capacity, uint64_t 
     Buffer<uint32_t> out;
     out.allocate(std::max<uint64_t>(2 * capacity, 2));
     const StepParams p = step_params(1.0);
-    hipLaunchKernelGGL(dev::k_fill_u64, dim3(1), dim3(64), 0, d.stream, d.key64.get(), size_t(1), 0ull);
-    hipLaunchKernelGGL(dev::k_broadphase_pairs, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.rods(),
+    KHS_LAUNCH(d, dev::k_fill_u64, dim3(1), dim3(64), 0, d.stream, d.key64.get(), size_t(1), 0ull);
+    KHS_LAUNCH(d, dev::k_broadphase_pairs, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.rods(),
                        d.collider(), p, d.q.get(), out.get(), (unsigned long long)capacity, d.key64.get());
     HIP_TRY(hipGetLastError());
     unsigned long long total = 0;
@@ -2177,8 +2213,8 @@ KhsResult Solver::broadphase_pairs_no_alloc(uint32_t *pairs, uint64_t capacity, 
     uint32_t *staged = staging + kNoAllocHeader;
     rebuild_bvh(false);
     const StepParams p = step_params(1.0);
-    hipLaunchKernelGGL(dev::k_fill_u64, dim3(1), dim3(64), 0, d.stream, counter, size_t(1), 0ull);
-    hipLaunchKernelGGL(dev::k_broadphase_pairs, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.rods(),
+    KHS_LAUNCH(d, dev::k_fill_u64, dim3(1), dim3(64), 0, d.stream, counter, size_t(1), 0ull);
+    KHS_LAUNCH(d, dev::k_broadphase_pairs, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.rods(),
                        d.collider(), p, d.q.get(), staged, (unsigned long long)capacity, counter);
     HIP_TRY(hipGetLastError());
     unsigned long long total = 0;
@@ -2221,8 +2257,8 @@ KhsResult Solver::check_bvh(uint64_t *violations)
     rebuild_bvh(false);
     Buffer<unsigned long long> out;
     out.allocate(6);
-    hipLaunchKernelGGL(dev::k_fill_u64, dim3(1), dim3(64), 0, d.stream, out.get(), size_t(6), 0ull);
-    hipLaunchKernelGGL(dev::k_check_bvh, dim3(grid_for(d.T, kBlock)), dim3(kBlock), 0, d.stream, d.T,
+    KHS_LAUNCH(d, dev::k_fill_u64, dim3(1), dim3(64), 0, d.stream, out.get(), size_t(6), 0ull);
+    KHS_LAUNCH(d, dev::k_check_bvh, dim3(grid_for(d.T, kBlock)), dim3(kBlock), 0, d.stream, d.T,
                        d.keys_sorted.get(), d.ids_sorted.get(), d.tri_box.get(), d.parent_internal.get(),
                        d.parent_leaf.get(), d.bvh.get(), out.get());
     HIP_TRY(hipGetLastError());
@@ -2366,7 +2402,7 @@ void Solver::apply_strand_materials()
         d.strand_density.upload(density.data(), density.size());
     }
     const MassDensity rho = mass_density(material_);
-    hipLaunchKernelGGL(dev::k_model_masses, dim3(grid_for(d.S, kBlock)), dim3(kBlock), 0, d.stream, d.rods(),
+    KHS_LAUNCH(d, dev::k_model_masses, dim3(grid_for(d.S, kBlock)), dim3(kBlock), 0, d.stream, d.rods(),
                        (const double *)d.rest.get(), rho.translational, rho.rotational,
                        d.strand_density.size() ? (const double *)d.strand_density.get() : nullptr, d.mass.get(),
                        d.rmass.get());
@@ -2443,11 +2479,11 @@ KhsResult Solver::step_bound(const double *direction, uint64_t count, double *al
     refresh_contact_cache(dev::kAtState, true, true);
     const StepParams cp = step_params(1.0);
     refresh_contact_cache(dev::kSwept, false);
-    hipLaunchKernelGGL(dev::k_list_ccd, dim3(kPairBlocks), dim3(kBlock), 0, d.stream, d.flags.get(), d.collider(),
+    KHS_LAUNCH(d, dev::k_list_ccd, dim3(kPairBlocks), dim3(kBlock), 0, d.stream, d.flags.get(), d.collider(),
                        d.cache(), cp, d.q.get(), d.dir.get(), d.list_alpha.get(), d.ccd_bound.get(), int(d.ccd_slot),
                        (const unsigned *)nullptr);
     d.ccd_slot ^= 1u;
-    hipLaunchKernelGGL(dev::k_ccd_slope, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.rods(), d.collider(),
+    KHS_LAUNCH(d, dev::k_ccd_slope, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.rods(), d.collider(),
                        d.cache(), 1, cp, d.q.get(), d.grad.get(), d.dir.get(), d.partials.get());
     HIP_TRY(hipGetLastError());
     d.reduce_async(d.node_blocks(), DeviceState::kCcdSlot, dev::ConvergenceTest{0.0, 0.0, 0.0, nullptr},
@@ -2502,6 +2538,14 @@ KhsResult Solver::direction_counts(uint64_t *block, uint64_t *steepest) const
     if (!block || !steepest) return KHS_ERROR_INVALID_ARGUMENT;
     *block = block_directions_;
     *steepest = steepest_directions_;
+    return KHS_OK;
+}
+
+KhsResult Solver::launch_counts(uint64_t *dispatches, uint64_t *host_waits) const
+{
+    if (!dispatches || !host_waits) return KHS_ERROR_INVALID_ARGUMENT;
+    *dispatches = built_ ? dev_->launch_count : 0;
+    *host_waits = built_ ? dev_->wait_count : 0;
     return KHS_OK;
 }
 

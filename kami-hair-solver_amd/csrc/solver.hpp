@@ -83,6 +83,7 @@ public:
     KhsResult sweep_check(uint32_t *feasible, KhsFailureDiagnostics *diagnostics);
     KhsResult set_direction_mode(uint32_t mode);
     KhsResult direction_counts(uint64_t *block, uint64_t *steepest) const;
+    KhsResult launch_counts(uint64_t *dispatches, uint64_t *host_waits) const;
 
     const std::string &last_error() const { return error_; }
     void set_error(std::string message) { error_ = std::move(message); }

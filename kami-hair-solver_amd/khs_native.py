@@ -319,6 +319,7 @@ EXT_FUNCTIONS = {
     "khsxSetDirectionMode": (C.c_int, [_P, C.c_uint32]),
     "khsxSetSolveAlgorithm": (C.c_int, [_P, C.c_uint32]),
     "khsxGetDirectionCounts": (C.c_int, [_P, _D(C.c_uint64), _D(C.c_uint64)]),
+    "khsxGetLaunchCounts": (C.c_int, [_P, _D(C.c_uint64), _D(C.c_uint64)]),
 }
 
 
@@ -627,6 +628,12 @@ class HairSolver:
 
     def set_direction_mode(self, mode: int) -> int:
         return self.lib.khsxSetDirectionMode(self.handle, mode)
+
+    def launch_counts(self):
+        """(dispatches, host waits) enqueued on the solver's stream so far."""
+        dispatches, waits = C.c_uint64(0), C.c_uint64(0)
+        self.check(self.lib.khsxGetLaunchCounts(self.handle, C.byref(dispatches), C.byref(waits)))
+        return dispatches.value, waits.value
 
     def direction_counts(self):
         block, steepest = C.c_uint64(0), C.c_uint64(0)

@@ -2829,4 +2829,11 @@ KhsResult khsxGetDirectionCounts(const KhsSolver *s, uint64_t *block, uint64_t *
     return s ? s->impl.direction_counts(block, steepest) : KHS_ERROR_INVALID_ARGUMENT;
 }
 
+KhsResult khsxGetLaunchCounts(const KhsSolver *s, uint64_t *dispatches, uint64_t *host_waits)
+{
+    if (!s || !dispatches || !host_waits) return KHS_ERROR_INVALID_ARGUMENT;
+    *dispatches = *host_waits = 0;  // no device
+    return KHS_OK;
+}
+
 }  // extern "C"
