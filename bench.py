@@ -122,13 +122,15 @@ def cpu_baseline(scene, strands, frames, threads):
     }
 
 
-def run_batched(lib, scene, jobs, warmup, frames):
+def run_batched(lib, scene, jobs, warmup, frames, algorithm):
     """`jobs` copies of the workload stepped concurrently, each solver on
     its own host thread and HIP stream (the library call releases the
     interpreter lock). Returns aggregate node-substeps/s over the timed
     frames and the per-job substep latencies."""
     import threading
     solvers = [setup_animation(lib, scene, warmup + frames + 1, **LIFTED) for _ in range(jobs)]
+    for s in solvers:
+        s.check(s.set_solve_algorithm(algorithm))
     for s in solvers:
         for f in range(1, warmup + 1):
             s.check(s.step_frame(f, scene.frame_dt))
@@ -293,7 +295,7 @@ def run_solve_algorithm(lib, scene, warmup, frames, algorithm, reference_positio
     }
 
 
-def run_uneven(lib, warmup, frames):
+def run_uneven(lib, warmup, frames, algorithm):
     """The uneven-length groom (50,000 strands of 4..256 nodes, about 6.5M
     nodes, per-strand E and radius) on the rotating head + torso body, plain
     FP64 to the 1e-6 residual: node-substeps/s over `frames` frames after
@@ -302,6 +304,7 @@ def run_uneven(lib, warmup, frames):
     t = time.perf_counter()
     scene, young, radius = grooms.uneven_scene()
     s = grooms.build_solver(lib, scene, **LIFTED)
+    s.check(s.set_solve_algorithm(algorithm))
     s.check(s.set_convergence_mode(1))
     s.check(s.set_strand_materials(young, radius))
     total = warmup + frames + 1
@@ -578,13 +581,16 @@ def main():
             f"{result['mixed_precision']['value']:.4g} node-substeps/s")
     if args.batch_jobs > 0:
         t = time.perf_counter()
-        result["batched"] = run_batched(lib, scene, args.batch_jobs, args.warmup, args.batch_frames or args.steps)
+        result["batched"] = run_batched(lib, scene, args.batch_jobs, args.warmup, args.batch_frames or args.steps,
+                                        args.solve_algorithm)
+        result["batched"]["solve_algorithm"] = SOLVE_ALGORITHMS[args.solve_algorithm]
         log(f"[bench] batched form: {args.batch_jobs} jobs in {time.perf_counter() - t:.1f} s, "
             f"{result['batched']['value']:.4g} node-substeps/s")
     if args.uneven_frames > 0:
         t = time.perf_counter()
         try:
-            result["uneven"] = run_uneven(lib, 1, args.uneven_frames)
+            result["uneven"] = run_uneven(lib, 1, args.uneven_frames, args.solve_algorithm)
+            result["uneven"]["solve_algorithm"] = SOLVE_ALGORITHMS[args.solve_algorithm]
             log(f"[bench] uneven groom in {time.perf_counter() - t:.1f} s, "
                 f"{result['uneven']['value']:.4g} node-substeps/s")
         except kn.SolverError as e:  # reported in the line; the headline stands
@@ -597,12 +603,14 @@ def main():
                                   frames=(10, 30))
         log(f"[bench] serving: {len(jobs)} jobs generated in {time.perf_counter() - t:.1f} s")
         t = time.perf_counter()
-        served = serve.BakeServer(lib, slots=args.serve_slots, hbm_budget=200e9).run(jobs)
+        served = serve.BakeServer(lib, slots=args.serve_slots, hbm_budget=200e9,
+                                  solve_algorithm=args.solve_algorithm).run(jobs)
         served["arrivals"] = (f"Poisson, {args.serve_rate}/s over {args.serve_window} s; strands log-uniform "
                               f"500..{args.serve_max_strands}, nodes lognormal median 40 (8..80), frames 10..30, "
                               f"each on its own rotating ellipsoid (icosphere subdivision 5)")
         served["form"] = ("continuous admission by slots and a 200 GB HBM budget; each job resident with a "
                           "10-frame rollback ring on its own stream; plain FP64")
+        served["solve_algorithm"] = SOLVE_ALGORITHMS[args.solve_algorithm]
         result["served"] = served
         log(f"[bench] served {served['completed']} jobs in {time.perf_counter() - t:.1f} s, "
             f"{served['value']:.4g} node-substeps/s")

@@ -152,11 +152,12 @@ def job_bytes(job: Job) -> int:
 # ----------------------------------------------------------------- server
 class BakeServer:
     def __init__(self, lib, *, slots: int = 4, hbm_budget: float = 200e9, precision_mode: int = 0,
-                 keep_positions: bool = False, **desc):
+                 solve_algorithm: int = 0, keep_positions: bool = False, **desc):
         self.lib = lib
         self.slots = slots
         self.budget = int(hbm_budget)
         self.precision_mode = precision_mode
+        self.solve_algorithm = solve_algorithm
         self.keep_positions = keep_positions
         self.desc = {**LIFTED, **desc}
         self._lock = threading.Condition()
@@ -174,6 +175,8 @@ class BakeServer:
             s.check(s.set_convergence_mode(1))
             if self.precision_mode:
                 s.check(s.set_precision_mode(self.precision_mode))
+            if self.solve_algorithm:
+                s.check(s.set_solve_algorithm(self.solve_algorithm))
             s.check(s.allocate_animation(job.frames + 1))
             for f in range(job.frames + 1):
                 s.check(s.set_root_frame(f, job.scene.frame_points(f)))
