@@ -1928,6 +1928,8 @@ __global__ __launch_bounds__(kWave) void k_block_solve_twisted(RodView m, const 
 // elimination, against the reference form's inverse, its products and
 // its moves: about a third of the instructions.
 constexpr int kFusedAhead = 4;  // ring slots; even, as the lane halves alternate
+constexpr int kFusedWindow = 20;  // nodes per strand whose factors stay in LDS
+constexpr int kFusedRecord = 42;  // a node's C' (36, row-major) and e (6)
 
 // v_fmac_f64_dpp ACC, X, -Y: ACC += X(lane L of the row) * -Y
 #define KHS_FMAC_NEG(ACC, X, Y, L) "v_fmac_f64_dpp " ACC ", " X ", -" Y " row_newbcast:" L " row_mask:0xf bank_mask:0xf\n\t"
@@ -2056,6 +2058,14 @@ __global__ __launch_bounds__(kWave) void k_block_solve_fused(RodView m, const ui
         for (int r = 0; r < 6; ++r) f[r] = buffer_load(rB, base + stride * r);
     };
 
+    // The factors of each strand's last kFusedWindow nodes stay in LDS for
+    // the back substitution (slot W is where lanes with nothing to keep
+    // write); only earlier nodes' factors go through memory.
+    __shared__ double window[kSolveGroups][kFusedWindow + 1][kFusedRecord];
+    const bool ulane0 = half == 1, ulane1 = half == 0;  // U lanes at parity 0 and 1
+    const int rec_at[2] = {ulane0 ? ci : (rlane ? 36 : 41), ulane1 ? ci : (rlane ? 36 : 41)};
+    const int rec_stride[2] = {ulane0 ? 6 : (rlane ? 1 : 0), ulane1 ? 6 : (rlane ? 1 : 0)};
+
     bool ok = true;
     double col[6];
 #pragma unroll
@@ -2095,17 +2105,22 @@ __global__ __launch_bounds__(kWave) void k_block_solve_fused(RodView m, const ui
         const bool bad = !(smallest >= 1.0e-30) || !isfinite(e0);
         if (live && bad) ok = false;
         const bool keep = live && ok;
-        const uint32_t oc = (keep && half == 1 - P) ? 288 * n + 8 * ci : kNoAccess;
-        const uint32_t op = (keep && rlane) ? 48 * n : kNoAccess;
+        const bool on_chip = t + kFusedWindow >= len;  // one of the strand's last kFusedWindow nodes
+        const bool keeper = (half == 1 - P) || rlane;   // C' columns and e
+        const uint32_t oc = (keep && !on_chip && half == 1 - P) ? 288 * n + 8 * ci : kNoAccess;
+        const uint32_t op = (keep && !on_chip && rlane) ? 48 * n : kNoAccess;
+        double *rec = window[g][(keep && on_chip && keeper) ? t + kFusedWindow - len : kFusedWindow];
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
             buffer_store(rC, oc + 48 * r, col[r]);
             buffer_store(rP, op + 8 * r, col[r]);
+            rec[rec_at[P] + rec_stride[P] * r] = col[r];
         }
     };
     for (uint32_t t0 = 0; t0 < steps; t0 += kFusedAhead)
         static_for<kFusedAhead>([&](auto A) { forward(t0 + A, A); });
     if (!ok && j == 0) atomicOr(failed, 1u);
+    __syncthreads();  // the window's factors, written by other lanes, before they are read
 
     // Back substitution x_n = d'_n - C'_n x_{n+1} with d' = -e, lane r < 6
     // owning row r (as in k_block_solve).
@@ -2113,13 +2128,22 @@ __global__ __launch_bounds__(kWave) void k_block_solve_fused(RodView m, const ui
     struct BackSlot {
         double c[6], e, g;
     };
+    // the first kFusedWindow steps (each strand's last nodes) from LDS, the
+    // rest from memory (t is the same for every row, so is the choice)
     auto fetch_back = [&](BackSlot &b, uint32_t t) __attribute__((always_inline)) {
         const uint32_t n = end - 1 - t;
         const bool bl = acol && t < len;
-        const uint32_t oc = bl ? 288 * n + 48 * r : kNoAccess;
+        if (t < uint32_t(kFusedWindow)) {
+            const double *rec = window[g][kFusedWindow - 1 - t];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) b.c[k] = buffer_load(rC, oc + 8 * k);
-        b.e = buffer_load(rP, bl ? 48 * n + 8 * r : kNoAccess);
+            for (int k = 0; k < 6; ++k) b.c[k] = rec[6 * r + k];
+            b.e = rec[36 + r];
+        } else {
+            const uint32_t oc = bl ? 288 * n + 48 * r : kNoAccess;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) b.c[k] = buffer_load(rC, oc + 8 * k);
+            b.e = buffer_load(rP, bl ? 48 * n + 8 * r : kNoAccess);
+        }
         b.g = buffer_load(rG, bl ? 48 * n + 8 * r : kNoAccess);
     };
     BackSlot back[kBackAhead];
