@@ -2365,12 +2365,7 @@ KhsResult Solver::contact_list_stats(KhsxContactListStats *out)
 
 double Solver::max_radius() const
 {
-    double r = material_.radius;
-    if (!strand_r_.empty()) {
-        r = strand_r_[0];
-        for (double x : strand_r_) r = std::fmax(r, x);
-    }
-    return r;
+    return strand_r_.empty() ? material_.radius : strand_r_max_;
 }
 
 // Uploads the per-strand materials of a built solver: (E, r) of each node's
@@ -2431,6 +2426,7 @@ KhsResult Solver::set_strand_materials(const double *E, const double *r, uint32_
             }
         strand_E_.assign(E, E + count);
         strand_r_.assign(r, r + count);
+        strand_r_max_ = *std::max_element(strand_r_.begin(), strand_r_.end());
     }
     if (built_) apply_strand_materials();
     return KHS_OK;

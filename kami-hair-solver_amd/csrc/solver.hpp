@@ -152,6 +152,7 @@ private:
     uint32_t direction_mode_ = 0;
     uint64_t block_directions_ = 0, steepest_directions_ = 0;
     std::vector<double> strand_E_, strand_r_;  // per-strand materials (empty: the material's)
+    double strand_r_max_ = 0.0;                 // the largest of strand_r_
     double trust_radius_ = 0.0;
     uint32_t animation_frames_ = 0, current_frame_ = 0;
     std::vector<uint32_t> ring_frame_;  // animation frame held by each rollback slot (~0u: empty)
