@@ -71,6 +71,13 @@ struct ContactCache {
     uint32_t *pending;      // elements to gather (the swept pass)
     uint32_t *pending_count;
     const double *tri_box;  // 6 per triangle: the box at the current collider position
+    // The swept check marks every element it queues with the pass's epoch
+    // (a count of swept passes, never 0) and records the listed count; the
+    // CCD kernel after it bounds the queued elements where it gathers them,
+    // and the rest from their lists.
+    uint32_t *pending_epoch;    // per element
+    uint32_t epoch;
+    uint32_t *listed_snapshot;  // list_count when the swept check ran
 };
 
 // The Newton direction's choice on the device: the block direction if it

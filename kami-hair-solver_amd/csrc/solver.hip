@@ -47,6 +47,9 @@ const double kInf = std::numeric_limits<double>::infinity();
 // first, then run nearly alone on their SIMDs (measured 8% faster than
 // letting two wavefronts share each SIMD).
 constexpr unsigned kSolveLdsReserve = 28 * 1024;
+// Blocks of the swept step-bound kernel: its gathering blocks, then the
+// list blocks.
+constexpr unsigned kSweptBoundBlocks = khs::dev::kSweptGatherBlocks + khs::kPairBlocks;
 
 template <class T>
 class Buffer {
@@ -200,8 +203,9 @@ struct DeviceState {
     size_t sort_temp_bytes = 0;
     // per-element contact candidate lists (valid for the current substep)
     Buffer<uint32_t> cache_count, cache_tris, cache_listed, cache_list, cache_list_count, cache_pending,
-        cache_pending_count;
+        cache_pending_count, cache_pending_epoch, cache_snapshot;
     uint32_t pending_slot = 0;  // which pending counter the next swept pass uses
+    uint32_t swept_epoch = 0;   // swept passes so far (marks of the elements each one queues)
     Buffer<double> cache_box;
     double cache_skin = 0.0;
     bool cache_valid = false;
@@ -309,7 +313,8 @@ struct DeviceState {
     {
         return ContactCache{cache_skin,       cache_count.get(),      cache_tris.get(),        cache_box.get(),
                             cache_listed.get(), cache_list.get(),  cache_list_count.get(), cache_pending.get(),
-                            cache_pending_count.get() + pending_slot, tri_box_static.get()};
+                            cache_pending_count.get() + pending_slot, tri_box_static.get(),
+                            cache_pending_epoch.get(), swept_epoch, cache_snapshot.get()};
     }
 
     size_t resident_bytes() const
@@ -327,6 +332,7 @@ struct DeviceState {
         b += bvh.bytes() + parent_internal.bytes() + parent_leaf.bytes() + counters.bytes() + sort_temp.bytes() +
              partials.bytes() + reduced.bytes() + cache_count.bytes() + cache_tris.bytes() + cache_box.bytes() +
              cache_listed.bytes() + cache_list.bytes() + cache_list_count.bytes() + cache_pending.bytes() +
+             cache_pending_epoch.bytes() + cache_snapshot.bytes() +
              best_tri.bytes();
         return b;
     }
@@ -765,7 +771,7 @@ KhsResult Solver::upload_model()
     const uint32_t max_blocks = std::max({d.tiled_blocks(), d.node_blocks(), d.solve_blocks(), d.twisted_blocks()}) + 1;
     d.partials.allocate(max_blocks + 16);
     d.list_words.allocate(kPairBlocks);
-    d.list_alpha.allocate(kPairBlocks);
+    d.list_alpha.allocate(kSweptBoundBlocks);
     d.reduced.allocate(6);  // reductions, with the direction choice in slot 3 (at most one Partial)
     {
         const unsigned zero = 0;
@@ -815,6 +821,10 @@ KhsResult Solver::upload_model()
         d.cache_pending.allocate(d.N);
         d.cache_pending_count.allocate(2);
         HIP_TRY(hipMemset(d.cache_pending_count.get(), 0, d.cache_pending_count.bytes()));
+        d.cache_pending_epoch.allocate(d.N);
+        HIP_TRY(hipMemset(d.cache_pending_epoch.get(), 0, d.cache_pending_epoch.bytes()));
+        d.cache_snapshot.allocate(1);
+        d.swept_epoch = 0;
         d.cache_listed.allocate(d.N);
         d.cache_list.allocate(d.N);
         d.cache_list_count.allocate(1);
@@ -884,7 +894,6 @@ void Solver::refit_bvh(bool swept)
 // Brings the candidate lists up to date for a query pass (all of them
 // after the collider boxes changed): elements whose query box left their
 // cached box gather again.
-constexpr unsigned kGatherBlocks = 512;
 void Solver::refresh_contact_cache(int mode, bool full, bool swept_leaves, bool with_roots)
 {
     DeviceState &d = *dev_;
@@ -922,18 +931,38 @@ void Solver::refresh_contact_cache(int mode, bool full, bool swept_leaves, bool 
             d.trial_step.source = dev::kAlphaStored;
         }
         else {
-            // the swept pass queues its elements; the two pending counters
-            // alternate, each cleared by the gather before its next use
-            KHS_LAUNCH(d, dev::k_contact_check<dev::kSwept>, grid, block, 0, d.stream, d.rods(), col, cc,
+            // The swept pass queues its elements, marked with a new epoch,
+            // for the gathers of the step-bound kernel after it
+            // (swept_bound_async).
+            if (++d.swept_epoch == 0) {
+                KHS_LAUNCH(d, dev::k_fill_u32, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream,
+                                   d.cache_pending_epoch.get(), size_t(d.N), 0u);
+                d.swept_epoch = 1;
+            }
+            KHS_LAUNCH(d, dev::k_contact_check<dev::kSwept>, grid, block, 0, d.stream, d.rods(), col, d.cache(),
                                d.q.get(), d.q_base.get(), d.dir.get(), d.alpha.get(), dev::TrialStep{}, pad,
                                with_roots ? 1 : 0);
-            KHS_LAUNCH(d, dev::k_contact_gather, dim3(kGatherBlocks), block, 0, d.stream, d.flags.get(), col, cc,
-                               d.cache_pending_count.get() + (d.pending_slot ^ 1u));
-            d.pending_slot ^= 1u;
         }
     }
     HIP_TRY(hipGetLastError());
     d.cache_valid = true;
+}
+
+// The line-search step bound along dir over the listed elements: the swept
+// check, then one kernel that gathers the elements it queued (bounding each
+// where it gathers it) and bounds the others from their lists; the two
+// pending counters alternate, each cleared by the kernel before its next
+// use. Leaves kSweptBoundBlocks per-block steps in list_alpha.
+void Solver::swept_bound_async(const StepParams &cp, const unsigned *stop)
+{
+    DeviceState &d = *dev_;
+    refresh_contact_cache(dev::kSwept, false);
+    KHS_LAUNCH(d, dev::k_swept_ccd, dim3(kSweptBoundBlocks), dim3(kBlock), 0, d.stream, d.flags.get(),
+                       d.collider(), d.cache(), cp, d.q.get(), d.dir.get(), d.list_alpha.get(), d.ccd_bound.get(),
+                       int(d.ccd_slot), d.cache_pending_count.get() + (d.pending_slot ^ 1u), stop);
+    HIP_TRY(hipGetLastError());
+    d.ccd_slot ^= 1u;
+    d.pending_slot ^= 1u;
 }
 
 // Closest triangle of every listed element at q (or at alpha d + q_base
@@ -1413,19 +1442,14 @@ KhsResult Solver::optimize_substep(double dt)
         phase_.store(KHS_PHASE_CCD);
         hipEvent_t t0 = d.tic();
         const bool list_pass = d.cache_valid;
-        if (list_pass) {
-            refresh_contact_cache(dev::kSwept, false);
-            KHS_LAUNCH(d, dev::k_list_ccd, dim3(kPairBlocks), dim3(kBlock), 0, d.stream, d.flags.get(),
-                               d.collider(), d.cache(), cp, d.q.get(), d.dir.get(), d.list_alpha.get(),
-                               d.ccd_bound.get(), int(d.ccd_slot), d.skip());
-            d.ccd_slot ^= 1u;
-        }
+        if (list_pass) swept_bound_async(cp, d.skip());
         KHS_LAUNCH(d, dev::k_ccd_slope, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, d.collider(),
                            d.cache(), list_pass ? 1 : 0, cp, d.q.get(), d.grad.get(), d.dir.get(), d.partials.get());
         HIP_TRY(hipGetLastError());
         d.toc(t0, d.totals.collision_ms, d.totals.collision_launches);
         d.reduce_async(d.node_blocks(), DeviceState::kCcdSlot, dev::ConvergenceTest{0.0, 0.0, 0.0, nullptr},
-                       dev::ListPartials{nullptr, list_pass ? d.list_alpha.get() : nullptr, list_pass ? kPairBlocks : 0u});
+                       dev::ListPartials{nullptr, list_pass ? d.list_alpha.get() : nullptr,
+                                         list_pass ? kSweptBoundBlocks : 0u});
         set_trial_step(dev::kAlphaFromCcd, 0.0);
         // The first trial runs at the device's step before the host has
         // seen the bound; one read-back returns direction choice, bound and
@@ -2475,16 +2499,12 @@ KhsResult Solver::step_bound(const double *direction, uint64_t count, double *al
     rebuild_bvh(true);
     refresh_contact_cache(dev::kAtState, true, true);
     const StepParams cp = step_params(1.0);
-    refresh_contact_cache(dev::kSwept, false);
-    KHS_LAUNCH(d, dev::k_list_ccd, dim3(kPairBlocks), dim3(kBlock), 0, d.stream, d.flags.get(), d.collider(),
-                       d.cache(), cp, d.q.get(), d.dir.get(), d.list_alpha.get(), d.ccd_bound.get(), int(d.ccd_slot),
-                       (const unsigned *)nullptr);
-    d.ccd_slot ^= 1u;
+    swept_bound_async(cp, nullptr);
     KHS_LAUNCH(d, dev::k_ccd_slope, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.rods(), d.collider(),
                        d.cache(), 1, cp, d.q.get(), d.grad.get(), d.dir.get(), d.partials.get());
     HIP_TRY(hipGetLastError());
     d.reduce_async(d.node_blocks(), DeviceState::kCcdSlot, dev::ConvergenceTest{0.0, 0.0, 0.0, nullptr},
-                   dev::ListPartials{nullptr, d.list_alpha.get(), kPairBlocks});
+                   dev::ListPartials{nullptr, d.list_alpha.get(), kSweptBoundBlocks});
     *alpha = d.read_back().ccd.alpha;
     return KHS_OK;
 }

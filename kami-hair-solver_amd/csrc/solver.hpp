@@ -106,6 +106,7 @@ private:
     void refit_boxes();
     void refresh_contact_cache(int mode, bool full, bool swept_leaves = false, bool with_roots = false);
     void closest_pass(bool line_search, unsigned long long *block_candidates, bool with_roots = false);
+    void swept_bound_async(const StepParams &cp, const unsigned *stop);
     // With `stop`, the gradient's reduction also sets *stop (device) when
     // |g| <= abs_tol or |g| / initial <= rel_tol (initial < 0: this one).
     void evaluate_async(bool gradient, double dt, bool line_search, unsigned *stop = nullptr, double abs_tol = 0.0,

@@ -470,6 +470,8 @@ __global__ __launch_bounds__(kBlock) void k_contact_check(RodView m, ColliderVie
     const int lane = threadIdx.x & (kWave - 1);
     const double alpha = MODE == kAtTrial ? resolve_step(step, alpha_at) : 0.0;
     const uint32_t L = blockIdx.x * blockDim.x + threadIdx.x;
+    // (the swept check appends nothing to the listed queue)
+    if (MODE == kSwept && L == 0) *cc.listed_snapshot = *cc.list_count;
     // every lane stays to the end: the gathers use the whole wavefront
     bool need = false;
     V3 nlo = mk(0, 0, 0), nhi = nlo;
@@ -486,6 +488,7 @@ __global__ __launch_bounds__(kBlock) void k_contact_check(RodView m, ColliderVie
                 if constexpr (MODE == kSwept) {
                     store3(cc.box + 6 * size_t(L), nlo);
                     store3(cc.box + 6 * size_t(L) + 3, nhi);
+                    cc.pending_epoch[L] = cc.epoch;
                     cc.pending[atomicAdd(cc.pending_count, 1u)] = L;
                 }
             }
@@ -929,6 +932,206 @@ __global__ __launch_bounds__(kBlock) void k_list_ccd(const uint32_t *flags, Coll
             limit = fmin(limit, r);
         }
         __syncthreads();
+    }
+    __shared__ double wave_alpha[kBlock / kWave];
+#pragma unroll
+    for (int o = kWave / 2; o >= 1; o >>= 1) limit = fmin(limit, __shfl_xor(limit, o, kWave));
+    if ((threadIdx.x & (kWave - 1)) == 0) wave_alpha[threadIdx.x / kWave] = limit;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = wave_alpha[0];
+        for (int w = 1; w < kBlock / kWave; ++w) a = fmin(a, wave_alpha[w]);
+        block_alpha[blockIdx.x] = a;
+    }
+}
+
+// The swept pass's gathers and the line-search step bound in one launch.
+// The first kSweptGatherBlocks blocks take the elements the swept check
+// queued, one wavefront each: the wavefront gathers the element's list
+// (into LDS, then memory, as k_contact_gather does) and bounds the element
+// right away from that list (an overflowed list is walked by k_ccd_slope
+// after this kernel, as every overflowed list is). The other blocks bound
+// every listed element the check did not queue from its list, as
+// k_list_ccd does. A queued element's gather thus overlaps the others'
+// advancement loops instead of preceding them. Both parts take the same
+// pairs as the separate kernels (the minimum is exact) and share the
+// launch-wide bound. The gathers run also when the Newton loop has stopped,
+// so every list matches its box.
+constexpr unsigned kSweptGatherBlocks = 512;
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_swept_ccd(const uint32_t *flags, ColliderView col, ContactCache cc,
+                                                      StepParams p, const double *q, const double *dir,
+                                                      double *block_alpha, unsigned long long *bound, int slot,
+                                                      uint32_t *next_pending_count, const unsigned *stop)
+{
+    const bool stopped = stop && *stop;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        bound[slot ^ 1] = __double_as_longlong(1.0);
+        *next_pending_count = 0;
+    }
+    double limit = 1.0;
+    if (blockIdx.x < kSweptGatherBlocks) {
+        __shared__ int queues[kTilesPerBlock][kGatherQueue];
+        __shared__ uint32_t found_lists[kTilesPerBlock][kCacheCapacity];
+        const int wave_in_block = threadIdx.x / kWave;
+        volatile int *queue = queues[wave_in_block];
+        uint32_t *found = found_lists[wave_in_block];
+        uint32_t *hits = reinterpret_cast<uint32_t *>(const_cast<int *>(queue));  // after the walk
+        const int lane = threadIdx.x & (kWave - 1);
+        const unsigned long long below = __lanemask_lt();
+        const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+        const uint32_t waves = kSweptGatherBlocks * kBlock / kWave;
+        const uint32_t pending = *cc.pending_count;
+        for (uint32_t j = wave; j < pending; j += waves) {
+            const uint32_t L = cc.pending[j];
+            double *box = cc.box + 6 * size_t(L);
+            V3 lo = mk(box[0], box[1], box[2]), hi = mk(box[3], box[4], box[5]);
+            uint32_t n = wave_gather(col, lo, hi, found, kCacheCapacity, queue, false);
+            if (n > kCacheCapacity) {
+                thin_box(cc.skin, lo, hi);
+                n = wave_gather(col, lo, hi, found, kCacheCapacity, queue, false);
+                if (lane == 0) {
+                    store3(box, lo);
+                    store3(box + 3, hi);
+                }
+            }
+            if (n == 0) {
+                const double far = 15.0 * cc.skin;
+                const V3 flo = mk(lo.x - far, lo.y - far, lo.z - far), fhi = mk(hi.x + far, hi.y + far, hi.z + far);
+                uint32_t none;
+                if (wave_gather(col, flo, fhi, &none, 0, queue, true) == 0 && lane == 0) {
+                    store3(box, flo);
+                    store3(box + 3, fhi);
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            const uint32_t kept = n <= kCacheCapacity ? n : 0u;
+            uint32_t *tris = cc.tris + size_t(L) * kCacheCapacity;
+            for (uint32_t k = lane; k < kept; k += kWave) tris[k] = found[k];
+            if (lane == 0) {
+                cc.count[L] = n <= kCacheCapacity ? n : kCacheOverflow;
+                if (n > 0 && !(flags[L] & kNodeFixed) && !cc.listed[L]) {
+                    cc.listed[L] = 1;
+                    cc.list[atomicAdd(cc.list_count, 1u)] = L;
+                }
+            }
+            // the element's step bound from its new list (the swept check
+            // queues only elements with a free left node; an overflowed list
+            // is walked by k_ccd_slope, as every overflowed list is)
+            if (stopped || n == 0 || n > kCacheCapacity) continue;
+            const V3 p0 = load3(q + 6 * size_t(L)), p1 = load3(q + 6 * size_t(L + 1));
+            const V3 d0 = load3(dir + 6 * size_t(L)), d1 = load3(dir + 6 * size_t(L + 1));
+            const double speed = fmax(norm(d0), norm(d1));
+            if (!(speed >= 1e-30)) continue;
+            const double target = sweep_target_of(p, L);
+            const V3 slo = vmin(vmin(p0, p1), vmin(add(p0, d0), add(p1, d1)));
+            const V3 shi = vmax(vmax(p0, p1), vmax(add(p0, d0), add(p1, d1)));
+            double own = 1.0;
+            uint32_t taken = 0;
+            for (uint32_t k0 = 0; k0 < n; k0 += kWave) {
+                const uint32_t k = k0 + lane;
+                uint32_t t = 0;
+                bool hit = false;
+                if (k < n) {
+                    t = found[k];
+                    const double *b = cc.tri_box + 6 * size_t(t);
+                    hit = overlap(slo, shi, mk(b[0], b[1], b[2]), mk(b[3], b[4], b[5]), target);
+                }
+                const unsigned long long mask = __ballot(hit);
+                if (hit) hits[taken + __popcll(mask & below)] = t;
+                taken += __popcll(mask);
+            }
+            __builtin_amdgcn_wave_barrier();
+            for (uint32_t i = lane; i < taken; i += kWave) {
+                const uint32_t *t3 = col.triangles + 3 * size_t(hits[i]);
+                const double seen = fmin(own, __longlong_as_double(__hip_atomic_load(
+                                                  bound + slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
+                const double r = advancement_limit(p0, p1, d0, d1, speed, vertex(col, t3[0]), vertex(col, t3[1]),
+                                                   vertex(col, t3[2]), target, seen / 0.9);
+                if (r < seen) atomicMin(bound + slot, (unsigned long long)__double_as_longlong(r));
+                own = fmin(own, r);
+            }
+            limit = fmin(limit, own);
+            __builtin_amdgcn_wave_barrier();
+        }
+    } else if (!stopped) {
+        // every listed element the check did not queue, from its list
+        constexpr int G = kCcdLanes;
+        __shared__ uint32_t pair_tri[kCcdBatch * kCacheCapacity];
+        __shared__ uint8_t pair_elem[kCcdBatch * kCacheCapacity];
+        __shared__ double elem[kCcdBatch][14];  // p0, p1, d0, d1, speed, target clearance
+        __shared__ uint32_t pair_count;
+        const uint32_t cb = blockIdx.x - kSweptGatherBlocks, cblocks = gridDim.x - kSweptGatherBlocks;
+        const int lane = threadIdx.x & (G - 1), my = threadIdx.x / G;
+        const int shift = (threadIdx.x & (kWave - 1)) & ~(G - 1);
+        const unsigned long long below = (1ull << lane) - 1;
+        const uint32_t listed = *cc.listed_snapshot;
+        for (uint32_t first = cb * kCcdBatch; first < listed; first += cblocks * kCcdBatch) {
+            if (threadIdx.x == 0) pair_count = 0;
+            __syncthreads();
+            const uint32_t j = first + my;
+            uint32_t L = 0, c = 0;
+            if (j < listed) {
+                L = cc.list[j];
+                if (cc.pending_epoch[L] != cc.epoch) {
+                    c = cc.count[L];
+                    if (c == kCacheOverflow || (flags[L] & kNodeFixed)) c = 0;
+                }
+            }
+            V3 p0 = mk(0, 0, 0), p1 = p0, d0 = p0, d1 = p0;
+            double speed = 0.0;
+            const double target = sweep_target_of(p, L);
+            if (c > 0) {
+                p0 = load3(q + 6 * size_t(L));
+                p1 = load3(q + 6 * size_t(L + 1));
+                d0 = load3(dir + 6 * size_t(L));
+                d1 = load3(dir + 6 * size_t(L + 1));
+                speed = fmax(norm(d0), norm(d1));
+                if (speed < 1e-30) c = 0;
+            }
+            if (c > 0 && lane == 0) {
+                const double v[14] = {p0.x, p0.y, p0.z, p1.x, p1.y, p1.z, d0.x, d0.y, d0.z, d1.x, d1.y, d1.z, speed,
+                                      target};
+#pragma unroll
+                for (int k = 0; k < 14; ++k) elem[my][k] = v[k];
+            }
+            const V3 lo = vmin(vmin(p0, p1), vmin(add(p0, d0), add(p1, d1)));
+            const V3 hi = vmax(vmax(p0, p1), vmax(add(p0, d0), add(p1, d1)));
+            const uint32_t *tris = cc.tris + size_t(L) * kCacheCapacity;
+            for (uint32_t k0 = 0; k0 < c; k0 += G) {
+                const uint32_t k = k0 + lane;
+                uint32_t t = 0;
+                bool hit = false;
+                if (k < c) {
+                    t = tris[k];
+                    const double *b = cc.tri_box + 6 * size_t(t);
+                    hit = overlap(lo, hi, mk(b[0], b[1], b[2]), mk(b[3], b[4], b[5]), target);
+                }
+                const unsigned long long mask = (__ballot(hit) >> shift) & ((G == 64) ? ~0ull : ((1ull << G) - 1));
+                uint32_t base = 0;
+                if (lane == 0 && mask) base = atomicAdd(&pair_count, uint32_t(__popcll(mask)));
+                base = __shfl(base, shift, kWave);
+                if (hit) {
+                    const uint32_t at = base + __popcll(mask & below);
+                    pair_tri[at] = t;
+                    pair_elem[at] = uint8_t(my);
+                }
+            }
+            __syncthreads();
+            const uint32_t pairs = pair_count;
+            for (uint32_t i = threadIdx.x; i < pairs; i += kBlock) {
+                const uint32_t t = pair_tri[i];
+                const double *e = elem[pair_elem[i]];
+                const uint32_t *t3 = col.triangles + 3 * size_t(t);
+                const double seen = fmin(limit, __longlong_as_double(__hip_atomic_load(
+                                                    bound + slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
+                const double r = advancement_limit(mk(e[0], e[1], e[2]), mk(e[3], e[4], e[5]), mk(e[6], e[7], e[8]),
+                                                   mk(e[9], e[10], e[11]), e[12], vertex(col, t3[0]),
+                                                   vertex(col, t3[1]), vertex(col, t3[2]), e[13], seen / 0.9);
+                if (r < seen) atomicMin(bound + slot, (unsigned long long)__double_as_longlong(r));
+                limit = fmin(limit, r);
+            }
+            __syncthreads();
+        }
     }
     __shared__ double wave_alpha[kBlock / kWave];
 #pragma unroll
