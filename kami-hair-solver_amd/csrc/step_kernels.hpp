@@ -458,7 +458,8 @@ __device__ __forceinline__ double resolve_step(const TrialStep &s, double *alpha
 // reader of their lists), and keep lists but stay off the listed queue.
 // The swept pass (boxes over a whole step, where many neighbouring elements
 // leave their boxes at once) stores the new box and queues the element for
-// k_contact_gather instead, which spreads the gathers over all wavefronts.
+// the step-bound kernel after it (k_swept_ccd) instead, which spreads the
+// gathers over its wavefronts.
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_contact_check(RodView m, ColliderView col, ContactCache cc,
                                                           const double *q, const double *q_base, const double *dir,
@@ -524,53 +525,6 @@ __global__ __launch_bounds__(kBlock) void k_contact_check(RodView m, ColliderVie
             if (n > 0 && !(m.flags[E] & kNodeFixed) && !cc.listed[E]) {
                 cc.listed[E] = 1;
                 cc.list[atomicAdd(cc.list_count, 1u)] = E;
-            }
-        }
-    }
-}
-
-// Second half of the swept pass: one wavefront per queued element gathers
-// the triangles overlapping its new box (stored by the check); an empty
-// result keeps a box fifteen skins wider still when that is empty too. It also
-// clears the other pending counter, which the next swept pass appends to.
-__global__ __launch_bounds__(kBlock) void k_contact_gather(const uint32_t *flags, ColliderView col, ContactCache cc,
-                                                           uint32_t *next_count)
-{
-    __shared__ int queues[kTilesPerBlock][kGatherQueue];
-    volatile int *queue = queues[threadIdx.x / kWave];
-    const int lane = threadIdx.x & (kWave - 1);
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-    const uint32_t waves = gridDim.x * blockDim.x / kWave;
-    const uint32_t pending = *cc.pending_count;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *next_count = 0;
-    for (uint32_t j = wave; j < pending; j += waves) {
-        const uint32_t L = cc.pending[j];
-        double *box = cc.box + 6 * size_t(L);
-        V3 lo = mk(box[0], box[1], box[2]), hi = mk(box[3], box[4], box[5]);
-        uint32_t *tris = cc.tris + size_t(L) * kCacheCapacity;
-        uint32_t n = wave_gather(col, lo, hi, tris, kCacheCapacity, queue, false);
-        if (n > kCacheCapacity) {
-            thin_box(cc.skin, lo, hi);
-            n = wave_gather(col, lo, hi, tris, kCacheCapacity, queue, false);
-            if (lane == 0) {
-                store3(box, lo);
-                store3(box + 3, hi);
-            }
-        }
-        if (n == 0) {
-            const double far = 15.0 * cc.skin;
-            const V3 flo = mk(lo.x - far, lo.y - far, lo.z - far), fhi = mk(hi.x + far, hi.y + far, hi.z + far);
-            uint32_t none;
-            if (wave_gather(col, flo, fhi, &none, 0, queue, true) == 0 && lane == 0) {
-                store3(box, flo);
-                store3(box + 3, fhi);
-            }
-        }
-        if (lane == 0) {
-            cc.count[L] = n <= kCacheCapacity ? n : kCacheOverflow;
-            if (n > 0 && !(flags[L] & kNodeFixed) && !cc.listed[L]) {
-                cc.listed[L] = 1;
-                cc.list[atomicAdd(cc.list_count, 1u)] = L;
             }
         }
     }
@@ -838,125 +792,31 @@ __device__ inline double advancement_limit(V3 p0, V3 p1, V3 d0, V3 d1, double sp
     return alpha;
 }
 
-// Line-search step bound over the listed elements with a free left node.
-// A block takes the listed elements kBlock / kCcdLanes at a time: each
-// group of kCcdLanes lanes box-tests one element's list against its
-// swept segment box and appends the overlapping (element, triangle) pairs
-// to the block's LDS queue; then every lane of the block takes pairs from
-// the queue, so an element with many candidates spreads over the block.
+constexpr int kCcdLanes = 32;
+constexpr int kCcdBatch = kBlock / kCcdLanes;
+// The swept pass's gathers and the line-search step bound in one launch.
+// The first kSweptGatherBlocks blocks take the elements the swept check
+// queued, one wavefront each: the wavefront gathers the triangles
+// overlapping the element's new box into LDS (an empty result keeps a box
+// fifteen skins wider still when that is empty too; a list over capacity
+// is gathered again for the thin box), stores the list, and bounds the
+// element right away from it (an overflowed list is walked by k_ccd_slope
+// after this kernel, as every overflowed list is). The other blocks bound
+// every listed element the check did not queue from its list: kBlock /
+// kCcdLanes elements at a time, each group of kCcdLanes lanes box-tests one
+// element's list against its swept segment box and appends the overlapping
+// (element, triangle) pairs to the block's LDS queue, and every lane of the
+// block takes pairs from the queue, so an element with many candidates
+// spreads over the block. A queued element's gather thus overlaps the
+// others' advancement loops instead of preceding them.
 // Only the minimum step matters, so each advancement loop stops once it
 // cannot go below the smallest step found so far: the lane's own, or the
 // launch-wide one in `bound[slot]` (bits of a non-negative double, lowered
 // with atomicMin and read without ordering; any value read is some
-// candidate's step or 1, so the minimum is unchanged). The kernel resets
-// the other slot to 1 for the next launch.
-constexpr int kCcdLanes = 32;
-constexpr int kCcdBatch = kBlock / kCcdLanes;
-__global__ __launch_bounds__(kBlock) void k_list_ccd(const uint32_t *flags, ColliderView col, ContactCache cc,
-                                                     StepParams p, const double *q, const double *dir,
-                                                     double *block_alpha, unsigned long long *bound, int slot,
-                                                     const unsigned *stop)
-{
-    constexpr int G = kCcdLanes;
-    __shared__ uint32_t pair_tri[kCcdBatch * kCacheCapacity];
-    __shared__ uint8_t pair_elem[kCcdBatch * kCacheCapacity];
-    __shared__ double elem[kCcdBatch][14];  // p0, p1, d0, d1, speed, target clearance
-    __shared__ uint32_t pair_count;
-    const int lane = threadIdx.x & (G - 1), my = threadIdx.x / G;
-    const int shift = (threadIdx.x & (kWave - 1)) & ~(G - 1);
-    const unsigned long long below = (1ull << lane) - 1;
-    const uint32_t listed = *cc.list_count;
-    if (blockIdx.x == 0 && threadIdx.x == 0) bound[slot ^ 1] = __double_as_longlong(1.0);
-    if (stop && *stop) return;
-    double limit = 1.0;
-    for (uint32_t first = blockIdx.x * kCcdBatch; first < listed; first += gridDim.x * kCcdBatch) {
-        if (threadIdx.x == 0) pair_count = 0;
-        __syncthreads();
-        const uint32_t j = first + my;
-        uint32_t L = 0, c = 0;
-        if (j < listed) {
-            L = cc.list[j];
-            c = cc.count[L];
-            if (c == kCacheOverflow || (flags[L] & kNodeFixed)) c = 0;
-        }
-        V3 p0 = mk(0, 0, 0), p1 = p0, d0 = p0, d1 = p0;
-        double speed = 0.0;
-        const double target = sweep_target_of(p, L);
-        if (c > 0) {
-            p0 = load3(q + 6 * size_t(L));
-            p1 = load3(q + 6 * size_t(L + 1));
-            d0 = load3(dir + 6 * size_t(L));
-            d1 = load3(dir + 6 * size_t(L + 1));
-            speed = fmax(norm(d0), norm(d1));
-            if (speed < 1e-30) c = 0;
-        }
-        if (c > 0 && lane == 0) {
-            const double v[14] = {p0.x, p0.y, p0.z, p1.x, p1.y, p1.z, d0.x, d0.y, d0.z, d1.x, d1.y, d1.z, speed, target};
-#pragma unroll
-            for (int k = 0; k < 14; ++k) elem[my][k] = v[k];
-        }
-        const V3 lo = vmin(vmin(p0, p1), vmin(add(p0, d0), add(p1, d1)));
-        const V3 hi = vmax(vmax(p0, p1), vmax(add(p0, d0), add(p1, d1)));
-        const uint32_t *tris = cc.tris + size_t(L) * kCacheCapacity;
-        for (uint32_t k0 = 0; k0 < c; k0 += G) {
-            const uint32_t k = k0 + lane;
-            uint32_t t = 0;
-            bool hit = false;
-            if (k < c) {
-                t = tris[k];
-                const double *b = cc.tri_box + 6 * size_t(t);
-                hit = overlap(lo, hi, mk(b[0], b[1], b[2]), mk(b[3], b[4], b[5]), target);
-            }
-            const unsigned long long mask = (__ballot(hit) >> shift) & ((G == 64) ? ~0ull : ((1ull << G) - 1));
-            uint32_t base = 0;
-            if (lane == 0 && mask) base = atomicAdd(&pair_count, uint32_t(__popcll(mask)));
-            base = __shfl(base, shift, kWave);
-            if (hit) {
-                const uint32_t at = base + __popcll(mask & below);
-                pair_tri[at] = t;
-                pair_elem[at] = uint8_t(my);
-            }
-        }
-        __syncthreads();
-        const uint32_t pairs = pair_count;
-        for (uint32_t i = threadIdx.x; i < pairs; i += kBlock) {
-            const uint32_t t = pair_tri[i];
-            const double *e = elem[pair_elem[i]];
-            const uint32_t *t3 = col.triangles + 3 * size_t(t);
-            const double seen = fmin(limit, __longlong_as_double(__hip_atomic_load(
-                                                bound + slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
-            const double r = advancement_limit(mk(e[0], e[1], e[2]), mk(e[3], e[4], e[5]), mk(e[6], e[7], e[8]),
-                                               mk(e[9], e[10], e[11]), e[12], vertex(col, t3[0]), vertex(col, t3[1]),
-                                               vertex(col, t3[2]), e[13], seen / 0.9);
-            if (r < seen) atomicMin(bound + slot, (unsigned long long)__double_as_longlong(r));
-            limit = fmin(limit, r);
-        }
-        __syncthreads();
-    }
-    __shared__ double wave_alpha[kBlock / kWave];
-#pragma unroll
-    for (int o = kWave / 2; o >= 1; o >>= 1) limit = fmin(limit, __shfl_xor(limit, o, kWave));
-    if ((threadIdx.x & (kWave - 1)) == 0) wave_alpha[threadIdx.x / kWave] = limit;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double a = wave_alpha[0];
-        for (int w = 1; w < kBlock / kWave; ++w) a = fmin(a, wave_alpha[w]);
-        block_alpha[blockIdx.x] = a;
-    }
-}
-
-// The swept pass's gathers and the line-search step bound in one launch.
-// The first kSweptGatherBlocks blocks take the elements the swept check
-// queued, one wavefront each: the wavefront gathers the element's list
-// (into LDS, then memory, as k_contact_gather does) and bounds the element
-// right away from that list (an overflowed list is walked by k_ccd_slope
-// after this kernel, as every overflowed list is). The other blocks bound
-// every listed element the check did not queue from its list, as
-// k_list_ccd does. A queued element's gather thus overlaps the others'
-// advancement loops instead of preceding them. Both parts take the same
-// pairs as the separate kernels (the minimum is exact) and share the
-// launch-wide bound. The gathers run also when the Newton loop has stopped,
-// so every list matches its box.
+// candidate's step or 1, so the minimum is unchanged). Block 0 resets the
+// other slot to 1, and the other pending counter to 0, for the next pass.
+// The gathers run also when the Newton loop has stopped, so every list
+// matches its box.
 constexpr unsigned kSweptGatherBlocks = 512;
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_swept_ccd(const uint32_t *flags, ColliderView col, ContactCache cc,
                                                       StepParams p, const double *q, const double *dir,
