@@ -46,7 +46,7 @@ struct ColliderView {
     const double *previous;
     const uint32_t *triangles;
     const BvhNode *bvh;
-    const double *leaf_box;  // 6 per triangle at `current`, or null: leaves use the tree's boxes
+    const double *leaf_box;  // triangle records at `current` (kTriRecord each), or null: leaves use the tree's boxes
 };
 
 // Per-element lists of the collider triangles whose leaf box overlaps a
@@ -70,7 +70,7 @@ struct ContactCache {
     uint32_t *list_count;
     uint32_t *pending;      // elements to gather (the swept pass)
     uint32_t *pending_count;
-    const double *tri_box;  // 6 per triangle: the box at the current collider position
+    const double *tri_box;  // triangle records (kTriRecord each): box and vertices at the current position
     // The swept check marks every element it queues with the pass's epoch
     // (a count of swept passes, never 0) and records the listed count; the
     // CCD kernel after it bounds the queued elements where it gathers them,

@@ -18,6 +18,14 @@ struct alignas(16) BvhNode {
     int range[2];    // first and last sorted leaf below the node
 };
 
+// Per-triangle record at the collider's current position: the triangle's
+// box (lo, hi), its three vertices and one unused word, 128 bytes, one cache
+// line. A query that has just tested a leaf's box reads the triangle's
+// vertices from the line it already fetched, not through the index and
+// vertex arrays (two more dependent loads).
+constexpr int kTriRecord = 16;
+constexpr int kRecordVertices = 6;  // offset of vertex 0
+
 namespace dev {
 
 constexpr int kBvhStack = 64;
@@ -26,10 +34,11 @@ constexpr int kBvhStack = 64;
 // below 64, so the traversal stack never overflows.
 
 // Calls f(triangle) for every triangle whose box overlaps [lo, hi] grown by
-// pad, until f returns false. With leaf_box (6 doubles per triangle) a leaf
-// is tested against its own entry there instead of the box stored in the
-// tree, which must contain it: a tree refit with boxes swept over a collider
-// motion then answers queries on the collider's current position exactly.
+// pad, until f returns false. With leaf_box (the triangle records above,
+// kTriRecord doubles each, box first) a leaf is tested against its record's
+// box instead of the box stored in the tree, which must contain it: a tree
+// refit with boxes swept over a collider motion then answers queries on the
+// collider's current position exactly.
 template <class F>
 __device__ inline void bvh_visit(const BvhNode *nodes, int internal_count, const double *leaf_box, V3 lo, V3 hi,
                                  double pad, F &&f)
@@ -51,7 +60,7 @@ __device__ inline void bvh_visit(const BvhNode *nodes, int internal_count, const
             if (ch < 0) {
                 const uint32_t t = uint32_t(-1 - ch);
                 if (leaf_box) {
-                    const double *b = leaf_box + 6 * size_t(t);
+                    const double *b = leaf_box + kTriRecord * size_t(t);
                     if (!overlap(lo, hi, mk(b[0], b[1], b[2]), mk(b[3], b[4], b[5]), pad)) continue;
                 }
                 if (!f(t)) return;

@@ -16,6 +16,19 @@ __device__ __forceinline__ V3 vertex_prev(const ColliderView &c, uint32_t i)
 {
     return load3(c.previous + 3 * size_t(i));
 }
+// Vertex k (0, 1, 2) of triangle t from its record (kTriRecord doubles per
+// triangle, the collider's current position).
+__device__ __forceinline__ V3 record_vertex(const double *records, uint32_t t, int k)
+{
+    return load3(records + kTriRecord * size_t(t) + kRecordVertices + 3 * k);
+}
+// Vertex k of triangle t at the current position: from the triangle's
+// record when the view carries records, else through the index array (the
+// same values).
+__device__ __forceinline__ V3 triangle_vertex(const ColliderView &c, uint32_t t, int k)
+{
+    return c.leaf_box ? record_vertex(c.leaf_box, t, k) : vertex(c, c.triangles[3 * size_t(t) + k]);
+}
 
 // ------------------------------------------------------------------ LBVH
 __device__ __forceinline__ uint32_t spread_bits(uint32_t v)
@@ -27,10 +40,11 @@ __device__ __forceinline__ uint32_t spread_bits(uint32_t v)
     return v;
 }
 
-// Triangle boxes (static, or swept over previous and current positions)
+// Triangle boxes (static, or swept over previous and current positions),
+// the triangle records at the current position (when `records` is given)
 // and per-block centroid bounds.
 __global__ __launch_bounds__(256) void k_triangle_boxes(ColliderView col, int swept, double *tri_box,
-                                                           double *static_box, double *bound_partials)
+                                                           double *records, double *bound_partials)
 {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     __shared__ double lo_s[256][3], hi_s[256][3];
@@ -39,16 +53,21 @@ __global__ __launch_bounds__(256) void k_triangle_boxes(ColliderView col, int sw
         const uint32_t *t3 = col.triangles + 3 * size_t(t);
         V3 lo = mk(1e300, 1e300, 1e300), hi = mk(-1e300, -1e300, -1e300);
         V3 centroid = mk(0, 0, 0);
+        V3 v[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const V3 v = vertex(col, t3[c]);
-            lo = vmin(lo, v);
-            hi = vmax(hi, v);
-            centroid = add(centroid, v);
+            v[c] = vertex(col, t3[c]);
+            lo = vmin(lo, v[c]);
+            hi = vmax(hi, v[c]);
+            centroid = add(centroid, v[c]);
         }
-        if (static_box) {
-            store3(static_box + 6 * size_t(t), lo);
-            store3(static_box + 6 * size_t(t) + 3, hi);
+        if (records) {
+            double *r = records + kTriRecord * size_t(t);
+            store3(r, lo);
+            store3(r + 3, hi);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) store3(r + kRecordVertices + 3 * c, v[c]);
+            r[kTriRecord - 1] = 0.0;
         }
         if (swept) {
 #pragma unroll

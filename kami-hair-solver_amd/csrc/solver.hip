@@ -796,7 +796,7 @@ KhsResult Solver::upload_model()
         d.tris.upload(collider_.triangles.data(), collider_.triangles.size());
         const size_t T = d.T;
         d.tri_box.allocate(6 * T);
-        d.tri_box_static.allocate(6 * T);
+        d.tri_box_static.allocate(kTriRecord * T);  // the triangle records
         d.bound_partials.allocate(6 * size_t(grid_for(T, kBlock)));
         d.bounds.allocate(6);
         d.keys.allocate(T);

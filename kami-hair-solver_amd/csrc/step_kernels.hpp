@@ -140,26 +140,49 @@ struct ListPartials {
 
 // Folds per-block partials in index order (one block of kBlock threads),
 // and the list passes' per-block words, then applies the convergence test to
-// the result when one is given.
+// the result when one is given. Thread t folds partials t, t + kReduceBlock,
+// ... in that order; each takes kReduceBatch of its partials (and of its
+// list words) from memory before folding any, so their loads overlap
+// instead of each waiting for the one before.
 constexpr int kReduceBlock = 256;
+constexpr int kReduceBatch = 4;
 __global__ __launch_bounds__(kReduceBlock) void k_reduce(const Partial *partials, uint32_t count, Partial *out,
                                                          Partial *mirror, ConvergenceTest test, ListPartials lists)
 {
     Partial acc = empty_partial();
-    for (uint32_t i = threadIdx.x; i < lists.count; i += kReduceBlock) {
-        if (lists.candidates) acc.candidates += lists.candidates[i];
-        if (lists.alpha) acc.alpha = fmin(acc.alpha, lists.alpha[i]);
+    constexpr uint32_t kStride = kReduceBlock, kSpan = kReduceBlock * kReduceBatch;
+    for (uint32_t i0 = threadIdx.x; i0 < lists.count; i0 += kSpan) {
+        unsigned long long c[kReduceBatch];
+        double a[kReduceBatch];
+#pragma unroll
+        for (int b = 0; b < kReduceBatch; ++b) {
+            const uint32_t i = i0 + b * kStride;
+            c[b] = (lists.candidates && i < lists.count) ? lists.candidates[i] : 0ull;
+            a[b] = (lists.alpha && i < lists.count) ? lists.alpha[i] : 1.0;
+        }
+#pragma unroll
+        for (int b = 0; b < kReduceBatch; ++b) {
+            acc.candidates += c[b];
+            acc.alpha = fmin(acc.alpha, a[b]);
+        }
     }
-    for (uint32_t i = threadIdx.x; i < count; i += kReduceBlock) {
-        const Partial &o = partials[i];
-        for (int k = 0; k < 8; ++k) acc.sum[k] += o.sum[k];
-        acc.min_gap = fmin(acc.min_gap, o.min_gap);
-        acc.alpha = fmin(acc.alpha, o.alpha);
-        acc.max_translation = fmax(acc.max_translation, o.max_translation);
-        acc.candidates += o.candidates;
-        acc.active += o.active;
-        acc.infeasible |= o.infeasible;
-        acc.failed |= o.failed;
+    for (uint32_t i0 = threadIdx.x; i0 < count; i0 += kSpan) {
+        Partial o[kReduceBatch];
+#pragma unroll
+        for (int b = 0; b < kReduceBatch; ++b)
+            o[b] = i0 + b * kStride < count ? partials[i0 + b * kStride] : empty_partial();
+#pragma unroll
+        for (int b = 0; b < kReduceBatch; ++b) {
+            if (i0 + b * kStride >= count) break;
+            for (int k = 0; k < 8; ++k) acc.sum[k] += o[b].sum[k];
+            acc.min_gap = fmin(acc.min_gap, o[b].min_gap);
+            acc.alpha = fmin(acc.alpha, o[b].alpha);
+            acc.max_translation = fmax(acc.max_translation, o[b].max_translation);
+            acc.candidates += o[b].candidates;
+            acc.active += o[b].active;
+            acc.infeasible |= o[b].infeasible;
+            acc.failed |= o[b].failed;
+        }
     }
     block_reduce<kReduceBlock>(acc, out, mirror);
     if (test.stop && threadIdx.x == 0) converged_test(*out, test);
@@ -393,7 +416,7 @@ __device__ inline uint32_t wave_gather(const ColliderView &col, V3 lo, V3 hi, ui
                 if (ch >= 0) {
                     push[c] = true;
                 } else if (col.leaf_box) {
-                    const double *b = col.leaf_box + 6 * size_t(-1 - ch);
+                    const double *b = col.leaf_box + kTriRecord * size_t(-1 - ch);
                     emit[c] = overlap(lo, hi, mk(b[0], b[1], b[2]), mk(b[3], b[4], b[5]), 0.0);
                 } else {
                     emit[c] = true;
@@ -582,7 +605,7 @@ __device__ inline bool wave_walk(const ColliderView &col, V3 lo, V3 hi, double p
                 }
                 const uint32_t t = uint32_t(-1 - ch);
                 if (col.leaf_box) {
-                    const double *b = col.leaf_box + 6 * size_t(t);
+                    const double *b = col.leaf_box + kTriRecord * size_t(t);
                     if (!overlap(lo, hi, mk(b[0], b[1], b[2]), mk(b[3], b[4], b[5]), pad)) continue;
                 }
                 f(t);
@@ -612,9 +635,9 @@ __device__ inline bool wave_closest(const ColliderView &col, V3 p0, V3 p1, doubl
     uint32_t bt = kNoTriangle, n = 0;
     const bool ok = wave_walk(col, vmin(p0, p1), vmax(p0, p1), pad, queue, qcap, [&](uint32_t t) {
         ++n;
-        const uint32_t *t3 = col.triangles + 3 * size_t(t);
-        const double d =
-            closest_segment_triangle(p0, p1, vertex(col, t3[0]), vertex(col, t3[1]), vertex(col, t3[2])).distance;
+        const double d = closest_segment_triangle(p0, p1, triangle_vertex(col, t, 0), triangle_vertex(col, t, 1),
+                                                  triangle_vertex(col, t, 2))
+                             .distance;
         if (d < bd || (d == bd && t < bt)) {
             bd = d;
             bt = t;
@@ -706,7 +729,7 @@ __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, 
             bool hit = false;
             if (k < c) {
                 t = tris[k];
-                const double *b = cc.tri_box + 6 * size_t(t);
+                const double *b = cc.tri_box + kTriRecord * size_t(t);
                 hit = overlap(lo, hi, mk(b[0], b[1], b[2]), mk(b[3], b[4], b[5]), pad);
             }
             const unsigned long long mask = (__ballot(hit) >> shift) & ((G == 64) ? ~0ull : ((1ull << G) - 1));
@@ -718,9 +741,9 @@ __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, 
         uint32_t bt = kNoTriangle;
         for (uint32_t i = lane; i < n; i += G) {
             const uint32_t t = cand[i];
-            const uint32_t *t3 = col.triangles + 3 * size_t(t);
-            const double d =
-                closest_segment_triangle(p0, p1, vertex(col, t3[0]), vertex(col, t3[1]), vertex(col, t3[2])).distance;
+            const double d = closest_segment_triangle(p0, p1, record_vertex(cc.tri_box, t, 0),
+                                                      record_vertex(cc.tri_box, t, 1), record_vertex(cc.tri_box, t, 2))
+                                 .distance;
             if (d < bd || (d == bd && t < bt)) {
                 bd = d;
                 bt = t;
@@ -893,7 +916,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
                 bool hit = false;
                 if (k < n) {
                     t = found[k];
-                    const double *b = cc.tri_box + 6 * size_t(t);
+                    const double *b = cc.tri_box + kTriRecord * size_t(t);
                     hit = overlap(slo, shi, mk(b[0], b[1], b[2]), mk(b[3], b[4], b[5]), target);
                 }
                 const unsigned long long mask = __ballot(hit);
@@ -902,11 +925,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
             }
             __builtin_amdgcn_wave_barrier();
             for (uint32_t i = lane; i < taken; i += kWave) {
-                const uint32_t *t3 = col.triangles + 3 * size_t(hits[i]);
+                const uint32_t t = hits[i];
                 const double seen = fmin(own, __longlong_as_double(__hip_atomic_load(
                                                   bound + slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
-                const double r = advancement_limit(p0, p1, d0, d1, speed, vertex(col, t3[0]), vertex(col, t3[1]),
-                                                   vertex(col, t3[2]), target, seen / 0.9);
+                const double r = advancement_limit(p0, p1, d0, d1, speed, record_vertex(cc.tri_box, t, 0),
+                                                   record_vertex(cc.tri_box, t, 1), record_vertex(cc.tri_box, t, 2),
+                                                   target, seen / 0.9);
                 if (r < seen) atomicMin(bound + slot, (unsigned long long)__double_as_longlong(r));
                 own = fmin(own, r);
             }
@@ -963,7 +987,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
                 bool hit = false;
                 if (k < c) {
                     t = tris[k];
-                    const double *b = cc.tri_box + 6 * size_t(t);
+                    const double *b = cc.tri_box + kTriRecord * size_t(t);
                     hit = overlap(lo, hi, mk(b[0], b[1], b[2]), mk(b[3], b[4], b[5]), target);
                 }
                 const unsigned long long mask = (__ballot(hit) >> shift) & ((G == 64) ? ~0ull : ((1ull << G) - 1));
@@ -981,12 +1005,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
             for (uint32_t i = threadIdx.x; i < pairs; i += kBlock) {
                 const uint32_t t = pair_tri[i];
                 const double *e = elem[pair_elem[i]];
-                const uint32_t *t3 = col.triangles + 3 * size_t(t);
                 const double seen = fmin(limit, __longlong_as_double(__hip_atomic_load(
                                                     bound + slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
                 const double r = advancement_limit(mk(e[0], e[1], e[2]), mk(e[3], e[4], e[5]), mk(e[6], e[7], e[8]),
-                                                   mk(e[9], e[10], e[11]), e[12], vertex(col, t3[0]),
-                                                   vertex(col, t3[1]), vertex(col, t3[2]), e[13], seen / 0.9);
+                                                   mk(e[9], e[10], e[11]), e[12], record_vertex(cc.tri_box, t, 0),
+                                                   record_vertex(cc.tri_box, t, 1), record_vertex(cc.tri_box, t, 2),
+                                                   e[13], seen / 0.9);
                 if (r < seen) atomicMin(bound + slot, (unsigned long long)__double_as_longlong(r));
                 limit = fmin(limit, r);
             }
@@ -1109,10 +1133,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GRAD ? 2
             // the list pass counted the candidates and chose the triangle
             tri = cc.count[L] ? best_tri[L] : kNoTriangle;
             found = tri != kNoTriangle;
-            if (found) {
-                const uint32_t *t3 = col.triangles + 3 * size_t(tri);
-                pair = closest_segment_triangle(p0, p1, vertex(col, t3[0]), vertex(col, t3[1]), vertex(col, t3[2]));
-            }
+            if (found)
+                pair = closest_segment_triangle(p0, p1, triangle_vertex(col, tri, 0), triangle_vertex(col, tri, 1),
+                                                triangle_vertex(col, tri, 2));
         } else {
             unsigned long long count = 0;
             found = closest_triangle(col, p0, p1, contact_pad_of(p, L), pair, tri, count);
@@ -1128,8 +1151,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GRAD ? 2
                 if (owner) acc.infeasible = 1;
             } else if (gap < p.barrier) {
                 if (owner) acc.active += 1;
-                const uint32_t *t3 = col.triangles + 3 * size_t(tri);
-                const V3 A = vertex(col, t3[0]), B = vertex(col, t3[1]), C = vertex(col, t3[2]);
+                const V3 A = triangle_vertex(col, tri, 0), B = triangle_vertex(col, tri, 1),
+                         C = triangle_vertex(col, tri, 2);
                 const V3 bp = add(add(mul(A, pair.bary.x), mul(B, pair.bary.y)), mul(C, pair.bary.z));
                 const double w0 = 1.0 - pair.t, w1 = pair.t;
                 const V3 hp = add(mul(p0, w0), mul(p1, w1));
@@ -2523,9 +2546,9 @@ __global__ __launch_bounds__(kBlock) void k_ccd_slope(RodView m, ColliderView co
         const V3 hi = vmax(vmax(a0, a1), vmax(add(a0, e0), add(a1, e1)));
         double limit = 1.0;
         const bool ok = wave_walk(col, lo, hi, tg, queue, kGatherQueue, [&](uint32_t t) {
-            const uint32_t *t3 = col.triangles + 3 * size_t(t);
-            limit = fmin(limit, advancement_limit(a0, a1, e0, e1, sp, vertex(col, t3[0]), vertex(col, t3[1]),
-                                                  vertex(col, t3[2]), tg, limit / 0.9));
+            limit = fmin(limit, advancement_limit(a0, a1, e0, e1, sp, triangle_vertex(col, t, 0),
+                                                  triangle_vertex(col, t, 1), triangle_vertex(col, t, 2), tg,
+                                                  limit / 0.9));
         });
 #pragma unroll
         for (int o = kWave / 2; o >= 1; o >>= 1) limit = fmin(limit, __shfl_xor(limit, o, kWave));
@@ -2716,10 +2739,9 @@ __global__ __launch_bounds__(kBlock) void k_friction(RodView m, ColliderView col
             if (listed_query(cc, best_tri, uint32_t(L))) {
                 tri = cc.count[L] ? best_tri[L] : kNoTriangle;
                 found = tri != kNoTriangle;
-                if (found) {
-                    const uint32_t *t3 = col.triangles + 3 * size_t(tri);
-                    pair = closest_segment_triangle(p0, p1, vertex(col, t3[0]), vertex(col, t3[1]), vertex(col, t3[2]));
-                }
+                if (found)
+                    pair = closest_segment_triangle(p0, p1, triangle_vertex(col, tri, 0), triangle_vertex(col, tri, 1),
+                                                    triangle_vertex(col, tri, 2));
             } else {
                 unsigned long long count = 0;
                 found = closest_triangle(col, p0, p1, contact_pad_of(p, L), pair, tri, count);
@@ -2730,8 +2752,9 @@ __global__ __launch_bounds__(kBlock) void k_friction(RodView m, ColliderView col
                     const uint32_t *t3 = col.triangles + 3 * size_t(tri);
                     const double w0 = 1.0 - pair.t, w1 = pair.t;
                     const V3 hair = add(mul(p0, w0), mul(p1, w1));
-                    const V3 body = add(add(mul(vertex(col, t3[0]), pair.bary.x), mul(vertex(col, t3[1]), pair.bary.y)),
-                                        mul(vertex(col, t3[2]), pair.bary.z));
+                    const V3 body = add(add(mul(triangle_vertex(col, tri, 0), pair.bary.x),
+                                            mul(triangle_vertex(col, tri, 1), pair.bary.y)),
+                                        mul(triangle_vertex(col, tri, 2), pair.bary.z));
                     const V3 old_body =
                         add(add(mul(vertex_prev(col, t3[0]), pair.bary.x), mul(vertex_prev(col, t3[1]), pair.bary.y)),
                             mul(vertex_prev(col, t3[2]), pair.bary.z));

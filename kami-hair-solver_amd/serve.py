@@ -131,11 +131,11 @@ def estimated_device_bytes(nodes: int, strands: int, vertices: int, triangles: i
     buffer layout: per node the state and solver vectors, model, Gauss-Newton
     blocks and Thomas factors, root targets and the 512-entry contact list
     (about 3.9 KB); 48 B per node and 24 B per collider vertex per resident
-    animation frame; 96 B per node per rollback slot; about 0.3 KB per
+    animation frame; 96 B per node per rollback slot; about 0.4 KB per
     triangle for the boxes, sort keys and hierarchy; and a fixed 16 MiB for
     small buffers and the sort's temporary storage."""
     per_node = 212 * 8 + 15 * 4 + 512 * 4 + 64
-    per_triangle = 12 + 2 * 48 + 4 * 4 + 112 + 8 + 8 + 16 + 40
+    per_triangle = 12 + 48 + 128 + 4 * 4 + 112 + 8 + 8 + 16 + 40
     per_vertex = 4 * 24
     return int(nodes * per_node + strands * 32 + triangles * per_triangle + vertices * per_vertex
                + frames * (48 * nodes + 24 * vertices) + ring * 96 * nodes + (16 << 20))

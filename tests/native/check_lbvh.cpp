@@ -142,6 +142,9 @@ int main()
     // swept over a collider motion is), queried with the ungrown boxes as
     // leaf boxes, must report exactly the ungrown overlaps.
     std::vector<double> grown(tri_box);
+    std::vector<double> records(kTriRecord * size_t(T), 0.0);  // leaf boxes in the triangle records' layout
+    for (uint32_t t = 0; t < T; ++t)
+        for (int k = 0; k < 6; ++k) records[kTriRecord * size_t(t) + k] = tri_box[6 * t + k];
     std::uniform_real_distribution<double> grow(0.0, 0.02);
     for (uint32_t t = 0; t < T; ++t)
         for (int k = 0; k < 3; ++k) {
@@ -162,7 +165,7 @@ int main()
         const double *x = &queries[7 * size_t(q)];
         const V3 lo = mk(x[0], x[1], x[2]), hi = mk(x[3], x[4], x[5]);
         std::vector<uint32_t> got;
-        bvh_query(nodes.data(), internal, tri_box.data(), lo, hi, x[6], [&](uint32_t t) { got.push_back(t); });
+        bvh_query(nodes.data(), internal, records.data(), lo, hi, x[6], [&](uint32_t t) { got.push_back(t); });
         std::sort(got.begin(), got.end());
         if (got != want_all[q]) ++leaf_mismatched;
     }
