@@ -209,6 +209,11 @@ struct DeviceState {
     Buffer<double> cache_box;
     double cache_skin = 0.0;
     bool cache_valid = false;
+    // best_tri and list_words hold the list pass at the positions q now has:
+    // the accepted line-search trial's, whose check and list pass ran at
+    // q_base + alpha d, the expression the trial evaluation wrote into q. The
+    // gradient evaluation at that iterate then needs neither again.
+    bool closest_at_q = false;
     Buffer<uint32_t> best_tri;  // closest triangle per element from the list pass
     // reductions and flags
     Buffer<Partial> partials, reduced;
@@ -1230,9 +1235,11 @@ void Solver::evaluate_async(bool gradient, double dt, bool line_search, unsigned
     const ContactCache cc = d.cache();
     hipEvent_t t0 = d.tic();
     // With current candidate lists the list pass finds the closest
-    // triangles first; its partials follow the evaluation's.
+    // triangles first; its partials follow the evaluation's. At an
+    // accepted trial's iterate its own list pass already did.
     const bool list_pass = d.cache_valid;
-    if (list_pass) closest_pass(line_search, d.list_words.get());
+    if (list_pass && !(gradient && d.closest_at_q)) closest_pass(line_search, d.list_words.get());
+    d.closest_at_q = false;
     const uint32_t *best = list_pass ? d.best_tri.get() : nullptr;
     // the variants without the tree walk when the list pass answered every query
     auto launch = [&](auto kernel, const unsigned *stop) {
@@ -1557,9 +1564,11 @@ KhsResult Solver::optimize_substep(double dt)
             HIP_TRY(hipMemcpyAsync(d.q.get(), d.q_base.get(), d.q.bytes(), hipMemcpyDeviceToDevice, d.stream));
             return fail(KHS_ERROR_NOT_CONVERGED, "Gauss-Newton line search failed.");
         }
-        // The gradient at the accepted iterate; unless the budget ends here,
-        // the next iteration's direction, bound and first trial follow before
-        // the one wait (they skip themselves if this gradient converged).
+        // The gradient at the accepted iterate (whose closest triangles the
+        // trial's list pass left); unless the budget ends here, the next
+        // iteration's direction, bound and first trial follow before the one
+        // wait (they skip themselves if this gradient converged).
+        d.closest_at_q = true;
         evaluate_async(true, dt, false, d.stop.get(), abs_tol, rel_tol, initial);
         if (it + 1 < desc_.newton_iterations) {
             direction_async();
