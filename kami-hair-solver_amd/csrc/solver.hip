@@ -214,6 +214,9 @@ struct DeviceState {
     // q_base + alpha d, the expression the trial evaluation wrote into q. The
     // gradient evaluation at that iterate then needs neither again.
     bool closest_at_q = false;
+    // Every contact list covers the swept boxes of q_base to q_base + dir
+    // (set by the swept pass, cleared when dir or q_base change otherwise).
+    bool swept_covers = false;
     Buffer<uint32_t> best_tri;  // closest triangle per element from the list pass
     // reductions and flags
     Buffer<Partial> partials, reduced;
@@ -911,6 +914,7 @@ void Solver::refresh_contact_cache(int mode, bool full, bool swept_leaves, bool 
     d.cache_skin = 0.5 * pad;
     const ContactCache cc = d.cache();
     if (full) {
+        d.swept_covers = false;
         KHS_LAUNCH(d, dev::k_fill_u32, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.cache_listed.get(),
                            size_t(d.N), 0u);
         KHS_LAUNCH(d, dev::k_fill_u32, dim3(1), dim3(64), 0, d.stream, d.cache_list_count.get(), size_t(1), 0u);
@@ -962,6 +966,7 @@ void Solver::swept_bound_async(const StepParams &cp, const unsigned *stop)
 {
     DeviceState &d = *dev_;
     refresh_contact_cache(dev::kSwept, false);
+    d.swept_covers = true;
     KHS_LAUNCH(d, dev::k_swept_ccd, dim3(kSweptBoundBlocks), dim3(kBlock), 0, d.stream, d.flags.get(),
                        d.collider(), d.cache(), cp, d.q.get(), d.dir.get(), d.list_alpha.get(), d.ccd_bound.get(),
                        int(d.ccd_slot), d.cache_pending_count.get() + (d.pending_slot ^ 1u), stop);
@@ -978,20 +983,27 @@ void Solver::closest_pass(bool line_search, unsigned long long *block_candidates
 {
     DeviceState &d = *dev_;
     const StepParams sp = step_params(1.0);
-    refresh_contact_cache(line_search ? dev::kAtTrial : dev::kAtState, false, false, with_roots);
+    // A trial's query boxes lie inside the boxes the swept check made every
+    // list cover (each trial point alpha d + q_base, 0 < alpha <= 1, lies
+    // between q_base and q_base + d in every coordinate, rounding included),
+    // and the coverage test is monotone: its check would gather nothing.
+    const bool covered = line_search && d.swept_covers && !with_roots;
+    if (!covered) refresh_contact_cache(line_search ? dev::kAtTrial : dev::kAtState, false, false, with_roots);
     const ContactCache cc = d.cache();
-    if (line_search)
+    if (line_search) {
         KHS_LAUNCH(d, dev::k_list_closest<dev::kAtTrial>, dim3(kPairBlocks), dim3(kBlock), 0, d.stream,
                            d.flags.get(), d.collider(), cc, sp, d.q.get(), d.q_base.get(), d.dir.get(),
-                           d.alpha.get(), d.best_tri.get(), block_candidates,
+                           d.alpha.get(), d.trial_step, d.best_tri.get(), block_candidates,
                            with_roots ? d.root_elements.get() : nullptr, with_roots ? d.root_element_count : 0u,
                            d.skip());
-    else
+        d.trial_step.source = dev::kAlphaStored;
+    } else {
         KHS_LAUNCH(d, dev::k_list_closest<dev::kAtState>, dim3(kPairBlocks), dim3(kBlock), 0, d.stream,
                            d.flags.get(), d.collider(), cc, sp, d.q.get(), d.q_base.get(), d.dir.get(),
-                           d.alpha.get(), d.best_tri.get(), block_candidates,
+                           d.alpha.get(), dev::TrialStep{}, d.best_tri.get(), block_candidates,
                            with_roots ? d.root_elements.get() : nullptr, with_roots ? d.root_element_count : 0u,
                            (const unsigned *)nullptr);
+    }
     HIP_TRY(hipGetLastError());
 }
 
@@ -1299,6 +1311,7 @@ Solver::Reduced Solver::evaluate(bool gradient, double dt, bool line_search)
 void Solver::steepest_direction_async()
 {
     DeviceState &d = *dev_;
+    d.swept_covers = false;
     const RodView rv = d.rods();
     const size_t dofs = 6 * size_t(d.N);
     KHS_LAUNCH(d, dev::k_steepest, dim3(grid_for(dofs, kBlock)), dim3(kBlock), 0, d.stream, rv, d.choice(),
@@ -1374,6 +1387,7 @@ uint32_t Solver::launch_block_solve(unsigned *failed, unsigned *next_failed, con
 void Solver::make_direction_async()
 {
     DeviceState &d = *dev_;
+    d.swept_covers = false;
     const RodView rv = d.rods();
     unsigned *failed = d.solve_failed.get() + d.failed_slot;
     hipEvent_t t0 = d.tic();
@@ -1397,6 +1411,7 @@ void Solver::direction_async()
         return;
     }
     DeviceState &d = *dev_;
+    d.swept_covers = false;
     ++d.launch_count;
     HIP_TRY(hipMemcpyAsync(d.q_base.get(), d.q.get(), d.q.bytes(), hipMemcpyDeviceToDevice, d.stream));
     KHS_LAUNCH(d, dev::k_force_steepest, dim3(1), dim3(1), 0, d.stream, d.choice(), d.mirror_choice());

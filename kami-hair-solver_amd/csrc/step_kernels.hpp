@@ -290,14 +290,17 @@ __device__ __forceinline__ void query_box(uint32_t L, const double *q, const dou
 
 // [lo - pad, hi + pad] inside the cached box with a margin far above the
 // rounding of either box test, so every triangle passing the query's test
-// passed the list's (false for NaN).
+// passed the list's (false for NaN). The margin follows the cached box's own
+// coordinate, so the test is monotone in the query box: a query box inside
+// one that is covered is covered (each rounded subtraction and addition is
+// monotone in the query coordinate).
 __device__ __forceinline__ bool box_covered(const double *box, V3 lo, V3 hi, double pad)
 {
     const double l[3] = {lo.x - pad, lo.y - pad, lo.z - pad}, h[3] = {hi.x + pad, hi.y + pad, hi.z + pad};
     bool ok = true;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const double ml = 1.0e-12 + 1.0e-9 * fabs(l[k]), mh = 1.0e-12 + 1.0e-9 * fabs(h[k]);
+        const double ml = 1.0e-12 + 1.0e-9 * fabs(box[k]), mh = 1.0e-12 + 1.0e-9 * fabs(box[3 + k]);
         ok = ok && box[k] <= l[k] - ml && box[3 + k] >= h[k] + mh;
     }
     return ok;
@@ -671,7 +674,7 @@ constexpr uint32_t kOverflowSlots = 16;
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, ColliderView col, ContactCache cc,
                                                          StepParams sp, const double *q, const double *q_base,
-                                                         const double *dir, const double *alpha_at,
+                                                         const double *dir, double *alpha_at, TrialStep step,
                                                          uint32_t *best_tri, unsigned long long *block_candidates,
                                                          const uint32_t *extra, uint32_t extra_count,
                                                          const unsigned *stop)
@@ -684,7 +687,8 @@ __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, 
     const int wave = threadIdx.x / kWave;
     if ((threadIdx.x & (kWave - 1)) == 0) deferred_count[wave] = 0;
     __builtin_amdgcn_wave_barrier();
-    const double alpha = MODE == kAtTrial ? *alpha_at : 0.0;
+    // a trial's step (resolved here when no check ran before this pass)
+    const double alpha = MODE == kAtTrial ? resolve_step(step, alpha_at) : 0.0;
     const int lane = threadIdx.x & (G - 1);
     const int slot = threadIdx.x / G;
     uint32_t *cand = packed[slot];
