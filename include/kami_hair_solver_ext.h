@@ -257,6 +257,15 @@ KHS_API KhsResult khsxSetDirectionMode(KhsSolver *solver, uint32_t mode);
    Precision mode 1 always runs the FP32 form of 0. */
 KHS_API KhsResult khsxSetSolveAlgorithm(KhsSolver *solver, uint32_t algorithm);
 
+/* Reuse of contact-list work inside the Newton loop. 1 (default): the
+   gradient evaluation at an accepted line-search trial takes the closest
+   triangles the trial's list pass found at the same positions, and a trial
+   whose query boxes the step bound's swept pass already made every list
+   cover runs no contact check of its own. 0: every evaluation runs its own
+   check and list pass (the result is the same; this mode exists so that
+   tests can show it). The CPU oracle keeps no lists and accepts both. */
+KHS_API KhsResult khsxSetContactReuse(KhsSolver *solver, uint32_t mode);
+
 /* Directions the Newton loop used in the last stepped frame (its final
    adaptive attempt): block-tridiagonal directions and steepest-descent
    directions (forced or fallback). */

@@ -308,6 +308,11 @@ KhsResult khsxSetSolveAlgorithm(KhsSolver *s, uint32_t algorithm)
     return guarded(s, [&] { return s->impl.set_solve_algorithm(algorithm); });
 }
 
+KhsResult khsxSetContactReuse(KhsSolver *s, uint32_t mode)
+{
+    return guarded(s, [&] { return s->impl.set_contact_reuse(mode); });
+}
+
 KhsResult khsxSetKernelTiming(KhsSolver *s, uint32_t enable)
 {
     return guarded(s, [&] { return s->impl.set_kernel_timing(enable); });

@@ -987,7 +987,7 @@ void Solver::closest_pass(bool line_search, unsigned long long *block_candidates
     // list cover (each trial point alpha d + q_base, 0 < alpha <= 1, lies
     // between q_base and q_base + d in every coordinate, rounding included),
     // and the coverage test is monotone: its check would gather nothing.
-    const bool covered = line_search && d.swept_covers && !with_roots;
+    const bool covered = contact_reuse_ && line_search && d.swept_covers && !with_roots;
     if (!covered) refresh_contact_cache(line_search ? dev::kAtTrial : dev::kAtState, false, false, with_roots);
     const ContactCache cc = d.cache();
     if (line_search) {
@@ -1583,7 +1583,7 @@ KhsResult Solver::optimize_substep(double dt)
         // trial's list pass left); unless the budget ends here, the next
         // iteration's direction, bound and first trial follow before the one
         // wait (they skip themselves if this gradient converged).
-        d.closest_at_q = true;
+        d.closest_at_q = contact_reuse_;
         evaluate_async(true, dt, false, d.stop.get(), abs_tol, rel_tol, initial);
         if (it + 1 < desc_.newton_iterations) {
             direction_async();
@@ -2332,6 +2332,16 @@ KhsResult Solver::set_precision_mode(uint32_t mode)
         return KHS_ERROR_INVALID_ARGUMENT;
     }
     precision_mode_ = mode;
+    return KHS_OK;
+}
+
+KhsResult Solver::set_contact_reuse(uint32_t mode)
+{
+    if (mode > 1) {
+        error_ = "Contact reuse must be 0 (off) or 1 (on).";
+        return KHS_ERROR_INVALID_ARGUMENT;
+    }
+    contact_reuse_ = mode == 1;
     return KHS_OK;
 }
 

@@ -69,6 +69,7 @@ public:
     KhsResult set_convergence_mode(uint32_t mode);
     KhsResult set_precision_mode(uint32_t mode);
     KhsResult set_solve_algorithm(uint32_t algorithm);
+    KhsResult set_contact_reuse(uint32_t mode);
     KhsResult set_kernel_timing(uint32_t enable);
     KhsResult kernel_timing(KhsxKernelTiming *out) const;
     KhsResult substep_times(double *ms, uint32_t capacity, uint32_t *count) const;
@@ -149,6 +150,7 @@ private:
     std::unique_ptr<DeviceState> dev_;
     uint32_t convergence_mode_ = 0;
     uint32_t precision_mode_ = 0;
+    bool contact_reuse_ = true;     // khsxSetContactReuse
     uint32_t solve_algorithm_ = 0;  // 0 block Thomas, 1 twisted (FP64 only)
     uint32_t direction_mode_ = 0;
     uint64_t block_directions_ = 0, steepest_directions_ = 0;

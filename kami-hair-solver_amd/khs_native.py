@@ -318,6 +318,7 @@ EXT_FUNCTIONS = {
     "khsxSweepCheck": (C.c_int, [_P, _D(C.c_uint32), _D(KhsFailureDiagnostics)]),
     "khsxSetDirectionMode": (C.c_int, [_P, C.c_uint32]),
     "khsxSetSolveAlgorithm": (C.c_int, [_P, C.c_uint32]),
+    "khsxSetContactReuse": (C.c_int, [_P, C.c_uint32]),
     "khsxGetDirectionCounts": (C.c_int, [_P, _D(C.c_uint64), _D(C.c_uint64)]),
     "khsxGetLaunchCounts": (C.c_int, [_P, _D(C.c_uint64), _D(C.c_uint64)]),
 }
@@ -625,6 +626,9 @@ class HairSolver:
 
     def set_solve_algorithm(self, algorithm: int) -> int:
         return self.lib.khsxSetSolveAlgorithm(self.handle, algorithm)
+
+    def set_contact_reuse(self, mode: int) -> int:
+        return self.lib.khsxSetContactReuse(self.handle, mode)
 
     def set_direction_mode(self, mode: int) -> int:
         return self.lib.khsxSetDirectionMode(self.handle, mode)

@@ -2817,6 +2817,13 @@ KhsResult khsxSetSolveAlgorithm(KhsSolver *s, uint32_t algorithm)
     return KHS_OK;
 }
 
+KhsResult khsxSetContactReuse(KhsSolver *s, uint32_t mode)
+{
+    // no candidate lists here: both modes are the plain step
+    if (!s || mode > 1) return KHS_ERROR_INVALID_ARGUMENT;
+    return KHS_OK;
+}
+
 KhsResult khsxSetDirectionMode(KhsSolver *s, uint32_t mode)
 {
     if (!s || mode > 1) return KHS_ERROR_INVALID_ARGUMENT;

@@ -347,6 +347,20 @@ def groom_subset():
     return grooms.head_scene(strands=GROOM_STRANDS)
 
 
+@pytest.fixture(scope="module")
+def groom_contact_checkpoint(gpu_lib, groom_subset):
+    """The groom subset's checkpoint after frames 1-12 stepped on the GPU,
+    when its hair lies on the body."""
+    scene = groom_subset
+    print(f"[gpu] groom subset: frames 1-{GROOM_WARM} on the GPU", flush=True)
+    s = _groom_solver(gpu_lib, scene, GROOM_WARM + 1)
+    for f in range(1, GROOM_WARM + 1):
+        s.check(s.step_frame(f, scene.frame_dt))
+    state = s.save_checkpoint()
+    s.close()
+    return state
+
+
 def test_groom_subset_first_frames_match_oracle(oracle_lib, gpu_lib, groom_subset):
     """The first 600 strands of the 6,757-strand groom on the full rotating
     225,184-vertex body, frames 1-2 at 8 substeps to the 1e-6 residual with
@@ -369,7 +383,7 @@ def test_groom_subset_first_frames_match_oracle(oracle_lib, gpu_lib, groom_subse
     _frames_match(results, "groom frames 1-2")
 
 
-def test_groom_contact_state_stages_match_oracle(oracle_lib, gpu_lib, groom_subset):
+def test_groom_contact_state_stages_match_oracle(oracle_lib, gpu_lib, groom_subset, groom_contact_checkpoint):
     """The same groom once its hair lies on the body (the GPU steps frames
     1-12 alone, and both libraries restore its checkpoint): at the first
     substep of frame 13, with the body between its frame-12 and substep
@@ -382,12 +396,7 @@ def test_groom_contact_state_stages_match_oracle(oracle_lib, gpu_lib, groom_subs
     device libm (log, sin, cos) and the oracle's dual-number Jacobian against
     the device's analytic one differ there."""
     scene = groom_subset
-    print(f"[gpu] groom subset: frames 1-{GROOM_WARM} on the GPU", flush=True)
-    s = _groom_solver(gpu_lib, scene, GROOM_WARM + 1)
-    for f in range(1, GROOM_WARM + 1):
-        s.check(s.step_frame(f, scene.frame_dt))
-    state = s.save_checkpoint()
-    s.close()
+    state = groom_contact_checkpoint
     substeps = scene.desc["substeps"]
     dt = scene.frame_dt / substeps
     prev = scene.frame_vertices(GROOM_WARM)
@@ -436,6 +445,34 @@ def test_groom_contact_state_stages_match_oracle(oracle_lib, gpu_lib, groom_subs
     assert np.count_nonzero(dv[0]) > 20
     assert np.count_nonzero(dv[1]) == np.count_nonzero(dv[0])
     assert _rel(dv[1], dv[0]) <= 1e-10
+
+
+def test_contact_reuse_is_exact(gpu_lib, groom_subset, groom_contact_checkpoint):
+    """The device's reuse of contact-list work (the gradient at an accepted
+    trial takes the trial's closest triangles; a trial inside the boxes the
+    swept pass covered runs no check) changes nothing: from the groom
+    subset's frame-12 contact state, frames 13-14 stepped with the reuse on
+    and off give the same positions bit for bit and every count equal."""
+    scene = groom_subset
+    results = []
+    for mode in (1, 0):
+        print(f"[gpu] groom contact frames 13-14, contact reuse {mode}", flush=True)
+        s = _groom_solver(gpu_lib, scene, GROOM_WARM + 3)
+        s.check(s.set_contact_reuse(mode))
+        s.check(s.restore_checkpoint(groom_contact_checkpoint))
+        stats = []
+        for f in (GROOM_WARM + 1, GROOM_WARM + 2):
+            s.check(s.step_frame(f, scene.frame_dt))
+            st = s.step_stats()
+            stats.append((st.newton_iterations, st.line_search_evaluations, st.contact_candidate_count,
+                          st.active_contact_count, st.minimum_gap))
+        results.append((s.internal_positions(), stats))
+        s.close()
+    (x_on, st_on), (x_off, st_off) = results
+    print(f"reuse on {st_on}\nreuse off {st_off}")
+    assert st_on[0][3] > 20  # active barriers: the frames are in contact
+    assert st_on == st_off
+    assert np.array_equal(x_on, x_off)
 
 
 def test_fp32_block_solve_matches_oracle(oracle_lib, gpu_lib):
