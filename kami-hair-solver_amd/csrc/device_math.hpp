@@ -274,16 +274,34 @@ struct ElementJacobian {
     double Si[9], Sj[9], Ki[9], Kj[9];
 };
 
+// A node's rotation Rk = exp(uk) Rbar_k and, with the Jacobian,
+// Mk = Rbar_k^T Jr(uk): the per-node half of an element's strains, which the
+// two elements sharing the node use alike.
+struct NodeRotation {
+    double R[9];
+    double M[9];
+};
+
 template <bool WITH_JACOBIAN>
-__device__ inline void element_strains(const double *qi, const double *qj, const double *Fi, const double *Fj,
-                                       const double *rest, ElementJacobian &out)
+__device__ inline void node_rotation(const double *qk, const double *Fk, NodeRotation &out)
 {
-    const V3 ui = mk(qi[3], qi[4], qi[5]), uj = mk(qj[3], qj[4], qj[5]);
-    double Ei[9], Ej[9], Ri[9], Rj[9];
-    exp_so3(ui, Ei);
-    exp_so3(uj, Ej);
-    mat_mul(Ei, Fi, Ri);
-    mat_mul(Ej, Fj, Rj);
+    const V3 u = mk(qk[3], qk[4], qk[5]);
+    double E[9];
+    exp_so3(u, E);
+    mat_mul(E, Fk, out.R);
+    if (!WITH_JACOBIAN) return;
+    double Jr[9];
+    right_jacobian(u, Jr);
+    mat_tmul(Fk, Jr, out.M);
+}
+
+// The element's strains (and Jacobian) from its nodes' positions and
+// rotations.
+template <bool WITH_JACOBIAN>
+__device__ inline void element_strains_from(const double *qi, const double *qj, const NodeRotation &ni,
+                                            const NodeRotation &nj, const double *rest, ElementJacobian &out)
+{
+    const double *Ri = ni.R, *Rj = nj.R;
     const double inv_L = 1.0 / rest[0];
     const V3 t = mk((qj[0] - qi[0]) * inv_L, (qj[1] - qi[1]) * inv_L, (qj[2] - qi[2]) * inv_L);
     double A2[9];  // Ri^T + Rj^T
@@ -308,12 +326,7 @@ __device__ inline void element_strains(const double *qi, const double *qj, const
 #pragma unroll
     for (int k = 0; k < 9; ++k) out.Sx[k] = (A2[k] * inv_L) * 0.5;
 
-    double Jr[9], Mi[9], Mj[9];
-    right_jacobian(ui, Jr);
-    mat_tmul(Fi, Jr, Mi);
-    right_jacobian(uj, Jr);
-    mat_tmul(Fj, Jr, Mj);
-
+    const double *Mi = ni.M, *Mj = nj.M;
     // a_k = Rk^T t; dgamma/duk = 1/2 [a_k]x Mk
     const V3 ai = mk(Ri[0] * t.x + Ri[3] * t.y + Ri[6] * t.z, Ri[1] * t.x + Ri[4] * t.y + Ri[7] * t.z,
                      Ri[2] * t.x + Ri[5] * t.y + Ri[8] * t.z);
@@ -339,6 +352,16 @@ __device__ inline void element_strains(const double *qi, const double *qj, const
         }
     mat_mul(Bi, Mi, out.Ki);
     mat_mul(Bj, Mj, out.Kj);
+}
+
+template <bool WITH_JACOBIAN>
+__device__ inline void element_strains(const double *qi, const double *qj, const double *Fi, const double *Fj,
+                                       const double *rest, ElementJacobian &out)
+{
+    NodeRotation ni, nj;
+    node_rotation<WITH_JACOBIAN>(qi, Fi, ni);
+    node_rotation<WITH_JACOBIAN>(qj, Fj, nj);
+    element_strains_from<WITH_JACOBIAN>(qi, qj, ni, nj, rest, out);
 }
 
 }  // namespace dev

@@ -16,6 +16,10 @@ constexpr int kTile = 63;   // nodes owned by one wavefront in tiled kernels
 constexpr int kBlock = 256;  // four wavefronts
 constexpr int kTilesPerBlock = kBlock / kWave;
 constexpr int kNodesPerBlock = kTile * kTilesPerBlock;
+// The evaluation's tiles own one node fewer: its lane 63 lends its node's
+// rotation to lane 62 (step_kernels.hpp, k_evaluate).
+constexpr int kEvalTile = 62;
+constexpr int kEvalNodesPerBlock = kEvalTile * kTilesPerBlock;
 
 // One block's share of every reduction the host needs after a pass; the
 // reduce kernel folds them in a fixed order, so results are reproducible.

@@ -398,6 +398,7 @@ struct DeviceState {
     double *cache_host = nullptr;      // pinned staging of one cache frame (3P doubles)
 
     uint32_t tiled_blocks() const { return grid_for(size_t(N) + 1, kNodesPerBlock); }
+    uint32_t eval_blocks() const { return grid_for(size_t(N) + 1, kEvalNodesPerBlock); }
     uint32_t solve_blocks() const { return grid_for(S, dev::kSolveGroups); }
     uint32_t twisted_blocks() const { return grid_for(S, dev::kTwistGroups); }
     uint32_t node_blocks() const { return grid_for(N, kBlock); }
@@ -776,7 +777,8 @@ KhsResult Solver::upload_model()
     d.rot_cur.upload(zero3.data(), zero3.size());
     d.rot_pend.upload(zero3.data(), zero3.size());
     d.out3.allocate(3 * std::max<size_t>(N, d.P));
-    const uint32_t max_blocks = std::max({d.tiled_blocks(), d.node_blocks(), d.solve_blocks(), d.twisted_blocks()}) + 1;
+    const uint32_t max_blocks =
+        std::max({d.tiled_blocks(), d.eval_blocks(), d.node_blocks(), d.solve_blocks(), d.twisted_blocks()}) + 1;
     d.partials.allocate(max_blocks + 16);
     d.list_words.allocate(kPairBlocks);
     d.list_alpha.allocate(kSweptBoundBlocks);
@@ -1241,7 +1243,7 @@ void Solver::evaluate_async(bool gradient, double dt, bool line_search, unsigned
 {
     DeviceState &d = *dev_;
     const StepParams p = step_params(dt);
-    const unsigned blocks = d.tiled_blocks();
+    const unsigned blocks = d.eval_blocks();
     const RodView rv = d.rods();
     const ColliderView cv = d.collider();
     const ContactCache cc = d.cache();

@@ -30,6 +30,14 @@ __device__ __forceinline__ double from_lane_below(double v)
     const int rhi = __builtin_amdgcn_update_dpp(0, hi, 0x138, 0xf, 0xf, false);
     return __hiloint2double(rhi, rlo);
 }
+// Value of the lane above (lane 63 receives 0), a DPP wavefront shift left.
+__device__ __forceinline__ double from_lane_above(double v)
+{
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    const int rlo = __builtin_amdgcn_update_dpp(0, lo, 0x130, 0xf, 0xf, false);
+    const int rhi = __builtin_amdgcn_update_dpp(0, hi, 0x130, 0xf, 0xf, false);
+    return __hiloint2double(rhi, rlo);
+}
 
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -1065,6 +1073,12 @@ __device__ inline void store_tile_blocks(const double *st, double *out, long fir
 // per element; with GRAD also the gradient and the Gauss-Newton blocks (the
 // diagonal block D and the upper coupling block U of each node). With LS the
 // state is q = alpha d + q_base, written back by the owners.
+// Tiles of kEvalTile nodes: wavefront w covers nodes 62 w - 1 .. 62 w + 62,
+// one per lane. Every lane forms its node's rotation (and with GRAD its
+// right-Jacobian product) once; lanes 0..62 take the next node's from the
+// lane above by a DPP shift and evaluate the element from their node to the
+// next, lanes 1..62 own their nodes (lane 0's element is the previous tile's
+// last, for its coupling terms; lane 63 only lends its node's rotation).
 // The assembly is held to 256 registers so two wavefronts share a SIMD
 // (a few values spill); at one wavefront per SIMD its loads are exposed.
 // With LISTS the list pass before it has chosen every element's closest
@@ -1079,11 +1093,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GRAD ? 2
     const double alpha = LS ? *alpha_at : 0.0;
     const int lane = threadIdx.x & (kWave - 1);
     const long tile = long(blockIdx.x) * kTilesPerBlock + threadIdx.x / kWave;
-    const long L = tile * kTile - 1 + lane;
+    const long L = tile * kEvalTile - 1 + lane;
     const bool in_range = L >= 0 && L < long(m.node_count);
-    const bool owner = lane >= 1 && in_range;
+    const bool owner = lane >= 1 && lane <= kEvalTile && in_range;
     const uint32_t fl = in_range ? m.flags[L] : 0u;
-    const bool elem = (fl & kNodeHasElement) != 0;
+    const bool elem = lane <= kEvalTile && (fl & kNodeHasElement) != 0;
     const bool fixed_i = (fl & kNodeFixed) != 0;
     const bool fixed_j = elem && (m.flags[L + 1] & kNodeFixed) != 0;
 
@@ -1189,13 +1203,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GRAD ? 2
         }
     }
 
-    // Cosserat elasticity (cuda_backend.cu:905-919).
+    // Cosserat elasticity (cuda_backend.cu:905-919). This lane's node
+    // rotation, and the next node's from the lane above.
+    NodeRotation ni = {}, nj;
+    if (in_range) node_rotation<GRAD>(qi, m.frame + 9 * L, ni);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        nj.R[k] = from_lane_above(ni.R[k]);
+        if (GRAD) nj.M[k] = from_lane_above(ni.M[k]);
+    }
     ElementJacobian J;
     double elastic = 0.0;
     double weight[6];
     weights_of<MAT>(p, elem ? L : 0, weight);
     if (elem) {
-        element_strains<GRAD>(qi, qj, m.frame + 9 * L, m.frame + 9 * (L + 1), m.rest + 8 * L, J);
+        element_strains_from<GRAD>(qi, qj, ni, nj, m.rest + 8 * L, J);
         const double L0 = m.rest[8 * L];
 #pragma unroll
         for (int s = 0; s < 6; ++s) {
@@ -1279,11 +1301,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GRAD ? 2
         // records, whole lines at a time: first D
         // (inertia, both elements' j- and i-parts, both contact terms), then
         // U (this element's coupling and contact term).
-        __shared__ double stage[kTilesPerBlock][kTile * kStageStride];
+        __shared__ double stage[kTilesPerBlock][kEvalTile * kStageStride];
         double *st = stage[threadIdx.x / kWave];
         const int slot = lane - 1;
-        const long first_node = tile * kTile;
-        const int tile_nodes = first_node < long(m.node_count) ? int(min(long(kTile), long(m.node_count) - first_node)) : 0;
+        const long first_node = tile * kEvalTile;
+        const int tile_nodes =
+            first_node < long(m.node_count) ? int(min(long(kEvalTile), long(m.node_count) - first_node)) : 0;
 #pragma unroll
         for (int a = 0; a < 6; ++a) {
 #pragma unroll
