@@ -218,6 +218,7 @@ struct DeviceState {
     // (set by the swept pass, cleared when dir or q_base change otherwise).
     bool swept_covers = false;
     Buffer<uint32_t> best_tri;  // closest triangle per element from the list pass
+    Buffer<double> best_pair;   // and its closest pair (dev::kPairWords per element)
     // reductions and flags
     Buffer<Partial> partials, reduced;
     Buffer<unsigned> flag;
@@ -341,7 +342,7 @@ struct DeviceState {
              partials.bytes() + reduced.bytes() + cache_count.bytes() + cache_tris.bytes() + cache_box.bytes() +
              cache_listed.bytes() + cache_list.bytes() + cache_list_count.bytes() + cache_pending.bytes() +
              cache_pending_epoch.bytes() + cache_snapshot.bytes() +
-             best_tri.bytes();
+             best_tri.bytes() + best_pair.bytes();
         return b;
     }
 
@@ -839,6 +840,7 @@ KhsResult Solver::upload_model()
         d.cache_list.allocate(d.N);
         d.cache_list_count.allocate(1);
         d.best_tri.allocate(d.N);
+        d.best_pair.allocate(dev::kPairWords * size_t(d.N));
         rebuild_bvh(false);
     }
     trust_radius_ = std::max(0.25 * model_.maximum_rest_length,
@@ -995,14 +997,14 @@ void Solver::closest_pass(bool line_search, unsigned long long *block_candidates
     if (line_search) {
         KHS_LAUNCH(d, dev::k_list_closest<dev::kAtTrial>, dim3(kPairBlocks), dim3(kBlock), 0, d.stream,
                            d.flags.get(), d.collider(), cc, sp, d.q.get(), d.q_base.get(), d.dir.get(),
-                           d.alpha.get(), d.trial_step, d.best_tri.get(), block_candidates,
+                           d.alpha.get(), d.trial_step, d.best_tri.get(), d.best_pair.get(), block_candidates,
                            with_roots ? d.root_elements.get() : nullptr, with_roots ? d.root_element_count : 0u,
                            d.skip());
         d.trial_step.source = dev::kAlphaStored;
     } else {
         KHS_LAUNCH(d, dev::k_list_closest<dev::kAtState>, dim3(kPairBlocks), dim3(kBlock), 0, d.stream,
                            d.flags.get(), d.collider(), cc, sp, d.q.get(), d.q_base.get(), d.dir.get(),
-                           d.alpha.get(), dev::TrialStep{}, d.best_tri.get(), block_candidates,
+                           d.alpha.get(), dev::TrialStep{}, d.best_tri.get(), d.best_pair.get(), block_candidates,
                            with_roots ? d.root_elements.get() : nullptr, with_roots ? d.root_element_count : 0u,
                            (const unsigned *)nullptr);
     }
@@ -1257,7 +1259,8 @@ void Solver::evaluate_async(bool gradient, double dt, bool line_search, unsigned
     const uint32_t *best = list_pass ? d.best_tri.get() : nullptr;
     // the variants without the tree walk when the list pass answered every query
     auto launch = [&](auto kernel, const unsigned *stop) {
-        KHS_LAUNCH(d, kernel, dim3(blocks), dim3(kBlock), 0, d.stream, rv, cv, cc, best, p, d.q.get(),
+        KHS_LAUNCH(d, kernel, dim3(blocks), dim3(kBlock), 0, d.stream, rv, cv, cc, best,
+                           (const double *)d.best_pair.get(), p, d.q.get(),
                            d.q_old.get(), d.vel.get(), d.q_base.get(), d.dir.get(), d.alpha.get(), d.grad.get(),
                            d.Dblk.get(), d.Ublk.get(), d.partials.get(), stop);
     };

@@ -677,15 +677,43 @@ __device__ inline bool wave_closest(const ColliderView &col, V3 p0, V3 p1, doubl
 // as the ring). One the wavefront cannot take (more than kOverflowSlots, or
 // a ring overflow) is walked by one lane (closest_triangle). Every element
 // of the pass thus gets its best_tri, and the consumers walk nothing.
+// With the triangle the pass also leaves the element's closest pair as the
+// evaluations use it (kPairWords per element: the distance, the segment
+// parameter t and the closest point on the triangle, formed from the
+// barycentric weights as the evaluation forms it), so they compute no
+// segment-triangle distance of their own.
 constexpr int kClosestLanes = 32;
 constexpr uint32_t kOverflowSlots = 16;
+constexpr int kPairWords = 5;
+__device__ __forceinline__ void pair_words(const Closest &c, V3 A, V3 B, V3 C, double (&w)[kPairWords])
+{
+    const V3 bp = add(add(mul(A, c.bary.x), mul(B, c.bary.y)), mul(C, c.bary.z));
+    w[0] = c.distance;
+    w[1] = c.t;
+    w[2] = bp.x;
+    w[3] = bp.y;
+    w[4] = bp.z;
+}
+__device__ __forceinline__ void store_pair(double *best_pair, uint32_t L, const double (&w)[kPairWords])
+{
+#pragma unroll
+    for (int k = 0; k < kPairWords; ++k) best_pair[kPairWords * size_t(L) + k] = w[k];
+}
+// The closest pair of segment (p0, p1) and triangle t, into best_pair.
+__device__ inline void store_pair_of(const ColliderView &col, double *best_pair, uint32_t L, V3 p0, V3 p1, uint32_t t)
+{
+    const V3 A = triangle_vertex(col, t, 0), B = triangle_vertex(col, t, 1), C = triangle_vertex(col, t, 2);
+    double w[kPairWords];
+    pair_words(closest_segment_triangle(p0, p1, A, B, C), A, B, C, w);
+    store_pair(best_pair, L, w);
+}
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, ColliderView col, ContactCache cc,
                                                          StepParams sp, const double *q, const double *q_base,
                                                          const double *dir, double *alpha_at, TrialStep step,
-                                                         uint32_t *best_tri, unsigned long long *block_candidates,
-                                                         const uint32_t *extra, uint32_t extra_count,
-                                                         const unsigned *stop)
+                                                         uint32_t *best_tri, double *best_pair,
+                                                         unsigned long long *block_candidates, const uint32_t *extra,
+                                                         uint32_t extra_count, const unsigned *stop)
 {
     if (stop && *stop) return;
     constexpr int G = kClosestLanes;
@@ -723,6 +751,7 @@ __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, 
                     unsigned long long n = 0;
                     closest_triangle(col, p0, p1, contact_pad_of(sp, L), pair, tri, n);
                     best_tri[L] = tri;
+                    if (tri != kNoTriangle) store_pair_of(col, best_pair, L, p0, p1, tri);
                     if (!(flags[L] & kNodeFixed)) candidates += n;
                 }
             }
@@ -751,22 +780,27 @@ __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, 
         __builtin_amdgcn_wave_barrier();
         double bd = 1.0e300;
         uint32_t bt = kNoTriangle;
+        double bw[kPairWords] = {0, 0, 0, 0, 0};  // this lane's closest pair so far
         for (uint32_t i = lane; i < n; i += G) {
             const uint32_t t = cand[i];
-            const double d = closest_segment_triangle(p0, p1, record_vertex(cc.tri_box, t, 0),
-                                                      record_vertex(cc.tri_box, t, 1), record_vertex(cc.tri_box, t, 2))
-                                 .distance;
-            if (d < bd || (d == bd && t < bt)) {
-                bd = d;
+            const V3 A = record_vertex(cc.tri_box, t, 0), B = record_vertex(cc.tri_box, t, 1),
+                     C = record_vertex(cc.tri_box, t, 2);
+            const Closest c = closest_segment_triangle(p0, p1, A, B, C);
+            if (c.distance < bd || (c.distance == bd && t < bt)) {
+                bd = c.distance;
                 bt = t;
+                pair_words(c, A, B, C, bw);
             }
         }
         __builtin_amdgcn_wave_barrier();
+        const uint32_t own = bt;
         group_closest<G>(bd, bt);
-        if (lane == 0) {
+        // the lane that found the closest triangle leaves it and its pair
+        if (bt == kNoTriangle ? lane == 0 : own == bt) {
             best_tri[L] = bt;
-            if (counted) candidates += n;
+            if (bt != kNoTriangle) store_pair(best_pair, L, bw);
         }
+        if (lane == 0 && counted) candidates += n;
     }
     __builtin_amdgcn_wave_barrier();
     const uint32_t walks = min(deferred_count[wave], kOverflowSlots);
@@ -787,6 +821,7 @@ __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, 
                 closest_triangle(col, p0, p1, pad, pair, bt, count);
             }
             best_tri[L] = bt;
+            if (bt != kNoTriangle) store_pair_of(col, best_pair, L, p0, p1, bt);
             if (!(flags[L] & kNodeFixed)) candidates += count;
         }
         __builtin_amdgcn_wave_barrier();
@@ -1085,7 +1120,8 @@ __device__ inline void store_tile_blocks(const double *st, double *out, long fir
 // triangle, and the tree walk (and its stack) is not compiled in.
 template <bool GRAD, bool LS, bool LISTS, bool MAT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GRAD ? 2 : 1))) void k_evaluate(
-    RodView m, ColliderView col, ContactCache cc, const uint32_t *best_tri, StepParams p, double *q,
+    RodView m, ColliderView col, ContactCache cc, const uint32_t *best_tri, const double *best_pair, StepParams p,
+    double *q,
     const double *q_old, const double *vel, const double *q_base, const double *dir, const double *alpha_at,
     double *grad, double *Dblk, double *Ublk, Partial *partials, const unsigned *stop)
 {
@@ -1145,15 +1181,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GRAD ? 2
     if (elem && p.has_collider && !fixed_i && (fl & kNodeContact)) {
         const V3 p0 = mk(qi[0], qi[1], qi[2]), p1 = mk(qj[0], qj[1], qj[2]);
         Closest pair;
+        V3 listed_bp = mk(0, 0, 0);
         uint32_t tri = 0;
         bool found;
         if constexpr (LISTS) {
-            // the list pass counted the candidates and chose the triangle
+            // the list pass counted the candidates, chose the triangle and
+            // left its closest pair
             tri = cc.count[L] ? best_tri[L] : kNoTriangle;
             found = tri != kNoTriangle;
-            if (found)
-                pair = closest_segment_triangle(p0, p1, triangle_vertex(col, tri, 0), triangle_vertex(col, tri, 1),
-                                                triangle_vertex(col, tri, 2));
+            if (found) {
+                const double *w = best_pair + kPairWords * size_t(L);
+                pair.distance = w[0];
+                pair.t = w[1];
+                listed_bp = mk(w[2], w[3], w[4]);
+            }
         } else {
             unsigned long long count = 0;
             found = closest_triangle(col, p0, p1, contact_pad_of(p, L), pair, tri, count);
@@ -1169,9 +1210,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GRAD ? 2
                 if (owner) acc.infeasible = 1;
             } else if (gap < p.barrier) {
                 if (owner) acc.active += 1;
-                const V3 A = triangle_vertex(col, tri, 0), B = triangle_vertex(col, tri, 1),
-                         C = triangle_vertex(col, tri, 2);
-                const V3 bp = add(add(mul(A, pair.bary.x), mul(B, pair.bary.y)), mul(C, pair.bary.z));
+                V3 bp = listed_bp;
+                if constexpr (!LISTS) {
+                    const V3 A = triangle_vertex(col, tri, 0), B = triangle_vertex(col, tri, 1),
+                             C = triangle_vertex(col, tri, 2);
+                    bp = add(add(mul(A, pair.bary.x), mul(B, pair.bary.y)), mul(C, pair.bary.z));
+                }
                 const double w0 = 1.0 - pair.t, w1 = pair.t;
                 const V3 hp = add(mul(p0, w0), mul(p1, w1));
                 const V3 diff = sub(hp, bp);

@@ -134,7 +134,7 @@ def estimated_device_bytes(nodes: int, strands: int, vertices: int, triangles: i
     animation frame; 96 B per node per rollback slot; about 0.4 KB per
     triangle for the boxes, sort keys and hierarchy; and a fixed 16 MiB for
     small buffers and the sort's temporary storage."""
-    per_node = 212 * 8 + 15 * 4 + 512 * 4 + 64
+    per_node = 217 * 8 + 15 * 4 + 512 * 4 + 64
     per_triangle = 12 + 48 + 128 + 4 * 4 + 112 + 8 + 8 + 16 + 40
     per_vertex = 4 * 24
     return int(nodes * per_node + strands * 32 + triangles * per_triangle + vertices * per_vertex
