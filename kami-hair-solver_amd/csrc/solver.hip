@@ -1371,7 +1371,7 @@ uint32_t Solver::launch_block_solve(unsigned *failed, unsigned *next_failed, con
                            d.solve_order.get(), d.Dblk.get(), d.Ublk.get(), d.grad.get(), d.Cp.get(), d.dp.get(),
                            d.dir.get(), failed, next_failed, d.partials.get(), stop);
     } else if (solve_algorithm_ == 2 && d.solve_in.bytes() < dev::kMaxBufferBytes) {
-        // its factor window is the LDS reservation (static, 27.6 KB)
+        // its factor window is the LDS reservation (static, 38.9 KB)
         KHS_LAUNCH(d, dev::k_block_solve_fused, dim3(blocks), dim3(kWave), 0, d.stream, rv,
                            d.solve_order.get(), (const double *)d.solve_in.get(), d.Cp.get(), d.dp.get(), d.dir.get(),
                            failed, next_failed, d.partials.get(), stop);

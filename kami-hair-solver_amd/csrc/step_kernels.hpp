@@ -2085,7 +2085,11 @@ __global__ __launch_bounds__(kWave) void k_block_solve_twisted(RodView m, const 
 // elimination, against the reference form's inverse, its products and
 // its moves: about a third of the instructions.
 constexpr int kFusedAhead = 4;  // ring slots; even, as the lane halves alternate
-constexpr int kFusedWindow = 20;  // nodes per strand whose factors stay in LDS
+// Nodes per strand whose factors stay in LDS: 28 is the largest window that
+// keeps four of these one-wavefront workgroups on a CU (38.9 KB each);
+// measured on the 6,757-strand groom against 12-40, the solve takes 88 us
+// (20: 93 us, 32: 104 us)
+constexpr int kFusedWindow = 28;
 constexpr int kFusedRecord = 42;  // a node's C' (36, row-major) and e (6)
 
 // v_fmac_f64_dpp ACC, X, -Y: ACC += X(lane L of the row) * -Y
