@@ -2084,7 +2084,7 @@ __global__ __launch_bounds__(kWave) void k_block_solve_twisted(RodView m, const 
 // for the Schur complement and 6 x (1 reciprocal, 5 DPP-FMAs) for the
 // elimination, against the reference form's inverse, its products and
 // its moves: about a third of the instructions.
-constexpr int kFusedAhead = 4;  // ring slots; even, as the lane halves alternate
+constexpr int kFusedAhead = 6;  // ring slots; even, as the lane halves alternate (4: 2.5 us slower, 8: 1 us)
 // Nodes per strand whose factors stay in LDS: 28 is the largest window that
 // keeps four of these one-wavefront workgroups on a CU (38.9 KB each);
 // measured on the 6,757-strand groom against 12-40, the solve takes 88 us
