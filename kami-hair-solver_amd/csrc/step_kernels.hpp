@@ -2153,6 +2153,13 @@ __device__ __forceinline__ void fused_gj_step(double (&col)[6], double inv)
 #undef KHS_SCHUR_K
 #undef KHS_FMAC_NEG
 
+// s_waitcnt vmcnt(0). The compiler's wait at a loop header covers every
+// path into it: when the loads of a prefetch ring's first slot are issued
+// last before the loop, it drains all memory operations at the top of every
+// iteration. Waiting for them once before the loop leaves the header the
+// back edge's wait alone (slots issued several steps earlier).
+__device__ __forceinline__ void wait_memory_all() { __builtin_amdgcn_s_waitcnt(0x0f70); }
+
 // 1 / x from the hardware reciprocal and two Newton steps (pivots are
 // normal numbers here: the shift keeps them above 1e-24 unless the solve
 // fails anyway)
@@ -2238,6 +2245,15 @@ __global__ __launch_bounds__(kWave) void k_block_solve_fused(RodView m, const ui
 #pragma unroll
     for (int r = 0; r < 6; ++r) ring[kFusedAhead - 1][r] = 0.0;
     static_for<kFusedAhead - 1>([&](auto A) { fetch(ring[A], A, std::integral_constant<int, A & 1>{}); });
+    // g of each strand's last kFusedWindow nodes for the back substitution
+    // (lane r < 6 its row), held from here so that its latency is long past
+    // when the back substitution's window steps, which run from LDS, use it
+    double g_window[kFusedWindow];
+    static_for<kFusedWindow>([&](auto T) {
+        const uint32_t n = end - 1 - T;
+        g_window[T] = buffer_load(rG, (acol && T < len) ? 48 * n + 8 * j : kNoAccess);
+    });
+    wait_memory_all();
 
     auto forward = [&](uint32_t t, auto A) __attribute__((always_inline)) {
         constexpr int P = A & 1, B = P ? 6 : 0, prev = (A + kFusedAhead - 1) % kFusedAhead;
@@ -2289,44 +2305,55 @@ __global__ __launch_bounds__(kWave) void k_block_solve_fused(RodView m, const ui
     struct BackSlot {
         double c[6], e, g;
     };
-    // the first kFusedWindow steps (each strand's last nodes) from LDS, the
-    // rest from memory (t is the same for every row, so is the choice)
+    // Two loops, so that no register is loaded from memory on one path and
+    // from LDS on the other (the compiler then drains memory before every
+    // LDS read): the first kFusedWindow steps (each strand's last nodes) take
+    // C' and e from LDS and prefetch g; the rest run from memory through a
+    // ring whose first slots load while the window steps run.
     auto fetch_back = [&](BackSlot &b, uint32_t t) __attribute__((always_inline)) {
         const uint32_t n = end - 1 - t;
         const bool bl = acol && t < len;
-        if (t < uint32_t(kFusedWindow)) {
-            const double *rec = window[g][kFusedWindow - 1 - t];
+        const uint32_t oc = bl ? 288 * n + 48 * r : kNoAccess;
 #pragma unroll
-            for (int k = 0; k < 6; ++k) b.c[k] = rec[6 * r + k];
-            b.e = rec[36 + r];
-        } else {
-            const uint32_t oc = bl ? 288 * n + 48 * r : kNoAccess;
-#pragma unroll
-            for (int k = 0; k < 6; ++k) b.c[k] = buffer_load(rC, oc + 8 * k);
-            b.e = buffer_load(rP, bl ? 48 * n + 8 * r : kNoAccess);
-        }
+        for (int k = 0; k < 6; ++k) b.c[k] = buffer_load(rC, oc + 8 * k);
+        b.e = buffer_load(rP, bl ? 48 * n + 8 * r : kNoAccess);
         b.g = buffer_load(rG, bl ? 48 * n + 8 * r : kNoAccess);
     };
+    const bool beyond = steps > uint32_t(kFusedWindow);  // some strand has nodes outside the window
     BackSlot back[kBackAhead];
 #pragma unroll
-    for (int a = 0; a < kBackAhead; ++a) fetch_back(back[a], a);
+    for (int a = 0; a < kBackAhead; ++a) fetch_back(back[a], kFusedWindow + a);  // no access unless beyond
     double next = 0.0, slope = 0.0, reach2 = 0.0;
-    auto backward = [&](BackSlot &b, uint32_t t) __attribute__((always_inline)) {
+    auto backward = [&](const double (&c)[6], double e, double gv, uint32_t t) __attribute__((always_inline)) {
         const uint32_t n = end - 1 - t;
-        double value = -b.e;
-        if (t > 0) static_for<6>([&](auto K) { value = fma(-b.c[K], row_bcast<K>(next), value); });
+        double value = -e;
+        if (t > 0) static_for<6>([&](auto K) { value = fma(-c[K], row_bcast<K>(next), value); });
         next = value;
         buffer_store(rX, (t < len && ok && acol) ? 48 * n + 8 * j : kNoAccess, value);
         const double sq = value * value;
         const double t2 = (sq + row_bcast<1>(sq)) + row_bcast<2>(sq);
         if (t < len) {
-            slope += b.g * value;
+            slope += gv * value;
             reach2 = fmax(reach2, t2);
         }
-        fetch_back(b, t + kBackAhead);
     };
-    for (uint32_t t0 = 0; t0 < steps; t0 += kBackAhead)
-        static_for<kBackAhead>([&](auto A) { backward(back[A], t0 + A); });
+    static_for<kFusedWindow>([&](auto T) {
+        if (T < steps) {
+            const double *rec = window[g][kFusedWindow - 1 - T];
+            double c[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) c[k] = rec[6 * r + k];
+            backward(c, rec[36 + r], g_window[T], T);
+        }
+    });
+    if (beyond) {
+        wait_memory_all();  // the ring's first slots, loaded during the window steps
+        for (uint32_t t0 = kFusedWindow; t0 < steps; t0 += kBackAhead)
+            static_for<kBackAhead>([&](auto A) {
+                backward(back[A].c, back[A].e, back[A].g, t0 + A);
+                fetch_back(back[A], t0 + A + kBackAhead);
+            });
+    }
     double row_slope = 0.0;
     static_for<6>([&](auto K) { row_slope += row_bcast<K>(acol ? slope : 0.0); });
     Partial acc = empty_partial();
