@@ -1132,20 +1132,46 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GRAD ? 2
     const long L = tile * kEvalTile - 1 + lane;
     const bool in_range = L >= 0 && L < long(m.node_count);
     const bool owner = lane >= 1 && lane <= kEvalTile && in_range;
-    const uint32_t fl = in_range ? m.flags[L] : 0u;
+    // Every input of the lane's node is requested here, before any is used
+    // (a lane outside the rods reads node 0 and discards it), so the loads
+    // are in flight together instead of one round trip per phase below.
+    const size_t Lc = in_range ? size_t(L) : 0;
+    const uint32_t fl_loaded = m.flags[Lc];
+    double q_in[6], dir_in[6], qo_in[6], v_in[6], F_in[9], rest_in[8];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        q_in[k] = LS ? q_base[6 * Lc + k] : q[6 * Lc + k];
+        dir_in[k] = LS ? dir[6 * Lc + k] : 0.0;
+        qo_in[k] = q_old[6 * Lc + k];
+        v_in[k] = vel[6 * Lc + k];
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) F_in[k] = m.frame[9 * Lc + k];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) rest_in[k] = m.rest[8 * Lc + k];
+    const double mass_in = m.mass[Lc], rmass_in = m.rotational_mass[Lc];
+    uint32_t count_in = 0, tri_in = kNoTriangle;
+    if constexpr (LISTS) {
+        count_in = cc.count[Lc];
+        tri_in = best_tri[Lc];
+    }
+    const uint32_t fl = in_range ? fl_loaded : 0u;
     const bool elem = lane <= kEvalTile && (fl & kNodeHasElement) != 0;
     const bool fixed_i = (fl & kNodeFixed) != 0;
-    const bool fixed_j = elem && (m.flags[L + 1] & kNodeFixed) != 0;
+    // the next node's flags and state are the lane above's (an element's
+    // lane is at most 62, its next node is lane + 1's)
+    const uint32_t fl_above = uint32_t(__builtin_amdgcn_update_dpp(0, int(fl), 0x130, 0xf, 0xf, false));
+    const bool fixed_j = elem && (fl_above & kNodeFixed) != 0;
 
     double qi[6] = {0, 0, 0, 0, 0, 0}, qj[6] = {0, 0, 0, 0, 0, 0};
     if (in_range) {
 #pragma unroll
-        for (int k = 0; k < 6; ++k) qi[k] = LS ? alpha * dir[6 * L + k] + q_base[6 * L + k] : q[6 * L + k];
+        for (int k = 0; k < 6; ++k) qi[k] = LS ? alpha * dir_in[k] + q_in[k] : q_in[k];
     }
-    if (elem) {
 #pragma unroll
-        for (int k = 0; k < 6; ++k)
-            qj[k] = LS ? alpha * dir[6 * (L + 1) + k] + q_base[6 * (L + 1) + k] : q[6 * (L + 1) + k];
+    for (int k = 0; k < 6; ++k) {
+        const double above = from_lane_above(qi[k]);
+        qj[k] = elem ? above : 0.0;
     }
     if (LS && owner) {
 #pragma unroll
@@ -1157,18 +1183,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GRAD ? 2
     // Inertia around q_old + dt v + g dt^2 and damping (cuda_backend.cu:883-897).
     double gnode[6] = {0, 0, 0, 0, 0, 0}, dnode[6] = {0, 0, 0, 0, 0, 0};
     if (owner && !fixed_i) {
-        const double kx = m.mass[L] * p.inv_dt2, kr = m.rotational_mass[L] * p.inv_dt2;
-        const double cx = m.mass[L] * p.damping * p.inv_dt, cr = m.rotational_mass[L] * p.damping * p.inv_dt;
+        const double kx = mass_in * p.inv_dt2, kr = rmass_in * p.inv_dt2;
+        const double cx = mass_in * p.damping * p.inv_dt, cr = rmass_in * p.damping * p.inv_dt;
         double obj = 0.0, kin = 0.0;
 #pragma unroll
         for (int d = 0; d < 6; ++d) {
             const double gr = d < 3 ? p.gravity[d] * p.dt * p.dt : 0.0;
-            const double predicted = q_old[6 * L + d] + p.dt * vel[6 * L + d] + gr;
+            const double predicted = qo_in[d] + p.dt * v_in[d] + gr;
             const double delta = qi[d] - predicted;
-            const double damp = qi[d] - q_old[6 * L + d];
+            const double damp = qi[d] - qo_in[d];
             const double k = d < 3 ? kx : kr, c = d < 3 ? cx : cr;
             obj += 0.5 * k * delta * delta + 0.5 * c * damp * damp;
-            if (d < 3) kin += 0.5 * m.mass[L] * damp * damp * p.inv_dt2;
+            if (d < 3) kin += 0.5 * mass_in * damp * damp * p.inv_dt2;
             gnode[d] = k * delta + c * damp;
             dnode[d] = k + c;
         }
@@ -1187,7 +1213,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GRAD ? 2
         if constexpr (LISTS) {
             // the list pass counted the candidates, chose the triangle and
             // left its closest pair
-            tri = cc.count[L] ? best_tri[L] : kNoTriangle;
+            tri = count_in ? tri_in : kNoTriangle;
             found = tri != kNoTriangle;
             if (found) {
                 const double *w = best_pair + kPairWords * size_t(L);
@@ -1250,7 +1276,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GRAD ? 2
     // Cosserat elasticity (cuda_backend.cu:905-919). This lane's node
     // rotation, and the next node's from the lane above.
     NodeRotation ni = {}, nj;
-    if (in_range) node_rotation<GRAD>(qi, m.frame + 9 * L, ni);
+    if (in_range) node_rotation<GRAD>(qi, F_in, ni);
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         nj.R[k] = from_lane_above(ni.R[k]);
@@ -1261,8 +1287,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GRAD ? 2
     double weight[6];
     weights_of<MAT>(p, elem ? L : 0, weight);
     if (elem) {
-        element_strains_from<GRAD>(qi, qj, ni, nj, m.rest + 8 * L, J);
-        const double L0 = m.rest[8 * L];
+        element_strains_from<GRAD>(qi, qj, ni, nj, rest_in, J);
+        const double L0 = rest_in[0];
 #pragma unroll
         for (int s = 0; s < 6; ++s) {
             const double c = (0.5 * L0) * weight[s];
@@ -1277,7 +1303,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GRAD ? 2
     }
 
     if constexpr (GRAD) {
-        const double L0 = elem ? m.rest[8 * L] : 0.0;
+        const double L0 = elem ? rest_in[0] : 0.0;
         double Lw[6], cw[6];
 #pragma unroll
         for (int s = 0; s < 6; ++s) {
