@@ -683,6 +683,7 @@ __device__ inline bool wave_closest(const ColliderView &col, V3 p0, V3 p1, doubl
 // barycentric weights as the evaluation forms it), so they compute no
 // segment-triangle distance of their own.
 constexpr int kClosestLanes = 32;
+static_assert(kClosestLanes <= int(kCacheCapacity), "a group's first list entries lie in the element's list");
 constexpr uint32_t kOverflowSlots = 16;
 constexpr int kPairWords = 5;
 __device__ __forceinline__ void pair_words(const Closest &c, V3 A, V3 B, V3 C, double (&w)[kPairWords])
@@ -736,7 +737,15 @@ __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, 
     unsigned long long candidates = 0;
     for (uint32_t j = group; j < listed + extra_count; j += groups) {
         const uint32_t L = j < listed ? cc.list[j] : extra[j - listed];
+        // The element's count, flags, end points and first kClosestLanes
+        // list entries depend on L alone: requested together, before the
+        // count decides what happens to them.
         const uint32_t c = cc.count[L];
+        const uint32_t fl = flags[L];
+        V3 p0, p1;
+        element_points<MODE>(L, q, q_base, dir, alpha, p0, p1);
+        const uint32_t *tris = cc.tris + size_t(L) * kCacheCapacity;
+        const uint32_t first = tris[lane];
         if (c == 0) continue;
         if (c == kCacheOverflow) {
             if (lane == 0) {
@@ -744,32 +753,27 @@ __global__ __launch_bounds__(kBlock) void k_list_closest(const uint32_t *flags, 
                 if (at < kOverflowSlots) {
                     deferred[wave][at] = L;
                 } else {
-                    V3 p0, p1;
-                    element_points<MODE>(L, q, q_base, dir, alpha, p0, p1);
                     Closest pair;
                     uint32_t tri;
                     unsigned long long n = 0;
                     closest_triangle(col, p0, p1, contact_pad_of(sp, L), pair, tri, n);
                     best_tri[L] = tri;
                     if (tri != kNoTriangle) store_pair_of(col, best_pair, L, p0, p1, tri);
-                    if (!(flags[L] & kNodeFixed)) candidates += n;
+                    if (!(fl & kNodeFixed)) candidates += n;
                 }
             }
             continue;
         }
-        const bool counted = !(flags[L] & kNodeFixed);
+        const bool counted = !(fl & kNodeFixed);
         const double pad = contact_pad_of(sp, L);
-        V3 p0, p1;
-        element_points<MODE>(L, q, q_base, dir, alpha, p0, p1);
         const V3 lo = vmin(p0, p1), hi = vmax(p0, p1);
-        const uint32_t *tris = cc.tris + size_t(L) * kCacheCapacity;
         uint32_t n = 0;
         for (uint32_t k0 = 0; k0 < c; k0 += G) {
             const uint32_t k = k0 + lane;
             uint32_t t = 0;
             bool hit = false;
             if (k < c) {
-                t = tris[k];
+                t = k0 == 0 ? first : tris[k];
                 const double *b = cc.tri_box + kTriRecord * size_t(t);
                 hit = overlap(lo, hi, mk(b[0], b[1], b[2]), mk(b[3], b[4], b[5]), pad);
             }
@@ -1000,22 +1004,29 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
             if (threadIdx.x == 0) pair_count = 0;
             __syncthreads();
             const uint32_t j = first + my;
-            uint32_t L = 0, c = 0;
-            if (j < listed) {
-                L = cc.list[j];
-                if (cc.pending_epoch[L] != cc.epoch) {
-                    c = cc.count[L];
-                    if (c == kCacheOverflow || (flags[L] & kNodeFixed)) c = 0;
-                }
+            const uint32_t L = j < listed ? cc.list[j] : 0u;
+            // everything of element L is requested together, before the
+            // mark, count and flags decide whether it is bounded here (a
+            // group past the end reads element 0, which exists when any is
+            // listed, and discards it)
+            const uint32_t mark = cc.pending_epoch[L], count = cc.count[L], fl = flags[L];
+            const V3 q0 = load3(q + 6 * size_t(L)), q1 = load3(q + 6 * size_t(L + 1));
+            const V3 e0 = load3(dir + 6 * size_t(L)), e1 = load3(dir + 6 * size_t(L + 1));
+            const uint32_t *tris = cc.tris + size_t(L) * kCacheCapacity;
+            const uint32_t first_tri = tris[lane];
+            uint32_t c = 0;
+            if (j < listed && mark != cc.epoch) {
+                c = count;
+                if (c == kCacheOverflow || (fl & kNodeFixed)) c = 0;
             }
             V3 p0 = mk(0, 0, 0), p1 = p0, d0 = p0, d1 = p0;
             double speed = 0.0;
             const double target = sweep_target_of(p, L);
             if (c > 0) {
-                p0 = load3(q + 6 * size_t(L));
-                p1 = load3(q + 6 * size_t(L + 1));
-                d0 = load3(dir + 6 * size_t(L));
-                d1 = load3(dir + 6 * size_t(L + 1));
+                p0 = q0;
+                p1 = q1;
+                d0 = e0;
+                d1 = e1;
                 speed = fmax(norm(d0), norm(d1));
                 if (speed < 1e-30) c = 0;
             }
@@ -1027,13 +1038,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
             }
             const V3 lo = vmin(vmin(p0, p1), vmin(add(p0, d0), add(p1, d1)));
             const V3 hi = vmax(vmax(p0, p1), vmax(add(p0, d0), add(p1, d1)));
-            const uint32_t *tris = cc.tris + size_t(L) * kCacheCapacity;
             for (uint32_t k0 = 0; k0 < c; k0 += G) {
                 const uint32_t k = k0 + lane;
                 uint32_t t = 0;
                 bool hit = false;
                 if (k < c) {
-                    t = tris[k];
+                    t = k0 == 0 ? first_tri : tris[k];
                     const double *b = cc.tri_box + kTriRecord * size_t(t);
                     hit = overlap(lo, hi, mk(b[0], b[1], b[2]), mk(b[3], b[4], b[5]), target);
                 }
