@@ -510,12 +510,19 @@ __global__ __launch_bounds__(kBlock) void k_contact_check(RodView m, ColliderVie
     // every lane stays to the end: the gathers use the whole wavefront
     bool need = false;
     V3 nlo = mk(0, 0, 0), nhi = nlo;
-    if (L < m.node_count) {
+    if (L < m.node_count && m.node_count >= 2) {
+        // The flags, the element's state and its cached box are requested
+        // together (the last node, which has no element, reads the element
+        // before it and discards it).
+        const uint32_t Lq = L + 1 < m.node_count ? L : m.node_count - 2;
         const uint32_t fl = m.flags[L];
+        V3 lo, hi;
+        query_box<MODE>(Lq, q, q_base, dir, alpha, lo, hi);
+        double box[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) box[k] = cc.box[6 * size_t(Lq) + k];
         if ((fl & kNodeHasElement) && (fl & kNodeContact) && (!(fl & kNodeFixed) || with_roots)) {
-            V3 lo, hi;
-            query_box<MODE>(L, q, q_base, dir, alpha, lo, hi);
-            if (!box_covered(cc.box + 6 * size_t(L), lo, hi, pad)) {
+            if (!box_covered(box, lo, hi, pad)) {
                 const double grow = pad + cc.skin;
                 need = true;
                 nlo = mk(lo.x - grow, lo.y - grow, lo.z - grow);
@@ -2642,9 +2649,10 @@ __global__ __launch_bounds__(kBlock) void k_ccd_slope(RodView m, ColliderView co
         for (int k = 0; k < 6; ++k) gd += grad[6 * size_t(L) + k] * dir[6 * size_t(L) + k];
         acc.sum[kSlope] = gd;
         const uint32_t fl = m.flags[L];
+        const uint32_t count = list_pass ? cc.count[L] : 0u;  // requested beside the flags
         // elements with a list were bounded by the list pass
         if (p.has_collider && (fl & kNodeHasElement) && (fl & kNodeContact) && !(fl & kNodeFixed) &&
-            !(list_pass && cc.count[L] != kCacheOverflow)) {
+            !(list_pass && count != kCacheOverflow)) {
             p0 = load3(q + 6 * size_t(L));
             p1 = load3(q + 6 * size_t(L + 1));
             d0 = load3(dir + 6 * size_t(L));
