@@ -936,11 +936,11 @@ void Solver::refresh_contact_cache(int mode, bool full, bool swept_leaves, bool 
         if (mode == dev::kAtState)
             KHS_LAUNCH(d, dev::k_contact_check<dev::kAtState>, grid, block, 0, d.stream, d.rods(), col, cc,
                                d.q.get(), d.q_base.get(), d.dir.get(), d.alpha.get(), dev::TrialStep{}, pad,
-                               with_roots ? 1 : 0);
+                               with_roots ? 1 : 0, (const double *)nullptr, (Partial *)nullptr);
         else if (mode == dev::kAtTrial) {
             KHS_LAUNCH(d, dev::k_contact_check<dev::kAtTrial>, grid, block, 0, d.stream, d.rods(), col, cc,
                                d.q.get(), d.q_base.get(), d.dir.get(), d.alpha.get(), d.trial_step, pad,
-                               with_roots ? 1 : 0);
+                               with_roots ? 1 : 0, (const double *)nullptr, (Partial *)nullptr);
             d.trial_step.source = dev::kAlphaStored;
         }
         else {
@@ -954,7 +954,7 @@ void Solver::refresh_contact_cache(int mode, bool full, bool swept_leaves, bool 
             }
             KHS_LAUNCH(d, dev::k_contact_check<dev::kSwept>, grid, block, 0, d.stream, d.rods(), col, d.cache(),
                                d.q.get(), d.q_base.get(), d.dir.get(), d.alpha.get(), dev::TrialStep{}, pad,
-                               with_roots ? 1 : 0);
+                               with_roots ? 1 : 0, (const double *)d.grad.get(), d.partials.get());
         }
     }
     HIP_TRY(hipGetLastError());
@@ -1469,9 +1469,13 @@ KhsResult Solver::optimize_substep(double dt)
         phase_.store(KHS_PHASE_CCD);
         hipEvent_t t0 = d.tic();
         const bool list_pass = d.cache_valid;
+        // With lists the swept check leaves the slope's partials and the
+        // bound kernel bounds every listed element (overflowed lists too);
+        // without, one pass takes the slope and walks the tree per element.
         if (list_pass) swept_bound_async(cp, d.skip());
-        KHS_LAUNCH(d, dev::k_ccd_slope, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, d.collider(),
-                           d.cache(), list_pass ? 1 : 0, cp, d.q.get(), d.grad.get(), d.dir.get(), d.partials.get());
+        else
+            KHS_LAUNCH(d, dev::k_ccd_slope, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv, d.collider(), cp,
+                               d.q.get(), d.grad.get(), d.dir.get(), d.partials.get());
         HIP_TRY(hipGetLastError());
         d.toc(t0, d.totals.collision_ms, d.totals.collision_launches);
         d.reduce_async(d.node_blocks(), DeviceState::kCcdSlot, dev::ConvergenceTest{0.0, 0.0, 0.0, nullptr},
@@ -2523,8 +2527,8 @@ KhsResult Solver::friction_stage(double dt)
 }
 
 // The line search's step bound along a given direction, by the step's own
-// path: lists gathered at the state, the swept-box refresh, the list pass
-// and the pass over overflowed lists.
+// path: lists gathered at the state, then the swept check and the bound
+// kernel (lists, new gathers and overflowed lists).
 KhsResult Solver::step_bound(const double *direction, uint64_t count, double *alpha)
 {
     if (!built_ || !direction || !alpha || count != dof_count()) return KHS_ERROR_INVALID_ARGUMENT;
@@ -2539,9 +2543,6 @@ KhsResult Solver::step_bound(const double *direction, uint64_t count, double *al
     refresh_contact_cache(dev::kAtState, true, true);
     const StepParams cp = step_params(1.0);
     swept_bound_async(cp, nullptr);
-    KHS_LAUNCH(d, dev::k_ccd_slope, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, d.rods(), d.collider(),
-                       d.cache(), 1, cp, d.q.get(), d.grad.get(), d.dir.get(), d.partials.get());
-    HIP_TRY(hipGetLastError());
     d.reduce_async(d.node_blocks(), DeviceState::kCcdSlot, dev::ConvergenceTest{0.0, 0.0, 0.0, nullptr},
                    dev::ListPartials{nullptr, d.list_alpha.get(), kSweptBoundBlocks});
     *alpha = d.read_back().ccd.alpha;

@@ -498,7 +498,7 @@ template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_contact_check(RodView m, ColliderView col, ContactCache cc,
                                                           const double *q, const double *q_base, const double *dir,
                                                           double *alpha_at, TrialStep step, double pad,
-                                                          int with_roots)
+                                                          int with_roots, const double *grad, Partial *partials)
 {
     __shared__ int queues[kTilesPerBlock][kGatherQueue];
     volatile int *queue = queues[threadIdx.x / kWave];
@@ -536,7 +536,19 @@ __global__ __launch_bounds__(kBlock) void k_contact_check(RodView m, ColliderVie
             }
         }
     }
-    if constexpr (MODE == kSwept) return;
+    if constexpr (MODE == kSwept) {
+        // The slope g.d of the step the swept pass bounds, a partial per
+        // block (the step-bound kernel after it leaves the smallest step).
+        Partial acc = empty_partial();
+        if (L < m.node_count) {
+            double gd = 0.0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) gd += grad[6 * size_t(L) + k] * dir[6 * size_t(L) + k];
+            acc.sum[kSlope] = gd;
+        }
+        block_reduce(acc, partials + blockIdx.x);
+        return;
+    }
     for (unsigned long long mask = __ballot(need); mask; mask &= mask - 1) {
         const int src = __ffsll((long long)mask) - 1;
         const uint32_t E = uint32_t(__shfl(int(L), src, kWave));
@@ -881,8 +893,8 @@ constexpr int kCcdBatch = kBlock / kCcdLanes;
 // overlapping the element's new box into LDS (an empty result keeps a box
 // fifteen skins wider still when that is empty too; a list over capacity
 // is gathered again for the thin box), stores the list, and bounds the
-// element right away from it (an overflowed list is walked by k_ccd_slope
-// after this kernel, as every overflowed list is). The other blocks bound
+// element right away from it (an overflowed list by the wavefront's walk of
+// the swept box, overflow_bound). The other blocks bound
 // every listed element the check did not queue from its list: kBlock /
 // kCcdLanes elements at a time, each group of kCcdLanes lanes box-tests one
 // element's list against its swept segment box and appends the overlapping
@@ -899,6 +911,35 @@ constexpr int kCcdBatch = kBlock / kCcdLanes;
 // The gathers run also when the Newton loop has stopped, so every list
 // matches its box.
 constexpr unsigned kSweptGatherBlocks = 512;
+// Step bound of an element whose list overflowed, by its wavefront: a
+// cooperative walk of the swept box, each lane advancing against the leaves
+// it reaches (a loop stops once it cannot lower the lane's minimum, which
+// leaves the minimum unchanged); an incomplete walk (ring overflow) is
+// redone by lane 0 alone. Every lane returns the element's bound.
+__device__ inline double overflow_bound(const ColliderView &col, V3 p0, V3 p1, V3 d0, V3 d1, double speed, V3 lo,
+                                        V3 hi, double target, volatile int *queue)
+{
+    double limit = 1.0;
+    const bool ok = wave_walk(col, lo, hi, target, queue, kGatherQueue, [&](uint32_t t) {
+        limit = fmin(limit, advancement_limit(p0, p1, d0, d1, speed, triangle_vertex(col, t, 0),
+                                              triangle_vertex(col, t, 1), triangle_vertex(col, t, 2), target,
+                                              limit / 0.9));
+    });
+#pragma unroll
+    for (int o = kWave / 2; o >= 1; o >>= 1) limit = fmin(limit, __shfl_xor(limit, o, kWave));
+    if (!ok) {
+        double full = 1.0;
+        if ((threadIdx.x & (kWave - 1)) == 0)
+            bvh_query(col.bvh, internal_count(col), col.leaf_box, lo, hi, target, [&](uint32_t t) {
+                const uint32_t *t3 = col.triangles + 3 * size_t(t);
+                full = fmin(full, advancement_limit(p0, p1, d0, d1, speed, vertex(col, t3[0]), vertex(col, t3[1]),
+                                                    vertex(col, t3[2]), target));
+            });
+        limit = __shfl(full, 0, kWave);
+    }
+    return limit;
+}
+
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_swept_ccd(const uint32_t *flags, ColliderView col, ContactCache cc,
                                                       StepParams p, const double *q, const double *dir,
                                                       double *block_alpha, unsigned long long *bound, int slot,
@@ -956,9 +997,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
                 }
             }
             // the element's step bound from its new list (the swept check
-            // queues only elements with a free left node; an overflowed list
-            // is walked by k_ccd_slope, as every overflowed list is)
-            if (stopped || n == 0 || n > kCacheCapacity) continue;
+            // queues only elements with a free left node)
+            if (stopped || n == 0) continue;
             const V3 p0 = load3(q + 6 * size_t(L)), p1 = load3(q + 6 * size_t(L + 1));
             const V3 d0 = load3(dir + 6 * size_t(L)), d1 = load3(dir + 6 * size_t(L + 1));
             const double speed = fmax(norm(d0), norm(d1));
@@ -967,6 +1007,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
             const V3 slo = vmin(vmin(p0, p1), vmin(add(p0, d0), add(p1, d1)));
             const V3 shi = vmax(vmax(p0, p1), vmax(add(p0, d0), add(p1, d1)));
             double own = 1.0;
+            if (n > kCacheCapacity) {
+                // an overflowed list: the wavefront walks the swept box
+                limit = fmin(limit, overflow_bound(col, p0, p1, d0, d1, speed, slo, shi, target, queue));
+                __builtin_amdgcn_wave_barrier();
+                continue;
+            }
             uint32_t taken = 0;
             for (uint32_t k0 = 0; k0 < n; k0 += kWave) {
                 const uint32_t k = k0 + lane;
@@ -1022,20 +1068,32 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
             const uint32_t *tris = cc.tris + size_t(L) * kCacheCapacity;
             const uint32_t first_tri = tris[lane];
             uint32_t c = 0;
-            if (j < listed && mark != cc.epoch) {
+            bool overflowed = false;
+            if (j < listed && mark != cc.epoch && !(fl & kNodeFixed)) {
                 c = count;
-                if (c == kCacheOverflow || (fl & kNodeFixed)) c = 0;
+                overflowed = c == kCacheOverflow;
+                if (overflowed) c = 0;
             }
             V3 p0 = mk(0, 0, 0), p1 = p0, d0 = p0, d1 = p0;
             double speed = 0.0;
             const double target = sweep_target_of(p, L);
-            if (c > 0) {
+            if (c > 0 || overflowed) {
                 p0 = q0;
                 p1 = q1;
                 d0 = e0;
                 d1 = e1;
                 speed = fmax(norm(d0), norm(d1));
                 if (speed < 1e-30) c = 0;
+            }
+            if (overflowed && lane == 0 && speed >= 1e-30) {
+                // an overflowed list (rare): this lane walks the swept box
+                const V3 lo = vmin(vmin(p0, p1), vmin(add(p0, d0), add(p1, d1)));
+                const V3 hi = vmax(vmax(p0, p1), vmax(add(p0, d0), add(p1, d1)));
+                bvh_query(col.bvh, internal_count(col), col.leaf_box, lo, hi, target, [&](uint32_t t) {
+                    const uint32_t *t3 = col.triangles + 3 * size_t(t);
+                    limit = fmin(limit, advancement_limit(p0, p1, d0, d1, speed, vertex(col, t3[0]),
+                                                          vertex(col, t3[1]), vertex(col, t3[2]), target));
+                });
             }
             if (c > 0 && lane == 0) {
                 const double v[14] = {p0.x, p0.y, p0.z, p1.x, p1.y, p1.z, d0.x, d0.y, d0.z, d1.x, d1.y, d1.z, speed,
@@ -2628,86 +2686,36 @@ __global__ void k_copy(double *dst, const double *src, size_t count)
     if (i < count) dst[i] = src[i];
 }
 
-// ---------------------------------------------------------------- CCD
-// Line-search step bound: conservative advancement of each contact-enabled
-// segment along the direction against the static collider
-// (cuda_backend.cu:989-1000), plus the slope g.d.
-__global__ __launch_bounds__(kBlock) void k_ccd_slope(RodView m, ColliderView col, ContactCache cc, int list_pass,
-                                                      StepParams p, const double *q, const double *grad,
-                                                      const double *dir, Partial *partials)
+// Slope g.d and the step bound of every contact element by its own tree
+// walk, one lane each: the path without candidate lists (with lists the
+// swept check takes the slope and k_swept_ccd the bound).
+__global__ __launch_bounds__(kBlock) void k_ccd_slope(RodView m, ColliderView col, StepParams p, const double *q,
+                                                      const double *grad, const double *dir, Partial *partials)
 {
-    __shared__ int queues[kTilesPerBlock][kGatherQueue];
-    volatile int *queue = queues[threadIdx.x / kWave];
     const uint32_t L = blockIdx.x * blockDim.x + threadIdx.x;
     Partial acc = empty_partial();
-    bool need = false;
-    V3 p0 = mk(0, 0, 0), p1 = p0, d0 = p0, d1 = p0;
-    double speed = 0.0, target = p.sweep_target;
     if (L < m.node_count) {
         double gd = 0.0;
 #pragma unroll
         for (int k = 0; k < 6; ++k) gd += grad[6 * size_t(L) + k] * dir[6 * size_t(L) + k];
         acc.sum[kSlope] = gd;
         const uint32_t fl = m.flags[L];
-        const uint32_t count = list_pass ? cc.count[L] : 0u;  // requested beside the flags
-        // elements with a list were bounded by the list pass
-        if (p.has_collider && (fl & kNodeHasElement) && (fl & kNodeContact) && !(fl & kNodeFixed) &&
-            !(list_pass && count != kCacheOverflow)) {
-            p0 = load3(q + 6 * size_t(L));
-            p1 = load3(q + 6 * size_t(L + 1));
-            d0 = load3(dir + 6 * size_t(L));
-            d1 = load3(dir + 6 * size_t(L + 1));
-            speed = fmax(norm(d0), norm(d1));
-            need = speed >= 1e-30;
-            target = sweep_target_of(p, L);
-        }
-    }
-    // After a list pass the few elements left (lists that overflowed) each
-    // walk the swept box with their whole wavefront, every lane advancing
-    // against the leaves it reaches (each loop stops once it cannot lower
-    // the lane's own minimum; the minimum is unchanged). An incomplete walk
-    // (ring overflow) falls back to the element's own lane. Without lists
-    // every contact element walks, one lane each.
-    const int lane = threadIdx.x & (kWave - 1);
-    if (!list_pass && need) {
-        const V3 lo = vmin(vmin(p0, p1), vmin(add(p0, d0), add(p1, d1)));
-        const V3 hi = vmax(vmax(p0, p1), vmax(add(p0, d0), add(p1, d1)));
-        double limit = 1.0;
-        bvh_query(col.bvh, internal_count(col), col.leaf_box, lo, hi, target, [&](uint32_t t) {
-            const uint32_t *t3 = col.triangles + 3 * size_t(t);
-            limit = fmin(limit, advancement_limit(p0, p1, d0, d1, speed, vertex(col, t3[0]), vertex(col, t3[1]),
-                                                  vertex(col, t3[2]), target));
-        });
-        acc.alpha = limit;
-        need = false;
-    }
-    for (unsigned long long mask = __ballot(need); mask; mask &= mask - 1) {
-        const int src = __ffsll((long long)mask) - 1;
-        const V3 a0 = mk(__shfl(p0.x, src, kWave), __shfl(p0.y, src, kWave), __shfl(p0.z, src, kWave));
-        const V3 a1 = mk(__shfl(p1.x, src, kWave), __shfl(p1.y, src, kWave), __shfl(p1.z, src, kWave));
-        const V3 e0 = mk(__shfl(d0.x, src, kWave), __shfl(d0.y, src, kWave), __shfl(d0.z, src, kWave));
-        const V3 e1 = mk(__shfl(d1.x, src, kWave), __shfl(d1.y, src, kWave), __shfl(d1.z, src, kWave));
-        const double sp = __shfl(speed, src, kWave), tg = __shfl(target, src, kWave);
-        const V3 lo = vmin(vmin(a0, a1), vmin(add(a0, e0), add(a1, e1)));
-        const V3 hi = vmax(vmax(a0, a1), vmax(add(a0, e0), add(a1, e1)));
-        double limit = 1.0;
-        const bool ok = wave_walk(col, lo, hi, tg, queue, kGatherQueue, [&](uint32_t t) {
-            limit = fmin(limit, advancement_limit(a0, a1, e0, e1, sp, triangle_vertex(col, t, 0),
-                                                  triangle_vertex(col, t, 1), triangle_vertex(col, t, 2), tg,
-                                                  limit / 0.9));
-        });
-#pragma unroll
-        for (int o = kWave / 2; o >= 1; o >>= 1) limit = fmin(limit, __shfl_xor(limit, o, kWave));
-        if (lane == src) {
-            if (!ok) {
-                limit = 1.0;
-                bvh_query(col.bvh, internal_count(col), col.leaf_box, lo, hi, tg, [&](uint32_t t) {
+        if (p.has_collider && (fl & kNodeHasElement) && (fl & kNodeContact) && !(fl & kNodeFixed)) {
+            const V3 p0 = load3(q + 6 * size_t(L)), p1 = load3(q + 6 * size_t(L + 1));
+            const V3 d0 = load3(dir + 6 * size_t(L)), d1 = load3(dir + 6 * size_t(L + 1));
+            const double speed = fmax(norm(d0), norm(d1));
+            const double target = sweep_target_of(p, L);
+            if (speed >= 1e-30) {
+                const V3 lo = vmin(vmin(p0, p1), vmin(add(p0, d0), add(p1, d1)));
+                const V3 hi = vmax(vmax(p0, p1), vmax(add(p0, d0), add(p1, d1)));
+                double limit = 1.0;
+                bvh_query(col.bvh, internal_count(col), col.leaf_box, lo, hi, target, [&](uint32_t t) {
                     const uint32_t *t3 = col.triangles + 3 * size_t(t);
-                    limit = fmin(limit, advancement_limit(a0, a1, e0, e1, sp, vertex(col, t3[0]), vertex(col, t3[1]),
-                                                          vertex(col, t3[2]), tg));
+                    limit = fmin(limit, advancement_limit(p0, p1, d0, d1, speed, vertex(col, t3[0]),
+                                                          vertex(col, t3[1]), vertex(col, t3[2]), target));
                 });
+                acc.alpha = limit;
             }
-            acc.alpha = limit;
         }
     }
     block_reduce(acc, partials + blockIdx.x);
