@@ -150,6 +150,52 @@ def test_step_bound_plane_matches_oracle(oracle_lib, gpu_lib):
     assert out[1] == out[0]
 
 
+def test_step_bound_with_overflowed_lists_matches_oracle(oracle_lib, gpu_lib):
+    """A strand 1 mm above a plane of 0.4 mm quads, where an element's thin
+    box holds more triangles than a list can: the elements whose lists
+    overflow at the state, and those whose swept box leaves their list and
+    overflow when regathered, are bounded by walks of the swept box."""
+    import test_gpu_parity as par
+
+    print("[gpu] step bound with overflowed lists, strand over a 0.4 mm grid", flush=True)
+    verts, tris = par._grid_plane(200, 4.0e-4)
+    pts = np.array([[-0.03 + 0.01 * k, 0.0013 * k, 0.006] for k in range(7)])
+    solvers = []
+    for lib in (oracle_lib, gpu_lib):
+        desc = kn.default_desc(lib)
+        desc.substeps = 4
+        desc.fixed_root_nodes = 1
+        desc.maximum_element_length = 0.01
+        mat = kn.default_material(lib)
+        mat.radius = 1.0e-4
+        mat.barrier_distance = 2.0e-3
+        s = kn.HairSolver(lib, desc)
+        s.check(s.set_hair(pts, [0, len(pts)], mat))
+        s.check(s.set_collider(verts, tris))
+        s.check(s.build())
+        solvers.append(s)
+    q, _ = solvers[0].get_state()
+    x = q.reshape(-1, 6)
+    _, _, _, flags = solvers[0].node_model()
+    free = (flags & 1) == 0
+    x[free, 2] = 1.0e-3
+    rng = np.random.default_rng(12)
+    for trial in range(3):
+        d = np.zeros_like(q)
+        dv = d.reshape(-1, 6)
+        dv[free, 2] = -rng.uniform(1.5e-3, 3.0e-3, int(free.sum()))
+        dv[free, :2] = rng.normal(scale=2.0e-3, size=(int(free.sum()), 2))
+        alphas = []
+        for s in solvers:
+            s.check(s.set_state(q, np.zeros_like(q), q))
+            alphas.append(s.step_bound(d))
+        lists = solvers[1].contact_list_stats()
+        print(f"trial {trial}: alpha oracle {alphas[0]!r} gpu {alphas[1]!r}; overflowed lists {lists.overflowed}")
+        assert lists.overflowed > 0
+        assert alphas[0] < 1.0
+        assert abs(alphas[1] - alphas[0]) <= 1e-12
+
+
 def test_sweep_matches_oracle(oracle_lib, gpu_lib):
     """The collider swelling outward at the pulled-in state: small swells
     pass, larger ones fail on the same (element, triangle)."""
