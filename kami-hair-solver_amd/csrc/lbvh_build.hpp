@@ -302,12 +302,26 @@ __global__ void k_refit(uint32_t count, const uint32_t *sorted_ids, const double
 // about 120 us less per substep.
 constexpr int kRefitChunk = 1024;
 
+// A node whose range lies inside a chunk has one of its range's ends as
+// its index (Karras), so the chunk's nodes are among internal nodes
+// [base, base + kRefitChunk): their ranges and parent links are read into
+// LDS once, and the climb reads no global memory but the leaf boxes.
 struct RefitLocal {
     double slot[kRefitChunk][12];  // child boxes of the chunk's nodes
     unsigned count[kRefitChunk];
+    int first[kRefitChunk], last[kRefitChunk], parent[kRefitChunk];  // of internal nodes base + i
 };
 
-__device__ inline void refit_local_init(RefitLocal &s) { s.count[threadIdx.x] = 0; }
+__device__ inline void refit_local_init(RefitLocal &s, uint32_t count, const int *parent_internal,
+                                        const BvhNode *nodes)
+{
+    const uint32_t i = threadIdx.x, node = blockIdx.x * kRefitChunk + i;
+    s.count[i] = 0;
+    const bool inside = count > 1 && node < count - 1;
+    s.first[i] = inside ? nodes[node].range[0] : -1;
+    s.last[i] = inside ? nodes[node].range[1] : -1;
+    s.parent[i] = inside ? parent_internal[node] : -1;
+}
 
 __device__ inline void refit_local_climb(RefitLocal &s, uint32_t count, const uint32_t *sorted_ids,
                                          const double *tri_box, const int *parent_internal, const int *parent_leaf,
@@ -324,7 +338,9 @@ __device__ inline void refit_local_climb(RefitLocal &s, uint32_t count, const ui
     int link = parent_leaf[k];
     for (int level = 0; level < 128; ++level) {
         const int parent = link >> 1, side = link & 1;
-        const bool local = nodes[parent].range[0] >= int(base) && nodes[parent].range[1] < int(base) + kRefitChunk;
+        const int li = parent - int(base);
+        const bool local = li >= 0 && li < kRefitChunk && s.first[li] >= int(base) &&
+                           s.last[li] < int(base) + kRefitChunk;
         if (!local) {
 #pragma unroll
             for (int c = 0; c < 6; ++c) nodes[parent].box[6 * side + c] = box[c];
@@ -333,7 +349,6 @@ __device__ inline void refit_local_climb(RefitLocal &s, uint32_t count, const ui
                     uint32_t(parent);
             return;
         }
-        const int li = parent - int(base);
 #pragma unroll
         for (int c = 0; c < 6; ++c)
             __hip_atomic_store(&s.slot[li][6 * side + c], box[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -349,7 +364,7 @@ __device__ inline void refit_local_climb(RefitLocal &s, uint32_t count, const ui
             box[c] = fmin(both[c], both[6 + c]);
             box[3 + c] = fmax(both[3 + c], both[9 + c]);
         }
-        link = parent_internal[parent];
+        link = s.parent[li];
         if (link < 0) return;
     }
 }
@@ -361,7 +376,7 @@ __global__ __launch_bounds__(kRefitChunk) void k_refit_local(uint32_t count, con
                                                              unsigned *ready_count)
 {
     __shared__ RefitLocal s;
-    refit_local_init(s);
+    refit_local_init(s, count, parent_internal, nodes);
     __syncthreads();
     refit_local_climb(s, count, sorted_ids, tri_box, parent_internal, parent_leaf, nodes, counters, ready,
                       ready_count);

@@ -44,7 +44,7 @@ static void refit_two_phase(uint32_t T, const std::vector<uint32_t> &arrival, co
         for (uint32_t t = 0; t < uint32_t(kRefitChunk); ++t) {
             threadIdx.x = t;
             blockIdx.x = b;
-            refit_local_init(local[b]);
+            refit_local_init(local[b], T, parent_internal, nodes);
         }
     for (uint32_t k : arrival) {
         threadIdx.x = k % kRefitChunk;
