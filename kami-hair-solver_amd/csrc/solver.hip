@@ -1527,6 +1527,7 @@ KhsResult Solver::optimize_substep(double dt)
         ++step_stats_.linear_solves;
         // with the forced rule the first direction already is steepest descent
         bool accepted = false, block_direction = direction_mode_ == 0, counted = false;
+        double step_dir_norm2 = 0.0;  // |d|^2 of the accepted trial's direction
         double alpha = 1.0;
         for (int attempt = 0; attempt < 2;) {
             if (!have_view) {
@@ -1567,6 +1568,7 @@ KhsResult Solver::optimize_substep(double dt)
                     if (trial.feasible && std::isfinite(trial.objective) &&
                         trial.objective <= current.objective + 1e-4 * alpha * slope) {
                         accepted = true;
+                        step_dir_norm2 = trial.dir_norm2;
                         break;
                     }
                     alpha *= 0.5;
@@ -1601,7 +1603,7 @@ KhsResult Solver::optimize_substep(double dt)
         }
         ahead = d.read_back();
         current = unpack(ahead.grad);
-        const double dnorm = std::sqrt(current.dir_norm2);
+        const double dnorm = std::sqrt(step_dir_norm2);
         step_stats_.increment_norm = alpha * dnorm;
         step_stats_.accepted_step_length = alpha;
         ++step_stats_.newton_iterations;

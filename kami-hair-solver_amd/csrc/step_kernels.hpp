@@ -1224,8 +1224,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GRAD ? 2
     }
 #pragma unroll
     for (int k = 0; k < 9; ++k) F_in[k] = m.frame[9 * Lc + k];
+    // (the assembly, which holds more registers live here, requests the rest
+    // strains below, before its node rotation)
+    if constexpr (!GRAD) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) rest_in[k] = m.rest[8 * Lc + k];
+        for (int k = 0; k < 8; ++k) rest_in[k] = m.rest[8 * Lc + k];
+    }
     const double mass_in = m.mass[Lc], rmass_in = m.rotational_mass[Lc];
     uint32_t count_in = 0, tri_in = kNoTriangle;
     if constexpr (LISTS) {
@@ -1352,6 +1356,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GRAD ? 2
 
     // Cosserat elasticity (cuda_backend.cu:905-919). This lane's node
     // rotation, and the next node's from the lane above.
+    if constexpr (GRAD) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) rest_in[k] = m.rest[8 * Lc + k];
+    }
     NodeRotation ni = {}, nj;
     if (in_range) node_rotation<GRAD>(qi, F_in, ni);
 #pragma unroll
@@ -1377,6 +1385,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GRAD ? 2
         acc.sum[kElastic] += elastic;
         acc.sum[kObjective] += contact_energy;
         acc.sum[kContact] += contact_energy;
+        if (LS) {
+            // |d|^2 of the step this trial takes (the host's increment test)
+            double dd = 0.0;
+#pragma unroll
+            for (int d = 0; d < 6; ++d) dd += dir_in[d] * dir_in[d];
+            acc.sum[kDirNorm2] += dd;
+        }
     }
 
     if constexpr (GRAD) {
@@ -1429,17 +1444,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GRAD ? 2
 #pragma unroll
         for (int k = 0; k < 3; ++k) dg_left[k] = from_lane_below(dg[3 + k]);
         if (owner) {
-            double gg = 0.0, dd = 0.0;
+            double gg = 0.0;
 #pragma unroll
             for (int d = 0; d < 6; ++d) {
                 const double gv = fixed_i ? 0.0 : g_out[d];
                 grad[6 * L + d] = gv;
                 gg += gv * gv;
-                const double dv = dir[6 * L + d];
-                dd += dv * dv;
             }
             acc.sum[kGradNorm2] += gg;
-            acc.sum[kDirNorm2] += dd;
         }
         const bool write_u = owner && elem && !fixed_i && !fixed_j;
         // Each element term L0 w_s J_s[a] J_s[b] is added by a fused
