@@ -149,36 +149,40 @@ struct ListPartials {
 // Folds per-block partials in index order (one block of kBlock threads),
 // and the list passes' per-block words, then applies the convergence test to
 // the result when one is given. Thread t folds partials t, t + kReduceBlock,
-// ... in that order; each takes kReduceBatch of its partials (and of its
-// list words) from memory before folding any, so their loads overlap
-// instead of each waiting for the one before.
+// ... in that order. A round requests up to kReduceBatch of its partials and
+// kListBatch of its list words from memory before folding any, so their
+// loads overlap instead of each waiting for the one before; one round covers
+// the 1,280 partials and 3,072 list words of the step's passes (the
+// evaluations leave one partial per 248 nodes, the swept step bound 3,072
+// words), so the kernel waits on memory once.
 constexpr int kReduceBlock = 256;
-constexpr int kReduceBatch = 4;
+constexpr int kReduceBatch = 5;
+constexpr int kListBatch = 12;
 __global__ __launch_bounds__(kReduceBlock) void k_reduce(const Partial *partials, uint32_t count, Partial *out,
                                                          Partial *mirror, ConvergenceTest test, ListPartials lists)
 {
     Partial acc = empty_partial();
-    constexpr uint32_t kStride = kReduceBlock, kSpan = kReduceBlock * kReduceBatch;
-    for (uint32_t i0 = threadIdx.x; i0 < lists.count; i0 += kSpan) {
-        unsigned long long c[kReduceBatch];
-        double a[kReduceBatch];
+    constexpr uint32_t kStride = kReduceBlock, kSpan = kReduceBlock * kReduceBatch,
+                       kListSpan = kReduceBlock * kListBatch;
+    for (uint32_t r = 0; r * kListSpan < lists.count || r * kSpan < count; ++r) {
+        unsigned long long c[kListBatch];
+        double a[kListBatch];
+        Partial o[kReduceBatch];
+        const uint32_t l0 = threadIdx.x + r * kListSpan, i0 = threadIdx.x + r * kSpan;
 #pragma unroll
-        for (int b = 0; b < kReduceBatch; ++b) {
-            const uint32_t i = i0 + b * kStride;
+        for (int b = 0; b < kListBatch; ++b) {
+            const uint32_t i = l0 + b * kStride;
             c[b] = (lists.candidates && i < lists.count) ? lists.candidates[i] : 0ull;
             a[b] = (lists.alpha && i < lists.count) ? lists.alpha[i] : 1.0;
         }
 #pragma unroll
-        for (int b = 0; b < kReduceBatch; ++b) {
+        for (int b = 0; b < kReduceBatch; ++b)
+            o[b] = i0 + b * kStride < count ? partials[i0 + b * kStride] : empty_partial();
+#pragma unroll
+        for (int b = 0; b < kListBatch; ++b) {
             acc.candidates += c[b];
             acc.alpha = fmin(acc.alpha, a[b]);
         }
-    }
-    for (uint32_t i0 = threadIdx.x; i0 < count; i0 += kSpan) {
-        Partial o[kReduceBatch];
-#pragma unroll
-        for (int b = 0; b < kReduceBatch; ++b)
-            o[b] = i0 + b * kStride < count ? partials[i0 + b * kStride] : empty_partial();
 #pragma unroll
         for (int b = 0; b < kReduceBatch; ++b) {
             if (i0 + b * kStride >= count) break;
