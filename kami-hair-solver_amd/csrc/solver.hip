@@ -73,6 +73,13 @@ public:
         count_ = count;
         owned_ = false;
     }
+    // Exchanges the two buffers' memory (no copy).
+    void swap(Buffer &o)
+    {
+        std::swap(ptr_, o.ptr_);
+        std::swap(count_, o.count_);
+        std::swap(owned_, o.owned_);
+    }
     void reset()
     {
         if (ptr_ && owned_) (void)hipFree(ptr_);
@@ -181,7 +188,7 @@ struct DeviceState {
     uint32_t failed_slot = 0;
     dev::TrialStep trial_step{};     // how the next trial's first kernel sets alpha
     // state and solver vectors
-    Buffer<double> q, q_old, vel, q_base, snap_q, snap_v, grad, dir, precond, Dblk, Ublk, Cp, dp, dv;
+    Buffer<double> q, q_old, vel, q_trial, snap_q, snap_v, grad, dir, precond, Dblk, Ublk, Cp, dp, dv;
     // D, U and the gradient in one allocation (D | U | g), which the fused
     // block solve reads through one buffer resource; Dblk, Ublk and grad
     // are views of it
@@ -211,11 +218,12 @@ struct DeviceState {
     bool cache_valid = false;
     // best_tri and list_words hold the list pass at the positions q now has:
     // the accepted line-search trial's, whose check and list pass ran at
-    // q_base + alpha d, the expression the trial evaluation wrote into q. The
+    // alpha d + base, the expression the trial evaluation wrote into q_trial
+    // (now q). The
     // gradient evaluation at that iterate then needs neither again.
     bool closest_at_q = false;
-    // Every contact list covers the swept boxes of q_base to q_base + dir
-    // (set by the swept pass, cleared when dir or q_base change otherwise).
+    // Every contact list covers the swept boxes of q to q + dir (set by the
+    // swept pass, cleared when dir or q change otherwise).
     bool swept_covers = false;
     Buffer<uint32_t> best_tri;  // closest triangle per element from the list pass
     Buffer<double> best_pair;   // and its closest pair (dev::kPairWords per element)
@@ -329,7 +337,7 @@ struct DeviceState {
     size_t resident_bytes() const
     {
         size_t b = 0;
-        for (const Buffer<double> *x : {&frame, &rest, &mass, &rmass, &q, &q_old, &vel, &q_base, &snap_q, &snap_v,
+        for (const Buffer<double> *x : {&frame, &rest, &mass, &rmass, &q, &q_old, &vel, &q_trial, &snap_q, &snap_v,
                                         &grad, &dir, &precond, &Dblk, &Ublk, &Cp, &dp, &dv, &root_cur, &rot_cur,
                                         &root_pend, &rot_pend, &root_anim, &rot_anim, &ring_q, &ring_v, &col_cur,
                                         &col_old, &col_start,
@@ -744,7 +752,7 @@ KhsResult Solver::upload_model()
     // q = (position, zero rotation) from the model build, v = 0 (reference
     // cuda_backend.cu:360-371)
     const std::vector<double> zeros(6 * N, 0.0);
-    for (Buffer<double> *b : {&d.q_old, &d.vel, &d.q_base, &d.snap_q, &d.snap_v, &d.dir, &d.precond, &d.dp})
+    for (Buffer<double> *b : {&d.q_old, &d.vel, &d.q_trial, &d.snap_q, &d.snap_v, &d.dir, &d.precond, &d.dp})
         b->allocate(6 * N);
     d.solve_in.allocate(78 * N);
     d.Dblk.view(d.solve_in.get(), 36 * N);
@@ -927,7 +935,7 @@ void Solver::refresh_contact_cache(int mode, bool full, bool swept_leaves, bool 
     if (full) {
         // every element gathers: one lane each
         KHS_LAUNCH(d, dev::k_contact_refresh<dev::kAtState>, grid, block, 0, d.stream, d.rods(),
-                           swept_leaves ? d.swept_collider() : d.collider(), cc, d.q.get(), d.q_base.get(),
+                           swept_leaves ? d.swept_collider() : d.collider(), cc, d.q.get(), d.q.get(),
                            d.dir.get(), d.alpha.get(), pad, 1);
     } else {
         // the few elements that left their box gather, each with its
@@ -935,11 +943,11 @@ void Solver::refresh_contact_cache(int mode, bool full, bool swept_leaves, bool 
         const ColliderView col = d.collider();
         if (mode == dev::kAtState)
             KHS_LAUNCH(d, dev::k_contact_check<dev::kAtState>, grid, block, 0, d.stream, d.rods(), col, cc,
-                               d.q.get(), d.q_base.get(), d.dir.get(), d.alpha.get(), dev::TrialStep{}, pad,
+                               d.q.get(), d.q.get(), d.dir.get(), d.alpha.get(), dev::TrialStep{}, pad,
                                with_roots ? 1 : 0, (const double *)nullptr, (Partial *)nullptr);
         else if (mode == dev::kAtTrial) {
             KHS_LAUNCH(d, dev::k_contact_check<dev::kAtTrial>, grid, block, 0, d.stream, d.rods(), col, cc,
-                               d.q.get(), d.q_base.get(), d.dir.get(), d.alpha.get(), d.trial_step, pad,
+                               d.q.get(), d.q.get(), d.dir.get(), d.alpha.get(), d.trial_step, pad,
                                with_roots ? 1 : 0, (const double *)nullptr, (Partial *)nullptr);
             d.trial_step.source = dev::kAlphaStored;
         }
@@ -953,7 +961,7 @@ void Solver::refresh_contact_cache(int mode, bool full, bool swept_leaves, bool 
                 d.swept_epoch = 1;
             }
             KHS_LAUNCH(d, dev::k_contact_check<dev::kSwept>, grid, block, 0, d.stream, d.rods(), col, d.cache(),
-                               d.q.get(), d.q_base.get(), d.dir.get(), d.alpha.get(), dev::TrialStep{}, pad,
+                               d.q.get(), d.q.get(), d.dir.get(), d.alpha.get(), dev::TrialStep{}, pad,
                                with_roots ? 1 : 0, (const double *)d.grad.get(), d.partials.get());
         }
     }
@@ -996,14 +1004,14 @@ void Solver::closest_pass(bool line_search, unsigned long long *block_candidates
     const ContactCache cc = d.cache();
     if (line_search) {
         KHS_LAUNCH(d, dev::k_list_closest<dev::kAtTrial>, dim3(kPairBlocks), dim3(kBlock), 0, d.stream,
-                           d.flags.get(), d.collider(), cc, sp, d.q.get(), d.q_base.get(), d.dir.get(),
+                           d.flags.get(), d.collider(), cc, sp, d.q.get(), d.q.get(), d.dir.get(),
                            d.alpha.get(), d.trial_step, d.best_tri.get(), d.best_pair.get(), block_candidates,
                            with_roots ? d.root_elements.get() : nullptr, with_roots ? d.root_element_count : 0u,
                            d.skip());
         d.trial_step.source = dev::kAlphaStored;
     } else {
         KHS_LAUNCH(d, dev::k_list_closest<dev::kAtState>, dim3(kPairBlocks), dim3(kBlock), 0, d.stream,
-                           d.flags.get(), d.collider(), cc, sp, d.q.get(), d.q_base.get(), d.dir.get(),
+                           d.flags.get(), d.collider(), cc, sp, d.q.get(), d.q.get(), d.dir.get(),
                            d.alpha.get(), dev::TrialStep{}, d.best_tri.get(), d.best_pair.get(), block_candidates,
                            with_roots ? d.root_elements.get() : nullptr, with_roots ? d.root_element_count : 0u,
                            (const unsigned *)nullptr);
@@ -1257,11 +1265,14 @@ void Solver::evaluate_async(bool gradient, double dt, bool line_search, unsigned
     if (list_pass && !(gradient && d.closest_at_q)) closest_pass(line_search, d.list_words.get());
     d.closest_at_q = false;
     const uint32_t *best = list_pass ? d.best_tri.get() : nullptr;
+    // A line-search trial reads its base q and writes alpha d + q into
+    // q_trial; the host swaps the two when it accepts the trial.
+    const bool trial_out = line_search && !gradient;
     // the variants without the tree walk when the list pass answered every query
     auto launch = [&](auto kernel, const unsigned *stop) {
         KHS_LAUNCH(d, kernel, dim3(blocks), dim3(kBlock), 0, d.stream, rv, cv, cc, best,
-                           (const double *)d.best_pair.get(), p, d.q.get(),
-                           d.q_old.get(), d.vel.get(), d.q_base.get(), d.dir.get(), d.alpha.get(), d.grad.get(),
+                           (const double *)d.best_pair.get(), p, trial_out ? d.q_trial.get() : d.q.get(),
+                           d.q_old.get(), d.vel.get(), d.q.get(), d.dir.get(), d.alpha.get(), d.grad.get(),
                            d.Dblk.get(), d.Ublk.get(), d.partials.get(), stop);
     };
     // variants with per-strand stiffness weights only when they are set
@@ -1401,11 +1412,11 @@ void Solver::make_direction_async()
     d.toc(t0, d.totals.solve_ms, d.totals.solve_launches);
     // the solve leaves the direction's statistics in one partial per block
     d.reduce_async(blocks, DeviceState::kDirSlot);
-    // the choice, and the line search's base state q_base = q
+    // the choice and the trust scaling (the line search's base is q itself)
     const size_t dofs = 6 * size_t(d.N);
     KHS_LAUNCH(d, dev::k_apply_direction_choice, dim3(grid_for(dofs, kBlock)), dim3(kBlock), 0, d.stream,
                        (const Partial *)(d.reduced.get() + DeviceState::kDirSlot), trust_radius_, d.choice(),
-                       d.mirror_choice(), d.dir.get(), dofs, (const double *)d.q.get(), d.q_base.get());
+                       d.mirror_choice(), d.dir.get(), dofs);
     HIP_TRY(hipGetLastError());
 }
 
@@ -1417,8 +1428,6 @@ void Solver::direction_async()
     }
     DeviceState &d = *dev_;
     d.swept_covers = false;
-    ++d.launch_count;
-    HIP_TRY(hipMemcpyAsync(d.q_base.get(), d.q.get(), d.q.bytes(), hipMemcpyDeviceToDevice, d.stream));
     KHS_LAUNCH(d, dev::k_force_steepest, dim3(1), dim3(1), 0, d.stream, d.choice(), d.mirror_choice());
     HIP_TRY(hipGetLastError());
     steepest_direction_async();
@@ -1485,7 +1494,7 @@ KhsResult Solver::optimize_substep(double dt)
         // The first trial runs at the device's step before the host has
         // seen the bound; one read-back returns direction choice, bound and
         // trial. A trial the host would not have run is discarded, and every
-        // path that does not accept restores q from q_base.
+        // path that does not accept leaves q as it is (trials write q_trial).
         phase_.store(KHS_PHASE_LINE_SEARCH);
         evaluate_async(false, dt, true);
     };
@@ -1522,7 +1531,7 @@ KhsResult Solver::optimize_substep(double dt)
             have_view = true;
             head_ready = false;
         } else {
-            direction_async();  // also q_base = q
+            direction_async();
         }
         ++step_stats_.linear_solves;
         // with the forced rule the first direction already is steepest descent
@@ -1538,8 +1547,6 @@ KhsResult Solver::optimize_substep(double dt)
             if (attempt == 0 && block_direction && !v.choice.used) {
                 // The block direction failed or does not descend: the step
                 // starts from steepest descent instead, with no retry.
-                ++d.launch_count;
-                HIP_TRY(hipMemcpyAsync(d.q.get(), d.q_base.get(), d.q.bytes(), hipMemcpyDeviceToDevice, d.stream));
                 steepest_direction_async();
                 block_direction = false;
                 continue;
@@ -1569,6 +1576,7 @@ KhsResult Solver::optimize_substep(double dt)
                         trial.objective <= current.objective + 1e-4 * alpha * slope) {
                         accepted = true;
                         step_dir_norm2 = trial.dir_norm2;
+                        d.q.swap(d.q_trial);  // the trial is the new iterate
                         break;
                     }
                     alpha *= 0.5;
@@ -1577,8 +1585,6 @@ KhsResult Solver::optimize_substep(double dt)
             }
             if (accepted) break;
             if (!block_direction) break;
-            ++d.launch_count;
-            HIP_TRY(hipMemcpyAsync(d.q.get(), d.q_base.get(), d.q.bytes(), hipMemcpyDeviceToDevice, d.stream));
             KHS_LAUNCH(d, dev::k_force_steepest, dim3(1), dim3(1), 0, d.stream, d.choice(), d.mirror_choice());
             steepest_direction_async();
             ++steepest_directions_;
@@ -1586,8 +1592,6 @@ KhsResult Solver::optimize_substep(double dt)
             ++attempt;
         }
         if (!accepted) {
-            ++d.launch_count;
-            HIP_TRY(hipMemcpyAsync(d.q.get(), d.q_base.get(), d.q.bytes(), hipMemcpyDeviceToDevice, d.stream));
             return fail(KHS_ERROR_NOT_CONVERGED, "Gauss-Newton line search failed.");
         }
         // The gradient at the accepted iterate (whose closest triangles the
@@ -1609,11 +1613,7 @@ KhsResult Solver::optimize_substep(double dt)
         ++step_stats_.newton_iterations;
         phase_.store(KHS_PHASE_ASSEMBLING);
         if (step_stats_.increment_norm / std::sqrt(dofs) <= 2.0 * desc_.increment_tolerance) {
-            // a trial enqueued ahead may have moved q: back to the iterate
-            if (head_ready) {
-                ++d.launch_count;
-                HIP_TRY(hipMemcpyAsync(d.q.get(), d.q_base.get(), d.q.bytes(), hipMemcpyDeviceToDevice, d.stream));
-            }
+            // (a trial enqueued ahead wrote only q_trial)
             KHS_LAUNCH(d, dev::k_update_velocity, dim3(d.node_blocks()), dim3(kBlock), 0, d.stream, rv,
                                d.q.get(), d.q_old.get(), d.vel.get(), 1.0 / dt);
             HIP_TRY(hipGetLastError());

@@ -2530,11 +2530,9 @@ __global__ void k_choose_direction(const Partial *stats, double trust, Direction
 
 // k_choose_direction and the block direction's scaling in one pass: every
 // thread derives the choice from the reduced statistics, one records it.
-// Also saves the line search's base state, q_base = q. The choice is
-// written to the device copy and to its host mirror.
+// The choice is written to the device copy and to its host mirror.
 __global__ void k_apply_direction_choice(const Partial *stats, double trust, DirectionChoice *choice,
-                                         DirectionChoice *mirror, double *x, size_t count, const double *q,
-                                         double *q_base)
+                                         DirectionChoice *mirror, double *x, size_t count)
 {
     const double slope = stats->sum[kSlope];
     const int used = !stats->failed && slope < 0 && isfinite(slope);
@@ -2546,10 +2544,7 @@ __global__ void k_apply_direction_choice(const Partial *stats, double trust, Dir
         mirror->used = used;
         mirror->scale = scale;
     }
-    if (i < count) {
-        if (used && scale != 1.0) x[i] *= scale;
-        if (q_base) q_base[i] = q[i];
-    }
+    if (i < count && used && scale != 1.0) x[i] *= scale;
 }
 
 __global__ void k_force_steepest(DirectionChoice *choice, DirectionChoice *mirror)
